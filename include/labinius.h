@@ -1,0 +1,186 @@
+/*
+ * labinius on MI355X: the C ABI of the commit / evaluate / fold prover.
+ *
+ * Each entry point replaces one entry point of the reference's Rust API (paths relative to the
+ * reference checkout, crates/pcs/src/...). Signatures use plain pointers and sizes only, so that
+ * a Rust `extern "C"` block (or ctypes, or cgo) binds them unchanged.
+ *
+ * Conventions
+ *   - Every function returns an int status (LBN_OK = 0) unless noted; no exception or panic
+ *     crosses the ABI. Where the reference panics (wrong witness length, a point of the wrong
+ *     shape, a mismatched opening), these return LBN_ERR_SHAPE / LBN_ERR_WITNESS_LENGTH.
+ *     lbn_last_error() gives a thread-local message for the last failure.
+ *   - An lbn_pp is immutable after creation and may be shared by provers; an lbn_prover is
+ *     single-threaded (it owns one HIP stream), as the Rust Prover takes &mut self.
+ *   - An lbn_opening is move-only: lbn_fold and lbn_recycle consume it.
+ *   - Layouts
+ *       F162            3 little-endian u64, bits 0..161, the top 30 bits zero (fields/scalar.rs:10)
+ *       components      [column c][row t][limb k][slot s] int16, centered: entry (t, c) of the
+ *                       4 x r VerticallyAlignedMatrix, one R_162 element per limb in primes()
+ *                       order (key.rs:226-233, ring/element.rs:74-117)
+ *       raw             [limb k][column c][648] u32 in [0, q_k) (key.rs:279-280)
+ *       key             [limb k][ring element i][slot j] int16, centered (key.rs:135-141)
+ *       folded witness  [ring element i][coefficient 648] int16, centered mod the base prime
+ *                       (commitment.rs:253-279)
+ *       folded comm.    [row t][limb k][slot s] int16, centered (commitment.rs:283-286)
+ */
+#ifndef LABINIUS_H
+#define LABINIUS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LBN_OK = 0,
+    /* ParamError, scheme/params.rs:22-33 */
+    LBN_ERR_COLUMNS_EXCEED_WITNESS = 1,
+    LBN_ERR_TOO_FEW_COLUMNS = 2,
+    LBN_ERR_COLUMN_TOO_SHORT = 3,
+    LBN_ERR_DUPLICATE_MODULUS = 4,
+    LBN_ERR_BASE_IS_ALSO_EXTRA = 5,
+    LBN_ERR_UNKNOWN_MODULUS = 6,     /* not one of ring/mod.rs:87-95 */
+    /* WitnessError::WrongLength, scheme/witness.rs:12-15 */
+    LBN_ERR_WITNESS_LENGTH = 7,
+    /* a reference assert!: mismatched shapes, points, openings */
+    LBN_ERR_SHAPE = 8,
+    LBN_ERR_HIP = 9,
+    LBN_ERR_OOM = 10,
+    LBN_ERR_UNSUPPORTED = 11,        /* e.g. Opening::Recursive (LaBRADOR stays out of scope) */
+    LBN_ERR_ARG = 12,
+    /* VerificationError::Rejected, scheme/commitment.rs:449-451 */
+    LBN_REJECTED = 13,
+    LBN_ERR_NO_DEVICE = 14
+};
+
+enum { LBN_OPENING_CLEAR = 0, LBN_OPENING_BIT_DROPPED = 1, LBN_OPENING_RECURSIVE = 2 };
+
+typedef struct { uint64_t w[3]; } lbn_f162;
+
+/* ShortChallenge, challenge.rs:231-235 */
+typedef struct {
+    uint8_t positions[32];
+    uint32_t signs;
+    uint32_t weight;
+} lbn_challenge;
+
+/* Params, scheme/params.rs:62-68 (moduli by prime: 2917 3889 4861 9721 12637 17497 19441) */
+typedef struct {
+    uint32_t witness_log_len;
+    uint32_t column_log_len;
+    uint32_t base;
+    uint32_t n_extra;
+    uint32_t extra[6];
+    uint32_t opening;
+    uint32_t dropped_bits;
+} lbn_params;
+
+typedef struct lbn_pp lbn_pp;
+typedef struct lbn_prover lbn_prover;
+typedef struct lbn_opening lbn_opening;
+typedef struct lbn_dwitness lbn_dwitness;
+typedef struct lbn_transcript lbn_transcript;
+
+const char *lbn_last_error(void);
+
+/* Params::with_base's checks, scheme/params.rs:172-203; returns a ParamError code or LBN_OK */
+int lbn_params_check(const lbn_params *params);
+
+/* PublicParameters::from_seed, scheme/params.rs:320-339: expands A (key.rs:48-88) and uploads it
+ * to the current HIP device in the layout the commit kernel streams; refcounted. */
+int lbn_pp_create(const lbn_params *params, const uint8_t matrix_seed[32], lbn_pp **out);
+void lbn_pp_release(lbn_pp *pp);
+/* CommitmentKey::row for every limb, key.rs:139-141 ([limb][i][648] int16, centered) */
+int lbn_pp_key(const lbn_pp *pp, int16_t *a, size_t len);
+
+/* Witness::random, scheme/witness.rs:66-87 (host only) */
+int lbn_witness_random(const lbn_params *params, const uint8_t seed[32], lbn_f162 *out, size_t n);
+
+/* Prover::new, scheme/prover.rs:31-53: the device workspace, one stream, and a warm-up round over
+ * a zero witness that runs every kernel once. */
+int lbn_prover_create(lbn_pp *pp, lbn_prover **out);
+void lbn_prover_release(lbn_prover *prover);
+
+/* A witness resident in HBM (Witness::from_elements, scheme/witness.rs:54-62, uploaded once).
+ * lbn_dwitness_device_ptr exposes the device copy (n x 24 bytes) for callers that fill it on the
+ * device themselves. */
+int lbn_witness_upload(lbn_prover *prover, const lbn_f162 *w, size_t n, lbn_dwitness **out);
+void *lbn_dwitness_device_ptr(lbn_dwitness *w);
+void lbn_dwitness_release(lbn_dwitness *w);
+
+/* Prover::commit, scheme/prover.rs:55-91 (Clear mode; the commitment matrix and the raw rows).
+ * The opening keeps the witness's bit layout the fold reads on the device. Either output pointer
+ * may be NULL. lbn_commit takes a host witness (uploaded on every call); lbn_commit_resident a
+ * resident one. */
+int lbn_commit(lbn_prover *prover, const lbn_f162 *w, size_t n, int16_t *components,
+               uint32_t *raw, lbn_opening **opening);
+int lbn_commit_resident(lbn_prover *prover, const lbn_dwitness *w, int16_t *components,
+                        uint32_t *raw, lbn_opening **opening);
+
+/* Prover::fold, scheme/prover.rs:120-133: consumes the opening; v is len_ring x 648 int16. */
+int lbn_fold(lbn_prover *prover, lbn_opening *opening, const lbn_challenge *challenges, size_t r,
+             int16_t *v);
+/* Prover::recycle, scheme/prover.rs:116-118 */
+int lbn_recycle(lbn_prover *prover, lbn_opening *opening);
+
+/* Witness::row_evaluate / mle_evaluate, scheme/witness.rs:94-105: u has 2^column_log_len F162;
+ * p0 has row_log_len coordinates, p1 column_log_len. */
+int lbn_row_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len,
+                     lbn_f162 *u);
+int lbn_mle_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len,
+                     const lbn_f162 *p1, size_t p1_len, lbn_f162 *t);
+
+/* Device time of each stage of the prover's last commit / fold / row evaluation, in ms, measured
+ * with HIP events on the prover's stream:
+ *   [0] commit total  [1] commit layout pass  [2] commit GEMM  [3] commit finish
+ *   [4] fold total    [5] challenge transform [6] fold GEMM    [7] inverse NTT
+ *   [8] eval total    [9] eq table            [10] row reduce */
+int lbn_prover_timings(const lbn_prover *prover, float *ms, size_t n);
+
+/* Transcript, challenge.rs:66-160 */
+lbn_transcript *lbn_transcript_new(const uint8_t *domain, size_t len);
+lbn_transcript *lbn_transcript_clone(const lbn_transcript *t);
+void lbn_transcript_free(lbn_transcript *t);
+void lbn_transcript_absorb_bytes(lbn_transcript *t, const uint8_t *b, size_t len);
+void lbn_transcript_absorb_u64(lbn_transcript *t, uint64_t x);
+void lbn_transcript_fill(lbn_transcript *t, const uint8_t *label, size_t label_len, uint8_t *out,
+                         size_t len);
+/* sample_short_challenge, challenge.rs:517-533; returns the number of attempts */
+uint64_t lbn_sample_short_challenge(lbn_transcript *t, uint32_t weight, double bound,
+                                    lbn_challenge *out);
+
+/* The verifier (scheme/verifier.rs), on the host. */
+/* verifier.rs:61-100 */
+int lbn_derive_evaluation_point(const lbn_pp *pp, lbn_transcript *t, const int16_t *components,
+                                lbn_f162 *p0, lbn_f162 *p1);
+/* FoldingChallenges::derive, commitment.rs:217-228, over a RowEvaluation (commitment.rs:320-331) */
+int lbn_derive_folding_challenges(const lbn_pp *pp, lbn_transcript *t, const lbn_f162 *u, size_t r,
+                                  lbn_challenge *out);
+/* verifier.rs:114-146 */
+int lbn_fold_commitment(const lbn_pp *pp, const int16_t *components, const lbn_challenge *c,
+                        size_t r, int16_t *out);
+/* verifier.rs:149-155 */
+int lbn_fold_row_evaluation(const lbn_f162 *u, const lbn_challenge *c, size_t r, lbn_f162 *out);
+/* verifier.rs:158-174: LBN_OK or LBN_REJECTED */
+int lbn_verify_evaluation(const lbn_pp *pp, const lbn_f162 *p1, size_t p1_len,
+                          const lbn_f162 *claimed, const lbn_f162 *u, size_t r);
+/* verifier.rs:178-231 (Clear mode): LBN_OK or LBN_REJECTED */
+int lbn_verify_opening_clear(const lbn_pp *pp, const int16_t *folded_commitment, const int16_t *v,
+                             size_t len_ring, const lbn_f162 *p0, size_t p0_len,
+                             const lbn_f162 *folded_row_value);
+
+/* BLAKE3 as the transcript and the seed expansions use it: mode 0 hash, 1 keyed (key = 32 bytes),
+ * 2 derive_key (key = NUL-terminated context). */
+int lbn_blake3(int mode, const uint8_t *key, const uint8_t *in, size_t len, uint8_t *out,
+               size_t out_len);
+/* F162 product, fields/scalar.rs:92-116 */
+void lbn_f162_mul(const lbn_f162 *a, const lbn_f162 *b, lbn_f162 *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

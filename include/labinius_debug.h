@@ -1,0 +1,31 @@
+/*
+ * Test-only introspection of liblabinius.so. No counterpart in the reference: it exists so that
+ * the CPU test suite can replay the GEMM tiles' tables (built on the host) without a device.
+ */
+#ifndef LABINIUS_DEBUG_H
+#define LABINIUS_DEBUG_H
+
+#include "labinius.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* which: 0 commit A fragments (int8), 1 commit LUTs (u16), 2 commit epilogue weights (u16),
+ * 3 commit output slots (i16), 4 commit tile descriptors, 5 fold LUTs, 6 fold epilogue weights,
+ * 7 fold output slots, 8 fold root exponents (u16), 9 fold tile descriptors, 10 finish constants,
+ * 11 the base root's powers (u16), 12 inverse-transform constants, 13 inverse twiddles (u32),
+ * 14 the expanded key A ([limb][len_ring][648] int16).
+ * With dst == NULL only *bytes is set. */
+int lbn_debug_tables(const lbn_params *params, const uint8_t seed[32], int which, void *dst, size_t cap,
+                     size_t *bytes);
+
+/* One v_mfma_i32_16x16x64_i8 on the device with the operand and result lane layouts the tile GEMM
+ * uses: a is 16 x 64 (row-major), b is 64 x 16, d receives the 16 x 16 int32 product. */
+int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,131 @@
+// Internal declarations shared by the host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/labinius.h"
+#include "../../include/labinius_debug.h"
+#include "blake3.hpp"
+#include "device.hpp"
+#include "ring.hpp"
+
+namespace lbn {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+// Params as the host uses them (scheme/params.rs:62-68, 254-287).
+struct Shape {
+    uint32_t wl = 0, cl = 0, rows = 0;
+    size_t n = 0, r = 0, column_len = 0, len_ring = 0;
+    std::vector<uint32_t> primes;  // base first
+    uint32_t dropped_bits = 0;
+    uint32_t opening = 0;
+};
+int make_shape(const lbn_params *p, Shape *s);
+
+// xorshift64* (rng.rs:1-20)
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull | 1) {}
+    uint64_t next() {
+        uint64_t x = s;
+        x ^= x >> 12;
+        x ^= x << 25;
+        x ^= x >> 27;
+        s = x;
+        return x * 0x2545F4914F6CDD1Dull;
+    }
+    uint32_t below(uint32_t n) { return (uint32_t)(((next() >> 32) * (uint64_t)n) >> 32); }
+};
+
+// CommitmentKey::random (key.rs:48-88): a[k][i][j], i = 32 b + p drawn b, j, p.
+std::vector<int16_t> expand_key(const std::vector<uint32_t> &primes, size_t len_ring,
+                                const uint8_t seed[32]);
+
+// The transcript (challenge.rs:66-160).
+struct Transcript {
+    Blake3 state = Blake3::hasher();
+    uint64_t counter = 0;
+    void absorb_bytes(const void *b, size_t n) {
+        state.update_u64(n);
+        state.update(b, n);
+    }
+    void absorb_u64(uint64_t x) { state.update_u64(x); }
+    Blake3::Output reader(const char *label, size_t label_len);
+    void fill(const char *label, size_t label_len, uint8_t *out, size_t n) {
+        reader(label, label_len).read(0, out, n);
+    }
+};
+
+uint64_t sample_short_challenge(Transcript &t, uint32_t weight, double bound, lbn_challenge *out);
+
+}  // namespace lbn
+
+struct lbn_transcript {
+    lbn::Transcript t;
+};
+
+// PublicParameters plus everything the kernels read, resident on one device.
+struct lbn_pp {
+    std::atomic<int> refs{1};
+    lbn_params params{};
+    lbn::Shape shape;
+    uint8_t seed[32];
+    std::vector<int16_t> key;  // [limb][len_ring][648], centered
+    int device = 0;
+    uint32_t lpad = 0, rpad = 0, kb_c = 0, kb_f = 0;
+    // commit GEMM
+    uint32_t ntiles = 0;
+    int8_t *d_afrag = nullptr;
+    uint16_t *d_lut = nullptr, *d_epi = nullptr;
+    int16_t *d_out = nullptr;
+    lbn::TileDesc *d_tiles = nullptr;
+    lbn::FinishLimb *d_finish = nullptr;
+    uint16_t *d_slot648 = nullptr, *d_qcs = nullptr;
+    // fold GEMM (base limb)
+    uint32_t ntiles_f = 0;
+    uint16_t *d_lut_f = nullptr, *d_epi_f = nullptr, *d_expo_f = nullptr, *d_pw_f = nullptr;
+    int16_t *d_out_f = nullptr;
+    lbn::TileDesc *d_tiles_f = nullptr;
+    uint32_t cond_f = 0, mult_f = 0;
+    lbn::InttConsts intt{};
+    uint32_t *d_twi = nullptr;
+    std::vector<void *> allocations;
+};
+
+struct lbn_dwitness {
+    lbn_prover *owner = nullptr;
+    uint64_t *d_w = nullptr;
+    size_t n = 0;
+};
+
+struct lbn_opening {
+    lbn_prover *owner = nullptr;
+    uint8_t *d_bytes_f = nullptr;  // the witness in the fold's byte layout
+};
+
+struct lbn_prover {
+    lbn_pp *pp = nullptr;
+    hipStream_t stream = nullptr;
+    // workspace
+    uint8_t *d_bytes_c = nullptr;
+    std::vector<uint8_t *> free_bytes_f;  // recycled fold layouts
+    uint32_t *d_yacc = nullptr, *d_raw = nullptr;
+    int16_t *d_comp = nullptr;
+    lbn_challenge *d_ch = nullptr;
+    int8_t *d_afrag_f = nullptr;
+    uint32_t *d_vhat = nullptr;
+    int16_t *d_v = nullptr;
+    uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_u = nullptr;
+    uint64_t *d_wscratch = nullptr;  // host-witness uploads
+    hipEvent_t ev[12] = {};
+    float timings[11] = {};
+    uint32_t splits_c = 1;
+};

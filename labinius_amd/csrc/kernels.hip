@@ -1,0 +1,586 @@
+// HIP kernels for gfx950 (MI355X): the commit / fold / row-evaluation path.
+//
+//   prepass_kernel        witness (F162, 24 B each) -> the two byte layouts the GEMMs stream
+//   tile_gemm_kernel      the int8-MFMA tile GEMM behind both the commitment and the fold
+//   commit_finish_kernel  y mod q -> raw rows and the 648 -> 4 x 162 decomposition
+//   chat_kernel           the challenges' transform c_j(-X^4) at every slot, as MFMA A fragments
+//   intt_kernel           the base prime's inverse transform -> centered folded witness
+//   eq_kernel             the eq tensor over F162
+//   row_eval_kernel       u_j = sum_i eq_i W[i + wdim j] over F162
+//
+// The algebra of the tile GEMM is in device.hpp.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "launch.hpp"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+namespace lbn {
+
+// =============================================================================================
+// layout pass
+// =============================================================================================
+
+// The byte of ring element i of column c at position n < 81: bit t is coefficient 81 t + n, and
+// coefficient 4 m + k of a ring element is bit m of its F162 element k (f162.rs:1-16). Written
+// twice: bytes_c[c][n][lpad] (the commit's K = ring elements) and bytes_f[i][n][rpad] (the fold's
+// K = columns). A block covers 8 ring elements x 32 columns and stages its 2592 bytes in LDS so
+// both stores are runs of contiguous bytes.
+__global__ __launch_bounds__(256) void prepass_kernel(const uint64_t *__restrict__ w,
+                                                      uint32_t column_len, uint32_t L, uint32_t r,
+                                                      uint8_t *__restrict__ bytes_c, uint32_t lpad,
+                                                      uint8_t *__restrict__ bytes_f, uint32_t rpad) {
+    __shared__ __attribute__((aligned(16))) uint8_t st[32][8][84];  // [c][i][n]
+    const int ii = threadIdx.x >> 5, cc = threadIdx.x & 31;
+    const uint32_t i = blockIdx.y * 8 + ii, c = blockIdx.x * 32 + cc;
+    uint32_t wd[4][6];
+    if (i < L && c < r) {
+        const uint64_t *e = w + 3 * ((size_t)c * column_len + 4 * (size_t)i);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                uint64_t x = e[3 * k + h];
+                wd[k][2 * h] = (uint32_t)x;
+                wd[k][2 * h + 1] = (uint32_t)(x >> 32);
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int h = 0; h < 6; h++) wd[k][h] = 0;
+    }
+#pragma unroll
+    for (int n = 0; n < 81; n++) {
+        uint32_t b = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int m = 81 * t + n, k = m & 3, mp = m >> 2;
+            b |= ((wd[k][mp >> 5] >> (mp & 31)) & 1u) << t;
+        }
+        st[cc][ii][n] = (uint8_t)b;
+    }
+    __syncthreads();
+    // bytes_c: for each (c, n) the 8 bytes of ring elements i0..i0+7
+    const uint32_t i0 = blockIdx.y * 8, c0 = blockIdx.x * 32;
+    for (int u = threadIdx.x; u < 32 * 81; u += 256) {
+        const int cl = u / 81, n = u % 81;
+        const uint32_t cg = c0 + cl;
+        if (cg >= r || i0 >= L) continue;
+        uint64_t v = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v |= (uint64_t)st[cl][q][n] << (8 * q);
+        *(uint64_t *)(bytes_c + ((size_t)cg * 81 + n) * lpad + i0) = v;
+    }
+    if (!bytes_f) return;
+    // bytes_f: for each (i, n) the 32 bytes of columns c0..c0+31
+    for (int u = threadIdx.x; u < 8 * 81 * 2; u += 256) {
+        const int half = u & 1, rest = u >> 1, il = rest / 81, n = rest % 81;
+        const uint32_t ig = i0 + il;
+        if (ig >= L) continue;
+        uint32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int cb = 16 * half + 4 * q;
+            v[q] = (uint32_t)st[cb][il][n] | (uint32_t)st[cb + 1][il][n] << 8 |
+                   (uint32_t)st[cb + 2][il][n] << 16 | (uint32_t)st[cb + 3][il][n] << 24;
+        }
+        *(uint4 *)(bytes_f + ((size_t)ig * 81 + n) * rpad + c0 + 16 * half) =
+            make_uint4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+// =============================================================================================
+// the tile GEMM
+// =============================================================================================
+
+__device__ __forceinline__ long long shfl_xor64(long long v, int m) {
+    int lo = __shfl_xor((int)(unsigned long long)v, m, 64);
+    int hi = __shfl_xor((int)((unsigned long long)v >> 32), m, 64);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// One 96 x 96 tile of Z = sum_k A[k][m] x_k[n] for one batch element (a column for the commit,
+// a ring element for the fold) over the K steps of blockIdx.z, followed by the epilogue
+// y[out[m]] += sum_n E[m][n] Z[m][n] mod q.
+//
+// 256 threads = 4 waves in a 2 x 2 grid; wave (wm, wn) owns M blocks 3 wm .. 3 wm + 2 and N
+// blocks 3 wn .. 3 wn + 2 of 16. Operands are two signed 7-bit digits each, so every 16 x 16
+// block takes four v_mfma_i32_16x16x64_i8 per K step into three accumulators (low x low,
+// the two cross terms, high x high); the epilogue weights carry the 1, 2^7, 2^14 scale.
+// The B operand (x digits) is built once per K step into LDS by table lookup on the byte
+// layout; the A operand fragments are read straight from global memory in MFMA lane order.
+__global__ __launch_bounds__(256) void tile_gemm_kernel(GemmArgs g) {
+    // tiles vary fastest, so the blocks of one batch element run together and share its bytes
+    const TileDesc td = g.tiles[blockIdx.x % g.ntiles];
+    const uint32_t batch = blockIdx.x / g.ntiles;
+    const uint32_t kb0 = blockIdx.z * g.kb_per_split;
+    const uint32_t kb1 = min(kb0 + g.kb_per_split, g.kb_total);
+
+    __shared__ uint16_t lut[512];
+    __shared__ __attribute__((aligned(16))) uint8_t bt[NBLK * 2 * 64 * 16];
+    __shared__ long long rowsum[TILE];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    for (int x = tid; x < 512; x += 256) lut[x] = g.lut[td.lut_off + x];
+    if (tid < TILE) rowsum[tid] = 0;
+
+    v4i acc[3][3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) acc[a][b][d] = (v4i){0, 0, 0, 0};
+
+    const uint8_t *bbase = g.bytes + (size_t)batch * g.bytes_batch;
+    const int8_t *abase = g.afrag + td.a_off;
+    __syncthreads();
+
+    for (uint32_t kb = kb0; kb < kb1; kb++) {
+        // B tile: unit u = (nb, lane, half) -> 8 lookups
+#pragma unroll
+        for (int it = 0; it < 3; it++) {
+            const int u = tid + 256 * it;
+            const int nb = u >> 7, rem = u & 127, ln = rem >> 1, h = rem & 1;
+            const int n = nb * 16 + (ln & 15);
+            uint32_t xl0 = 0, xl1 = 0, xh0 = 0, xh1 = 0;
+            if (n < TILE_USED) {
+                const uint32_t k0 = kb * KSTEP + 16 * (ln >> 4) + 8 * h;
+                const uint2 raw = *(const uint2 *)(bbase + (size_t)n * g.kpad + k0);
+                const uint16_t *L = lut + (((td.par + n) & 1) << 8);
+                uint32_t v0 = L[raw.x & 255], v1 = L[(raw.x >> 8) & 255];
+                uint32_t v2 = L[(raw.x >> 16) & 255], v3 = L[raw.x >> 24];
+                uint32_t v4 = L[raw.y & 255], v5 = L[(raw.y >> 8) & 255];
+                uint32_t v6 = L[(raw.y >> 16) & 255], v7 = L[raw.y >> 24];
+                // p01 = [v0.b0, v1.b0, v0.b1, v1.b1], then split the low and high digits
+                uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
+                uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
+                uint32_t p45 = __builtin_amdgcn_perm(v5, v4, 0x05010400u);
+                uint32_t p67 = __builtin_amdgcn_perm(v7, v6, 0x05010400u);
+                xl0 = __builtin_amdgcn_perm(p23, p01, 0x05040100u);
+                xh0 = __builtin_amdgcn_perm(p23, p01, 0x07060302u);
+                xl1 = __builtin_amdgcn_perm(p67, p45, 0x05040100u);
+                xh1 = __builtin_amdgcn_perm(p67, p45, 0x07060302u);
+            }
+            *(uint2 *)(bt + ((nb * 2 + 0) * 64 + ln) * 16 + 8 * h) = make_uint2(xl0, xl1);
+            *(uint2 *)(bt + ((nb * 2 + 1) * 64 + ln) * 16 + 8 * h) = make_uint2(xh0, xh1);
+        }
+        v4i a[3][2];
+        const int8_t *ak = abase + (size_t)kb * g.a_kb_stride;
+#pragma unroll
+        for (int mbi = 0; mbi < 3; mbi++)
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+                a[mbi][d] = *(const v4i *)(ak + (((3 * wm + mbi) * 2 + d) * 64 + lane) * 16);
+        __syncthreads();
+#pragma unroll
+        for (int nbi = 0; nbi < 3; nbi++) {
+            const int nb = 3 * wn + nbi;
+            const v4i xl = *(const v4i *)(bt + ((nb * 2 + 0) * 64 + lane) * 16);
+            const v4i xh = *(const v4i *)(bt + ((nb * 2 + 1) * 64 + lane) * 16);
+#pragma unroll
+            for (int mbi = 0; mbi < 3; mbi++) {
+                acc[mbi][nbi][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][0], xl, acc[mbi][nbi][0], 0, 0, 0);
+                acc[mbi][nbi][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][0], xh, acc[mbi][nbi][1], 0, 0, 0);
+                acc[mbi][nbi][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][1], xl, acc[mbi][nbi][1], 0, 0, 0);
+                acc[mbi][nbi][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][1], xh, acc[mbi][nbi][2], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: the D fragment of a 16 x 16 block holds row 4 (lane >> 4) + r, column lane & 15
+    const uint16_t *ep = g.epi + td.epi_off;
+#pragma unroll
+    for (int mbi = 0; mbi < 3; mbi++) {
+        const int mb = 3 * wm + mbi;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            long long s = 0;
+#pragma unroll
+            for (int nbi = 0; nbi < 3; nbi++) {
+                const int nb = 3 * wn + nbi;
+                const uint16_t *e = ep + (((mb * NBLK + nb) * 64 + lane) * 4 + r) * 3;
+                s += (long long)e[0] * acc[mbi][nbi][0][r] + (long long)e[1] * acc[mbi][nbi][1][r] +
+                     (long long)e[2] * acc[mbi][nbi][2][r];
+            }
+            s += shfl_xor64(s, 8);
+            s += shfl_xor64(s, 4);
+            s += shfl_xor64(s, 2);
+            s += shfl_xor64(s, 1);
+            if ((lane & 15) == 0)
+                atomicAdd((unsigned long long *)&rowsum[mb * 16 + 4 * (lane >> 4) + r],
+                          (unsigned long long)s);
+        }
+    }
+    __syncthreads();
+    if (tid < TILE) {
+        const int slot = g.out[td.out_off + tid];
+        if (slot >= 0) {
+            const long long qq = td.q;
+            long long m = rowsum[tid] % qq;
+            if (m < 0) m += qq;
+            uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
+            if (g.accumulate)
+                atomicAdd(yp, (uint32_t)m);
+            else
+                *yp = (uint32_t)m;
+        }
+    }
+}
+
+// =============================================================================================
+// the commitment's finish: y mod q, raw rows, and the decomposition
+// =============================================================================================
+
+__device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b, uint32_t q) {
+    return (uint32_t)(((uint64_t)a * b) % q);
+}
+
+__device__ __forceinline__ int16_t centered(uint32_t x, uint32_t q) {
+    return x > (q - 1) / 2 ? (int16_t)((int)x - (int)q) : (int16_t)x;
+}
+
+// block (c, limb), thread s < 162 (ring/mod.rs:274-312 split, 383-417 quadratic)
+__global__ void commit_finish_kernel(const uint32_t *__restrict__ yacc, uint32_t limbs,
+                                     const FinishLimb *__restrict__ fl,
+                                     const uint16_t *__restrict__ slot648,
+                                     const uint16_t *__restrict__ qcs, uint32_t *__restrict__ raw,
+                                     int16_t *__restrict__ comp, uint32_t cols) {
+    const uint32_t c = blockIdx.x, k = blockIdx.y, s = threadIdx.x;
+    if (s >= 162) return;
+    const FinishLimb &f = fl[k];
+    const uint32_t q = f.q;
+    const uint32_t *y = yacc + ((size_t)c * limbs + k) * 648;
+    uint32_t *rw = raw + ((size_t)k * cols + c) * 648;
+    int16_t *out = comp + (size_t)c * 4 * limbs * 162;  // [t][limb][s]
+    if (!f.quad) {
+        uint32_t e[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const uint32_t j = slot648[t * 162 + s];
+            e[t] = y[j] % q;
+            rw[j] = e[t];
+        }
+        const uint32_t a = e[0] + e[2], d0 = e[0] + q - e[2], cc = e[1] + e[3], d1 = e[1] + q - e[3];
+        const uint32_t id = mulmod(d1, f.ii, q);
+        const uint32_t m[4] = {(a + cc) % q, (d0 + q - id) % q, (a + 2 * q - cc) % q, (d0 + id) % q};
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            out[((size_t)t * limbs + k) * 162 + s] = centered(mulmod(m[t], f.twist[t][s], q), q);
+    } else {
+        const uint32_t jp = qcs[s], jm = qcs[162 + s];
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+            const uint32_t ep = y[2 * jp + kk] % q, em = y[2 * jm + kk] % q;
+            rw[2 * jp + kk] = ep;
+            rw[2 * jm + kk] = em;
+            out[((size_t)kk * limbs + k) * 162 + s] = centered(mulmod(ep + em, f.inv2, q), q);
+            out[((size_t)(kk + 2) * limbs + k) * 162 + s] = centered(mulmod(ep + q - em, f.twist[0][s], q), q);
+        }
+    }
+}
+
+// =============================================================================================
+// the fold's challenge transform
+// =============================================================================================
+
+// The A fragments of the fold GEMM: tile tau, K = column c, M = row m; the value at a row is
+// c(-X^4) at the row's root, sum_i (-1)^{p_i} (1 - 2 s_i) root^{mult expo[m] p_i} (fold.rs:287-300
+// embeds c as the coefficient (-1)^m c_m of X^{4m}). One thread per (tile, column, row).
+__global__ void chat_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r, uint32_t rpad,
+                            const uint16_t *__restrict__ expo, const uint16_t *__restrict__ pw,
+                            uint32_t cond, uint32_t mult, uint32_t q, int8_t *__restrict__ afrag,
+                            uint32_t ntiles) {
+    const uint32_t total = ntiles * rpad * TILE;
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= total) return;
+    const uint32_t m = id % TILE, c = (id / TILE) % rpad, tau = id / (TILE * rpad);
+    int x = 0;
+    const uint32_t ex = expo[tau * TILE + m];
+    if (c < r && ex != 0xFFFF) {
+        const lbn_challenge_dev &cc = ch[c];
+        int acc = 0;
+        for (uint32_t i = 0; i < cc.weight; i++) {
+            const uint32_t p = cc.positions[i];
+            const int co = (1 - 2 * (int)((cc.signs >> i) & 1)) * ((p & 1) ? -1 : 1);
+            acc += co * (int)pw[(mult * ex * p) % cond];
+        }
+        int v = acc % (int)q;
+        if (v < 0) v += q;
+        x = v > (int)(q - 1) / 2 ? v - (int)q : v;
+    }
+    const int xl = ((x + 64) & 127) - 64;
+    const int xh = (x - xl) / 128;
+    const uint32_t kbs = rpad / KSTEP;
+    const uint32_t kb = c / KSTEP, kk = c % KSTEP, g4 = kk / 16, b = kk % 16;
+    const uint32_t mb = m / 16, lanev = 16 * g4 + (m % 16);
+    const size_t base = ((size_t)tau * kbs + kb) * (size_t)(NBLK * 2 * FRAG);
+    afrag[base + ((mb * 2 + 0) * 64 + lanev) * 16 + b] = (int8_t)xl;
+    afrag[base + ((mb * 2 + 1) * 64 + lanev) * 16 + b] = (int8_t)xh;
+}
+
+// =============================================================================================
+// the inverse transform of the base prime (scalar.rs:123-177, recursion/limbs.rs:76-127)
+// =============================================================================================
+
+__device__ __forceinline__ uint32_t mm(uint32_t a, uint32_t b, uint32_t q) {
+    return (uint32_t)(((uint64_t)a * b) % q);
+}
+
+__global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ vhat, InttConsts ic,
+                                                   const uint32_t *__restrict__ twi,
+                                                   int16_t *__restrict__ v) {
+    __shared__ uint32_t u[648];
+    const uint32_t i = blockIdx.x;
+    const uint32_t q = ic.q;
+    for (int x = threadIdx.x; x < 648; x += 256) u[x] = vhat[(size_t)i * 648 + x] % q;
+    __syncthreads();
+    const int radix_s[7] = {2, 2, 2, 3, 3, 3, 3};
+    const int degree_s[8] = {648, 324, 162, 81, 27, 9, 3, 1};
+    const int radix_q[6] = {2, 2, 3, 3, 3, 3};
+    const int degree_q[7] = {648, 324, 162, 54, 18, 6, 2};
+    const int top = ic.quad ? 5 : 6;
+    for (int level = top; level >= 1; level--) {
+        const int n = ic.quad ? degree_q[level] : degree_s[level];
+        const int rad = ic.quad ? radix_q[level] : radix_s[level];
+        const int m = n / rad;
+        const int nbf = 648 / rad;
+        for (int b = threadIdx.x; b < nbf; b += 256) {
+            const int k = b / m, j = b % m, base = k * n;
+            const uint32_t zi = twi[ic.twi_off[level] + k];
+            if (rad == 2) {
+                const uint32_t y0 = u[base + j], y1 = u[base + m + j];
+                u[base + j] = mm((y0 + y1) % q, ic.inv2, q);
+                u[base + m + j] = mm(mm((y0 + q - y1) % q, ic.inv2, q), zi, q);
+            } else {
+                const uint32_t y0 = u[base + j], y1 = u[base + m + j], y2 = u[base + 2 * m + j];
+                const uint32_t a0 = mm((y0 + y1 + y2) % q, ic.inv3, q);
+                const uint32_t t1 = mm((y0 + mm(ic.w2, y1, q) + mm(ic.w, y2, q)) % q, ic.inv3, q);
+                const uint32_t t2 = mm((y0 + mm(ic.w, y1, q) + mm(ic.w2, y2, q)) % q, ic.inv3, q);
+                u[base + j] = a0;
+                u[base + m + j] = mm(t1, zi, q);
+                u[base + 2 * m + j] = mm(t2, mm(zi, zi, q), q);
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < 324; j += 256) {
+        const uint32_t y0 = u[j], y1 = u[j + 324];
+        const uint32_t a1 = mm((y0 + q - y1) % q, ic.det, q);
+        const uint32_t a0 = (y0 + q - mm(ic.z6, a1, q)) % q;
+        v[(size_t)i * 648 + j] = centered(a0, q);
+        v[(size_t)i * 648 + j + 324] = centered(a1, q);
+    }
+}
+
+// =============================================================================================
+// F162 on the device: a carry-less 162 x 162 product by shifted XOR over the bits of a, then the
+// reduction x^162 = x^81 + 1 (the same two-step fold as the host's f162_mul).
+// =============================================================================================
+
+__device__ void f162_mul_dev(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    uint64_t p[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int wi = 0; wi < 3; wi++) {
+        const uint64_t aw = a[wi];
+#pragma unroll
+        for (int bs = 0; bs < 64; bs++) {
+            if (wi == 2 && bs >= 34) break;
+            const uint64_t mask = 0ull - ((aw >> bs) & 1ull);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                p[wi + k] ^= (b[k] << bs) & mask;
+                if (bs) p[wi + k + 1] ^= (b[k] >> (64 - bs)) & mask;
+            }
+        }
+    }
+    uint64_t H0 = (p[2] >> 34) | (p[3] << 30), H1 = (p[3] >> 34) | (p[4] << 30);
+    uint64_t H2w = (p[4] >> 34) | (p[5] << 30);
+    uint64_t Hl0 = H0, Hl1 = H1 & ((1ull << 17) - 1);
+    uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+    uint64_t S0 = Hl0 ^ G0, S1 = Hl1 ^ G1;
+    out[0] = p[0] ^ H0 ^ G0;
+    out[1] = p[1] ^ H1 ^ G1 ^ (S0 << 17);
+    out[2] = (p[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
+}
+
+// eq(p0)[b] = prod_k (bit k of b ? p_k : 1 + p_k) (eval.rs:51-63), one thread per entry
+__global__ void eq_kernel(const uint64_t *__restrict__ p0, uint32_t rows, uint64_t *__restrict__ eq) {
+    __shared__ uint64_t ps[32 * 3];
+    for (uint32_t x = threadIdx.x; x < rows * 3; x += blockDim.x) ps[x] = p0[x];
+    __syncthreads();
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= (1u << rows)) return;
+    uint64_t acc[3] = {1, 0, 0};
+    for (uint32_t k = 0; k < rows; k++) {
+        uint64_t f[3] = {ps[3 * k], ps[3 * k + 1], ps[3 * k + 2]};
+        if (!((b >> k) & 1)) f[0] ^= 1;
+        uint64_t t[3];
+        f162_mul_dev(acc, f, t);
+        acc[0] = t[0];
+        acc[1] = t[1];
+        acc[2] = t[2];
+    }
+    eq[3 * (size_t)b] = acc[0];
+    eq[3 * (size_t)b + 1] = acc[1];
+    eq[3 * (size_t)b + 2] = acc[2];
+}
+
+// u_j += sum over a chunk of i of eq_i W[i + wdim j] (eval.rs:171-182): thread b < 162 collects
+// S_b = XOR of eq_i over the i whose element has bit b set; the block then forms
+// sum_b x^b S_b mod x^162 + x^81 + 1 and XORs it into u_j.
+constexpr int EVAL_CHUNK = 256;
+__global__ __launch_bounds__(192) void row_eval_kernel(const uint64_t *__restrict__ w,
+                                                       const uint64_t *__restrict__ eq,
+                                                       uint32_t wdim, uint64_t *__restrict__ u) {
+    __shared__ uint64_t ws[EVAL_CHUNK * 3], es[EVAL_CHUNK * 3];
+    __shared__ uint64_t red[6];
+    const uint32_t j = blockIdx.x, i0 = blockIdx.y * EVAL_CHUNK;
+    const int b = threadIdx.x;
+    const uint32_t cnt = min((uint32_t)EVAL_CHUNK, wdim - i0);
+    for (uint32_t x = threadIdx.x; x < cnt * 3; x += blockDim.x) {
+        ws[x] = w[3 * ((size_t)j * wdim + i0) + x];
+        es[x] = eq[3 * (size_t)i0 + x];
+    }
+    if (threadIdx.x < 6) red[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t s0 = 0, s1 = 0, s2 = 0;
+    if (b < 162) {
+        const int wi = b >> 6, bi = b & 63;
+        for (uint32_t i = 0; i < cnt; i++) {
+            const uint64_t mask = 0ull - ((ws[3 * i + wi] >> bi) & 1ull);
+            s0 ^= es[3 * i] & mask;
+            s1 ^= es[3 * i + 1] & mask;
+            s2 ^= es[3 * i + 2] & mask;
+        }
+        // x^b S_b as a 324-bit value in 6 words: word w takes S[w - ws] << bs and
+        // S[w - ws - 1] >> (64 - bs)
+        const int wsh = b >> 6, bs = b & 63;
+        auto pick = [&](int k) -> uint64_t { return k == 0 ? s0 : k == 1 ? s1 : k == 2 ? s2 : 0ull; };
+#pragma unroll
+        for (int wd = 0; wd < 6; wd++) {
+            uint64_t x = pick(wd - wsh) << bs;
+            if (bs) x ^= pick(wd - wsh - 1) >> (64 - bs);
+            if (x) atomicXor((unsigned long long *)&red[wd], (unsigned long long)x);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t p[6] = {red[0], red[1], red[2], red[3], red[4], red[5]};
+        uint64_t H0 = (p[2] >> 34) | (p[3] << 30), H1 = (p[3] >> 34) | (p[4] << 30);
+        uint64_t H2w = (p[4] >> 34) | (p[5] << 30);
+        uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+        uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
+        uint64_t o0 = p[0] ^ H0 ^ G0;
+        uint64_t o1 = p[1] ^ H1 ^ G1 ^ (S0 << 17);
+        uint64_t o2 = (p[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
+        atomicXor((unsigned long long *)&u[3 * j], (unsigned long long)o0);
+        atomicXor((unsigned long long *)&u[3 * j + 1], (unsigned long long)o1);
+        atomicXor((unsigned long long *)&u[3 * j + 2], (unsigned long long)o2);
+    }
+}
+
+// One v_mfma_i32_16x16x64_i8 with the operand and result lane layouts the tile GEMM assumes:
+// A[m][k] row-major 16 x 64, B[k][n] row-major 64 x 16, D[m][n] 16 x 16. One wave.
+__global__ void mfma_probe_kernel(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
+                                  int32_t *__restrict__ D) {
+    const int lane = threadIdx.x;
+    int8_t ab[16], bb[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        ab[j] = A[(lane & 15) * 64 + 16 * (lane >> 4) + j];
+        bb[j] = B[(16 * (lane >> 4) + j) * 16 + (lane & 15)];
+    }
+    v4i a, b;
+    __builtin_memcpy(&a, ab, 16);
+    __builtin_memcpy(&b, bb, 16);
+    v4i acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) D[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[r];
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) p[id] = v;
+}
+
+// =============================================================================================
+// launchers
+// =============================================================================================
+
+#define LBN_LAUNCH(...)                                                     \
+    do {                                                                    \
+        hipLaunchKernelGGL(__VA_ARGS__);                                    \
+        hipError_t e_ = hipGetLastError();                                  \
+        if (e_ != hipSuccess) return e_;                                    \
+    } while (0)
+
+hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D) {
+    LBN_LAUNCH(mfma_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D);
+    return hipSuccess;
+}
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v) {
+    if (!n) return hipSuccess;
+    LBN_LAUNCH(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipSuccess;
+}
+
+hipError_t launch_prepass(hipStream_t s, const uint64_t *w, uint32_t column_len, uint32_t L,
+                          uint32_t r, uint8_t *bytes_c, uint32_t lpad, uint8_t *bytes_f,
+                          uint32_t rpad) {
+    dim3 grid((r + 31) / 32, (L + 7) / 8);
+    LBN_LAUNCH(prepass_kernel, grid, dim3(256), 0, s, w, column_len, L, r, bytes_c, lpad, bytes_f, rpad);
+    return hipSuccess;
+}
+
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t ntiles, uint32_t batches,
+                            uint32_t splits) {
+    GemmArgs a = g;
+    a.ntiles = ntiles;
+    LBN_LAUNCH(tile_gemm_kernel, dim3(ntiles * batches, 1, splits), dim3(256), 0, s, a);
+    return hipSuccess;
+}
+
+hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t limbs,
+                                const FinishLimb *fl, const uint16_t *slot648, const uint16_t *qcs,
+                                uint32_t *raw, int16_t *comp, uint32_t cols) {
+    LBN_LAUNCH(commit_finish_kernel, dim3(cols, limbs), dim3(192), 0, s, yacc, limbs, fl, slot648, qcs, raw,
+               comp, cols);
+    return hipSuccess;
+}
+
+hipError_t launch_chat(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, uint32_t rpad,
+                       const uint16_t *expo, const uint16_t *pw, uint32_t cond, uint32_t mult,
+                       uint32_t q, int8_t *afrag, uint32_t ntiles) {
+    const uint32_t total = ntiles * rpad * TILE;
+    LBN_LAUNCH(chat_kernel, dim3((total + 255) / 256), dim3(256), 0, s, ch, r, rpad, expo, pw, cond, mult, q,
+               afrag, ntiles);
+    return hipSuccess;
+}
+
+hipError_t launch_intt(hipStream_t s, const uint32_t *vhat, uint32_t L, const InttConsts &ic,
+                       const uint32_t *twi, int16_t *v) {
+    LBN_LAUNCH(intt_kernel, dim3(L), dim3(256), 0, s, vhat, ic, twi, v);
+    return hipSuccess;
+}
+
+hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *eq) {
+    const uint32_t n = 1u << rows;
+    LBN_LAUNCH(eq_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p0, rows, eq);
+    return hipSuccess;
+}
+
+hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
+                           uint32_t cols, uint64_t *u) {
+    LBN_LAUNCH(row_eval_kernel, dim3(cols, (wdim + EVAL_CHUNK - 1) / EVAL_CHUNK), dim3(192), 0, s, w, eq, wdim,
+               u);
+    return hipSuccess;
+}
+
+}  // namespace lbn

@@ -1,0 +1,38 @@
+// Host-callable launchers of the HIP kernels (kernels.hip). Each returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace lbn {
+
+// Same layout as lbn_challenge (include/labinius.h).
+struct lbn_challenge_dev {
+    uint8_t positions[32];
+    uint32_t signs;
+    uint32_t weight;
+};
+
+hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D);
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
+hipError_t launch_prepass(hipStream_t s, const uint64_t *w, uint32_t column_len, uint32_t L,
+                          uint32_t r, uint8_t *bytes_c, uint32_t lpad, uint8_t *bytes_f,
+                          uint32_t rpad);
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t ntiles, uint32_t batches,
+                            uint32_t splits);
+hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t limbs,
+                                const FinishLimb *fl, const uint16_t *slot648, const uint16_t *qcs,
+                                uint32_t *raw, int16_t *comp, uint32_t cols);
+hipError_t launch_chat(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, uint32_t rpad,
+                       const uint16_t *expo, const uint16_t *pw, uint32_t cond, uint32_t mult,
+                       uint32_t q, int8_t *afrag, uint32_t ntiles);
+hipError_t launch_intt(hipStream_t s, const uint32_t *vhat, uint32_t L, const InttConsts &ic,
+                       const uint32_t *twi, int16_t *v);
+hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *eq);
+hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
+                           uint32_t cols, uint64_t *u);
+
+}  // namespace lbn

@@ -1,0 +1,773 @@
+// PublicParameters on the device, the Prover's workspace, and the commit / fold / row-evaluation
+// orchestration (scheme/params.rs:320-339, scheme/prover.rs, scheme/witness.rs:94-105).
+#include <algorithm>
+#include <cstring>
+
+#include "internal.hpp"
+#include "launch.hpp"
+
+namespace lbn {
+
+#define HIP_TRY(x)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? LBN_ERR_OOM : LBN_ERR_HIP,            \
+                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+static size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------------------------
+// the tables of one GEMM tile (see device.hpp)
+// ---------------------------------------------------------------------------------------------
+
+struct HostTile {
+    int rows[TILE];     // A row (commit) or -1
+    int out[TILE];      // output slot or -1
+    uint32_t expo[TILE];  // fold: exponent of the row's root, 0xFFFF for padding
+    uint16_t lut[2][256];
+    std::vector<uint32_t> E;  // [TILE][TILE] mod q
+    uint32_t par = 0;
+};
+
+static uint16_t digits(int64_t x) {
+    const int xl = (int)(((x + 64) & 127) - 64);
+    const int xh = (int)((x - xl) / 128);
+    return (uint16_t)((uint8_t)(int8_t)xl | (uint16_t)(uint8_t)(int8_t)xh << 8);
+}
+
+// Splitting limb: sub-ring beta < 8 of level 3 is Z_q[X]/(X^81 - psi^e), so its coefficient n is
+// sum_t w[81 t + n] psi^{e t} (the LUT on byte n), and slot u of the sub-ring evaluates it at
+// psi^{SLOT_EXP[u]}: E[m][n] = psi^{SLOT_EXP[81 beta + m] n} (params.rs:1-16, scalar.rs:35-84).
+static HostTile split_tile(const Prime &P, int beta) {
+    const Tables &t = tables();
+    HostTile h;
+    h.E.assign(TILE * TILE, 0);
+    const uint32_t e = t.subring_exp3[beta];
+    for (int b = 0; b < 256; b++) {
+        uint64_t x = 0;
+        for (int k = 0; k < 8; k++)
+            if ((b >> k) & 1) x += P.pw[(e * k) % CONDUCTOR];
+        h.lut[0][b] = h.lut[1][b] = digits(center(x % P.q, P.q));
+    }
+    for (int m = 0; m < TILE; m++) {
+        const bool ok = m < TILE_USED;
+        h.rows[m] = ok ? 81 * beta + m : -1;
+        h.out[m] = ok ? 81 * beta + m : -1;
+        h.expo[m] = ok ? t.slot_exp[81 * beta + m] : 0xFFFF;
+        if (!ok) continue;
+        for (int n = 0; n < TILE_USED; n++)
+            h.E[m * TILE + n] = P.pw[((uint64_t)t.slot_exp[81 * beta + m] * n) % CONDUCTOR];
+    }
+    return h;
+}
+
+// The LUTs of a quadratic limb's level-2 sub-ring k = Z_q[X]/(X^162 - psi'^e): its coefficient
+// 81 a + n is sum_{t<4} w[162 t + 81 a + n] psi'^{e t}, i.e. bits 2 t + a of byte n.
+static void quad_luts(const Prime &P, int k, uint16_t lut[2][256]) {
+    const uint32_t e = tables().subring_exp_q2[k];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 256; b++) {
+            uint64_t x = 0;
+            for (int tt = 0; tt < 4; tt++)
+                if ((b >> (2 * tt + a)) & 1) x += P.pw[(e * tt) % CONDUCTOR_QUAD];
+            lut[a][b] = digits(center(x % P.q, P.q));
+        }
+}
+
+// Quadratic limb, commit: sub-ring k, half hh of its 81 leaves, input parity b. Leaf j's row
+// 2j + b is sum_{s = b mod 2} x[s] c_j^{(s - b)/2} with c_j = psi'^{QUAD_SLOT_EXP[j]}; the
+// leaf product (scalar.rs:254-265) sends A row 2j + rbit times that into output 2j + (rbit ^ b),
+// with the extra factor c_j when rbit = b = 1.
+static HostTile quad_commit_tile(const Prime &P, int k, int hh, int b) {
+    const Tables &t = tables();
+    HostTile h;
+    h.E.assign(TILE * TILE, 0);
+    h.par = (uint32_t)b;
+    quad_luts(P, k, h.lut);
+    const int start = 81 * k + 41 * hh, count = hh ? 40 : 41;
+    for (int m = 0; m < TILE; m++) {
+        h.rows[m] = h.out[m] = -1;
+        h.expo[m] = 0xFFFF;
+        if (m >= 2 * count) continue;
+        const int rho = 2 * start + m, j = rho / 2, rbit = rho & 1;
+        h.rows[m] = rho;
+        h.out[m] = 2 * j + (rbit ^ b);
+        for (int n = 0; n < TILE_USED; n++) {
+            const int s = 81 * ((b + n) & 1) + n;
+            const uint64_t ex = (uint64_t)t.quad_slot_exp[j] * ((s - b) / 2 + (rbit & b));
+            h.E[m * TILE + n] = P.pw[ex % CONDUCTOR_QUAD];
+        }
+    }
+    return h;
+}
+
+// Quadratic base, fold: sub-ring k, parity b; rows are its 81 leaves (the challenge's leaf image
+// is a scalar, fold.rs:338-352), output 2j + b.
+static HostTile quad_fold_tile(const Prime &P, int k, int b) {
+    const Tables &t = tables();
+    HostTile h;
+    h.E.assign(TILE * TILE, 0);
+    h.par = (uint32_t)b;
+    quad_luts(P, k, h.lut);
+    for (int m = 0; m < TILE; m++) {
+        h.rows[m] = h.out[m] = -1;
+        h.expo[m] = 0xFFFF;
+        if (m >= TILE_USED) continue;
+        const int j = 81 * k + m;
+        h.out[m] = 2 * j + b;
+        h.expo[m] = t.quad_slot_exp[j];
+        for (int n = 0; n < TILE_USED; n++) {
+            const int s = 81 * ((b + n) & 1) + n;
+            h.E[m * TILE + n] = P.pw[((uint64_t)t.quad_slot_exp[j] * ((s - b) / 2)) % CONDUCTOR_QUAD];
+        }
+    }
+    return h;
+}
+
+// E, 2^7 E and 2^14 E in the D-fragment order of the MFMA: [mb][nb][lane][r][3], row
+// 16 mb + 4 (lane >> 4) + r, column 16 nb + (lane & 15).
+static void pack_epi(const HostTile &h, uint32_t q, uint16_t *dst) {
+    for (int mb = 0; mb < NBLK; mb++)
+        for (int nb = 0; nb < NBLK; nb++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int r = 0; r < 4; r++) {
+                    const int m = 16 * mb + 4 * (lane >> 4) + r, n = 16 * nb + (lane & 15);
+                    const uint64_t e = h.E[m * TILE + n];
+                    uint16_t *o = dst + (((mb * NBLK + nb) * 64 + lane) * 4 + r) * 3;
+                    o[0] = (uint16_t)e;
+                    o[1] = (uint16_t)(e * 128 % q);
+                    o[2] = (uint16_t)(e * 16384 % q);
+                }
+}
+
+// The A operand of a commit tile in MFMA lane order: [kb][mb][digit][lane][16 bytes], lane
+// holding row 16 mb + (lane & 15) and K = 64 kb + 16 (lane >> 4) + byte.
+static void pack_key(const HostTile &h, const int16_t *a, size_t len_ring, uint32_t kbs, int8_t *dst) {
+    for (uint32_t kb = 0; kb < kbs; kb++)
+        for (int mb = 0; mb < NBLK; mb++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int m = 16 * mb + (lane & 15), rho = h.rows[m];
+                for (int byte = 0; byte < 16; byte++) {
+                    const size_t i = (size_t)KSTEP * kb + 16 * (lane >> 4) + byte;
+                    int64_t x = 0;
+                    if (rho >= 0 && i < len_ring) x = a[i * N + rho];
+                    const uint16_t d = digits(x);
+                    int8_t *o = dst + (((size_t)kb * NBLK + mb) * 2) * FRAG + lane * 16 + byte;
+                    o[0] = (int8_t)(uint8_t)(d & 255);
+                    o[FRAG] = (int8_t)(uint8_t)(d >> 8);
+                }
+            }
+}
+
+template <typename T>
+static int upload(lbn_pp *pp, T **dst, const T *src, size_t count) {
+    HIP_TRY(hipMalloc((void **)dst, std::max<size_t>(1, count) * sizeof(T)));
+    pp->allocations.push_back(*dst);
+    if (count) HIP_TRY(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return LBN_OK;
+}
+
+static void pp_free(lbn_pp *pp) {
+    for (void *p : pp->allocations) (void)hipFree(p);
+    pp->allocations.clear();
+    delete pp;
+}
+
+// Everything the GEMMs read, built on the host.
+struct HostTables {
+    uint32_t lpad = 0, rpad = 0, kb_c = 0, kb_f = 0;
+    std::vector<int8_t> afrag;
+    std::vector<uint16_t> lut, epi;
+    std::vector<int16_t> out;
+    std::vector<TileDesc> desc;
+    std::vector<FinishLimb> finish;
+    std::vector<uint16_t> lut_f, epi_f, expo_f, pw_f;
+    std::vector<int16_t> out_f;
+    std::vector<TileDesc> desc_f;
+    uint32_t cond_f = 0, mult_f = 0;
+    InttConsts intt{};
+    std::vector<uint32_t> twi;
+};
+
+static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, HostTables &H) {
+    const size_t limbs = s.primes.size();
+    H.lpad = (uint32_t)round_up(s.len_ring, KSTEP);
+    H.rpad = (uint32_t)round_up(s.r, KSTEP);
+    H.kb_c = H.lpad / KSTEP;
+    H.kb_f = H.rpad / KSTEP;
+
+    // commit tiles
+    std::vector<HostTile> tiles;
+    std::vector<uint32_t> tile_limb;
+    for (size_t k = 0; k < limbs; k++) {
+        const Prime &P = prime(s.primes[k]);
+        if (P.quad) {
+            for (int sub = 0; sub < 4; sub++)
+                for (int hh = 0; hh < 2; hh++)
+                    for (int b = 0; b < 2; b++) {
+                        tiles.push_back(quad_commit_tile(P, sub, hh, b));
+                        tile_limb.push_back((uint32_t)k);
+                    }
+        } else {
+            for (int beta = 0; beta < 8; beta++) {
+                tiles.push_back(split_tile(P, beta));
+                tile_limb.push_back((uint32_t)k);
+            }
+        }
+    }
+    const size_t a_tile = (size_t)H.kb_c * NBLK * 2 * FRAG;
+    const size_t epi_tile = (size_t)NBLK * NBLK * 64 * 4 * 3;
+    H.afrag.assign(a_tile * tiles.size(), 0);
+    H.lut.assign(512 * tiles.size(), 0);
+    H.epi.assign(epi_tile * tiles.size(), 0);
+    H.out.assign(TILE * tiles.size(), 0);
+    H.desc.resize(tiles.size());
+    for (size_t ti = 0; ti < tiles.size(); ti++) {
+        const uint32_t k = tile_limb[ti], q = s.primes[k];
+        pack_key(tiles[ti], key.data() + (size_t)k * s.len_ring * N, s.len_ring, H.kb_c,
+                 H.afrag.data() + ti * a_tile);
+        std::memcpy(H.lut.data() + 512 * ti, tiles[ti].lut, 1024);
+        pack_epi(tiles[ti], q, H.epi.data() + epi_tile * ti);
+        for (int m = 0; m < TILE; m++) H.out[TILE * ti + m] = (int16_t)tiles[ti].out[m];
+        H.desc[ti] = TileDesc{ti * a_tile, (uint32_t)(512 * ti), (uint32_t)(epi_tile * ti), (uint32_t)(TILE * ti),
+                              q, tiles[ti].par, (uint32_t)(k * N)};
+    }
+
+    // the commitment's finish
+    const Tables &t = tables();
+    H.finish.resize(limbs);
+    for (size_t k = 0; k < limbs; k++) {
+        const Prime &P = prime(s.primes[k]);
+        FinishLimb &f = H.finish[k];
+        std::memset(&f, 0, sizeof f);
+        f.q = P.q;
+        f.quad = P.quad;
+        f.inv2 = (uint32_t)P.inv2;
+        if (!P.quad) {
+            f.ii = P.pw[486];
+            const uint64_t inv4 = inv_mod(4, P.q);
+            for (int kk = 0; kk < 4; kk++)
+                for (int sl = 0; sl < N162; sl++) {
+                    const uint64_t v = t.pow3_slot_exp[sl];
+                    f.twist[kk][sl] = (uint16_t)(inv4 * P.pw[(CONDUCTOR - (v * kk) % CONDUCTOR) % CONDUCTOR] % P.q);
+                }
+        } else {
+            for (int sl = 0; sl < N162; sl++) {
+                const uint64_t v = t.pow3_slot_exp[sl];
+                f.twist[0][sl] = (uint16_t)(P.inv2 * P.pw[(CONDUCTOR_QUAD - v) % CONDUCTOR_QUAD] % P.q);
+            }
+        }
+    }
+
+    // fold tiles over the base limb
+    const Prime &B = prime(s.primes[0]);
+    std::vector<HostTile> ft;
+    if (B.quad) {
+        for (int sub = 0; sub < 4; sub++)
+            for (int b = 0; b < 2; b++) ft.push_back(quad_fold_tile(B, sub, b));
+    } else {
+        for (int beta = 0; beta < 8; beta++) ft.push_back(split_tile(B, beta));
+    }
+    const size_t af_tile = (size_t)H.kb_f * NBLK * 2 * FRAG;
+    H.lut_f.assign(512 * ft.size(), 0);
+    H.epi_f.assign(epi_tile * ft.size(), 0);
+    H.expo_f.assign(TILE * ft.size(), 0);
+    H.out_f.assign(TILE * ft.size(), 0);
+    H.desc_f.resize(ft.size());
+    for (size_t ti = 0; ti < ft.size(); ti++) {
+        std::memcpy(H.lut_f.data() + 512 * ti, ft[ti].lut, 1024);
+        pack_epi(ft[ti], B.q, H.epi_f.data() + epi_tile * ti);
+        for (int m = 0; m < TILE; m++) {
+            H.out_f[TILE * ti + m] = (int16_t)ft[ti].out[m];
+            H.expo_f[TILE * ti + m] = (uint16_t)ft[ti].expo[m];
+        }
+        H.desc_f[ti] = TileDesc{ti * af_tile, (uint32_t)(512 * ti), (uint32_t)(epi_tile * ti),
+                                (uint32_t)(TILE * ti), B.q, ft[ti].par, 0};
+    }
+    H.cond_f = B.quad ? CONDUCTOR_QUAD : CONDUCTOR;
+    H.mult_f = B.quad ? 2 : 4;
+    H.pw_f.assign(B.pw.begin(), B.pw.end());
+
+    // the base prime's inverse transform
+    InttConsts &ic = H.intt;
+    ic.q = B.q;
+    ic.quad = B.quad;
+    ic.w = (uint32_t)B.w;
+    ic.w2 = (uint32_t)B.w2;
+    ic.z6 = (uint32_t)B.z6;
+    ic.inv2 = (uint32_t)B.inv2;
+    ic.inv3 = (uint32_t)B.inv3;
+    ic.det = (uint32_t)B.det;
+    H.twi.clear();
+    for (int l = 0; l < 7; l++) {
+        ic.twi_off[l] = (uint32_t)H.twi.size();
+        H.twi.insert(H.twi.end(), B.twi[l].begin(), B.twi[l].end());
+    }
+}
+
+static int build_device(lbn_pp *pp) {
+    HostTables H;
+    build_host_tables(pp->shape, pp->key, H);
+    pp->lpad = H.lpad;
+    pp->rpad = H.rpad;
+    pp->kb_c = H.kb_c;
+    pp->kb_f = H.kb_f;
+    pp->ntiles = (uint32_t)H.desc.size();
+    pp->ntiles_f = (uint32_t)H.desc_f.size();
+    pp->cond_f = H.cond_f;
+    pp->mult_f = H.mult_f;
+    pp->intt = H.intt;
+    const Tables &t = tables();
+    int rc;
+    if ((rc = upload(pp, &pp->d_afrag, H.afrag.data(), H.afrag.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_lut, H.lut.data(), H.lut.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_epi, H.epi.data(), H.epi.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_out, H.out.data(), H.out.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_tiles, H.desc.data(), H.desc.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_finish, H.finish.data(), H.finish.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_slot648, &t.slot_648[0][0], 4 * N162))) return rc;
+    if ((rc = upload(pp, &pp->d_qcs, &t.quad_class_slot[0][0], 2 * N162))) return rc;
+    if ((rc = upload(pp, &pp->d_lut_f, H.lut_f.data(), H.lut_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_epi_f, H.epi_f.data(), H.epi_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_out_f, H.out_f.data(), H.out_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_expo_f, H.expo_f.data(), H.expo_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_tiles_f, H.desc_f.data(), H.desc_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_pw_f, H.pw_f.data(), H.pw_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_twi, H.twi.data(), H.twi.size()))) return rc;
+    return LBN_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// the prover
+// ---------------------------------------------------------------------------------------------
+
+template <typename T>
+static int dalloc(T **p, size_t count) {
+    HIP_TRY(hipMalloc((void **)p, std::max<size_t>(1, count) * sizeof(T)));
+    return LBN_OK;
+}
+
+static void prover_free(lbn_prover *pr) {
+    void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
+                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_u, pr->d_wscratch};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
+    for (hipEvent_t &e : pr->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (pr->stream) (void)hipStreamDestroy(pr->stream);
+    if (pr->pp && --pr->pp->refs == 0) pp_free(pr->pp);
+    delete pr;
+}
+
+static size_t bytes_f_size(const lbn_pp *pp) { return (size_t)81 * pp->rpad * pp->shape.len_ring; }
+
+static int take_bytes_f(lbn_prover *pr, uint8_t **out) {
+    if (!pr->free_bytes_f.empty()) {
+        *out = pr->free_bytes_f.back();
+        pr->free_bytes_f.pop_back();
+        return LBN_OK;
+    }
+    const size_t sz = bytes_f_size(pr->pp);
+    HIP_TRY(hipMalloc((void **)out, sz));
+    // padding columns are never written by the layout pass and must read as zero
+    HIP_TRY(hipMemset(*out, 0, sz));
+    return LBN_OK;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
+    lbn_pp *pp = pr->pp;
+    const Shape &s = pp->shape;
+    const size_t limbs = s.primes.size();
+    uint8_t *bytes_f = nullptr;
+    int rc = take_bytes_f(pr, &bytes_f);
+    if (rc) return rc;
+    hipStream_t st = pr->stream;
+    HIP_TRY(hipEventRecord(pr->ev[0], st));
+    HIP_TRY(launch_prepass(st, d_w, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r, pr->d_bytes_c,
+                           pp->lpad, bytes_f, pp->rpad));
+    HIP_TRY(hipEventRecord(pr->ev[1], st));
+    HIP_TRY(launch_fill_u32(st, pr->d_yacc, s.r * limbs * N, 0));
+    GemmArgs g{};
+    g.tiles = pp->d_tiles;
+    g.afrag = pp->d_afrag;
+    g.lut = pp->d_lut;
+    g.epi = pp->d_epi;
+    g.out = pp->d_out;
+    g.bytes = pr->d_bytes_c;
+    g.bytes_batch = (uint64_t)81 * pp->lpad;
+    g.kpad = pp->lpad;
+    g.kb_total = pp->kb_c;
+    g.kb_per_split = (pp->kb_c + pr->splits_c - 1) / pr->splits_c;
+    g.a_kb_stride = NBLK * 2 * FRAG;
+    g.y = pr->d_yacc;
+    g.y_batch = limbs * N;
+    g.accumulate = 1;
+    HIP_TRY(launch_tile_gemm(st, g, pp->ntiles, (uint32_t)s.r, pr->splits_c));
+    HIP_TRY(hipEventRecord(pr->ev[2], st));
+    HIP_TRY(launch_commit_finish(st, pr->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, pr->d_raw,
+                                 pr->d_comp, (uint32_t)s.r));
+    HIP_TRY(hipEventRecord(pr->ev[3], st));
+    if (comp) HIP_TRY(hipMemcpyAsync(comp, pr->d_comp, s.r * 4 * limbs * N162 * sizeof(int16_t), hipMemcpyDeviceToHost, st));
+    if (raw) HIP_TRY(hipMemcpyAsync(raw, pr->d_raw, limbs * s.r * N * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    pr->timings[0] = elapsed(pr->ev[0], pr->ev[3]);
+    pr->timings[1] = elapsed(pr->ev[0], pr->ev[1]);
+    pr->timings[2] = elapsed(pr->ev[1], pr->ev[2]);
+    pr->timings[3] = elapsed(pr->ev[2], pr->ev[3]);
+    if (opening) {
+        lbn_opening *op = new lbn_opening();
+        op->owner = pr;
+        op->d_bytes_f = bytes_f;
+        *opening = op;
+    } else {
+        pr->free_bytes_f.push_back(bytes_f);
+    }
+    return LBN_OK;
+}
+
+static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch, size_t r, int16_t *v) {
+    lbn_pp *pp = pr->pp;
+    const Shape &s = pp->shape;
+    hipStream_t st = pr->stream;
+    HIP_TRY(hipMemcpyAsync(pr->d_ch, ch, r * sizeof(lbn_challenge), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(pr->ev[4], st));
+    HIP_TRY(launch_chat(st, (const lbn_challenge_dev *)pr->d_ch, (uint32_t)r, pp->rpad, pp->d_expo_f, pp->d_pw_f,
+                        pp->cond_f, pp->mult_f, s.primes[0], pr->d_afrag_f, pp->ntiles_f));
+    HIP_TRY(hipEventRecord(pr->ev[5], st));
+    GemmArgs g{};
+    g.tiles = pp->d_tiles_f;
+    g.afrag = pr->d_afrag_f;
+    g.lut = pp->d_lut_f;
+    g.epi = pp->d_epi_f;
+    g.out = pp->d_out_f;
+    g.bytes = op->d_bytes_f;
+    g.bytes_batch = (uint64_t)81 * pp->rpad;
+    g.kpad = pp->rpad;
+    g.kb_total = pp->kb_f;
+    g.kb_per_split = pp->kb_f;
+    g.a_kb_stride = NBLK * 2 * FRAG;
+    g.y = pr->d_vhat;
+    g.y_batch = N;
+    g.accumulate = 0;
+    HIP_TRY(launch_tile_gemm(st, g, pp->ntiles_f, (uint32_t)s.len_ring, 1));
+    HIP_TRY(hipEventRecord(pr->ev[6], st));
+    HIP_TRY(launch_intt(st, pr->d_vhat, (uint32_t)s.len_ring, pp->intt, pp->d_twi, pr->d_v));
+    HIP_TRY(hipEventRecord(pr->ev[7], st));
+    if (v) HIP_TRY(hipMemcpyAsync(v, pr->d_v, s.len_ring * N * sizeof(int16_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    pr->timings[4] = elapsed(pr->ev[4], pr->ev[7]);
+    pr->timings[5] = elapsed(pr->ev[4], pr->ev[5]);
+    pr->timings[6] = elapsed(pr->ev[5], pr->ev[6]);
+    pr->timings[7] = elapsed(pr->ev[6], pr->ev[7]);
+    return LBN_OK;
+}
+
+static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, lbn_f162 *u) {
+    lbn_pp *pp = pr->pp;
+    const Shape &s = pp->shape;
+    hipStream_t st = pr->stream;
+    HIP_TRY(hipMemcpyAsync(pr->d_p0, p0, s.rows * sizeof(lbn_f162), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(pr->ev[8], st));
+    HIP_TRY(launch_eq(st, pr->d_p0, s.rows, pr->d_eq));
+    HIP_TRY(hipEventRecord(pr->ev[9], st));
+    HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_u, s.r * 6, 0));
+    HIP_TRY(launch_row_eval(st, d_w, pr->d_eq, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
+    HIP_TRY(hipEventRecord(pr->ev[10], st));
+    HIP_TRY(hipMemcpyAsync(u, pr->d_u, s.r * sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    pr->timings[8] = elapsed(pr->ev[8], pr->ev[10]);
+    pr->timings[9] = elapsed(pr->ev[8], pr->ev[9]);
+    pr->timings[10] = elapsed(pr->ev[9], pr->ev[10]);
+    return LBN_OK;
+}
+
+}  // namespace lbn
+
+using namespace lbn;
+
+extern "C" {
+
+int lbn_pp_create(const lbn_params *params, const uint8_t matrix_seed[32], lbn_pp **out) {
+    if (!out || !matrix_seed) return fail(LBN_ERR_ARG, "null argument");
+    *out = nullptr;
+    Shape s;
+    int rc = make_shape(params, &s);
+    if (rc) return rc;
+    if (s.opening == LBN_OPENING_RECURSIVE)
+        return fail(LBN_ERR_UNSUPPORTED, "Opening::Recursive (LaBRADOR) is outside this prover");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(LBN_ERR_NO_DEVICE, "no HIP device: the prover runs on the GPU only");
+    lbn_pp *pp = new lbn_pp();
+    pp->params = *params;
+    pp->shape = s;
+    std::memcpy(pp->seed, matrix_seed, 32);
+    pp->key = expand_key(s.primes, s.len_ring, matrix_seed);
+    (void)hipGetDevice(&pp->device);
+    rc = build_device(pp);
+    if (rc) {
+        pp_free(pp);
+        return rc;
+    }
+    *out = pp;
+    return LBN_OK;
+}
+
+void lbn_pp_release(lbn_pp *pp) {
+    if (pp && --pp->refs == 0) pp_free(pp);
+}
+
+int lbn_pp_key(const lbn_pp *pp, int16_t *a, size_t len) {
+    if (!pp || !a) return fail(LBN_ERR_ARG, "null argument");
+    if (len != pp->key.size()) return fail(LBN_ERR_SHAPE, "wrong key buffer length");
+    std::memcpy(a, pp->key.data(), len * sizeof(int16_t));
+    return LBN_OK;
+}
+
+int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
+    if (!pp || !out) return fail(LBN_ERR_ARG, "null argument");
+    *out = nullptr;
+    (void)hipSetDevice(pp->device);
+    lbn_prover *pr = new lbn_prover();
+    pr->pp = pp;
+    pp->refs++;
+    const Shape &s = pp->shape;
+    const size_t limbs = s.primes.size();
+    int rc = LBN_OK;
+#define TRY_RC(x)              \
+    do {                       \
+        rc = (x);              \
+        if (rc) {              \
+            prover_free(pr);   \
+            return rc;         \
+        }                      \
+    } while (0)
+    TRY_RC(hipStreamCreateWithFlags(&pr->stream, hipStreamNonBlocking) == hipSuccess ? LBN_OK
+                                                                                   : fail(LBN_ERR_HIP, "stream"));
+    for (hipEvent_t &e : pr->ev)
+        TRY_RC(hipEventCreate(&e) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "event"));
+    const size_t bc = (size_t)81 * pp->lpad * s.r;
+    TRY_RC(dalloc(&pr->d_bytes_c, bc));
+    TRY_RC(hipMemset(pr->d_bytes_c, 0, bc) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "memset"));
+    TRY_RC(dalloc(&pr->d_yacc, s.r * limbs * N));
+    TRY_RC(dalloc(&pr->d_raw, s.r * limbs * N));
+    TRY_RC(dalloc(&pr->d_comp, s.r * 4 * limbs * N162));
+    TRY_RC(dalloc(&pr->d_ch, s.r));
+    TRY_RC(dalloc(&pr->d_afrag_f, (size_t)pp->ntiles_f * pp->kb_f * NBLK * 2 * FRAG));
+    TRY_RC(dalloc(&pr->d_vhat, s.len_ring * N));
+    TRY_RC(dalloc(&pr->d_v, s.len_ring * N));
+    TRY_RC(dalloc(&pr->d_p0, 3 * (size_t)std::max<uint32_t>(s.rows, 1)));
+    TRY_RC(dalloc(&pr->d_eq, 3 * s.column_len));
+    TRY_RC(dalloc(&pr->d_u, 3 * s.r));
+    // split K when there are too few tiles x columns to fill the device
+    const size_t wgs = (size_t)pp->ntiles * s.r;
+    uint32_t splits = 1;
+    while (wgs * splits < 2048 && pp->kb_c / (splits * 2) >= 4) splits *= 2;
+    pr->splits_c = splits;
+
+    // warm-up (scheme/prover.rs:39-49): a zero witness through every kernel once
+    {
+        std::vector<lbn_f162> zero(s.n);
+        std::memset(zero.data(), 0, zero.size() * sizeof(lbn_f162));
+        lbn_dwitness *dw = nullptr;
+        TRY_RC(lbn_witness_upload(pr, zero.data(), zero.size(), &dw));
+        lbn_opening *op = nullptr;
+        rc = lbn_commit_resident(pr, dw, nullptr, nullptr, &op);
+        std::vector<lbn_f162> p0(s.rows), u(s.r);
+        std::memset(p0.data(), 0, p0.size() * sizeof(lbn_f162));
+        if (!rc) rc = lbn_row_evaluate(pr, dw, p0.data(), p0.size(), u.data());
+        std::vector<lbn_challenge> ch(s.r);
+        lbn_transcript *t = lbn_transcript_new((const uint8_t *)"labinius/warm-up", 16);
+        for (size_t j = 0; j < s.r; j++) sample_short_challenge(t->t, 28, 12.0, &ch[j]);
+        lbn_transcript_free(t);
+        if (!rc) rc = lbn_fold(pr, op, ch.data(), ch.size(), nullptr);
+        else if (op) lbn_recycle(pr, op);
+        lbn_dwitness_release(dw);
+        if (rc) {
+            prover_free(pr);
+            return rc;
+        }
+    }
+#undef TRY_RC
+    *out = pr;
+    return LBN_OK;
+}
+
+void lbn_prover_release(lbn_prover *pr) {
+    if (pr) prover_free(pr);
+}
+
+int lbn_witness_upload(lbn_prover *pr, const lbn_f162 *w, size_t n, lbn_dwitness **out) {
+    if (!pr || !w || !out) return fail(LBN_ERR_ARG, "null argument");
+    if (n != pr->pp->shape.n) return fail(LBN_ERR_WITNESS_LENGTH, "the witness is not witness_len() elements");
+    lbn_dwitness *d = new lbn_dwitness();
+    d->owner = pr;
+    d->n = n;
+    if (hipMalloc((void **)&d->d_w, n * sizeof(lbn_f162)) != hipSuccess) {
+        delete d;
+        return fail(LBN_ERR_OOM, "witness allocation");
+    }
+    if (hipMemcpy(d->d_w, w, n * sizeof(lbn_f162), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d->d_w);
+        delete d;
+        return fail(LBN_ERR_HIP, "witness upload");
+    }
+    *out = d;
+    return LBN_OK;
+}
+
+void *lbn_dwitness_device_ptr(lbn_dwitness *w) { return w ? (void *)w->d_w : nullptr; }
+
+void lbn_dwitness_release(lbn_dwitness *w) {
+    if (!w) return;
+    if (w->d_w) (void)hipFree(w->d_w);
+    delete w;
+}
+
+int lbn_commit_resident(lbn_prover *pr, const lbn_dwitness *w, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
+    if (!pr || !w) return fail(LBN_ERR_ARG, "null argument");
+    if (w->n != pr->pp->shape.n)
+        return fail(LBN_ERR_SHAPE, "the witness was built for other parameters");
+    if (opening) *opening = nullptr;
+    return commit_device(pr, w->d_w, comp, raw, opening);
+}
+
+int lbn_commit(lbn_prover *pr, const lbn_f162 *w, size_t n, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
+    if (!pr || !w) return fail(LBN_ERR_ARG, "null argument");
+    if (n != pr->pp->shape.n) return fail(LBN_ERR_SHAPE, "the witness was built for other parameters");
+    if (!pr->d_wscratch) {
+        if (hipMalloc((void **)&pr->d_wscratch, n * sizeof(lbn_f162)) != hipSuccess)
+            return fail(LBN_ERR_OOM, "witness scratch");
+    }
+    HIP_TRY(hipMemcpyAsync(pr->d_wscratch, w, n * sizeof(lbn_f162), hipMemcpyHostToDevice, pr->stream));
+    if (opening) *opening = nullptr;
+    return commit_device(pr, pr->d_wscratch, comp, raw, opening);
+}
+
+int lbn_fold(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch, size_t r, int16_t *v) {
+    if (!pr || !op || !ch) return fail(LBN_ERR_ARG, "null argument");
+    if (op->owner != pr) return fail(LBN_ERR_SHAPE, "the opening belongs to another prover");
+    if (r != pr->pp->shape.r) return fail(LBN_ERR_SHAPE, "one challenge per chunk");
+    for (size_t j = 0; j < r; j++)
+        if (ch[j].weight > 32) return fail(LBN_ERR_ARG, "challenge weight above 32");
+    int rc = fold_device(pr, op, ch, r, v);
+    pr->free_bytes_f.push_back(op->d_bytes_f);
+    delete op;
+    return rc;
+}
+
+int lbn_recycle(lbn_prover *pr, lbn_opening *op) {
+    if (!pr || !op) return fail(LBN_ERR_ARG, "null argument");
+    if (op->owner != pr) return fail(LBN_ERR_SHAPE, "the opening belongs to another prover");
+    pr->free_bytes_f.push_back(op->d_bytes_f);
+    delete op;
+    return LBN_OK;
+}
+
+int lbn_row_evaluate(lbn_prover *pr, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len, lbn_f162 *u) {
+    if (!pr || !w || !p0 || !u) return fail(LBN_ERR_ARG, "null argument");
+    if (w->n != pr->pp->shape.n || p0_len != pr->pp->shape.rows)
+        return fail(LBN_ERR_SHAPE, "the evaluation point does not match the witness");
+    return row_eval_device(pr, w->d_w, p0, u);
+}
+
+int lbn_mle_evaluate(lbn_prover *pr, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len, const lbn_f162 *p1,
+                     size_t p1_len, lbn_f162 *t) {
+    if (!pr || !w || !p0 || !p1 || !t) return fail(LBN_ERR_ARG, "null argument");
+    const Shape &s = pr->pp->shape;
+    if (w->n != s.n || p0_len != s.rows || p1_len != s.cl)
+        return fail(LBN_ERR_SHAPE, "the evaluation point does not match the witness");
+    std::vector<lbn_f162> u(s.r);
+    int rc = row_eval_device(pr, w->d_w, p0, u.data());
+    if (rc) return rc;
+    // eval.rs:185-193: u . eq(p1)
+    std::vector<F162> ps(p1_len);
+    for (size_t k = 0; k < p1_len; k++) std::memcpy(ps[k].w, p1[k].w, 24);
+    std::vector<F162> eq = eq_table(ps.data(), (int)p1_len);
+    F162 acc{{0, 0, 0}};
+    for (size_t j = 0; j < s.r; j++) {
+        F162 uj;
+        std::memcpy(uj.w, u[j].w, 24);
+        acc = f162_add(acc, f162_mul(eq[j], uj));
+    }
+    std::memcpy(t->w, acc.w, 24);
+    return LBN_OK;
+}
+
+// Test-only (include/labinius_debug.h): one MFMA with the tile GEMM's lane layouts.
+int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d) {
+    if (!a || !b || !d) return fail(LBN_ERR_ARG, "null argument");
+    int8_t *da = nullptr, *db = nullptr;
+    int32_t *dd = nullptr;
+    int rc = LBN_OK;
+    if (hipMalloc((void **)&da, 1024) != hipSuccess || hipMalloc((void **)&db, 1024) != hipSuccess ||
+        hipMalloc((void **)&dd, 1024) != hipSuccess) {
+        rc = fail(LBN_ERR_OOM, "probe buffers");
+    } else if (hipMemcpy(da, a, 1024, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(db, b, 1024, hipMemcpyHostToDevice) != hipSuccess ||
+               launch_mfma_probe(nullptr, da, db, dd) != hipSuccess ||
+               hipMemcpy(d, dd, 1024, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = fail(LBN_ERR_HIP, "probe launch");
+    }
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dd) (void)hipFree(dd);
+    return rc;
+}
+
+// Test-only introspection (include/labinius_debug.h): the host-built GEMM tables, no device.
+static int copy_out(void *dst, size_t cap, const void *src, size_t bytes) {
+    if (!dst) return LBN_OK;
+    if (cap < bytes) return fail(LBN_ERR_SHAPE, "debug buffer too small");
+    std::memcpy(dst, src, bytes);
+    return LBN_OK;
+}
+
+int lbn_debug_tables(const lbn_params *params, const uint8_t seed[32], int which, void *dst, size_t cap,
+                     size_t *bytes) {
+    Shape s;
+    int rc = make_shape(params, &s);
+    if (rc) return rc;
+    std::vector<int16_t> key = expand_key(s.primes, s.len_ring, seed);
+    HostTables H;
+    build_host_tables(s, key, H);
+    const void *src = nullptr;
+    size_t n = 0;
+    switch (which) {
+        case 0: src = H.afrag.data(); n = H.afrag.size(); break;
+        case 1: src = H.lut.data(); n = H.lut.size() * 2; break;
+        case 2: src = H.epi.data(); n = H.epi.size() * 2; break;
+        case 3: src = H.out.data(); n = H.out.size() * 2; break;
+        case 4: src = H.desc.data(); n = H.desc.size() * sizeof(TileDesc); break;
+        case 5: src = H.lut_f.data(); n = H.lut_f.size() * 2; break;
+        case 6: src = H.epi_f.data(); n = H.epi_f.size() * 2; break;
+        case 7: src = H.out_f.data(); n = H.out_f.size() * 2; break;
+        case 8: src = H.expo_f.data(); n = H.expo_f.size() * 2; break;
+        case 9: src = H.desc_f.data(); n = H.desc_f.size() * sizeof(TileDesc); break;
+        case 10: src = H.finish.data(); n = H.finish.size() * sizeof(FinishLimb); break;
+        case 11: src = H.pw_f.data(); n = H.pw_f.size() * 2; break;
+        case 12: src = &H.intt; n = sizeof(InttConsts); break;
+        case 13: src = H.twi.data(); n = H.twi.size() * 4; break;
+        case 14: src = key.data(); n = key.size() * 2; break;
+        default: return fail(LBN_ERR_ARG, "unknown table");
+    }
+    if (bytes) *bytes = n;
+    return copy_out(dst, cap, src, n);
+}
+
+int lbn_prover_timings(const lbn_prover *pr, float *ms, size_t n) {
+    if (!pr || !ms) return fail(LBN_ERR_ARG, "null argument");
+    for (size_t i = 0; i < n && i < 11; i++) ms[i] = pr->timings[i];
+    return LBN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+"""ctypes view of the oracle (oracle/oracle.c): a CPU restatement of labinius's commit / evaluate /
+fold path and the host protocol pieces its parity tests need.
+
+Test infrastructure only: the tests, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may use it, and only as the checker (or as the CPU time reported beside the HIP
+path). The product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+N = 648
+N162 = 162
+
+
+def build(force=False):
+    """Compile oracle/oracle.c with the committed Makefile."""
+    if force or not os.path.exists(LIB_PATH) or (
+        os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "oracle.c"))
+    ):
+        subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
+
+
+class Challenge(ctypes.Structure):
+    """The ShortChallenge layout (challenge.rs:231-235): sorted positions, sign bits, weight."""
+
+    _fields_ = [("positions", ctypes.c_uint8 * 32), ("signs", ctypes.c_uint32), ("weight", ctypes.c_uint32)]
+
+
+class OParams(ctypes.Structure):
+    _fields_ = [
+        ("witness_log_len", ctypes.c_uint32),
+        ("column_log_len", ctypes.c_uint32),
+        ("n_primes", ctypes.c_uint32),
+        ("primes", ctypes.c_uint32 * 8),
+        ("dropped_bits", ctypes.c_uint32),
+        ("recursion", ctypes.c_uint32),
+        ("matrix_seed", ctypes.c_uint8 * 32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.orc_transcript_size.restype = ctypes.c_size_t
+        _lib.orc_matrix_seed.restype = ctypes.c_uint64
+        _lib.orc_eval_at.restype = ctypes.c_uint32
+        _lib.orc_psi.restype = ctypes.c_uint32
+        _lib.orc_sample_short_challenge.restype = ctypes.c_uint64
+        _lib.orc_canonical_norm_sq.restype = ctypes.c_double
+        _lib.orc_verify_folded_opening.restype = ctypes.c_int
+        _lib.orc_is_quad.restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def is_quad(q):
+    return bool(lib().orc_is_quad(ctypes.c_uint32(q)))
+
+
+def psi(q):
+    return lib().orc_psi(ctypes.c_uint32(q))
+
+
+def tables():
+    se = np.zeros(N, np.uint16)
+    qse = np.zeros(324, np.uint16)
+    p3 = np.zeros(N162, np.uint16)
+    s648 = np.zeros((4, N162), np.uint16)
+    qcs = np.zeros((2, N162), np.uint16)
+    lib().orc_tables(_p(se), _p(qse), _p(p3), _p(s648), _p(qcs))
+    return dict(SLOT_EXP=se, QUAD_SLOT_EXP=qse, POW3_SLOT_EXP=p3, SLOT_648=s648, QUAD_CLASS_SLOT=qcs)
+
+
+def ntt(q, a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.zeros(N, np.uint32)
+    lib().orc_ntt(ctypes.c_uint32(q), _p(a), _p(out))
+    return out
+
+
+def intt(q, a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.zeros(N, np.uint32)
+    lib().orc_intt(ctypes.c_uint32(q), _p(a), _p(out))
+    return out
+
+
+def eval_at(q, a, u):
+    a = np.ascontiguousarray(a, np.uint32)
+    return lib().orc_eval_at(ctypes.c_uint32(q), _p(a), ctypes.c_uint32(u))
+
+
+def eval_quad_at(q, a, u):
+    a = np.ascontiguousarray(a, np.uint32)
+    r = np.zeros(2, np.uint32)
+    lib().orc_eval_quad_at(ctypes.c_uint32(q), _p(a), ctypes.c_uint32(u), _p(r))
+    return int(r[0]), int(r[1])
+
+
+def mul_mod_phi(a, b, q):
+    a = np.ascontiguousarray(a, np.uint32)
+    b = np.ascontiguousarray(b, np.uint32)
+    out = np.zeros(N, np.uint32)
+    lib().orc_mul_mod_phi(_p(a), _p(b), ctypes.c_uint32(q), _p(out))
+    return out
+
+
+def decompose(q, y):
+    y = np.ascontiguousarray(y, np.uint32)
+    out = np.zeros((4, N162), np.int16)
+    lib().orc_decompose(ctypes.c_uint32(q), _p(y), _p(out))
+    return out
+
+
+def blake3(data, out_len=32, key=None, context=None):
+    data = bytes(data)
+    buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
+    out = (ctypes.c_uint8 * out_len)()
+    if context is not None:
+        mode, k = 2, ctypes.create_string_buffer(context.encode())
+    elif key is not None:
+        mode, k = 1, (ctypes.c_uint8 * 32).from_buffer_copy(bytes(key))
+    else:
+        mode, k = 0, None
+    lib().orc_blake3(ctypes.c_int(mode), k, buf, ctypes.c_size_t(len(data)), out, ctypes.c_size_t(out_len))
+    return bytes(out)
+
+
+def matrix_seed(seed):
+    return lib().orc_matrix_seed((ctypes.c_uint8 * 32).from_buffer_copy(bytes(seed)))
+
+
+def key_expand(primes, len_ring, seed64):
+    pr = np.array(primes, np.uint32)
+    a = np.zeros((len(primes), len_ring, N), np.int16)
+    lib().orc_key_expand(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(len_ring), ctypes.c_uint64(seed64), _p(a))
+    return a
+
+
+def witness_random(seed, n):
+    w = np.zeros((n, 3), np.uint64)
+    lib().orc_witness_random((ctypes.c_uint8 * 32).from_buffer_copy(bytes(seed)), ctypes.c_size_t(n), _p(w))
+    return w
+
+
+def lift(elems4):
+    e = np.ascontiguousarray(elems4, np.uint64)
+    out = np.zeros(N, np.uint32)
+    lib().orc_lift(_p(e), _p(out))
+    return out
+
+
+def commit(primes, len_ring, cols, a, w):
+    pr = np.array(primes, np.uint32)
+    a = np.ascontiguousarray(a, np.int16)
+    w = np.ascontiguousarray(w, np.uint64)
+    raw = np.zeros((len(primes), cols, N), np.uint32)
+    comp = np.zeros((cols, 4, len(primes), N162), np.int16)
+    lib().orc_commit(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(len_ring), ctypes.c_int(cols), _p(a), _p(w),
+                     _p(raw), _p(comp))
+    return raw, comp
+
+
+def challenges_array(chs):
+    arr = (Challenge * len(chs))()
+    for i, c in enumerate(chs):
+        arr[i] = c
+    return arr
+
+
+def embed(ch):
+    e = np.zeros(N, np.int32)
+    lib().orc_embed(ctypes.byref(ch), _p(e))
+    return e
+
+
+def fold(q, len_ring, cols, w, chs):
+    w = np.ascontiguousarray(w, np.uint64)
+    v = np.zeros((len_ring, N), np.int16)
+    lib().orc_fold(ctypes.c_uint32(q), ctypes.c_int(len_ring), ctypes.c_int(cols), _p(w), challenges_array(chs), _p(v))
+    return v
+
+
+def fold_direct(q, len_ring, cols, w, chs, rows):
+    w = np.ascontiguousarray(w, np.uint64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    v = np.zeros((len(rows), N), np.int16)
+    lib().orc_fold_direct(ctypes.c_uint32(q), ctypes.c_int(len_ring), ctypes.c_int(cols), _p(w),
+                          challenges_array(chs), _p(rows), ctypes.c_int(len(rows)), _p(v))
+    return v
+
+
+def f162_mul(a, b):
+    a = np.ascontiguousarray(a, np.uint64)
+    b = np.ascontiguousarray(b, np.uint64)
+    out = np.zeros(3, np.uint64)
+    lib().orc_f162_mul(_p(a), _p(b), _p(out))
+    return out
+
+
+def eq_table(ps):
+    ps = np.ascontiguousarray(ps, np.uint64).reshape(-1, 3)
+    t = np.zeros((1 << len(ps), 3), np.uint64)
+    lib().orc_eq_table(_p(ps), ctypes.c_int(len(ps)), _p(t))
+    return t
+
+
+def row_evaluate(w, p0):
+    w = np.ascontiguousarray(w, np.uint64).reshape(-1, 3)
+    p0 = np.ascontiguousarray(p0, np.uint64).reshape(-1, 3)
+    cols = len(w) >> len(p0)
+    u = np.zeros((cols, 3), np.uint64)
+    lib().orc_row_evaluate(_p(w), ctypes.c_size_t(len(w)), _p(p0), ctypes.c_int(len(p0)), _p(u))
+    return u
+
+
+def claim(u, p1):
+    u = np.ascontiguousarray(u, np.uint64).reshape(-1, 3)
+    p1 = np.ascontiguousarray(p1, np.uint64).reshape(-1, 3)
+    t = np.zeros(3, np.uint64)
+    lib().orc_claim(_p(u), _p(p1), ctypes.c_int(len(p1)), _p(t))
+    return t
+
+
+def fold_binary(u, chs):
+    u = np.ascontiguousarray(u, np.uint64).reshape(-1, 3)
+    out = np.zeros(3, np.uint64)
+    lib().orc_fold_binary(_p(u), challenges_array(chs), ctypes.c_size_t(len(chs)), _p(out))
+    return out
+
+
+class Transcript:
+    """challenge.rs:66-160."""
+
+    def __init__(self, domain):
+        self.buf = ctypes.create_string_buffer(lib().orc_transcript_size())
+        d = bytes(domain)
+        lib().orc_transcript_new(self.buf, ctypes.c_char_p(d), ctypes.c_size_t(len(d)))
+
+    def clone(self):
+        t = Transcript.__new__(Transcript)
+        t.buf = ctypes.create_string_buffer(self.buf.raw, len(self.buf.raw))
+        return t
+
+    def absorb_bytes(self, b):
+        b = bytes(b)
+        lib().orc_absorb_bytes(self.buf, ctypes.c_char_p(b), ctypes.c_size_t(len(b)))
+
+    def absorb_u64(self, x):
+        lib().orc_absorb_u64(self.buf, ctypes.c_uint64(x))
+
+    def fill(self, label, n):
+        out = (ctypes.c_uint8 * n)()
+        lib().orc_fill(self.buf, ctypes.c_char_p(bytes(label)), out, ctypes.c_size_t(n))
+        return bytes(out)
+
+    def sample_short_challenge(self, weight=28, bound=12.0):
+        c = Challenge()
+        att = lib().orc_sample_short_challenge(self.buf, ctypes.c_uint32(weight), ctypes.c_double(bound), ctypes.byref(c))
+        return c, att
+
+
+def canonical_norm_sq(c):
+    return lib().orc_canonical_norm_sq(ctypes.byref(c))
+
+
+def make_params(wl, cl, primes, seed, dropped_bits=0):
+    p = OParams()
+    p.witness_log_len = wl
+    p.column_log_len = cl
+    p.n_primes = len(primes)
+    for i, q in enumerate(primes):
+        p.primes[i] = q
+    p.dropped_bits = dropped_bits
+    p.recursion = 0
+    for i, b in enumerate(bytes(seed)):
+        p.matrix_seed[i] = b
+    return p
+
+
+def derive_evaluation_point(t, params, comp):
+    comp = np.ascontiguousarray(comp, np.int16)
+    pt = np.zeros((params.witness_log_len, 3), np.uint64)
+    lib().orc_derive_evaluation_point(t.buf, ctypes.byref(params), _p(comp), _p(pt))
+    rows = params.witness_log_len - params.column_log_len
+    return pt[:rows].copy(), pt[rows:].copy()
+
+
+def derive_folding_challenges(t, u):
+    u = np.ascontiguousarray(u, np.uint64).reshape(-1, 3)
+    arr = (Challenge * len(u))()
+    lib().orc_derive_folding_challenges(t.buf, _p(u), ctypes.c_size_t(len(u)), arr)
+    return list(arr)
+
+
+def fold_commitment(primes, cols, comp, chs):
+    pr = np.array(primes, np.uint32)
+    comp = np.ascontiguousarray(comp, np.int16)
+    out = np.zeros((4, len(primes), N162), np.int16)
+    lib().orc_fold_commitment(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols), _p(comp), challenges_array(chs),
+                              _p(out))
+    return out
+
+
+def verify_folded_opening(params, a, folded_comm, v, p0, u_folded):
+    a = np.ascontiguousarray(a, np.int16)
+    fc = np.ascontiguousarray(folded_comm, np.int16)
+    v = np.ascontiguousarray(v, np.int16)
+    p0 = np.ascontiguousarray(p0, np.uint64)
+    uf = np.ascontiguousarray(u_folded, np.uint64)
+    return bool(lib().orc_verify_folded_opening(ctypes.byref(params), _p(a), _p(fc), _p(v), _p(p0), _p(uf)))

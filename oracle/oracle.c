@@ -1,0 +1,1361 @@
+/*
+ * The oracle: a plain-C, single-threaded CPU restatement of labinius's commit / evaluate / fold
+ * path and of the host-side protocol pieces the path's parity tests need (BLAKE3, the transcript,
+ * the short-challenge sampler, the verifier's checks).
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product links, loads or calls this file; only the
+ * tests, `__graft_entry__.smoke()` and the `cpu_baseline` leg of `bench.py` do, and only as the
+ * checker of (or the CPU time beside) the HIP path.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the reference
+ * checkout). Arithmetic is exact and fully reduced (u32 in [0, q)), following crates/pcs/src/
+ * scalar.rs: there is no lazy reduction anywhere, so any representative the HIP kernels keep
+ * internally has to come out identical here after the final reduction.
+ *
+ * Pinning (what checks each part; the tests under tests/ carry them out):
+ *   - trees, psi, SLOT_EXP, POW3 tables: the definitions and the reference's compile-time asserts
+ *     (units mod 1944 each once, psi of order exactly 1944, omega^2+omega+1 = 0, ...);
+ *   - ntt/ntt_quad: Horner evaluation (eval_at / eval_quad_at) and the ring homomorphism against
+ *     the schoolbook mul_mod_phi; intt/intt_quad: round trips;
+ *   - decomposition: the defining identity y(psi^u) = sum_k psi^{uk} y_k(theta^u) brute-forced;
+ *   - commit: the i64 sum of scalar NTTs (tests/commit.rs:86-101) and, independently, the
+ *     coefficient-domain ring product sum_i a_i w_i;
+ *   - fold: the coefficient-domain sum sum_j c_j(-X^4) w_j via mul_mod_phi;
+ *   - F162 mul: the bit-by-bit schoolbook of tests/fields.rs:33-48 is this implementation;
+ *     x has order 243, distributivity;
+ *   - eq/row/mle: eq_naive / row_naive / mle_naive of tests/eval.rs:15-46;
+ *   - BLAKE3: published test vectors of the spec (hash of "", "abc", keyed and derive-key of the
+ *     empty input) plus agreement with the product's independently structured implementation at
+ *     every input length around the chunk and tree boundaries;
+ *   - A expansion, Witness::random, the transcript framing and the challenge sampler: parity
+ *     unpinned (no reference execution is possible in this container); restated byte by byte.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define N 648
+#define N162 162
+#define CONDUCTOR 1944
+#define CONDUCTOR_QUAD 972
+
+/* ============================================================================================
+ * constants: crates/pcs/src/params.rs
+ * ========================================================================================== */
+
+/* params.rs:35-46 */
+static uint64_t pow_mod(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1;
+    b %= q;
+    while (e > 0) {
+        if (e & 1) r = r * b % q;
+        b = b * b % q;
+        e >>= 1;
+    }
+    return r;
+}
+
+static uint64_t inv_mod(uint64_t a, uint64_t q) { return pow_mod(a, q - 2, q); }
+
+/* params.rs:53-61 */
+static uint32_t find_psi(uint32_t q) {
+    for (uint64_t x = 2;; x++)
+        if (pow_mod(x, 1944, q) == 1 && pow_mod(x, 972, q) != 1 && pow_mod(x, 648, q) != 1)
+            return (uint32_t)x;
+}
+
+/* params.rs:291-299 */
+static uint32_t find_psi972(uint32_t q) {
+    for (uint64_t x = 2;; x++)
+        if (pow_mod(x, 972, q) == 1 && pow_mod(x, 486, q) != 1 && pow_mod(x, 324, q) != 1)
+            return (uint32_t)x;
+}
+
+static const int RADIX[7] = {2, 2, 2, 3, 3, 3, 3};
+static const int SUBRINGS[8] = {1, 2, 4, 8, 24, 72, 216, 648};
+static const int DEGREE[8] = {648, 324, 162, 81, 27, 9, 3, 1};
+static const int RADIX_Q[6] = {2, 2, 3, 3, 3, 3};
+static const int SUBRINGS_Q[7] = {1, 2, 4, 12, 36, 108, 324};
+static const int DEGREE_Q[7] = {648, 324, 162, 54, 18, 6, 2};
+
+/* params.rs:64-70 */
+static uint32_t subring_exp(int level, int k) {
+    if (level == 1) return k == 0 ? 324 : 1620;
+    uint32_t p = (uint32_t)RADIX[level - 1];
+    return (subring_exp(level - 1, k / (int)p) + CONDUCTOR * ((uint32_t)k % p)) / p;
+}
+/* params.rs:74-76 */
+static uint32_t twiddle_exp(int level, int k) { return subring_exp(level, k) / (uint32_t)RADIX[level]; }
+
+/* params.rs:305-311 */
+static uint32_t subring_exp_quad(int level, int k) {
+    if (level == 1) return k == 0 ? 162 : 810;
+    uint32_t p = (uint32_t)RADIX_Q[level - 1];
+    return (subring_exp_quad(level - 1, k / (int)p) + CONDUCTOR_QUAD * ((uint32_t)k % p)) / p;
+}
+/* params.rs:316-318 */
+static uint32_t twiddle_exp_quad(int level, int k) {
+    return subring_exp_quad(level, k) / (uint32_t)RADIX_Q[level];
+}
+
+/* params.rs:266-268 */
+int orc_is_quad(uint32_t q) { return (q - 1) % CONDUCTOR_QUAD == 0 && (q - 1) % CONDUCTOR != 0; }
+
+static uint16_t SLOT_EXP[N];          /* params.rs:78-89 */
+static uint16_t QUAD_SLOT_EXP[324];   /* params.rs:320-332 */
+static uint16_t POW3_SLOT_EXP[N162];  /* ring/mod.rs:149-176 */
+static uint16_t SLOT_648[4][N162];    /* ring/mod.rs:178-179 */
+static uint16_t QUAD_CLASS_SLOT[2][N162]; /* params.rs:357-401 */
+static int tables_ready = 0;
+
+static void init_tables(void) {
+    if (tables_ready) return;
+    for (int j = 0; j < N; j++) SLOT_EXP[j] = (uint16_t)subring_exp(7, j);
+    for (int j = 0; j < 324; j++) QUAD_SLOT_EXP[j] = (uint16_t)subring_exp_quad(6, j);
+    int pos[486];
+    for (int v = 0; v < 486; v++) pos[v] = -1;
+    int n = 0;
+    for (int j = 0; j < N; j++) {
+        int u = SLOT_EXP[j], v = u % 486, t = (u - v) / 486;
+        if (pos[v] < 0) {
+            pos[v] = n;
+            POW3_SLOT_EXP[n] = (uint16_t)v;
+            n++;
+        }
+        SLOT_648[t][pos[v]] = (uint16_t)j;
+    }
+    for (int s = 0; s < N162; s++) {
+        int v = POW3_SLOT_EXP[s];
+        for (int j = 0; j < 324; j++) {
+            if (QUAD_SLOT_EXP[j] == v) QUAD_CLASS_SLOT[0][s] = (uint16_t)j;
+            else if (QUAD_SLOT_EXP[j] == v + 486) QUAD_CLASS_SLOT[1][s] = (uint16_t)j;
+        }
+    }
+    tables_ready = 1;
+}
+
+void orc_tables(uint16_t *slot_exp, uint16_t *quad_slot_exp, uint16_t *pow3, uint16_t *slot648,
+                uint16_t *quad_class_slot) {
+    init_tables();
+    memcpy(slot_exp, SLOT_EXP, sizeof SLOT_EXP);
+    memcpy(quad_slot_exp, QUAD_SLOT_EXP, sizeof QUAD_SLOT_EXP);
+    memcpy(pow3, POW3_SLOT_EXP, sizeof POW3_SLOT_EXP);
+    memcpy(slot648, SLOT_648, sizeof SLOT_648);
+    memcpy(quad_class_slot, QUAD_CLASS_SLOT, sizeof QUAD_CLASS_SLOT);
+}
+
+uint32_t orc_psi(uint32_t q) { return orc_is_quad(q) ? find_psi972(q) : find_psi(q); }
+
+/* ============================================================================================
+ * per-prime tables: every twiddle of both trees, precomputed once per prime
+ * ========================================================================================== */
+
+typedef struct {
+    uint32_t q;
+    uint64_t bm;           /* floor(2^40 / q): x mod q for x < 2^40 with one multiply */
+    int quad;
+    uint64_t psi, w, w2, z6, inv2, inv3, det;
+    uint32_t tw[7][648];   /* tw[level][k] = psi^twiddle_exp(level, k) */
+    uint32_t twi[7][648];  /* inverses */
+    uint32_t leafc[324];   /* quadratic tree: LEAF_C[j] = psi'^QUAD_SLOT_EXP[j], params.rs:444-452 */
+} prime_tab;
+
+#define MAX_PRIMES 8
+static prime_tab PT[MAX_PRIMES];
+static int n_pt = 0;
+
+static const prime_tab *prime(uint32_t q) {
+    for (int i = 0; i < n_pt; i++)
+        if (PT[i].q == q) return &PT[i];
+    prime_tab *t = &PT[n_pt++];
+    memset(t, 0, sizeof *t);
+    t->q = q;
+    t->bm = (1ull << 40) / q;
+    t->quad = orc_is_quad(q);
+    if (t->quad) {
+        t->psi = find_psi972(q);
+        t->w = pow_mod(t->psi, 324, q);     /* params.rs:414 */
+        t->z6 = pow_mod(t->psi, 162, q);    /* params.rs:416 */
+        for (int l = 1; l <= 5; l++)
+            for (int k = 0; k < SUBRINGS_Q[l]; k++) {
+                t->tw[l][k] = (uint32_t)pow_mod(t->psi, twiddle_exp_quad(l, k), q);
+                t->twi[l][k] = (uint32_t)inv_mod(t->tw[l][k], q);
+            }
+    } else {
+        t->psi = find_psi(q);
+        t->w = pow_mod(t->psi, 648, q);     /* params.rs:122 */
+        t->z6 = pow_mod(t->psi, 324, q);    /* params.rs:124 */
+        for (int l = 1; l <= 6; l++)
+            for (int k = 0; k < SUBRINGS[l]; k++) {
+                t->tw[l][k] = (uint32_t)pow_mod(t->psi, twiddle_exp(l, k), q);
+                t->twi[l][k] = (uint32_t)inv_mod(t->tw[l][k], q);
+            }
+    }
+    if (t->quad)
+        for (int j = 0; j < 324; j++) t->leafc[j] = (uint32_t)pow_mod(t->psi, QUAD_SLOT_EXP[j], q);
+    t->w2 = t->w * t->w % q;
+    t->inv2 = inv_mod(2, q);
+    t->inv3 = inv_mod(3, q);
+    t->det = inv_mod((2 * t->z6 + q - 1) % q, q);
+    return t;
+}
+
+/* x mod q for x < 2^40 (Barrett with a 40-bit magic; the quotient estimate is low by at most 1) */
+static inline uint64_t md(const prime_tab *t, uint64_t x) {
+    uint64_t r = x - (uint64_t)(((unsigned __int128)x * t->bm) >> 40) * t->q;
+    return r >= t->q ? r - t->q : r;
+}
+
+/* ============================================================================================
+ * the exact scalar transforms: crates/pcs/src/scalar.rs, recursion/limbs.rs:76-127
+ * ========================================================================================== */
+
+/* scalar.rs:9-32: a * b in Z_q[X]/(X^648 - X^324 + 1) */
+void orc_mul_mod_phi(const uint32_t *a, const uint32_t *b, uint32_t q, uint32_t *out) {
+    uint64_t c[2 * N];
+    memset(c, 0, sizeof c);
+    uint64_t qq = q;
+    for (int i = 0; i < N; i++) {
+        if (a[i] == 0) continue;
+        for (int j = 0; j < N; j++) c[i + j] += (uint64_t)a[i] * b[j];
+        /* keep the running sums bounded: fold every row */
+        if ((i & 63) == 63)
+            for (int j = 0; j < 2 * N; j++) c[j] %= qq;
+    }
+    for (int k = 2 * N - 1; k >= N; k--) {
+        uint64_t v = c[k] % qq;
+        c[k] = 0;
+        c[k - 324] += v;
+        c[k - 648] += qq * qq - v;
+    }
+    for (int i = 0; i < N; i++) out[i] = (uint32_t)(c[i] % qq);
+}
+
+/* scalar.rs:35-84: forward NTT on the splitting tree, out[j] = a(psi^SLOT_EXP[j]) */
+static void ntt_split(const prime_tab *t, uint64_t *v) {
+    uint64_t q = t->q, w = t->w, w2 = t->w2, z6 = t->z6;
+    for (int i = 0; i < 324; i++) {
+        uint64_t a0 = v[i], a1 = v[i + 324], tt = md(t, a1 * z6);
+        v[i] = md(t, a0 + tt);
+        v[i + 324] = md(t, a0 + a1 + q - tt);
+    }
+    for (int level = 1; level <= 6; level++) {
+        int n = DEGREE[level], p = RADIX[level], m = n / p;
+        for (int k = 0; k < SUBRINGS[level]; k++) {
+            int base = k * n;
+            uint64_t zeta = t->tw[level][k];
+            if (p == 2) {
+                for (int i = 0; i < m; i++) {
+                    uint64_t a0 = v[base + i], tt = md(t, v[base + m + i] * zeta);
+                    v[base + i] = md(t, a0 + tt);
+                    v[base + m + i] = md(t, a0 + q - tt);
+                }
+            } else {
+                uint64_t zeta2 = md(t, zeta * zeta);
+                for (int i = 0; i < m; i++) {
+                    uint64_t a0 = v[base + i];
+                    uint64_t t1 = md(t, v[base + m + i] * zeta);
+                    uint64_t t2 = md(t, v[base + 2 * m + i] * zeta2);
+                    v[base + i] = md(t, a0 + t1 + t2);
+                    v[base + m + i] = md(t, a0 + w * t1 + w2 * t2);
+                    v[base + 2 * m + i] = md(t, a0 + w2 * t1 + w * t2);
+                }
+            }
+        }
+    }
+}
+
+/* scalar.rs:185-234: forward transform on the quadratic-slot tree */
+static void ntt_quad(const prime_tab *t, uint64_t *v) {
+    uint64_t q = t->q, w = t->w, w2 = t->w2, z6 = t->z6;
+    for (int i = 0; i < 324; i++) {
+        uint64_t a0 = v[i], a1 = v[i + 324], tt = md(t, a1 * z6);
+        v[i] = md(t, a0 + tt);
+        v[i + 324] = md(t, a0 + a1 + q - tt);
+    }
+    for (int level = 1; level <= 5; level++) {
+        int n = DEGREE_Q[level], p = RADIX_Q[level], m = n / p;
+        for (int k = 0; k < SUBRINGS_Q[level]; k++) {
+            int base = k * n;
+            uint64_t zeta = t->tw[level][k];
+            if (p == 2) {
+                for (int i = 0; i < m; i++) {
+                    uint64_t a0 = v[base + i], tt = md(t, v[base + m + i] * zeta);
+                    v[base + i] = md(t, a0 + tt);
+                    v[base + m + i] = md(t, a0 + q - tt);
+                }
+            } else {
+                uint64_t zeta2 = md(t, zeta * zeta);
+                for (int i = 0; i < m; i++) {
+                    uint64_t a0 = v[base + i];
+                    uint64_t t1 = md(t, v[base + m + i] * zeta);
+                    uint64_t t2 = md(t, v[base + 2 * m + i] * zeta2);
+                    v[base + i] = md(t, a0 + t1 + t2);
+                    v[base + m + i] = md(t, a0 + w * t1 + w2 * t2);
+                    v[base + 2 * m + i] = md(t, a0 + w2 * t1 + w * t2);
+                }
+            }
+        }
+    }
+}
+
+/* scalar.rs:123-177 (split) and recursion/limbs.rs:76-127 (quadratic): the exact inverses */
+static void intt_any(const prime_tab *t, uint64_t *u) {
+    uint64_t q = t->q, w = t->w, w2 = t->w2;
+    int top = t->quad ? 5 : 6;
+    for (int level = top; level >= 1; level--) {
+        int n = t->quad ? DEGREE_Q[level] : DEGREE[level];
+        int p = t->quad ? RADIX_Q[level] : RADIX[level];
+        int nk = t->quad ? SUBRINGS_Q[level] : SUBRINGS[level];
+        int m = n / p;
+        for (int k = 0; k < nk; k++) {
+            int base = k * n;
+            uint64_t zi = t->twi[level][k];
+            if (p == 2) {
+                for (int i = 0; i < m; i++) {
+                    uint64_t y0 = u[base + i], y1 = u[base + m + i];
+                    u[base + i] = (y0 + y1) % q * t->inv2 % q;
+                    u[base + m + i] = (y0 + q - y1) % q * t->inv2 % q * zi % q;
+                }
+            } else {
+                uint64_t zi2 = zi * zi % q;
+                for (int i = 0; i < m; i++) {
+                    uint64_t y0 = u[base + i], y1 = u[base + m + i], y2 = u[base + 2 * m + i];
+                    uint64_t a0 = (y0 + y1 + y2) % q * t->inv3 % q;
+                    uint64_t t1 = (y0 + w2 * y1 + w * y2) % q * t->inv3 % q;
+                    uint64_t t2 = (y0 + w * y1 + w2 * y2) % q * t->inv3 % q;
+                    u[base + i] = a0;
+                    u[base + m + i] = t1 * zi % q;
+                    u[base + 2 * m + i] = t2 * zi2 % q;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 324; i++) {
+        uint64_t y0 = u[i], y1 = u[i + 324];
+        uint64_t a1 = (y0 + q - y1) % q * t->det % q;
+        uint64_t a0 = (y0 + q - t->z6 * a1 % q) % q;
+        u[i] = a0;
+        u[i + 324] = a1;
+    }
+}
+
+/* The transform of limb q (split or quadratic tree), fully reduced. */
+void orc_ntt(uint32_t q, const uint32_t *a, uint32_t *out) {
+    init_tables();
+    const prime_tab *t = prime(q);
+    uint64_t v[N];
+    for (int i = 0; i < N; i++) v[i] = a[i] % q;
+    if (t->quad) ntt_quad(t, v);
+    else ntt_split(t, v);
+    for (int i = 0; i < N; i++) out[i] = (uint32_t)v[i];
+}
+
+void orc_intt(uint32_t q, const uint32_t *a, uint32_t *out) {
+    init_tables();
+    const prime_tab *t = prime(q);
+    uint64_t v[N];
+    for (int i = 0; i < N; i++) v[i] = a[i] % q;
+    intt_any(t, v);
+    for (int i = 0; i < N; i++) out[i] = (uint32_t)v[i];
+}
+
+/* scalar.rs:87-95: a(psi^u) by Horner */
+uint32_t orc_eval_at(uint32_t q, const uint32_t *a, uint32_t u) {
+    uint64_t x = pow_mod(find_psi(q), u, q), acc = 0;
+    for (int i = N - 1; i >= 0; i--) acc = (acc * x + a[i]) % q;
+    return (uint32_t)acc;
+}
+
+/* scalar.rs:237-248: a mod (X^2 - psi'^u) by Horner */
+void orc_eval_quad_at(uint32_t q, const uint32_t *a, uint32_t u, uint32_t *r01) {
+    uint64_t c = pow_mod(find_psi972(q), u, q), r0 = 0, r1 = 0;
+    for (int i = N - 1; i >= 0; i--) {
+        uint64_t n0 = (r1 * c + a[i]) % q, n1 = r0;
+        r0 = n0;
+        r1 = n1;
+    }
+    r01[0] = (uint32_t)r0;
+    r01[1] = (uint32_t)r1;
+}
+
+/* scalar.rs:254-265: leaf-wise product of two quadratic-slot transforms */
+static void mul_quad_slots(const prime_tab *t, const uint64_t *a, const uint64_t *b, uint64_t *c) {
+    uint64_t q = t->q;
+    for (int j = 0; j < 324; j++) {
+        uint64_t cj = t->leafc[j];
+        uint64_t a0 = a[2 * j], a1 = a[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
+        c[2 * j] = md(t, a0 * b0 + cj * md(t, a1 * b1));
+        c[2 * j + 1] = md(t, a0 * b1 + a1 * b0);
+    }
+}
+
+/* ============================================================================================
+ * the 648 -> 4 x 162 decomposition: ring/mod.rs:22-55, 217-312 (split), 363-417 (quadratic)
+ * ========================================================================================== */
+
+static int16_t center(uint64_t x, uint64_t q) {
+    x %= q;
+    return x > (q - 1) / 2 ? (int16_t)((int64_t)x - (int64_t)q) : (int16_t)x;
+}
+
+/* out[k][s], centered, for a fully reduced y */
+void orc_decompose(uint32_t q, const uint32_t *y, int16_t *out) {
+    init_tables();
+    const prime_tab *t = prime(q);
+    uint64_t qq = q;
+    if (!t->quad) {
+        /* Y_k(v) = 4^-1 psi^{-vk} sum_t i^{-tk} E_t, i = psi^486 */
+        uint64_t inv4 = inv_mod(4, qq), ii = pow_mod(t->psi, 486, qq);
+        uint64_t ipow[4] = {1, ii, ii * ii % qq, ii * ii % qq * ii % qq};
+        for (int s = 0; s < N162; s++) {
+            uint64_t v = POW3_SLOT_EXP[s];
+            for (int k = 0; k < 4; k++) {
+                uint64_t acc = 0;
+                for (int tt = 0; tt < 4; tt++) {
+                    uint64_t e = y[SLOT_648[tt][s]] % qq;
+                    /* i^{-tk} = i^{(4 - tk mod 4) mod 4} */
+                    acc += e * ipow[(4 - (tt * k) % 4) % 4] % qq;
+                }
+                uint64_t tw = inv4 * pow_mod(t->psi, (CONDUCTOR - (v * k) % CONDUCTOR) % CONDUCTOR, qq) % qq;
+                out[k * N162 + s] = center(acc % qq * tw, qq);
+            }
+        }
+    } else {
+        /* scalar.rs:287-309 */
+        uint64_t inv2 = t->inv2;
+        for (int s = 0; s < N162; s++) {
+            uint64_t v = POW3_SLOT_EXP[s];
+            int jp = QUAD_CLASS_SLOT[0][s], jm = QUAD_CLASS_SLOT[1][s];
+            uint64_t ipv = pow_mod(t->psi, (CONDUCTOR_QUAD - v) % CONDUCTOR_QUAD, qq);
+            for (int k = 0; k < 2; k++) {
+                uint64_t ep = y[2 * jp + k] % qq, em = y[2 * jm + k] % qq;
+                out[k * N162 + s] = center((ep + em) % qq * inv2, qq);
+                out[(k + 2) * N162 + s] = center((ep + qq - em) % qq * inv2 % qq * ipv, qq);
+            }
+        }
+    }
+}
+
+/* ============================================================================================
+ * BLAKE3 (the public specification: compression, chunk chaining, the binary tree and the XOF),
+ * written here as an incremental hasher with a chaining-value stack.
+ * ========================================================================================== */
+
+static const uint32_t B3_IV[8] = {0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A,
+                                  0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19};
+static const int B3_PERM[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+enum { CHUNK_START = 1, CHUNK_END = 2, PARENT = 4, ROOT = 8, KEYED_HASH = 16,
+       DERIVE_KEY_CONTEXT = 32, DERIVE_KEY_MATERIAL = 64 };
+
+static uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+static void b3_g(uint32_t *s, int a, int b, int c, int d, uint32_t mx, uint32_t my) {
+    s[a] = s[a] + s[b] + mx;
+    s[d] = rotr(s[d] ^ s[a], 16);
+    s[c] = s[c] + s[d];
+    s[b] = rotr(s[b] ^ s[c], 12);
+    s[a] = s[a] + s[b] + my;
+    s[d] = rotr(s[d] ^ s[a], 8);
+    s[c] = s[c] + s[d];
+    s[b] = rotr(s[b] ^ s[c], 7);
+}
+
+static void b3_compress(const uint32_t cv[8], const uint32_t block[16], uint64_t counter,
+                        uint32_t block_len, uint32_t flags, uint32_t out[16]) {
+    uint32_t s[16], m[16], t[16];
+    for (int i = 0; i < 8; i++) s[i] = cv[i];
+    for (int i = 0; i < 4; i++) s[8 + i] = B3_IV[i];
+    s[12] = (uint32_t)counter;
+    s[13] = (uint32_t)(counter >> 32);
+    s[14] = block_len;
+    s[15] = flags;
+    memcpy(m, block, sizeof m);
+    for (int r = 0; r < 7; r++) {
+        b3_g(s, 0, 4, 8, 12, m[0], m[1]);
+        b3_g(s, 1, 5, 9, 13, m[2], m[3]);
+        b3_g(s, 2, 6, 10, 14, m[4], m[5]);
+        b3_g(s, 3, 7, 11, 15, m[6], m[7]);
+        b3_g(s, 0, 5, 10, 15, m[8], m[9]);
+        b3_g(s, 1, 6, 11, 12, m[10], m[11]);
+        b3_g(s, 2, 7, 8, 13, m[12], m[13]);
+        b3_g(s, 3, 4, 9, 14, m[14], m[15]);
+        if (r < 6) {
+            for (int i = 0; i < 16; i++) t[i] = m[B3_PERM[i]];
+            memcpy(m, t, sizeof m);
+        }
+    }
+    for (int i = 0; i < 8; i++) {
+        out[i] = s[i] ^ s[i + 8];
+        out[i + 8] = s[i + 8] ^ cv[i];
+    }
+}
+
+static void words_le(const uint8_t *b, uint32_t *w, int n) {
+    for (int i = 0; i < n; i++)
+        w[i] = (uint32_t)b[4 * i] | (uint32_t)b[4 * i + 1] << 8 | (uint32_t)b[4 * i + 2] << 16 |
+               (uint32_t)b[4 * i + 3] << 24;
+}
+
+typedef struct {
+    uint32_t key[8];
+    uint32_t flags;
+    uint32_t cv[8];
+    uint64_t chunk_counter;
+    uint8_t block[64];
+    uint32_t block_len;
+    uint32_t blocks_done;
+    uint32_t stack[64][8];
+    int stack_len;
+} b3;
+
+/* the output of a node not yet finalised: enough to squeeze root bytes or take a chaining value */
+typedef struct {
+    uint32_t cv[8];
+    uint32_t block[16];
+    uint64_t counter;
+    uint32_t block_len;
+    uint32_t flags;
+} b3_out;
+
+static void b3_init(b3 *h, const uint32_t key[8], uint32_t flags) {
+    memset(h, 0, sizeof *h);
+    memcpy(h->key, key, 32);
+    memcpy(h->cv, key, 32);
+    h->flags = flags;
+}
+
+static uint32_t chunk_start_flag(const b3 *h) { return h->blocks_done == 0 ? CHUNK_START : 0; }
+
+static void chunk_output(const b3 *h, b3_out *o) {
+    memcpy(o->cv, h->cv, 32);
+    uint8_t blk[64];
+    memset(blk, 0, 64);
+    memcpy(blk, h->block, h->block_len);
+    words_le(blk, o->block, 16);
+    o->counter = h->chunk_counter;
+    o->block_len = h->block_len;
+    o->flags = h->flags | chunk_start_flag(h) | CHUNK_END;
+}
+
+static void parent_output(const uint32_t *l, const uint32_t *r, const uint32_t *key,
+                          uint32_t flags, b3_out *o) {
+    memcpy(o->cv, key, 32);
+    memcpy(o->block, l, 32);
+    memcpy(o->block + 8, r, 32);
+    o->counter = 0;
+    o->block_len = 64;
+    o->flags = PARENT | flags;
+}
+
+static void out_cv(const b3_out *o, uint32_t cv[8]) {
+    uint32_t w[16];
+    b3_compress(o->cv, o->block, o->counter, o->block_len, o->flags, w);
+    memcpy(cv, w, 32);
+}
+
+static void b3_update(b3 *h, const uint8_t *in, size_t len) {
+    while (len > 0) {
+        if (h->block_len == 64) {
+            uint32_t w[16], o[16];
+            words_le(h->block, w, 16);
+            if (h->blocks_done == 15) {
+                /* the chunk is full: finish it only once more input is known to follow */
+            } else {
+                b3_compress(h->cv, w, h->chunk_counter, 64, h->flags | chunk_start_flag(h), o);
+                memcpy(h->cv, o, 32);
+                h->blocks_done++;
+                h->block_len = 0;
+                continue;
+            }
+            /* 16th block of a chunk with more input following: close the chunk, push its CV */
+            b3_out co;
+            chunk_output(h, &co);
+            uint32_t cv[8];
+            out_cv(&co, cv);
+            uint64_t total = h->chunk_counter + 1;
+            while ((total & 1) == 0) {
+                b3_out po;
+                h->stack_len--;
+                parent_output(h->stack[h->stack_len], cv, h->key, h->flags, &po);
+                out_cv(&po, cv);
+                total >>= 1;
+            }
+            memcpy(h->stack[h->stack_len++], cv, 32);
+            memcpy(h->cv, h->key, 32);
+            h->chunk_counter++;
+            h->block_len = 0;
+            h->blocks_done = 0;
+        }
+        size_t take = 64 - h->block_len;
+        if (take > len) take = len;
+        memcpy(h->block + h->block_len, in, take);
+        h->block_len += (uint32_t)take;
+        in += take;
+        len -= take;
+    }
+}
+
+static void b3_final_out(const b3 *h, b3_out *o) {
+    chunk_output(h, o);
+    for (int i = h->stack_len - 1; i >= 0; i--) {
+        uint32_t cv[8];
+        out_cv(o, cv);
+        parent_output(h->stack[i], cv, h->key, h->flags, o);
+    }
+}
+
+/* root output bytes [pos, pos + len) */
+static void b3_squeeze(const b3_out *o, uint64_t pos, uint8_t *dst, size_t len) {
+    while (len > 0) {
+        uint64_t blk = pos / 64;
+        uint32_t w[16];
+        b3_compress(o->cv, o->block, blk, o->block_len, o->flags | ROOT, w);
+        uint8_t bytes[64];
+        for (int i = 0; i < 16; i++) {
+            bytes[4 * i] = (uint8_t)w[i];
+            bytes[4 * i + 1] = (uint8_t)(w[i] >> 8);
+            bytes[4 * i + 2] = (uint8_t)(w[i] >> 16);
+            bytes[4 * i + 3] = (uint8_t)(w[i] >> 24);
+        }
+        size_t off = (size_t)(pos % 64), take = 64 - off;
+        if (take > len) take = len;
+        memcpy(dst, bytes + off, take);
+        dst += take;
+        pos += take;
+        len -= take;
+    }
+}
+
+static void b3_new(b3 *h) { b3_init(h, B3_IV, 0); }
+static void b3_new_keyed(b3 *h, const uint8_t key[32]) {
+    uint32_t k[8];
+    words_le(key, k, 8);
+    b3_init(h, k, KEYED_HASH);
+}
+static void b3_new_derive(b3 *h, const char *context) {
+    b3 c;
+    b3_init(&c, B3_IV, DERIVE_KEY_CONTEXT);
+    b3_update(&c, (const uint8_t *)context, strlen(context));
+    b3_out o;
+    b3_final_out(&c, &o);
+    uint8_t ck[32];
+    b3_squeeze(&o, 0, ck, 32);
+    uint32_t k[8];
+    words_le(ck, k, 8);
+    b3_init(h, k, DERIVE_KEY_MATERIAL);
+}
+
+/* mode 0 hash, 1 keyed (key = 32 bytes), 2 derive_key (key = NUL-terminated context) */
+void orc_blake3(int mode, const uint8_t *key, const uint8_t *in, size_t len, uint8_t *out,
+                size_t out_len) {
+    b3 h;
+    if (mode == 0) b3_new(&h);
+    else if (mode == 1) b3_new_keyed(&h, key);
+    else b3_new_derive(&h, (const char *)key);
+    b3_update(&h, in, len);
+    b3_out o;
+    b3_final_out(&h, &o);
+    b3_squeeze(&o, 0, out, out_len);
+}
+
+/* ============================================================================================
+ * randomness and inputs: rng.rs:1-20, key.rs:48-88, scheme/params.rs:320-324,
+ * scheme/witness.rs:64-87
+ * ========================================================================================== */
+
+typedef struct { uint64_t s; } rng_t;
+static void rng_new(rng_t *r, uint64_t seed) { r->s = seed * 0x9E3779B97F4A7C15ull | 1; }
+static uint64_t rng_next(rng_t *r) {
+    uint64_t x = r->s;
+    x ^= x >> 12;
+    x ^= x << 25;
+    x ^= x >> 27;
+    r->s = x;
+    return x * 0x2545F4914F6CDD1Dull;
+}
+static uint32_t rng_below(rng_t *r, uint32_t n) {
+    return (uint32_t)(((rng_next(r) >> 32) * (uint64_t)n) >> 32);
+}
+
+/* scheme/params.rs:321-322 */
+uint64_t orc_matrix_seed(const uint8_t seed[32]) {
+    uint8_t d[32];
+    orc_blake3(0, 0, seed, 32, d, 32);
+    uint64_t s = 0;
+    for (int i = 7; i >= 0; i--) s = s << 8 | d[i];
+    return s;
+}
+
+/* key.rs:63-81: a[k][i][j] = A_k[i][j], i = 32 b + p, drawn b-major, then j, then p */
+void orc_key_expand(const uint32_t *primes, int limbs, int len_ring, uint64_t seed, int16_t *a) {
+    for (int k = 0; k < limbs; k++) {
+        uint32_t q = primes[k];
+        int16_t half = (int16_t)((q - 1) / 2);
+        rng_t r;
+        rng_new(&r, seed ^ (0x9E3779B9ull * (uint64_t)(k + 1)));
+        int16_t *ak = a + (size_t)k * len_ring * N;
+        for (int b = 0; b < len_ring / 32; b++)
+            for (int j = 0; j < N; j++)
+                for (int p = 0; p < 32; p++)
+                    ak[(size_t)(32 * b + p) * N + j] = (int16_t)((int16_t)rng_below(&r, q) - half);
+    }
+}
+
+/* scheme/witness.rs:66-87: n elements as 3 u64 each */
+void orc_witness_random(const uint8_t seed[32], size_t n, uint64_t *w) {
+    b3 h;
+    b3_new_keyed(&h, seed);
+    b3_update(&h, (const uint8_t *)"labinius/witness", 16);
+    b3_out o;
+    b3_final_out(&h, &o);
+    uint8_t buf[24 * 256];
+    for (size_t i0 = 0; i0 < n; i0 += 256) {
+        size_t cnt = n - i0 < 256 ? n - i0 : 256;
+        b3_squeeze(&o, 24 * i0, buf, 24 * cnt);
+        for (size_t i = 0; i < cnt; i++) {
+            for (int k = 0; k < 3; k++) {
+                uint64_t x = 0;
+                for (int b = 7; b >= 0; b--) x = x << 8 | buf[24 * i + 8 * k + b];
+                w[3 * (i0 + i) + k] = x;
+            }
+            w[3 * (i0 + i) + 2] &= (1ull << 34) - 1;
+        }
+    }
+}
+
+/* f162.rs:32-41: coefficient 4m + k of ring element r is bit m of element 4r + k */
+void orc_lift(const uint64_t *elems, uint32_t *c) {
+    for (int m = 0; m < 162; m++)
+        for (int k = 0; k < 4; k++) c[4 * m + k] = (uint32_t)(elems[3 * k + m / 64] >> (m % 64)) & 1;
+}
+
+/* ============================================================================================
+ * the commitment: tests/commit.rs:86-101 (split), scalar.rs:254-265 (quadratic), key.rs:192-234
+ * ========================================================================================== */
+
+/* raw[k][c][j] in [0, q_k); comp[c][t][k][s] centered (the column-major 4 x r matrix, each entry
+ * holding one R_162 element per limb) */
+void orc_commit(const uint32_t *primes, int limbs, int len_ring, int cols, const int16_t *a,
+                const uint64_t *w, uint32_t *raw, int16_t *comp) {
+    init_tables();
+    for (int k = 0; k < limbs; k++) {
+        const prime_tab *t = prime(primes[k]);
+        uint64_t q = t->q;
+        const int16_t *ak = a + (size_t)k * len_ring * N;
+        for (int c = 0; c < cols; c++) {
+            uint64_t y[N];
+            memset(y, 0, sizeof y);
+            for (int i = 0; i < len_ring; i++) {
+                uint32_t lift[N];
+                orc_lift(w + 3 * ((size_t)c * len_ring * 4 + 4 * (size_t)i), lift);
+                uint64_t v[N];
+                for (int j = 0; j < N; j++) v[j] = lift[j];
+                uint64_t av[N];
+                for (int j = 0; j < N; j++)
+                    av[j] = (uint64_t)(((int64_t)ak[(size_t)i * N + j] % (int64_t)q + (int64_t)q) % (int64_t)q);
+                if (t->quad) {
+                    ntt_quad(t, v);
+                    uint64_t pr[N];
+                    mul_quad_slots(t, av, v, pr);
+                    for (int j = 0; j < N; j++) y[j] = md(t, y[j] + pr[j]);
+                } else {
+                    ntt_split(t, v);
+                    for (int j = 0; j < N; j++) y[j] = md(t, y[j] + v[j] * av[j]);
+                }
+            }
+            uint32_t *rk = raw + ((size_t)k * cols + c) * N;
+            for (int j = 0; j < N; j++) rk[j] = (uint32_t)y[j];
+            int16_t d[4 * N162];
+            orc_decompose(primes[k], rk, d);
+            for (int tt = 0; tt < 4; tt++)
+                for (int s = 0; s < N162; s++)
+                    comp[(((size_t)c * 4 + tt) * limbs + k) * N162 + s] = d[tt * N162 + s];
+        }
+    }
+}
+
+/* ============================================================================================
+ * the fold: fold.rs:281-393 (embed, challenge transforms), 605-677 (fold_witness)
+ * ========================================================================================== */
+
+typedef struct {
+    uint8_t positions[32];
+    uint32_t signs;
+    uint32_t weight;
+} orc_challenge;
+
+/* fold.rs:287-300: c(-X^4), coefficient of X^{4m} = (-1)^m (1 - 2 sign) */
+void orc_embed(const orc_challenge *c, int32_t *e) {
+    memset(e, 0, N * sizeof(int32_t));
+    for (uint32_t i = 0; i < c->weight; i++) {
+        int m = c->positions[i];
+        int co = 1 - 2 * (int)((c->signs >> i) & 1);
+        e[4 * m] = (m % 2 == 0) ? co : -co;
+    }
+}
+
+/* v (len_ring x 648 centered coefficients mod the base prime) = sum_j c_j(-X^4) W_j, computed as
+ * the reference does: transform the embedded challenges and the kept witness modulo the base,
+ * multiply slot-wise, sum over the columns, invert. */
+void orc_fold(uint32_t q, int len_ring, int cols, const uint64_t *w, const orc_challenge *ch,
+              int16_t *v) {
+    init_tables();
+    const prime_tab *t = prime(q);
+    uint64_t qq = q;
+    uint64_t *chat = (uint64_t *)malloc(sizeof(uint64_t) * N * cols);
+    for (int j = 0; j < cols; j++) {
+        int32_t e[N];
+        orc_embed(&ch[j], e);
+        uint64_t *cj = chat + (size_t)j * N;
+        for (int i = 0; i < N; i++) cj[i] = (uint64_t)((e[i] % (int64_t)qq + (int64_t)qq) % (int64_t)qq);
+        if (t->quad) {
+            ntt_quad(t, cj);
+            /* fold.rs:346-352: the leaf image is a scalar; write it into both rows */
+            for (int l = 0; l < 324; l++) cj[2 * l + 1] = cj[2 * l];
+        } else {
+            ntt_split(t, cj);
+        }
+    }
+    for (int i = 0; i < len_ring; i++) {
+        uint64_t acc[N];
+        memset(acc, 0, sizeof acc);
+        for (int j = 0; j < cols; j++) {
+            uint32_t lift[N];
+            orc_lift(w + 3 * ((size_t)j * len_ring * 4 + 4 * (size_t)i), lift);
+            uint64_t x[N];
+            for (int u = 0; u < N; u++) x[u] = lift[u];
+            if (t->quad) ntt_quad(t, x);
+            else ntt_split(t, x);
+            const uint64_t *cj = chat + (size_t)j * N;
+            for (int u = 0; u < N; u++) acc[u] = md(t, acc[u] + cj[u] * x[u]);
+        }
+        intt_any(t, acc);
+        for (int u = 0; u < N; u++) v[(size_t)i * N + u] = center(acc[u], qq);
+    }
+    free(chat);
+}
+
+/* The same v straight from the definition: the ring products c_j(-X^4) w_{j,i} in
+ * Z[X]/(X^648 - X^324 + 1) by sparse convolution, summed over Z, then centered mod q. Used to
+ * check sampled output elements at sizes where orc_fold is too slow. */
+void orc_fold_direct(uint32_t q, int len_ring, int cols, const uint64_t *w,
+                     const orc_challenge *ch, const int32_t *rows, int n_rows, int16_t *v) {
+    for (int r = 0; r < n_rows; r++) {
+        int i = rows[r];
+        int64_t acc[2 * N];
+        memset(acc, 0, sizeof acc);
+        for (int j = 0; j < cols; j++) {
+            uint32_t lift[N];
+            orc_lift(w + 3 * ((size_t)j * len_ring * 4 + 4 * (size_t)i), lift);
+            int32_t e[N];
+            orc_embed(&ch[j], e);
+            for (int a = 0; a < N; a += 4) {
+                if (!e[a]) continue;
+                for (int b = 0; b < N; b++)
+                    if (lift[b]) acc[a + b] += e[a];
+            }
+        }
+        for (int k = 2 * N - 1; k >= N; k--) {
+            acc[k - 324] += acc[k];
+            acc[k - 648] -= acc[k];
+            acc[k] = 0;
+        }
+        for (int u = 0; u < N; u++) {
+            int64_t x = acc[u] % (int64_t)q;
+            if (x < 0) x += q;
+            v[(size_t)r * N + u] = center((uint64_t)x, q);
+        }
+    }
+}
+
+/* ============================================================================================
+ * F162 = GF(2)[x]/(x^162 + x^81 + 1): fields/scalar.rs:92-116, the schoolbook of
+ * tests/fields.rs:33-48
+ * ========================================================================================== */
+
+static int bit(const uint64_t *v, int i) { return (int)((v[i / 64] >> (i % 64)) & 1); }
+static void flip(uint64_t *v, int i) { v[i / 64] ^= 1ull << (i % 64); }
+
+void orc_f162_mul(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    uint64_t p[6] = {0};
+    for (int i = 0; i < 162; i++) {
+        if (!bit(a, i)) continue;
+        /* p ^= b << i */
+        int ws = i / 64, bs = i % 64;
+        for (int k = 0; k < 3; k++) {
+            p[k + ws] ^= b[k] << bs;
+            if (bs) p[k + ws + 1] ^= b[k] >> (64 - bs);
+        }
+    }
+    for (int j = 323; j >= 162; j--)
+        if (bit(p, j)) {
+            flip(p, j);
+            flip(p, j - 162);
+            flip(p, j - 81);
+        }
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = p[2];
+}
+
+/* eval.rs:51-63 */
+void orc_eq_table(const uint64_t *ps, int n, uint64_t *t) {
+    t[0] = 1;
+    t[1] = 0;
+    t[2] = 0;
+    size_t len = 1;
+    for (int k = 0; k < n; k++) {
+        const uint64_t *p = ps + 3 * k;
+        uint64_t p1[3] = {p[0] ^ 1, p[1], p[2]};
+        for (size_t b = 0; b < len; b++) {
+            uint64_t x[3] = {t[3 * b], t[3 * b + 1], t[3 * b + 2]};
+            orc_f162_mul(x, p, t + 3 * (b + len));
+            orc_f162_mul(x, p1, t + 3 * b);
+        }
+        len *= 2;
+    }
+}
+
+/* eval.rs:171-182: u_j = sum_i eq(p0, i) W[i + wdim j] */
+void orc_row_evaluate(const uint64_t *w, size_t n, const uint64_t *p0, int rows, uint64_t *u) {
+    size_t wdim = (size_t)1 << rows;
+    uint64_t *eq = (uint64_t *)malloc(sizeof(uint64_t) * 3 * wdim);
+    orc_eq_table(p0, rows, eq);
+    for (size_t j = 0; j < n / wdim; j++) {
+        uint64_t acc[3] = {0, 0, 0};
+        for (size_t i = 0; i < wdim; i++) {
+            uint64_t pr[3];
+            orc_f162_mul(eq + 3 * i, w + 3 * (i + wdim * j), pr);
+            acc[0] ^= pr[0];
+            acc[1] ^= pr[1];
+            acc[2] ^= pr[2];
+        }
+        memcpy(u + 3 * j, acc, 24);
+    }
+    free(eq);
+}
+
+/* eval.rs:185-193: u . eq(p1) */
+void orc_claim(const uint64_t *u, const uint64_t *p1, int cols_log, uint64_t *t) {
+    size_t r = (size_t)1 << cols_log;
+    uint64_t *eq = (uint64_t *)malloc(sizeof(uint64_t) * 3 * r);
+    orc_eq_table(p1, cols_log, eq);
+    uint64_t acc[3] = {0, 0, 0};
+    for (size_t j = 0; j < r; j++) {
+        uint64_t pr[3];
+        orc_f162_mul(eq + 3 * j, u + 3 * j, pr);
+        acc[0] ^= pr[0];
+        acc[1] ^= pr[1];
+        acc[2] ^= pr[2];
+    }
+    memcpy(t, acc, 24);
+    free(eq);
+}
+
+/* ============================================================================================
+ * the transcript and the short challenges: challenge.rs:66-221, 231-311, 332-533
+ * ========================================================================================== */
+
+typedef struct {
+    b3 state;
+    uint64_t counter;
+} orc_transcript;
+
+static void t_update(orc_transcript *t, const void *p, size_t n) { b3_update(&t->state, (const uint8_t *)p, n); }
+static void t_u64(orc_transcript *t, uint64_t x) {
+    uint8_t b[8];
+    for (int i = 0; i < 8; i++) b[i] = (uint8_t)(x >> (8 * i));
+    t_update(t, b, 8);
+}
+
+size_t orc_transcript_size(void) { return sizeof(orc_transcript); }
+
+/* challenge.rs:73-78 */
+void orc_transcript_new(orc_transcript *t, const uint8_t *domain, size_t len) {
+    b3_new(&t->state);
+    t->counter = 0;
+    t_u64(t, len);
+    t_update(t, domain, len);
+}
+/* challenge.rs:81-84 */
+void orc_absorb_bytes(orc_transcript *t, const uint8_t *b, size_t len) {
+    t_u64(t, len);
+    t_update(t, b, len);
+}
+/* challenge.rs:87-89 */
+void orc_absorb_u64(orc_transcript *t, uint64_t x) { t_u64(t, x); }
+
+/* challenge.rs:153-160: the XOF a derivation reads */
+static void t_reader(orc_transcript *t, const char *label, b3_out *o) {
+    orc_transcript c = *t;
+    t_u64(&c, t->counter);
+    size_t n = strlen(label);
+    t_u64(&c, n);
+    t_update(&c, label, n);
+    t->counter++;
+    b3_final_out(&c.state, o);
+}
+
+/* challenge.rs:109-111 */
+void orc_fill(orc_transcript *t, const char *label, uint8_t *out, size_t len) {
+    b3_out o;
+    t_reader(t, label, &o);
+    b3_squeeze(&o, 0, out, len);
+}
+
+typedef struct {
+    b3_out o;
+    uint64_t pos;
+    uint8_t buf[256];
+    int bpos;
+} xof_t;
+
+static uint8_t xof_byte(xof_t *x) {
+    if (x->bpos == 256) {
+        b3_squeeze(&x->o, x->pos, x->buf, 256);
+        x->pos += 256;
+        x->bpos = 0;
+    }
+    return x->buf[x->bpos++];
+}
+/* challenge.rs:189-196 */
+static uint16_t xof_u16(xof_t *x) {
+    if (x->bpos + 2 <= 256) {
+        uint16_t v = (uint16_t)(x->buf[x->bpos] | x->buf[x->bpos + 1] << 8);
+        x->bpos += 2;
+        return v;
+    }
+    uint8_t lo = xof_byte(x), hi = xof_byte(x);
+    return (uint16_t)(lo | hi << 8);
+}
+/* challenge.rs:198-221 */
+static uint16_t xof_below(xof_t *x, uint32_t n) {
+    uint32_t limit = (65536u / n) * n;
+    uint64_t magic = (1ull << 32) / n + 1;
+    for (;;) {
+        uint32_t r = xof_u16(x);
+        if (r < limit) {
+            uint32_t qt = (uint32_t)(((uint64_t)r * magic) >> 32);
+            return (uint16_t)(r - qt * n);
+        }
+    }
+}
+
+/* challenge.rs:259-269 */
+static void challenge_signed(orc_challenge *c) {
+    b3 h;
+    b3_new_derive(&h, "labinius 2026 challenge signs v1");
+    uint8_t wb[8];
+    for (int i = 0; i < 8; i++) wb[i] = (uint8_t)((uint64_t)c->weight >> (8 * i));
+    b3_update(&h, wb, 8);
+    b3_update(&h, c->positions, c->weight);
+    b3_out o;
+    b3_final_out(&h, &o);
+    uint8_t d[4];
+    b3_squeeze(&o, 0, d, 4);
+    uint32_t bits = (uint32_t)d[0] | (uint32_t)d[1] << 8 | (uint32_t)d[2] << 16 | (uint32_t)d[3] << 24;
+    uint32_t mask = (uint32_t)((1ull << c->weight) - 1);
+    c->signs = bits & mask;
+}
+
+/* challenge.rs:332-360: phase table over the first 81 units */
+static double PH_RE[N162][81], PH_IM[N162][81];
+static int ph_ready = 0;
+static void init_phase(void) {
+    if (ph_ready) return;
+    int units[N162], n = 0;
+    for (int x = 1; x < 243; x++)
+        if (x % 3 != 0) units[n++] = x;
+    for (int p = 0; p < N162; p++)
+        for (int k = 0; k < 81; k++) {
+            int m = (p * units[k]) % 243;
+            double angle = 2.0 * 3.141592653589793 * (double)m / 243.0;
+            PH_RE[p][k] = cos(angle);
+            PH_IM[p][k] = sin(angle);
+        }
+    ph_ready = 1;
+}
+
+/* challenge.rs:362-419: every root's |c(zeta^u)|^2 <= bound_sq, accumulated term by term */
+static int within(const orc_challenge *c, double bound_sq) {
+    init_phase();
+    for (int k = 0; k < 81; k++) {
+        double re = 0.0, im = 0.0;
+        for (uint32_t i = 0; i < c->weight; i++) {
+            int p = c->positions[i];
+            if ((c->signs >> i) & 1) {
+                re = re + (-PH_RE[p][k]);
+                im = im + (-PH_IM[p][k]);
+            } else {
+                re = re + PH_RE[p][k];
+                im = im + PH_IM[p][k];
+            }
+        }
+        volatile double a = re * re, b = im * im;
+        double sq = a + b;
+        if (sq > bound_sq) return 0;
+    }
+    return 1;
+}
+
+double orc_canonical_norm_sq(const orc_challenge *c) {
+    init_phase();
+    double best = 0.0;
+    for (int k = 0; k < 81; k++) {
+        double re = 0.0, im = 0.0;
+        for (uint32_t i = 0; i < c->weight; i++) {
+            int p = c->positions[i];
+            double s = ((c->signs >> i) & 1) ? -1.0 : 1.0;
+            re += s * PH_RE[p][k];
+            im += s * PH_IM[p][k];
+        }
+        double sq = re * re + im * im;
+        if (sq > best) best = sq;
+    }
+    return best;
+}
+
+/* challenge.rs:478-506 */
+static void attempt(xof_t *x, uint32_t weight, orc_challenge *out) {
+    uint8_t perm[N162];
+    for (int i = 0; i < N162; i++) perm[i] = (uint8_t)i;
+    memset(out, 0, sizeof *out);
+    out->weight = weight;
+    for (uint32_t i = 0; i < weight; i++) {
+        uint32_t j = i + xof_below(x, (uint32_t)(N162 - i));
+        uint8_t tmp = perm[i];
+        perm[i] = perm[j];
+        perm[j] = tmp;
+    }
+    uint64_t seen[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < weight; i++) seen[perm[i] >> 6] |= 1ull << (perm[i] & 63);
+    int n = 0;
+    for (int wd = 0; wd < 3; wd++)
+        for (int b = 0; b < 64; b++)
+            if ((seen[wd] >> b) & 1) out->positions[n++] = (uint8_t)(64 * wd + b);
+}
+
+/* challenge.rs:517-533; returns the number of attempts */
+uint64_t orc_sample_short_challenge(orc_transcript *t, uint32_t weight, double bound,
+                                    orc_challenge *out) {
+    double bound_sq = bound * bound + 1e-12;
+    xof_t x;
+    t_reader(t, "short-challenge", &x.o);
+    x.pos = 0;
+    x.bpos = 256;
+    uint64_t attempts = 0;
+    for (;;) {
+        attempts++;
+        attempt(&x, weight, out);
+        challenge_signed(out);
+        if (within(out, bound_sq)) return attempts;
+    }
+}
+
+/* ============================================================================================
+ * the protocol glue: scheme/verifier.rs:41-100, scheme/commitment.rs:217-228, 320-331
+ * ========================================================================================== */
+
+typedef struct {
+    uint32_t witness_log_len, column_log_len;
+    uint32_t n_primes;
+    uint32_t primes[8];
+    uint32_t dropped_bits;
+    uint32_t recursion;
+    uint8_t matrix_seed[32];
+} orc_params;
+
+/* verifier.rs:41-57 */
+static void absorb_parameters(orc_transcript *t, const orc_params *p) {
+    orc_absorb_bytes(t, (const uint8_t *)"labinius/parameters", 19);
+    orc_absorb_u64(t, p->witness_log_len);
+    orc_absorb_u64(t, p->column_log_len);
+    for (uint32_t i = 0; i < p->n_primes; i++) orc_absorb_u64(t, p->primes[i]);
+    orc_absorb_u64(t, p->recursion ? 1 : 0);
+    if (p->dropped_bits > 0) {
+        orc_absorb_bytes(t, (const uint8_t *)"labinius/dropped-bits", 21);
+        orc_absorb_u64(t, p->dropped_bits);
+    }
+    orc_absorb_bytes(t, p->matrix_seed, 32);
+}
+
+/* verifier.rs:61-100 for a Clear commitment (comp laid out as in orc_commit) */
+void orc_derive_evaluation_point(orc_transcript *t, const orc_params *p, const int16_t *comp,
+                                 uint64_t *point) {
+    absorb_parameters(t, p);
+    orc_absorb_bytes(t, (const uint8_t *)"labinius/commitment", 19);
+    size_t cols = (size_t)1 << p->column_log_len;
+    orc_absorb_u64(t, cols);
+    int limbs = (int)p->n_primes;
+    size_t per = 2 * N162 * (size_t)limbs;
+    uint8_t *buf = (uint8_t *)malloc(per);
+    for (size_t c = 0; c < cols; c++) {
+        t_u64(t, 4);
+        for (int row = 0; row < 4; row++) {
+            for (int k = 0; k < limbs; k++)
+                for (int s = 0; s < N162; s++) {
+                    int16_t x = comp[((c * 4 + row) * limbs + k) * N162 + s];
+                    buf[2 * (k * N162 + s)] = (uint8_t)x;
+                    buf[2 * (k * N162 + s) + 1] = (uint8_t)((uint16_t)x >> 8);
+                }
+            t_update(t, buf, per);
+        }
+    }
+    free(buf);
+    int nv = (int)(p->witness_log_len);
+    uint8_t *bytes = (uint8_t *)malloc(24 * (size_t)nv);
+    orc_fill(t, "labinius/evaluation-point", bytes, 24 * (size_t)nv);
+    for (int e = 0; e < nv; e++) {
+        for (int k = 0; k < 3; k++) {
+            uint64_t x = 0;
+            for (int b = 7; b >= 0; b--) x = x << 8 | bytes[24 * e + 8 * k + b];
+            point[3 * e + k] = x;
+        }
+        point[3 * e + 2] &= (1ull << 34) - 1;
+    }
+    free(bytes);
+}
+
+/* commitment.rs:217-228, 320-331 */
+void orc_derive_folding_challenges(orc_transcript *t, const uint64_t *u, size_t cols,
+                                   orc_challenge *out) {
+    orc_absorb_bytes(t, (const uint8_t *)"labinius/row-evaluation", 23);
+    uint8_t *bytes = (uint8_t *)malloc(24 * cols);
+    for (size_t j = 0; j < cols; j++)
+        for (int k = 0; k < 3; k++)
+            for (int b = 0; b < 8; b++) bytes[24 * j + 8 * k + b] = (uint8_t)(u[3 * j + k] >> (8 * b));
+    orc_absorb_bytes(t, bytes, 24 * cols);
+    free(bytes);
+    for (size_t j = 0; j < cols; j++) orc_sample_short_challenge(t, 28, 12.0, &out[j]);
+}
+
+/* ============================================================================================
+ * the verifier's checks: verifier.rs:114-231, fold.rs:799-873, eval.rs:196-249
+ * ========================================================================================== */
+
+/* The challenge's value at R_162 slot s: slot SLOT_648[0][s] of its transform (split), or the
+ * scalar of leaf QUAD_CLASS_SLOT[0][s] (quadratic); fold.rs:799-830. */
+static void challenge_slots(const prime_tab *t, const orc_challenge *c, uint64_t *out) {
+    int32_t e[N];
+    orc_embed(c, e);
+    uint64_t v[N];
+    for (int i = 0; i < N; i++) v[i] = (uint64_t)((e[i] % (int64_t)t->q + (int64_t)t->q) % (int64_t)t->q);
+    if (t->quad) {
+        ntt_quad(t, v);
+        for (int s = 0; s < N162; s++) out[s] = v[2 * QUAD_CLASS_SLOT[0][s]];
+    } else {
+        ntt_split(t, v);
+        for (int s = 0; s < N162; s++) out[s] = v[SLOT_648[0][s]];
+    }
+}
+
+/* verifier.rs:114-146: out[t][k][s] = sum_j c_j . C_j[t][k][s] mod q_k, centered */
+void orc_fold_commitment(const uint32_t *primes, int limbs, int cols, const int16_t *comp,
+                         const orc_challenge *ch, int16_t *out) {
+    init_tables();
+    for (int k = 0; k < limbs; k++) {
+        const prime_tab *t = prime(primes[k]);
+        int64_t q = t->q;
+        int64_t acc[4][N162];
+        memset(acc, 0, sizeof acc);
+        for (int j = 0; j < cols; j++) {
+            uint64_t cs[N162];
+            challenge_slots(t, &ch[j], cs);
+            for (int row = 0; row < 4; row++)
+                for (int s = 0; s < N162; s++) {
+                    int64_t x = comp[(((size_t)j * 4 + row) * limbs + k) * N162 + s];
+                    acc[row][s] = (acc[row][s] + (int64_t)cs[s] * x) % q;
+                }
+        }
+        for (int row = 0; row < 4; row++)
+            for (int s = 0; s < N162; s++) {
+                int64_t x = acc[row][s] % q;
+                if (x < 0) x += q;
+                out[(row * limbs + k) * N162 + s] = center((uint64_t)x, (uint64_t)q);
+            }
+    }
+}
+
+/* eval.rs:196-200 */
+void orc_fold_binary(const uint64_t *u, const orc_challenge *ch, size_t cols, uint64_t *out) {
+    uint64_t acc[3] = {0, 0, 0};
+    for (size_t j = 0; j < cols; j++) {
+        uint64_t c[3] = {0, 0, 0};
+        for (uint32_t i = 0; i < ch[j].weight; i++) c[ch[j].positions[i] >> 6] |= 1ull << (ch[j].positions[i] & 63);
+        uint64_t pr[3];
+        orc_f162_mul(c, u + 3 * j, pr);
+        acc[0] ^= pr[0];
+        acc[1] ^= pr[1];
+        acc[2] ^= pr[2];
+    }
+    memcpy(out, acc, 24);
+}
+
+/* verifier.rs:178-231 for the Clear mode: returns 1 on acceptance. fold_cap per
+ * scheme/params.rs:263-265. */
+int orc_verify_folded_opening(const orc_params *p, const int16_t *a, const int16_t *folded_comm,
+                              const int16_t *v, const uint64_t *p0, const uint64_t *u_folded) {
+    init_tables();
+    int limbs = (int)p->n_primes;
+    int rows = (int)(p->witness_log_len - p->column_log_len);
+    int len_ring = (1 << rows) / 4;
+    int32_t half = (int32_t)((p->primes[0] - 1) / 2);
+    uint64_t normsq = 0;
+    int32_t worst = 0;
+    for (size_t i = 0; i < (size_t)len_ring * N; i++) {
+        int32_t x = v[i];
+        int32_t ax = x < 0 ? -x : x;
+        if (ax > worst) worst = ax;
+        normsq += (uint64_t)((int64_t)x * x);
+    }
+    double capd = ceil(108.0 * (double)(((size_t)1 << p->witness_log_len) / 4 * N));
+    if (worst > half || normsq > (uint64_t)capd) return 0;
+    for (int k = 0; k < limbs; k++) {
+        const prime_tab *t = prime(p->primes[k]);
+        uint64_t q = t->q;
+        uint64_t y[N];
+        memset(y, 0, sizeof y);
+        for (int i = 0; i < len_ring; i++) {
+            uint64_t x[N], av[N];
+            for (int u = 0; u < N; u++) x[u] = (uint64_t)((v[(size_t)i * N + u] % (int64_t)q + (int64_t)q) % (int64_t)q);
+            for (int u = 0; u < N; u++)
+                av[u] = (uint64_t)((a[((size_t)k * len_ring + i) * N + u] % (int64_t)q + (int64_t)q) % (int64_t)q);
+            if (t->quad) {
+                ntt_quad(t, x);
+                uint64_t pr[N];
+                mul_quad_slots(t, av, x, pr);
+                for (int u = 0; u < N; u++) y[u] = (y[u] + pr[u]) % q;
+            } else {
+                ntt_split(t, x);
+                for (int u = 0; u < N; u++) y[u] = (y[u] + x[u] * av[u]) % q;
+            }
+        }
+        uint32_t y32[N];
+        for (int u = 0; u < N; u++) y32[u] = (uint32_t)y[u];
+        int16_t d[4 * N162];
+        orc_decompose(p->primes[k], y32, d);
+        for (int row = 0; row < 4; row++)
+            for (int s = 0; s < N162; s++)
+                if (d[row * N162 + s] != folded_comm[(row * limbs + k) * N162 + s]) return 0;
+    }
+    /* eval.rs:202-249: eq(p0) . (v mod 2) == u' with element 4m + k = component k of v_m */
+    size_t wdim = (size_t)1 << rows;
+    uint64_t *eq = (uint64_t *)malloc(sizeof(uint64_t) * 3 * wdim);
+    orc_eq_table(p0, rows, eq);
+    uint64_t acc[3] = {0, 0, 0};
+    for (int m = 0; m < len_ring; m++)
+        for (int k = 0; k < 4; k++) {
+            uint64_t e[3] = {0, 0, 0};
+            for (int b = 0; b < 162; b++)
+                if (v[(size_t)m * N + 4 * b + k] & 1) e[b / 64] |= 1ull << (b % 64);
+            uint64_t pr[3];
+            orc_f162_mul(eq + 3 * (4 * (size_t)m + k), e, pr);
+            acc[0] ^= pr[0];
+            acc[1] ^= pr[1];
+            acc[2] ^= pr[2];
+        }
+    free(eq);
+    return acc[0] == u_folded[0] && acc[1] == u_folded[1] && acc[2] == u_folded[2];
+}

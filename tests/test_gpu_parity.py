@@ -1,0 +1,172 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the same seeded inputs: the
+commitment's raw rows and components, the folded witness, the row and MLE evaluations, and the
+verifier's acceptance. Integer and F162 outputs are compared bit for bit."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MATRIX_SEED = bytes([0x5A]) * 32
+WITNESS_SEED = bytes([0xC7]) * 32
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    from tests.conftest import gpu_available
+    if not gpu_available():
+        pytest.skip("no GPU")
+
+
+def test_mfma_lane_layout(lbn):
+    say("case: mfma probe")
+    rng = np.random.default_rng(1)
+    a = rng.integers(-128, 128, (16, 64)).astype(np.int8)
+    b = rng.integers(-128, 128, (64, 16)).astype(np.int8)
+    d = np.zeros((16, 16), np.int32)
+    rc = lbn.lib().lbn_debug_mfma_probe(a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                                        d.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    assert np.array_equal(d, a.astype(np.int32) @ b.astype(np.int32))
+
+
+def _setup(lbn, orc, params):
+    pp = lbn.PublicParameters.from_seed(params, MATRIX_SEED)
+    prover = lbn.Prover(pp)
+    w = lbn.Witness.random(params, WITNESS_SEED)
+    a = orc.key_expand(params.primes(), params.len_ring(), orc.matrix_seed(MATRIX_SEED))
+    return pp, prover, w, a
+
+
+CASES = [
+    (10, 2, 3889, [9721]),
+    (11, 3, 3889, [2917, 4861]),
+    (10, 1, 3889, [12637, 17497, 19441]),
+    (12, 4, 2917, [3889]),
+    (11, 2, 19441, [4861]),
+    (9, 2, 4861, []),
+]
+
+
+@pytest.mark.parametrize("wl,cl,base,extra", CASES)
+def test_commit_fold_eval_parity(lbn, orc, wl, cl, base, extra):
+    say(f"case: parity wl={wl} cl={cl} base={base} extra={extra}")
+    params = lbn.Params.with_base(wl, cl, lbn.Modulus(base), [lbn.Modulus(q) for q in extra])
+    pp, prover, w, a = _setup(lbn, orc, params)
+    commitment, opening = prover.commit(w)
+    raw, comp = orc.commit(params.primes(), params.len_ring(), params.columns(), a, w.elements)
+    assert np.array_equal(commitment.raw, raw)
+    assert np.array_equal(commitment.components, comp)
+    verifier = lbn.Verifier(pp)
+    t = lbn.Transcript(b"labinius/test/parity")
+    point = verifier.derive_evaluation_point(t, commitment)
+    ot = orc.Transcript(b"labinius/test/parity")
+    op0, op1 = orc.derive_evaluation_point(ot, orc.make_params(wl, cl, params.primes(), MATRIX_SEED), comp)
+    assert np.array_equal(point.p0, op0) and np.array_equal(point.p1, op1)
+    row = prover.row_evaluate(w, point)
+    assert np.array_equal(row.values, orc.row_evaluate(w.elements, point.p0))
+    claimed = prover.mle_evaluate(w, point)
+    assert np.array_equal(claimed, orc.claim(row.values, point.p1))
+    challenges = verifier.derive_folding_challenges(t, row)
+    och = orc.derive_folding_challenges(ot, row.values)
+    assert all(bytes(x.positions) == bytes(y.positions) and x.signs == y.signs for x, y in zip(challenges.challenges, och))
+    folded = prover.fold(opening, challenges)
+    assert np.array_equal(folded.elements, orc.fold(base, params.len_ring(), params.columns(), w.elements, och))
+    fc = verifier.fold_commitment(commitment, challenges)
+    assert np.array_equal(fc.rows, orc.fold_commitment(params.primes(), params.columns(), comp, och))
+    fr = verifier.fold_row_evaluation(row, challenges)
+    verifier.verify_evaluation(point, claimed, row)
+    verifier.verify_opening(commitment, challenges, point, fc, folded, fr)
+    assert orc.verify_folded_opening(orc.make_params(wl, cl, params.primes(), MATRIX_SEED), a, fc.rows,
+                                     folded.elements, point.p0, fr)
+
+
+def test_basic_round_bit_exact(lbn, orc):
+    """Params::basic() (2^18 F162, 128 columns, {3889, 9721}) with the reference bench's seeds."""
+    say("case: basic round")
+    params = lbn.Params.basic()
+    pp, prover, w, a = _setup(lbn, orc, params)
+    rw = prover.upload(w)
+    commitment, opening = prover.commit_resident(rw)
+    raw, comp = orc.commit(params.primes(), params.len_ring(), params.columns(), a, w.elements)
+    assert np.array_equal(commitment.raw, raw)
+    assert np.array_equal(commitment.components, comp)
+    verifier = lbn.Verifier(pp)
+    t = lbn.Transcript(b"labinius/reference")
+    point = verifier.derive_evaluation_point(t, commitment)
+    row = prover.row_evaluate(rw, point)
+    assert np.array_equal(row.values, orc.row_evaluate(w.elements, point.p0))
+    claimed = prover.mle_evaluate(rw, point)
+    challenges = verifier.derive_folding_challenges(t, row)
+    folded = prover.fold(opening, challenges)
+    assert np.array_equal(folded.elements,
+                          orc.fold(3889, params.len_ring(), params.columns(), w.elements, challenges.challenges))
+    fc = verifier.fold_commitment(commitment, challenges)
+    fr = verifier.fold_row_evaluation(row, challenges)
+    verifier.verify_evaluation(point, claimed, row)
+    verifier.verify_opening(commitment, challenges, point, fc, folded, fr)
+
+
+def test_tampering_is_rejected(lbn, orc):
+    # tests/fold.rs:29-242: a corrupted v, u or claim rejects
+    say("case: tamper")
+    params = lbn.Params(10, 2, [lbn.Modulus.Q9721_FS_S])
+    pp, prover, w, a = _setup(lbn, orc, params)
+    verifier = lbn.Verifier(pp)
+    commitment, opening = prover.commit(w)
+    t = lbn.Transcript(b"labinius/test/tamper")
+    point = verifier.derive_evaluation_point(t, commitment)
+    row = prover.row_evaluate(w, point)
+    claimed = prover.mle_evaluate(w, point)
+    challenges = verifier.derive_folding_challenges(t, row)
+    folded = prover.fold(opening, challenges)
+    fc = verifier.fold_commitment(commitment, challenges)
+    fr = verifier.fold_row_evaluation(row, challenges)
+    verifier.verify_opening(commitment, challenges, point, fc, folded, fr)
+    bad = lbn.FoldedWitness(folded.elements.copy())
+    bad.elements[1, 7] += 2
+    with pytest.raises(lbn.VerificationError):
+        verifier.verify_opening(commitment, challenges, point, fc, bad, fr)
+    bad_claim = claimed.copy()
+    bad_claim[0] ^= np.uint64(1)
+    with pytest.raises(lbn.VerificationError):
+        verifier.verify_evaluation(point, bad_claim, row)
+    bad_row = lbn.RowEvaluation(row.values.copy())
+    bad_row.values[0, 1] ^= np.uint64(4)
+    with pytest.raises(lbn.VerificationError):
+        verifier.verify_evaluation(point, claimed, bad_row)
+
+
+def test_prover_runs_several_rounds_and_recycles(lbn, orc):
+    # tests/api.rs: one prover, several rounds; recycle hands the buffer back
+    say("case: rounds")
+    params = lbn.Params(10, 2, [lbn.Modulus.Q2917_Q_S])
+    pp, prover, w, a = _setup(lbn, orc, params)
+    c1, o1 = prover.commit(w)
+    prover.recycle(o1)
+    c2, o2 = prover.commit(w)
+    assert np.array_equal(c1.components, c2.components)
+    w2 = lbn.Witness.random(params, bytes([1]) * 32)
+    c3, o3 = prover.commit(w2)
+    raw, comp = orc.commit(params.primes(), params.len_ring(), params.columns(), a, w2.elements)
+    assert np.array_equal(c3.raw, raw)
+    prover.recycle(o2)
+    prover.recycle(o3)
+
+
+def test_zero_witness_commits_to_zero(lbn, orc):
+    # tests/commit.rs:344-355
+    say("case: zero")
+    params = lbn.Params(10, 2, [lbn.Modulus.Q9721_FS_S])
+    pp, prover, _, _ = _setup(lbn, orc, params)
+    w = lbn.Witness.from_elements(params, np.zeros((params.witness_len(), 3), np.uint64))
+    c, o = prover.commit(w)
+    assert not c.raw.any() and not c.components.any()
+    prover.recycle(o)

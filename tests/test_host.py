@@ -1,0 +1,101 @@
+"""The product's host side (C++ in liblabinius.so, no device needed) against the oracle: BLAKE3,
+the seed expansions, the transcript and its challenge sampler, F162, parameter checks and the
+verifier's pieces. Everything here is bit-exact."""
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def test_library_exports_every_declared_symbol(lbn):
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    names = set()
+    for h in ("labinius.h", "labinius_debug.h"):
+        src = open(os.path.join(root, "include", h)).read()
+        names |= set(re.findall(r"\b(lbn_[a-z0-9_]+)\s*\(", src))
+    assert len(names) > 30
+    L = lbn.lib()
+    for name in sorted(names):
+        assert hasattr(L, name), name
+
+
+def test_blake3_matches_oracle(lbn, orc):
+    for n in list(range(0, 200, 7)) + [1023, 1024, 1025, 2047, 2048, 2049, 4096, 4097, 65536 + 3, 200000]:
+        data = bytes((i * 7 + 3) % 256 for i in range(n))
+        assert lbn.blake3(data, 77) == orc.blake3(data, 77), n
+        key = bytes(range(32))
+        assert lbn.blake3(data, 40, key=key) == orc.blake3(data, 40, key=key)
+        assert lbn.blake3(data, 40, context="ctx") == orc.blake3(data, 40, context="ctx")
+
+
+def test_params_errors(lbn):
+    # scheme/params.rs:179-195 and tests/api.rs:46-80
+    P, M = lbn.Params, lbn.Modulus
+    cases = [((5, 6, []), "ColumnsExceedWitness"), ((10, 0, []), "TooFewColumns"), ((10, 4, []), "ColumnTooShort"),
+             ((10, 2, [M.Q9721_FS_S, M.Q9721_FS_S]), "DuplicateModulus"), ((10, 2, [M.Q3889_FS_S]), "BaseIsAlsoExtra")]
+    for args, kind in cases:
+        with pytest.raises(lbn.ParamError) as e:
+            P(*args)
+        assert e.value.kind == kind
+    p = P.basic()
+    assert (p.witness_log_len, p.column_log_len, p.primes()) == (18, 7, [3889, 9721])
+    pl = P.sized(lbn.SUITES[2])
+    assert pl.primes() == [3889, 2917, 4861] and pl.column_log_len == 9
+    pxl = P.sized(lbn.SUITES[3])
+    assert pxl.primes() == [3889, 2917, 9721] and pxl.column_log_len == 10
+
+
+def test_witness_random_matches_oracle(lbn, orc):
+    p = lbn.Params(11, 3, [lbn.Modulus.Q9721_FS_S])
+    w = lbn.Witness.random(p, bytes([0xC7]) * 32)
+    assert np.array_equal(w.elements, orc.witness_random(bytes([0xC7]) * 32, p.witness_len()))
+    assert (w.elements[:, 2] >> np.uint64(34)).max() == 0
+    w2 = lbn.Witness.random(p, bytes([0xC7]) * 32)
+    assert np.array_equal(w.elements, w2.elements)
+    with pytest.raises(lbn.WitnessError):
+        lbn.Witness.from_elements(p, np.zeros((5, 3), np.uint64))
+
+
+def _debug(lbn, params, seed, which, dtype):
+    L = lbn.lib()
+    n = ctypes.c_size_t()
+    s = (ctypes.c_uint8 * 32).from_buffer_copy(bytes(seed))
+    assert L.lbn_debug_tables(ctypes.byref(params.c()), s, which, None, 0, ctypes.byref(n)) == 0
+    buf = np.zeros(n.value // np.dtype(dtype).itemsize, dtype)
+    assert L.lbn_debug_tables(ctypes.byref(params.c()), s, which, buf.ctypes.data_as(ctypes.c_void_p),
+                              ctypes.c_size_t(n.value), None) == 0
+    return buf
+
+
+@pytest.mark.parametrize("extra", [[9721], [2917, 4861], [12637, 17497, 19441]])
+def test_key_expansion_matches_oracle(lbn, orc, extra):
+    p = lbn.Params(10, 2, [lbn.Modulus(q) for q in extra])
+    seed = bytes([0x5A]) * 32
+    a = _debug(lbn, p, seed, 14, np.int16).reshape(len(p.primes()), p.len_ring(), 648)
+    want = orc.key_expand(p.primes(), p.len_ring(), orc.matrix_seed(seed))
+    assert np.array_equal(a, want)
+
+
+def test_transcript_and_challenges_match_oracle(lbn, orc):
+    a, b = lbn.Transcript(b"labinius/test"), orc.Transcript(b"labinius/test")
+    for t in (a, b):
+        t.absorb_bytes(b"hello")
+        t.absorb_u64(12345)
+    assert a.fill(b"label", 100) == b.fill(b"label", 100)
+    ac, bc = a.clone(), b.clone()
+    for _ in range(40):
+        ca, na = a.sample_short_challenge()
+        cb, nb = b.sample_short_challenge()
+        assert bytes(ca.positions) == bytes(cb.positions) and ca.signs == cb.signs and na == nb
+    assert ac.fill(b"x", 33) == bc.fill(b"x", 33)
+
+
+def test_f162_matches_oracle(lbn, orc):
+    rng = np.random.default_rng(8)
+    for _ in range(50):
+        x, y = (rng.integers(0, 2**63, 3, dtype=np.uint64) for _ in range(2))
+        x[2] &= np.uint64((1 << 34) - 1)
+        y[2] &= np.uint64((1 << 34) - 1)
+        assert np.array_equal(lbn.f162_mul(x, y), orc.f162_mul(x, y))
