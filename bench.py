@@ -1,0 +1,220 @@
+"""Benchmark of the commit + evaluate + fold path at 2^24 F162 on one MI355X.
+
+One step = Prover::commit + Witness::row_evaluate (and the MLE claim) + Prover::fold of one
+Params::sized(SUITES[3], Clear) witness (2^24 uniform F162 in 1024 columns, moduli 3889 / 2917 /
+9721), the witness and A already resident in HBM. The evaluation point and the folding
+challenges are derived once up front by the host transcript (they are the verifier's steps;
+the reference bench times them on their own rows). Outputs stay in HBM inside the timed region;
+the copy-back of the commitment and the folded witness is timed separately (``d2h_ms``).
+
+Prints one JSON line. ``--gpus N`` with N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MATRIX_SEED = bytes([0x5A]) * 32
+WITNESS_SEED = bytes([0xC7]) * 32
+PEAK_INT8_TOPS = 5000.0   # MI355X dense int8 MFMA, TOP/s
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E, GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tile_ops(params):
+    """Algorithmic int8 ops of one commit GEMM launch: per ring element and limb, every 81 x 81
+    block (8 per splitting limb; 16 quarter-blocks of 81 x 81 rows for a quadratic one) times the
+    four digit products, 2 ops per multiply-add."""
+    per_ring = 0
+    for q in params.primes():
+        quad = (q - 1) % 972 == 0 and (q - 1) % 1944 != 0
+        per_ring += (2 * 4 * 162 * 81) if quad else (8 * 81 * 81)
+    return 2 * 4 * per_ring * (params.witness_len() // 4)
+
+
+def algorithmic_bytes(params):
+    """Algorithmic HBM bytes of one commit + fold + eval unit (the reference's kept transform
+    counted at 2 B per slot): W read by commit and eval, T written by commit and read by fold, A."""
+    n = params.witness_len()
+    w = 24 * n
+    t = 2 * 648 * (n // 4)
+    a = 2 * 648 * params.len_ring() * len(params.primes())
+    v = 2 * 648 * params.len_ring()
+    chat = 2 * 648 * params.columns()
+    return dict(commit=w + t + a, fold=t + chat + v, eval=w, total=(w + t + a) + (t + chat + v) + w)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.machine()
+
+
+def cpu_baseline(params):
+    """The oracle (a single-threaded plain-C restatement of the reference algorithm) on a sample of
+    the same workload, scaled to the whole step: commit and row evaluation of a few columns (each
+    column is independent), and the fold of a few output ring elements (each depends on every
+    column at its own index)."""
+    import numpy as np
+    import oracle as orc
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    cols, L = params.columns(), params.len_ring()
+    wdim = params.column_len()
+    ncols = 32
+    w = orc.witness_random(WITNESS_SEED, ncols * wdim)
+    a = orc.key_expand(params.primes(), L, orc.matrix_seed(MATRIX_SEED))
+    t0 = time.perf_counter()
+    orc.commit(params.primes(), L, ncols, a, w)
+    t_commit = (time.perf_counter() - t0) / ncols * cols
+    rng = np.random.default_rng(0)
+    p0 = rng.integers(0, 2**63, (params.row_log_len(), 3), dtype=np.uint64)
+    t0 = time.perf_counter()
+    orc.row_evaluate(w, p0)
+    t_eval = (time.perf_counter() - t0) / ncols * cols
+    # fold: output ring element i needs ring element i of every column; the sample's columns hold
+    # just the first nrows ring elements (same per-row work, smaller buffer)
+    nrows = 256
+    wf = orc.witness_random(bytes([0x3C]) * 32, cols * 4 * nrows)
+    tr = orc.Transcript(b"labinius/baseline")
+    ch = [tr.sample_short_challenge()[0] for _ in range(cols)]
+    t0 = time.perf_counter()
+    orc.fold_rows(params.primes()[0], nrows, cols, wf, ch, np.arange(nrows))
+    t_fold = (time.perf_counter() - t0) / nrows * L
+    total_ms = 1e3 * (t_commit + t_eval + t_fold)
+    return {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"oracle (plain C, one pinned thread) on {ncols} of {cols} columns for commit and row "
+                      f"evaluation and {nrows} of {L} output ring elements for the fold, scaled linearly; "
+                      f"commit {1e3 * t_commit:.0f} ms, eval {1e3 * t_eval:.0f} ms, fold {1e3 * t_fold:.0f} ms; "
+                      f"host {cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--suite", default="xl", choices=["s", "m", "l", "xl"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the prover runs on a single MI355X (columns would shard across "
+              "GPUs with one all-reduce in the fold; not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import labinius_amd as lbn
+
+    if args.suite == "s":
+        params = lbn.Params.basic()
+    else:
+        params = lbn.Params.sized(lbn.SUITES[{"m": 1, "l": 2, "xl": 3}[args.suite]])
+    log(f"params: 2^{params.witness_log_len} F162, {params.columns()} columns, moduli {params.primes()}")
+    t0 = time.time()
+    pp = lbn.PublicParameters.from_seed(params, MATRIX_SEED)
+    w = lbn.Witness.random(params, WITNESS_SEED)
+    prover = lbn.Prover(pp)
+    verifier = lbn.Verifier(pp)
+    rw = prover.upload(w)
+    log(f"setup {time.time() - t0:.1f} s")
+
+    # the point and the challenges, once (host transcript)
+    commitment, opening = prover.commit_resident(rw)
+    t = lbn.Transcript(b"labinius/reference")
+    point = verifier.derive_evaluation_point(t, commitment)
+    row = prover.row_evaluate(rw, point)
+    challenges = verifier.derive_folding_challenges(t, row)
+    prover.recycle(opening)
+    log("point and challenges derived")
+
+    L = lbn.lib()
+    p0 = np.ascontiguousarray(point.p0, np.uint64)
+    p1 = np.ascontiguousarray(point.p1, np.uint64)
+    u = np.zeros((params.columns(), 3), np.uint64)
+    tval = np.zeros(3, np.uint64)
+    arr = challenges.array()
+    hp, hw = ctypes.c_void_p(prover._h), ctypes.c_void_p(rw._h)
+    vp = ctypes.c_void_p
+
+    def step():
+        op = ctypes.c_void_p()
+        lbn._check(L.lbn_commit_resident(hp, hw, None, None, ctypes.byref(op)))
+        lbn._check(L.lbn_mle_evaluate(hp, hw, p0.ctypes.data_as(vp), ctypes.c_size_t(len(p0)),
+                                      p1.ctypes.data_as(vp), ctypes.c_size_t(len(p1)), tval.ctypes.data_as(vp)))
+        lbn._check(L.lbn_fold(hp, op, arr, ctypes.c_size_t(len(arr)), None))
+        return prover.timings()
+
+    for _ in range(args.warmup):
+        step()
+    import torch
+    torch.cuda.synchronize()
+    stage = {k: 0.0 for k in prover.timings()}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        tm = step()
+        for k in stage:
+            stage[k] += tm[k]
+        log(f"step {i + 1}/{args.steps}")
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    stage = {k: v / args.steps for k, v in stage.items()}
+
+    # the same step with the outputs copied back (PCIe included)
+    t0 = time.perf_counter()
+    c2, o2 = prover.commit_resident(rw)
+    prover.mle_evaluate(rw, point)
+    prover.fold(o2, challenges)
+    d2h_ms = 1e3 * (time.perf_counter() - t0)
+
+    gemm_ms = stage["commit_gemm"]
+    ops = tile_ops(params)
+    achieved = ops / (gemm_ms * 1e-3) / 1e12
+    ab = algorithmic_bytes(params)
+    line = {
+        "metric": "commit+fold+eval ms @2^24 bit-exact",
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8-mfma/u32 mod q; gf(2^162)",
+        "data": "synthetic: Witness::random([0xC7;32]), A from [0x5A;32]",
+        "config": {"workload": f"Params::sized(SUITES[{ {'s': 0, 'm': 1, 'l': 2, 'xl': 3}[args.suite] }], Clear)"
+                               if args.suite != "s" else "Params::basic()",
+                   "witness_log_len": params.witness_log_len, "columns": params.columns(),
+                   "moduli": params.primes()},
+        "stages_ms": {k: round(v, 4) for k, v in stage.items()},
+        "d2h_inclusive_ms": round(d2h_ms, 3),
+        "hbm_floor_ms": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3, 4),
+        "hbm_roofline_frac": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3 / ms, 4),
+        "roofline": {"kernel": "tile_gemm_kernel (commit)", "bound": "mfma", "achieved": round(achieved, 1),
+                     "peak": PEAK_INT8_TOPS, "unit": "TOP/s int8", "frac": round(achieved / PEAK_INT8_TOPS, 4),
+                     "traffic": None, "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4)},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline...")
+        line["cpu_baseline"] = cpu_baseline(params)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,6 +105,13 @@ def lib():
     """The HIP library; raises if it has not been built (there is no fallback)."""
     global _lib
     if _lib is None:
+        # When torch (ROCm build) is present, its HIP runtime must be the one in the process before
+        # the library resolves libamdhip64.so.7: loading the system runtime first and torch's
+        # second leaves the library without a device.
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         if not os.path.exists(LIB_PATH):
             raise LabiniusError(-1, f"{LIB_PATH} is not built (run __graft_entry__.build())")
         L = ctypes.CDLL(LIB_PATH)

@@ -394,8 +394,8 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     HIP_TRY(hipEventRecord(pr->ev[0], st));
     HIP_TRY(launch_prepass(st, d_w, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r, pr->d_bytes_c,
                            pp->lpad, bytes_f, pp->rpad));
-    HIP_TRY(hipEventRecord(pr->ev[1], st));
     HIP_TRY(launch_fill_u32(st, pr->d_yacc, s.r * limbs * N, 0));
+    HIP_TRY(hipEventRecord(pr->ev[1], st));
     GemmArgs g{};
     g.tiles = pp->d_tiles;
     g.afrag = pp->d_afrag;

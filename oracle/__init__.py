@@ -176,9 +176,13 @@ def commit(primes, len_ring, cols, a, w):
 
 
 def challenges_array(chs):
+    """Challenges as the oracle's struct (accepts any object with the same three fields)."""
     arr = (Challenge * len(chs))()
     for i, c in enumerate(chs):
-        arr[i] = c
+        if isinstance(c, Challenge):
+            arr[i] = c
+        else:
+            ctypes.memmove(ctypes.byref(arr[i]), ctypes.byref(c), ctypes.sizeof(Challenge))
     return arr
 
 
@@ -192,6 +196,15 @@ def fold(q, len_ring, cols, w, chs):
     w = np.ascontiguousarray(w, np.uint64)
     v = np.zeros((len_ring, N), np.int16)
     lib().orc_fold(ctypes.c_uint32(q), ctypes.c_int(len_ring), ctypes.c_int(cols), _p(w), challenges_array(chs), _p(v))
+    return v
+
+
+def fold_rows(q, len_ring, cols, w, chs, rows):
+    w = np.ascontiguousarray(w, np.uint64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    v = np.zeros((len(rows), N), np.int16)
+    lib().orc_fold_rows(ctypes.c_uint32(q), ctypes.c_int(len_ring), ctypes.c_int(cols), _p(w), challenges_array(chs),
+                        _p(rows), ctypes.c_int(len(rows)), _p(v))
     return v
 
 

@@ -837,6 +837,46 @@ void orc_fold(uint32_t q, int len_ring, int cols, const uint64_t *w, const orc_c
     free(chat);
 }
 
+/* orc_fold restricted to the output ring elements `rows` (each output element depends only on
+ * the ring elements with the same index in every column): the CPU baseline times a sample. */
+void orc_fold_rows(uint32_t q, int len_ring, int cols, const uint64_t *w, const orc_challenge *ch,
+                   const int32_t *rows, int n_rows, int16_t *v) {
+    init_tables();
+    const prime_tab *t = prime(q);
+    uint64_t qq = q;
+    uint64_t *chat = (uint64_t *)malloc(sizeof(uint64_t) * N * cols);
+    for (int j = 0; j < cols; j++) {
+        int32_t e[N];
+        orc_embed(&ch[j], e);
+        uint64_t *cj = chat + (size_t)j * N;
+        for (int i = 0; i < N; i++) cj[i] = (uint64_t)((e[i] % (int64_t)qq + (int64_t)qq) % (int64_t)qq);
+        if (t->quad) {
+            ntt_quad(t, cj);
+            for (int l = 0; l < 324; l++) cj[2 * l + 1] = cj[2 * l];
+        } else {
+            ntt_split(t, cj);
+        }
+    }
+    for (int r = 0; r < n_rows; r++) {
+        int i = rows[r];
+        uint64_t acc[N];
+        memset(acc, 0, sizeof acc);
+        for (int j = 0; j < cols; j++) {
+            uint32_t lift[N];
+            orc_lift(w + 3 * ((size_t)j * len_ring * 4 + 4 * (size_t)i), lift);
+            uint64_t x[N];
+            for (int u = 0; u < N; u++) x[u] = lift[u];
+            if (t->quad) ntt_quad(t, x);
+            else ntt_split(t, x);
+            const uint64_t *cj = chat + (size_t)j * N;
+            for (int u = 0; u < N; u++) acc[u] = md(t, acc[u] + cj[u] * x[u]);
+        }
+        intt_any(t, acc);
+        for (int u = 0; u < N; u++) v[(size_t)r * N + u] = center(acc[u], qq);
+    }
+    free(chat);
+}
+
 /* The same v straight from the definition: the ring products c_j(-X^4) w_{j,i} in
  * Z[X]/(X^648 - X^324 + 1) by sparse convolution, summed over Z, then centered mod q. Used to
  * check sampled output elements at sizes where orc_fold is too slow. */
