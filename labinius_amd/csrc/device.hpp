@@ -23,16 +23,19 @@ namespace lbn {
 
 constexpr int TILE = 96;         // padded M and N of a tile (81 rows / inputs used)
 constexpr int TILE_USED = 81;
-constexpr int KSTEP = 64;        // K per MFMA step (v_mfma_i32_16x16x64_i8)
-constexpr int BLK = 16;          // MFMA block
-constexpr int NBLK = TILE / BLK; // 6 blocks per tile side
-constexpr int FRAG = 1024;       // bytes of one 16 x 64 int8 operand fragment (64 lanes x 16 B)
+constexpr int KSTEP = 64;        // K per step of the GEMM loop: two v_mfma_i32_32x32x32_i8
+constexpr int BLK = 32;          // MFMA block (32 x 32)
+constexpr int NBLK = TILE / BLK; // 3 blocks per tile side
+constexpr int FRAG = 1024;       // bytes of one 32 x 32 int8 operand fragment (64 lanes x 16 B)
+constexpr int A_STEP = 2 * 2 * NBLK * FRAG;  // A fragments per K step: [kh][digit][mb]
+constexpr int B_TILE = 2 * 2 * NBLK * FRAG;  // x digits per K step: [kh][digit][nb]
+constexpr int EPI_TILE = NBLK * NBLK * 64 * 16 * 3;  // epilogue weights per tile
 
 // One tile of a GEMM launch.
 struct TileDesc {
-    uint64_t a_off;     // byte offset of the tile's A fragments: [kb][mb 6][digit 2][lane 64][16]
+    uint64_t a_off;     // byte offset of the tile's A fragments: [kb][kh 2][digit 2][mb 3][lane 64][16]
     uint32_t lut_off;   // element offset (u16) of the tile's two 256-entry LUTs
-    uint32_t epi_off;   // element offset (u16) of the tile's epilogue weights [mb][nb][lane][4][3]
+    uint32_t epi_off;   // element offset (u16) of the tile's epilogue weights [mb][nb][lane][16][3]
     uint32_t out_off;   // element offset (i16) of the tile's 96 output slots (-1 = padding row)
     uint32_t q;
     uint32_t par;       // input column n reads LUT (par + n) & 1
@@ -50,7 +53,7 @@ struct GemmArgs {
     uint32_t kpad;            // K padded to a multiple of KSTEP (the n stride of bytes)
     uint32_t kb_per_split;    // K steps per split (blockIdx.z)
     uint32_t kb_total;
-    uint32_t a_kb_stride;     // bytes of A fragments per K step of a tile = 12 * FRAG
+    uint32_t a_kb_stride;     // bytes of A fragments per K step of a tile (A_STEP)
     uint32_t *y;              // output rows
     uint64_t y_batch;         // words per batch in y
     int accumulate;           // 1: atomicAdd (commit, several contributions), 0: store (fold)

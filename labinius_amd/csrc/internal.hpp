@@ -123,7 +123,7 @@ struct lbn_prover {
     int8_t *d_afrag_f = nullptr;
     uint32_t *d_vhat = nullptr;
     int16_t *d_v = nullptr;
-    uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_u = nullptr;
+    uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_eqpart = nullptr, *d_u = nullptr;
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
     hipEvent_t ev[12] = {};
     float timings[11] = {};

@@ -15,6 +15,7 @@
 #include "launch.hpp"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 namespace lbn {
 
@@ -105,13 +106,16 @@ __device__ __forceinline__ long long shfl_xor64(long long v, int m) {
 // a ring element for the fold) over the K steps of blockIdx.z, followed by the epilogue
 // y[out[m]] += sum_n E[m][n] Z[m][n] mod q.
 //
-// 256 threads = 4 waves in a 2 x 2 grid; wave (wm, wn) owns M blocks 3 wm .. 3 wm + 2 and N
-// blocks 3 wn .. 3 wn + 2 of 16. Operands are two signed 7-bit digits each, so every 16 x 16
-// block takes four v_mfma_i32_16x16x64_i8 per K step into three accumulators (low x low,
-// the two cross terms, high x high); the epilogue weights carry the 1, 2^7, 2^14 scale.
-// The B operand (x digits) is built once per K step into LDS by table lookup on the byte
-// layout; the A operand fragments are read straight from global memory in MFMA lane order.
-__global__ __launch_bounds__(256) void tile_gemm_kernel(GemmArgs g) {
+// Operands are two signed 7-bit digits each (A = A_l + 2^7 A_h, x = x_l + 2^7 x_h), so Z is the
+// sum of four digit products. The 256 threads are four waves, one per digit product: wave w
+// multiplies A digit DA(w) by x digit DX(w) over all nine 32 x 32 blocks of the tile with
+// v_mfma_i32_32x32x32_i8 (waves 2 and 3 are the two cross terms, which the epilogue weights
+// by 2^7; wave 1's by 2^14). Every wave also takes a third of the work of turning the next K
+// step's bytes into x digits by table lookup, into the other half of a double-buffered LDS
+// tile; its A fragments come straight from global memory in MFMA lane order, one K step ahead.
+constexpr int GEMM_THREADS = 256;
+
+__global__ __launch_bounds__(GEMM_THREADS, 2) void tile_gemm_kernel(GemmArgs g) {
     // tiles vary fastest, so the blocks of one batch element run together and share its bytes
     const TileDesc td = g.tiles[blockIdx.x % g.ntiles];
     const uint32_t batch = blockIdx.x / g.ntiles;
@@ -119,102 +123,129 @@ __global__ __launch_bounds__(256) void tile_gemm_kernel(GemmArgs g) {
     const uint32_t kb1 = min(kb0 + g.kb_per_split, g.kb_total);
 
     __shared__ uint16_t lut[512];
-    __shared__ __attribute__((aligned(16))) uint8_t bt[NBLK * 2 * 64 * 16];
+    __shared__ __attribute__((aligned(16))) uint8_t bt[2][B_TILE];
     __shared__ long long rowsum[TILE];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    for (int x = tid; x < 512; x += 256) lut[x] = g.lut[td.lut_off + x];
+    const int da = wave & 1, dx = (wave == 1 || wave == 2) ? 1 : 0;
+    lut[tid] = g.lut[td.lut_off + tid];
+    lut[tid + 256] = g.lut[td.lut_off + tid + 256];
     if (tid < TILE) rowsum[tid] = 0;
-
-    v4i acc[3][3][3];
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) acc[a][b][d] = (v4i){0, 0, 0, 0};
 
     const uint8_t *bbase = g.bytes + (size_t)batch * g.bytes_batch;
     const int8_t *abase = g.afrag + td.a_off;
-    __syncthreads();
 
-    for (uint32_t kb = kb0; kb < kb1; kb++) {
-        // B tile: unit u = (nb, lane, half) -> 8 lookups
+    // lookup unit u = (nb, kh, lane, half): 8 bytes of input column n = 32 nb + (lane & 31) at
+    // K = 64 kb + 32 kh + 16 (lane >> 5) + 8 half
+    uint2 raw[3];
+    auto load_bytes = [&](uint32_t kb) {
 #pragma unroll
         for (int it = 0; it < 3; it++) {
             const int u = tid + 256 * it;
-            const int nb = u >> 7, rem = u & 127, ln = rem >> 1, h = rem & 1;
-            const int n = nb * 16 + (ln & 15);
-            uint32_t xl0 = 0, xl1 = 0, xh0 = 0, xh1 = 0;
-            if (n < TILE_USED) {
-                const uint32_t k0 = kb * KSTEP + 16 * (ln >> 4) + 8 * h;
-                const uint2 raw = *(const uint2 *)(bbase + (size_t)n * g.kpad + k0);
-                const uint16_t *L = lut + (((td.par + n) & 1) << 8);
-                uint32_t v0 = L[raw.x & 255], v1 = L[(raw.x >> 8) & 255];
-                uint32_t v2 = L[(raw.x >> 16) & 255], v3 = L[raw.x >> 24];
-                uint32_t v4 = L[raw.y & 255], v5 = L[(raw.y >> 8) & 255];
-                uint32_t v6 = L[(raw.y >> 16) & 255], v7 = L[raw.y >> 24];
-                // p01 = [v0.b0, v1.b0, v0.b1, v1.b1], then split the low and high digits
-                uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
-                uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
-                uint32_t p45 = __builtin_amdgcn_perm(v5, v4, 0x05010400u);
-                uint32_t p67 = __builtin_amdgcn_perm(v7, v6, 0x05010400u);
-                xl0 = __builtin_amdgcn_perm(p23, p01, 0x05040100u);
-                xh0 = __builtin_amdgcn_perm(p23, p01, 0x07060302u);
-                xl1 = __builtin_amdgcn_perm(p67, p45, 0x05040100u);
-                xh1 = __builtin_amdgcn_perm(p67, p45, 0x07060302u);
-            }
-            *(uint2 *)(bt + ((nb * 2 + 0) * 64 + ln) * 16 + 8 * h) = make_uint2(xl0, xl1);
-            *(uint2 *)(bt + ((nb * 2 + 1) * 64 + ln) * 16 + 8 * h) = make_uint2(xh0, xh1);
+            const int nb = u >> 8, rem = u & 255, kh = rem >> 7, ln = (rem >> 1) & 63, h = rem & 1;
+            const int n = nb * 32 + (ln & 31);
+            raw[it] = make_uint2(0, 0);
+            if (n < TILE_USED)
+                raw[it] = *(const uint2 *)(bbase + (size_t)n * g.kpad + kb * KSTEP + 32 * kh + 16 * (ln >> 5) + 8 * h);
         }
-        v4i a[3][2];
+    };
+    auto build = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 3; it++) {
+            const int u = tid + 256 * it;
+            const int nb = u >> 8, rem = u & 255, kh = rem >> 7, ln = (rem >> 1) & 63, h = rem & 1;
+            const int n = nb * 32 + (ln & 31);
+            const uint16_t *L = lut + (((td.par + n) & 1) << 8);
+            const uint2 r = raw[it];
+            const uint32_t v0 = L[r.x & 255], v1 = L[(r.x >> 8) & 255];
+            const uint32_t v2 = L[(r.x >> 16) & 255], v3 = L[r.x >> 24];
+            const uint32_t v4 = L[r.y & 255], v5 = L[(r.y >> 8) & 255];
+            const uint32_t v6 = L[(r.y >> 16) & 255], v7 = L[r.y >> 24];
+            // p01 = [v0.b0, v1.b0, v0.b1, v1.b1], then split the low and high digits; a padding
+            // column (n >= 81) reads byte 0, whose table entry is 0
+            const uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
+            const uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
+            const uint32_t p45 = __builtin_amdgcn_perm(v5, v4, 0x05010400u);
+            const uint32_t p67 = __builtin_amdgcn_perm(v7, v6, 0x05010400u);
+            uint8_t *o = &bt[buf][((kh * 2) * 3 + nb) * FRAG + ln * 16 + 8 * h];
+            *(uint2 *)o = make_uint2(__builtin_amdgcn_perm(p23, p01, 0x05040100u),
+                                     __builtin_amdgcn_perm(p67, p45, 0x05040100u));
+            *(uint2 *)(o + 3 * FRAG) = make_uint2(__builtin_amdgcn_perm(p23, p01, 0x07060302u),
+                                                  __builtin_amdgcn_perm(p67, p45, 0x07060302u));
+        }
+    };
+
+    v16i acc[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) acc[a][b] = (v16i){};
+    // A fragments [kh][digit][mb]: lane holds row 32 mb + (lane & 31), K 32 kh + 16 (lane >> 5) + byte
+    v4i af[2][3];
+    auto load_a = [&](uint32_t kb) {
         const int8_t *ak = abase + (size_t)kb * g.a_kb_stride;
 #pragma unroll
-        for (int mbi = 0; mbi < 3; mbi++)
+        for (int kh = 0; kh < 2; kh++)
 #pragma unroll
-            for (int d = 0; d < 2; d++)
-                a[mbi][d] = *(const v4i *)(ak + (((3 * wm + mbi) * 2 + d) * 64 + lane) * 16);
-        __syncthreads();
+            for (int mb = 0; mb < 3; mb++)
+                af[kh][mb] = *(const v4i *)(ak + (((kh * 2 + da) * 3 + mb) * 64 + lane) * 16);
+    };
+
+    __syncthreads();  // the LUT
+    if (kb0 < kb1) {
+        load_bytes(kb0);
+        load_a(kb0);
+        build(0);
+        if (kb0 + 1 < kb1) load_bytes(kb0 + 1);
+    }
+    __syncthreads();
+
+    for (uint32_t kb = kb0; kb < kb1; kb++) {
+        const int buf = (kb - kb0) & 1;
+        v4i a[2][3];
 #pragma unroll
-        for (int nbi = 0; nbi < 3; nbi++) {
-            const int nb = 3 * wn + nbi;
-            const v4i xl = *(const v4i *)(bt + ((nb * 2 + 0) * 64 + lane) * 16);
-            const v4i xh = *(const v4i *)(bt + ((nb * 2 + 1) * 64 + lane) * 16);
+        for (int kh = 0; kh < 2; kh++)
 #pragma unroll
-            for (int mbi = 0; mbi < 3; mbi++) {
-                acc[mbi][nbi][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][0], xl, acc[mbi][nbi][0], 0, 0, 0);
-                acc[mbi][nbi][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][0], xh, acc[mbi][nbi][1], 0, 0, 0);
-                acc[mbi][nbi][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][1], xl, acc[mbi][nbi][1], 0, 0, 0);
-                acc[mbi][nbi][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[mbi][1], xh, acc[mbi][nbi][2], 0, 0, 0);
+            for (int mb = 0; mb < 3; mb++) a[kh][mb] = af[kh][mb];
+        if (kb + 1 < kb1) load_a(kb + 1);
+#pragma unroll
+        for (int kh = 0; kh < 2; kh++)
+#pragma unroll
+            for (int nb = 0; nb < 3; nb++) {
+                const v4i x = *(const v4i *)(&bt[buf][((kh * 2 + dx) * 3 + nb) * FRAG + lane * 16]);
+#pragma unroll
+                for (int mb = 0; mb < 3; mb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kh][mb], x, acc[mb][nb], 0, 0, 0);
             }
+        if (kb + 1 < kb1) {
+            build(buf ^ 1);
+            if (kb + 2 < kb1) load_bytes(kb + 2);
         }
         __syncthreads();
     }
 
-    // epilogue: the D fragment of a 16 x 16 block holds row 4 (lane >> 4) + r, column lane & 15
-    const uint16_t *ep = g.epi + td.epi_off;
+    // epilogue: the D fragment of a 32 x 32 block holds, in register r, row
+    // 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
+    {
+        const int cls = wave == 0 ? 0 : (wave == 1 ? 2 : 1);
+        const uint16_t *ep = g.epi + td.epi_off;
 #pragma unroll
-    for (int mbi = 0; mbi < 3; mbi++) {
-        const int mb = 3 * wm + mbi;
+        for (int mb = 0; mb < 3; mb++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            long long s = 0;
+            for (int r = 0; r < 16; r++) {
+                long long s = 0;
 #pragma unroll
-            for (int nbi = 0; nbi < 3; nbi++) {
-                const int nb = 3 * wn + nbi;
-                const uint16_t *e = ep + (((mb * NBLK + nb) * 64 + lane) * 4 + r) * 3;
-                s += (long long)e[0] * acc[mbi][nbi][0][r] + (long long)e[1] * acc[mbi][nbi][1][r] +
-                     (long long)e[2] * acc[mbi][nbi][2][r];
+                for (int nb = 0; nb < 3; nb++)
+                    s += (long long)ep[(((mb * 3 + nb) * 64 + lane) * 16 + r) * 3 + cls] * acc[mb][nb][r];
+                s += shfl_xor64(s, 16);
+                s += shfl_xor64(s, 8);
+                s += shfl_xor64(s, 4);
+                s += shfl_xor64(s, 2);
+                s += shfl_xor64(s, 1);
+                if ((lane & 31) == 0)
+                    atomicAdd((unsigned long long *)&rowsum[32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)],
+                              (unsigned long long)s);
             }
-            s += shfl_xor64(s, 8);
-            s += shfl_xor64(s, 4);
-            s += shfl_xor64(s, 2);
-            s += shfl_xor64(s, 1);
-            if ((lane & 15) == 0)
-                atomicAdd((unsigned long long *)&rowsum[mb * 16 + 4 * (lane >> 4) + r],
-                          (unsigned long long)s);
-        }
     }
     __syncthreads();
     if (tid < TILE) {
@@ -316,11 +347,11 @@ __global__ void chat_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r
     const int xl = ((x + 64) & 127) - 64;
     const int xh = (x - xl) / 128;
     const uint32_t kbs = rpad / KSTEP;
-    const uint32_t kb = c / KSTEP, kk = c % KSTEP, g4 = kk / 16, b = kk % 16;
-    const uint32_t mb = m / 16, lanev = 16 * g4 + (m % 16);
-    const size_t base = ((size_t)tau * kbs + kb) * (size_t)(NBLK * 2 * FRAG);
-    afrag[base + ((mb * 2 + 0) * 64 + lanev) * 16 + b] = (int8_t)xl;
-    afrag[base + ((mb * 2 + 1) * 64 + lanev) * 16 + b] = (int8_t)xh;
+    const uint32_t kb = c / KSTEP, kk = c % KSTEP, kh = kk / 32, g2 = (kk % 32) / 16, b = kk % 16;
+    const uint32_t mb = m / 32, lanev = 32 * g2 + (m % 32);
+    const size_t base = ((size_t)tau * kbs + kb) * (size_t)A_STEP;
+    afrag[base + (((kh * 2 + 0) * 3 + mb) * 64 + lanev) * 16 + b] = (int8_t)xl;
+    afrag[base + (((kh * 2 + 1) * 3 + mb) * 64 + lanev) * 16 + b] = (int8_t)xh;
 }
 
 // =============================================================================================
@@ -383,44 +414,64 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
 // =============================================================================================
 
 __device__ void f162_mul_dev(const uint64_t *a, const uint64_t *b, uint64_t *out) {
-    uint64_t p[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int wi = 0; wi < 3; wi++) {
-        const uint64_t aw = a[wi];
-#pragma unroll
-        for (int bs = 0; bs < 64; bs++) {
-            if (wi == 2 && bs >= 34) break;
-            const uint64_t mask = 0ull - ((aw >> bs) & 1ull);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                p[wi + k] ^= (b[k] << bs) & mask;
-                if (bs) p[wi + k + 1] ^= (b[k] >> (64 - bs)) & mask;
-            }
-        }
+    uint64_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
+    uint64_t s0 = b[0], s1 = b[1], s2 = b[2], s3 = 0, s4 = 0, s5 = 0;  // b << i
+    for (int i = 0; i < 162; i++) {
+        const uint64_t aw = i < 64 ? a[0] : (i < 128 ? a[1] : a[2]);
+        const uint64_t m = 0ull - ((aw >> (i & 63)) & 1ull);
+        p0 ^= s0 & m;
+        p1 ^= s1 & m;
+        p2 ^= s2 & m;
+        p3 ^= s3 & m;
+        p4 ^= s4 & m;
+        p5 ^= s5 & m;
+        s5 = (s5 << 1) | (s4 >> 63);
+        s4 = (s4 << 1) | (s3 >> 63);
+        s3 = (s3 << 1) | (s2 >> 63);
+        s2 = (s2 << 1) | (s1 >> 63);
+        s1 = (s1 << 1) | (s0 >> 63);
+        s0 <<= 1;
     }
-    uint64_t H0 = (p[2] >> 34) | (p[3] << 30), H1 = (p[3] >> 34) | (p[4] << 30);
-    uint64_t H2w = (p[4] >> 34) | (p[5] << 30);
-    uint64_t Hl0 = H0, Hl1 = H1 & ((1ull << 17) - 1);
-    uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
-    uint64_t S0 = Hl0 ^ G0, S1 = Hl1 ^ G1;
-    out[0] = p[0] ^ H0 ^ G0;
-    out[1] = p[1] ^ H1 ^ G1 ^ (S0 << 17);
-    out[2] = (p[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
+    const uint64_t H0 = (p2 >> 34) | (p3 << 30), H1 = (p3 >> 34) | (p4 << 30);
+    const uint64_t H2w = (p4 >> 34) | (p5 << 30);
+    const uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+    const uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
+    out[0] = p0 ^ H0 ^ G0;
+    out[1] = p1 ^ H1 ^ G1 ^ (S0 << 17);
+    out[2] = (p2 & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
 }
 
-// eq(p0)[b] = prod_k (bit k of b ? p_k : 1 + p_k) (eval.rs:51-63), one thread per entry
-__global__ void eq_kernel(const uint64_t *__restrict__ p0, uint32_t rows, uint64_t *__restrict__ eq) {
-    __shared__ uint64_t ps[32 * 3];
-    for (uint32_t x = threadIdx.x; x < rows * 3; x += blockDim.x) ps[x] = p0[x];
-    __syncthreads();
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= (1u << rows)) return;
+// eq(p0)[b] = prod_k (bit k of b ? p_k : 1 + p_k) (eval.rs:51-63), as a product of sub-tables over
+// groups of 8 coordinates: part[c][x] for the coordinates 8c .. 8c + 7 (x < 256)
+__global__ void eq_part_kernel(const uint64_t *__restrict__ p0, uint32_t rows, uint64_t *__restrict__ part) {
+    const uint32_t c = blockIdx.x, x = threadIdx.x;
+    const uint32_t k0 = 8 * c, nk = min(8u, rows - k0);
+    if (x >= (1u << nk)) return;
     uint64_t acc[3] = {1, 0, 0};
-    for (uint32_t k = 0; k < rows; k++) {
-        uint64_t f[3] = {ps[3 * k], ps[3 * k + 1], ps[3 * k + 2]};
-        if (!((b >> k) & 1)) f[0] ^= 1;
+    for (uint32_t k = 0; k < nk; k++) {
+        uint64_t f[3] = {p0[3 * (k0 + k)], p0[3 * (k0 + k) + 1], p0[3 * (k0 + k) + 2]};
+        if (!((x >> k) & 1)) f[0] ^= 1;
         uint64_t t[3];
         f162_mul_dev(acc, f, t);
+        acc[0] = t[0];
+        acc[1] = t[1];
+        acc[2] = t[2];
+    }
+    uint64_t *o = part + 3 * ((size_t)c * 256 + x);
+    o[0] = acc[0];
+    o[1] = acc[1];
+    o[2] = acc[2];
+}
+
+__global__ void eq_kernel(const uint64_t *__restrict__ part, uint32_t rows, uint64_t *__restrict__ eq) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= (1u << rows)) return;
+    const uint64_t *f = part + 3 * (b & 255);
+    uint64_t acc[3] = {f[0], f[1], f[2]};
+    for (uint32_t c = 1; 8 * c < rows; c++) {
+        const uint64_t *g2 = part + 3 * ((size_t)c * 256 + ((b >> (8 * c)) & 255));
+        uint64_t t[3];
+        f162_mul_dev(acc, g2, t);
         acc[0] = t[0];
         acc[1] = t[1];
         acc[2] = t[2];
@@ -543,7 +594,7 @@ hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t ntiles, u
                             uint32_t splits) {
     GemmArgs a = g;
     a.ntiles = ntiles;
-    LBN_LAUNCH(tile_gemm_kernel, dim3(ntiles * batches, 1, splits), dim3(256), 0, s, a);
+    LBN_LAUNCH(tile_gemm_kernel, dim3(ntiles * batches, 1, splits), dim3(GEMM_THREADS), 0, s, a);
     return hipSuccess;
 }
 
@@ -570,9 +621,10 @@ hipError_t launch_intt(hipStream_t s, const uint32_t *vhat, uint32_t L, const In
     return hipSuccess;
 }
 
-hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *eq) {
+hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *part, uint64_t *eq) {
     const uint32_t n = 1u << rows;
-    LBN_LAUNCH(eq_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p0, rows, eq);
+    LBN_LAUNCH(eq_part_kernel, dim3((rows + 7) / 8), dim3(256), 0, s, p0, rows, part);
+    LBN_LAUNCH(eq_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, rows, eq);
     return hipSuccess;
 }
 

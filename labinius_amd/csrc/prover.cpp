@@ -127,38 +127,39 @@ static HostTile quad_fold_tile(const Prime &P, int k, int b) {
 }
 
 // E, 2^7 E and 2^14 E in the D-fragment order of the MFMA: [mb][nb][lane][r][3], row
-// 16 mb + 4 (lane >> 4) + r, column 16 nb + (lane & 15).
+// 32 mb + 8 (r >> 2) + 4 (lane >> 5) + (r & 3), column 32 nb + (lane & 31).
 static void pack_epi(const HostTile &h, uint32_t q, uint16_t *dst) {
     for (int mb = 0; mb < NBLK; mb++)
         for (int nb = 0; nb < NBLK; nb++)
             for (int lane = 0; lane < 64; lane++)
-                for (int r = 0; r < 4; r++) {
-                    const int m = 16 * mb + 4 * (lane >> 4) + r, n = 16 * nb + (lane & 15);
+                for (int r = 0; r < 16; r++) {
+                    const int m = 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), n = 32 * nb + (lane & 31);
                     const uint64_t e = h.E[m * TILE + n];
-                    uint16_t *o = dst + (((mb * NBLK + nb) * 64 + lane) * 4 + r) * 3;
+                    uint16_t *o = dst + (((mb * NBLK + nb) * 64 + lane) * 16 + r) * 3;
                     o[0] = (uint16_t)e;
                     o[1] = (uint16_t)(e * 128 % q);
                     o[2] = (uint16_t)(e * 16384 % q);
                 }
 }
 
-// The A operand of a commit tile in MFMA lane order: [kb][mb][digit][lane][16 bytes], lane
-// holding row 16 mb + (lane & 15) and K = 64 kb + 16 (lane >> 4) + byte.
+// The A operand of a commit tile in MFMA lane order: [kb][kh][digit][mb][lane][16 bytes], lane
+// holding row 32 mb + (lane & 31) and K = 64 kb + 32 kh + 16 (lane >> 5) + byte.
 static void pack_key(const HostTile &h, const int16_t *a, size_t len_ring, uint32_t kbs, int8_t *dst) {
     for (uint32_t kb = 0; kb < kbs; kb++)
-        for (int mb = 0; mb < NBLK; mb++)
-            for (int lane = 0; lane < 64; lane++) {
-                const int m = 16 * mb + (lane & 15), rho = h.rows[m];
-                for (int byte = 0; byte < 16; byte++) {
-                    const size_t i = (size_t)KSTEP * kb + 16 * (lane >> 4) + byte;
-                    int64_t x = 0;
-                    if (rho >= 0 && i < len_ring) x = a[i * N + rho];
-                    const uint16_t d = digits(x);
-                    int8_t *o = dst + (((size_t)kb * NBLK + mb) * 2) * FRAG + lane * 16 + byte;
-                    o[0] = (int8_t)(uint8_t)(d & 255);
-                    o[FRAG] = (int8_t)(uint8_t)(d >> 8);
+        for (int kh = 0; kh < 2; kh++)
+            for (int mb = 0; mb < NBLK; mb++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int m = 32 * mb + (lane & 31), rho = h.rows[m];
+                    for (int byte = 0; byte < 16; byte++) {
+                        const size_t i = (size_t)KSTEP * kb + 32 * kh + 16 * (lane >> 5) + byte;
+                        int64_t x = 0;
+                        if (rho >= 0 && i < len_ring) x = a[i * N + rho];
+                        const uint16_t d = digits(x);
+                        int8_t *o = dst + (size_t)kb * A_STEP + (size_t)((kh * 2) * NBLK + mb) * FRAG + lane * 16 + byte;
+                        o[0] = (int8_t)(uint8_t)(d & 255);
+                        o[NBLK * FRAG] = (int8_t)(uint8_t)(d >> 8);
+                    }
                 }
-            }
 }
 
 template <typename T>
@@ -217,8 +218,8 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
             }
         }
     }
-    const size_t a_tile = (size_t)H.kb_c * NBLK * 2 * FRAG;
-    const size_t epi_tile = (size_t)NBLK * NBLK * 64 * 4 * 3;
+    const size_t a_tile = (size_t)H.kb_c * A_STEP;
+    const size_t epi_tile = (size_t)EPI_TILE;
     H.afrag.assign(a_tile * tiles.size(), 0);
     H.lut.assign(512 * tiles.size(), 0);
     H.epi.assign(epi_tile * tiles.size(), 0);
@@ -270,7 +271,7 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     } else {
         for (int beta = 0; beta < 8; beta++) ft.push_back(split_tile(B, beta));
     }
-    const size_t af_tile = (size_t)H.kb_f * NBLK * 2 * FRAG;
+    const size_t af_tile = (size_t)H.kb_f * A_STEP;
     H.lut_f.assign(512 * ft.size(), 0);
     H.epi_f.assign(epi_tile * ft.size(), 0);
     H.expo_f.assign(TILE * ft.size(), 0);
@@ -351,7 +352,7 @@ static int dalloc(T **p, size_t count) {
 
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
-                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_u, pr->d_wscratch};
+                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_u, pr->d_wscratch};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
@@ -407,7 +408,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     g.kpad = pp->lpad;
     g.kb_total = pp->kb_c;
     g.kb_per_split = (pp->kb_c + pr->splits_c - 1) / pr->splits_c;
-    g.a_kb_stride = NBLK * 2 * FRAG;
+    g.a_kb_stride = A_STEP;
     g.y = pr->d_yacc;
     g.y_batch = limbs * N;
     g.accumulate = 1;
@@ -454,7 +455,7 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     g.kpad = pp->rpad;
     g.kb_total = pp->kb_f;
     g.kb_per_split = pp->kb_f;
-    g.a_kb_stride = NBLK * 2 * FRAG;
+    g.a_kb_stride = A_STEP;
     g.y = pr->d_vhat;
     g.y_batch = N;
     g.accumulate = 0;
@@ -477,7 +478,7 @@ static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
     hipStream_t st = pr->stream;
     HIP_TRY(hipMemcpyAsync(pr->d_p0, p0, s.rows * sizeof(lbn_f162), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(pr->ev[8], st));
-    HIP_TRY(launch_eq(st, pr->d_p0, s.rows, pr->d_eq));
+    HIP_TRY(launch_eq(st, pr->d_p0, s.rows, pr->d_eqpart, pr->d_eq));
     HIP_TRY(hipEventRecord(pr->ev[9], st));
     HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_u, s.r * 6, 0));
     HIP_TRY(launch_row_eval(st, d_w, pr->d_eq, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
@@ -562,11 +563,12 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     TRY_RC(dalloc(&pr->d_raw, s.r * limbs * N));
     TRY_RC(dalloc(&pr->d_comp, s.r * 4 * limbs * N162));
     TRY_RC(dalloc(&pr->d_ch, s.r));
-    TRY_RC(dalloc(&pr->d_afrag_f, (size_t)pp->ntiles_f * pp->kb_f * NBLK * 2 * FRAG));
+    TRY_RC(dalloc(&pr->d_afrag_f, (size_t)pp->ntiles_f * pp->kb_f * A_STEP));
     TRY_RC(dalloc(&pr->d_vhat, s.len_ring * N));
     TRY_RC(dalloc(&pr->d_v, s.len_ring * N));
     TRY_RC(dalloc(&pr->d_p0, 3 * (size_t)std::max<uint32_t>(s.rows, 1)));
     TRY_RC(dalloc(&pr->d_eq, 3 * s.column_len));
+    TRY_RC(dalloc(&pr->d_eqpart, 3 * 256 * (size_t)((s.rows + 7) / 8)));
     TRY_RC(dalloc(&pr->d_u, 3 * s.r));
     // split K when there are too few tiles x columns to fill the device
     const size_t wgs = (size_t)pp->ntiles * s.r;

@@ -170,3 +170,20 @@ def test_zero_witness_commits_to_zero(lbn, orc):
     c, o = prover.commit(w)
     assert not c.raw.any() and not c.components.any()
     prover.recycle(o)
+
+
+@pytest.mark.parametrize("wl,cl", [(18, 1), (12, 5)])
+def test_row_evaluation_many_rows(lbn, orc, wl, cl):
+    # eq tables of 17 and 7 row variables (several 8-coordinate sub-tables, and one)
+    say(f"case: row eval wl={wl} cl={cl}")
+    params = lbn.Params(wl, cl, [])
+    pp = lbn.PublicParameters.from_seed(params, MATRIX_SEED)
+    prover = lbn.Prover(pp)
+    w = lbn.Witness.random(params, WITNESS_SEED)
+    rng = np.random.default_rng(wl)
+    pt = rng.integers(0, 2**63, (wl, 3), dtype=np.uint64)
+    pt[:, 2] &= np.uint64((1 << 34) - 1)
+    point = lbn.EvaluationPoint(pt[:wl - cl], pt[wl - cl:])
+    row = prover.row_evaluate(w, point)
+    assert np.array_equal(row.values, orc.row_evaluate(w.elements, point.p0))
+    assert np.array_equal(prover.mle_evaluate(w, point), orc.claim(row.values, point.p1))

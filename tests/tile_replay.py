@@ -6,7 +6,9 @@ import ctypes
 import numpy as np
 
 N = 648
-TILE, KSTEP, NBLK, FRAG = 96, 64, 6, 1024
+TILE, KSTEP, NBLK, FRAG = 96, 64, 3, 1024
+A_STEP = 2 * 2 * NBLK * FRAG
+EPI_TILE = NBLK * NBLK * 64 * 16 * 3
 DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("epi_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
                  ("par", "<u4"), ("y_off", "<u4")])
 
@@ -37,27 +39,30 @@ def ring_bytes(w, cols, L):
 
 
 def a_matrix(afrag, kbs):
-    """[kb][mb][d][lane][16] -> A[d][k][m]"""
-    f = afrag.reshape(kbs, NBLK, 2, 64, 16).astype(np.int64)
+    """[kb][kh][d][mb][lane][16] -> A[d][k][m]: lane holds row 32 mb + (lane & 31) and
+    K = 64 kb + 32 kh + 16 (lane >> 5) + byte (the A operand of v_mfma_i32_32x32x32_i8)"""
+    f = afrag.reshape(kbs, 2, 2, NBLK, 64, 16).astype(np.int64)
     A = np.zeros((2, kbs * KSTEP, TILE), np.int64)
     for lane in range(64):
-        row = lane & 15
-        k0 = 16 * (lane >> 4)
-        for mb in range(NBLK):
-            for kb in range(kbs):
-                A[:, kb * KSTEP + k0:kb * KSTEP + k0 + 16, 16 * mb + row] = f[kb, mb, :, lane, :]
+        row = lane & 31
+        for kh in range(2):
+            k0 = 32 * kh + 16 * (lane >> 5)
+            for mb in range(NBLK):
+                for kb in range(kbs):
+                    A[:, kb * KSTEP + k0:kb * KSTEP + k0 + 16, 32 * mb + row] = f[kb, kh, :, mb, lane, :]
     return A
 
 
 def epi_matrix(epi):
-    """[mb][nb][lane][r][3] -> E[3][m][n]"""
-    e = epi.reshape(NBLK, NBLK, 64, 4, 3).astype(np.int64)
+    """[mb][nb][lane][r][3] -> E[3][m][n]: register r of lane holds row
+    32 mb + 8 (r >> 2) + 4 (lane >> 5) + (r & 3), column 32 nb + (lane & 31)"""
+    e = epi.reshape(NBLK, NBLK, 64, 16, 3).astype(np.int64)
     E = np.zeros((3, TILE, TILE), np.int64)
     for lane in range(64):
-        for r in range(4):
+        for r in range(16):
             for mb in range(NBLK):
                 for nb in range(NBLK):
-                    E[:, 16 * mb + 4 * (lane >> 4) + r, 16 * nb + (lane & 15)] = e[mb, nb, lane, r]
+                    E[:, 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), 32 * nb + (lane & 31)] = e[mb, nb, lane, r]
     return E
 
 
@@ -92,10 +97,10 @@ def replay_commit(lbn, params, seed, w):
     desc = debug(lbn, p, seed, 4, DESC)
     rb = ring_bytes(w, cols, L)
     y = np.zeros((cols, limbs * N), np.int64)
-    a_tile = kbs * NBLK * 2 * FRAG
+    a_tile = kbs * A_STEP
     for d in desc:
         A = a_matrix(afrag[d["a_off"]:d["a_off"] + a_tile], kbs)
-        E = epi_matrix(epi[d["epi_off"]:d["epi_off"] + NBLK * NBLK * 64 * 12])
+        E = epi_matrix(epi[d["epi_off"]:d["epi_off"] + EPI_TILE])
         lt = lut[d["lut_off"]:d["lut_off"] + 512].astype(np.int64)
         o = out[d["out_off"]:d["out_off"] + TILE]
         for c in range(cols):
@@ -145,7 +150,7 @@ def replay_fold(lbn, params, seed, w, challenges):
                 x = x - q if x > (q - 1) // 2 else x
                 xl = ((x + 64) & 127) - 64
                 A[0, c, m], A[1, c, m] = xl, (x - xl) // 128
-        E = epi_matrix(epi[d["epi_off"]:d["epi_off"] + NBLK * NBLK * 64 * 12])
+        E = epi_matrix(epi[d["epi_off"]:d["epi_off"] + EPI_TILE])
         lt = lut[d["lut_off"]:d["lut_off"] + 512].astype(np.int64)
         o = out[d["out_off"]:d["out_off"] + TILE]
         for i in range(L):
