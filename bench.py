@@ -111,11 +111,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--suite", default="xl", choices=["s", "m", "l", "xl"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) five times, to scale the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the prover runs on a single MI355X (columns would shard across "
               "GPUs with one all-reduce in the fold; not part of this build)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30, "launches": 5}))
+        return
 
     import numpy as np
     import labinius_amd as lbn

@@ -11,8 +11,8 @@
 extern "C" {
 #endif
 
-/* which: 0 commit A fragments (int8), 1 commit LUTs (u16), 2 commit epilogue weights (u16),
- * 3 commit output slots (i16), 4 commit tile descriptors, 5 fold LUTs, 6 fold epilogue weights,
+/* which: 0 commit A fragments (int8), 1 commit LUTs (u16), 2 commit E tables [n 81][m 96] (u16),
+ * 3 commit output slots (i16), 4 commit tile descriptors, 5 fold LUTs, 6 fold E tables,
  * 7 fold output slots, 8 fold root exponents (u16), 9 fold tile descriptors, 10 finish constants,
  * 11 the base root's powers (u16), 12 inverse-transform constants, 13 inverse twiddles (u32),
  * 14 the expanded key A ([limb][len_ring][648] int16).

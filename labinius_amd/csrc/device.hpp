@@ -12,33 +12,57 @@
 //     y[u] = sum_n E[u][n] * Z[u][n],        Z[u][n] = sum_i A[i][u] * x_i[n],
 //
 // where Z is an integer GEMM (K = ring elements) run on int8 MFMA with both operands split into
-// two signed 7-bit digits. A "tile" is one such 96 x 96 (81 x 81 padded) GEMM: 8 per splitting
+// two signed 7-bit digits. A "tile" is one such 96 x 81 (81 x 81 used) GEMM: 8 per splitting
 // limb, 16 per quadratic limb (4 sub-rings x 2 halves of the A rows x 2 parities of the inputs).
 // The fold is the same GEMM with K = columns, A = the challenges' transform, 8 tiles on the base.
+//
+// A tile's A is shared by every batch element (column for the commit, ring element for the
+// fold), so the GEMM's N runs over batch elements and sub-ring coefficients together: a "group"
+// is 3 batch elements = 243 inputs, padded to 256 = 8 fragments of 32 inputs.
 #pragma once
+
+#include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 namespace lbn {
 
-constexpr int TILE = 96;         // padded M and N of a tile (81 rows / inputs used)
+constexpr int TILE = 96;         // padded M of a tile (81 rows used)
 constexpr int TILE_USED = 81;
 constexpr int KSTEP = 64;        // K per step of the GEMM loop: two v_mfma_i32_32x32x32_i8
 constexpr int BLK = 32;          // MFMA block (32 x 32)
 constexpr int NBLK = TILE / BLK; // 3 blocks per tile side
 constexpr int FRAG = 1024;       // bytes of one 32 x 32 int8 operand fragment (64 lanes x 16 B)
 constexpr int A_STEP = 2 * 2 * NBLK * FRAG;  // A fragments per K step: [kh][digit][mb]
-constexpr int B_TILE = 2 * 2 * NBLK * FRAG;  // x digits per K step: [kh][digit][nb]
-constexpr int EPI_TILE = NBLK * NBLK * 64 * 16 * 3;  // epilogue weights per tile
+constexpr int GROUP_BATCH = 3;               // batch elements per group
+constexpr int GROUP_COLS = 256;              // N of a group (243 used)
+constexpr int GROUP_FRAGS = GROUP_COLS / BLK;
+constexpr int EE_TILE = TILE_USED * TILE;    // E of a tile, [n 81][m 96] u16
+
+// The input bytes in MFMA B-fragment order. Byte n of batch element b at K index k lives at
+//
+//   ((((g * 8 + jg) * kbs + kb) * 2 + kh) * 64 + lane) * 16 + (k & 15)
+//
+// with g = b / 3, j = 81 (b % 3) + n, jg = j / 32, lane = (j & 31) + 32 ((k >> 4) & 1),
+// kh = (k >> 5) & 1, kb = k / 64: one 16-byte load per lane gives a lane its B fragment.
+__host__ __device__ inline uint64_t byte_offset(uint32_t b, uint32_t n, uint32_t k, uint32_t kbs) {
+    const uint32_t g = b / GROUP_BATCH, j = TILE_USED * (b % GROUP_BATCH) + n;
+    const uint32_t lane = (j & 31) + 32 * ((k >> 4) & 1);
+    return ((((uint64_t)g * GROUP_FRAGS + (j >> 5)) * kbs + (k >> 6)) * 2 + ((k >> 5) & 1)) * 1024 +
+           lane * 16 + (k & 15);
+}
+inline uint64_t bytes_size(uint32_t batches, uint32_t kbs) {
+    return (uint64_t)((batches + GROUP_BATCH - 1) / GROUP_BATCH) * GROUP_FRAGS * kbs * 2 * 1024;
+}
 
 // One tile of a GEMM launch.
 struct TileDesc {
     uint64_t a_off;     // byte offset of the tile's A fragments: [kb][kh 2][digit 2][mb 3][lane 64][16]
     uint32_t lut_off;   // element offset (u16) of the tile's two 256-entry LUTs
-    uint32_t epi_off;   // element offset (u16) of the tile's epilogue weights [mb][nb][lane][16][3]
+    uint32_t ee_off;    // element offset (u16) of the tile's E, [n 81][m 96]
     uint32_t out_off;   // element offset (i16) of the tile's 96 output slots (-1 = padding row)
     uint32_t q;
-    uint32_t par;       // input column n reads LUT (par + n) & 1
+    uint32_t par;       // input n reads LUT (par + n) & 1
     uint32_t y_off;     // word offset of the tile's limb inside one batch's output rows
 };
 
@@ -46,18 +70,17 @@ struct GemmArgs {
     const TileDesc *tiles;
     const int8_t *afrag;      // all tiles' A fragments
     const uint16_t *lut;      // all tiles' LUTs (low byte x_l, high byte x_h, both int8)
-    const uint16_t *epi;      // all tiles' epilogue weights (E, 128 E, 16384 E mod q)
+    const uint16_t *ee;       // all tiles' E (mod q)
     const int16_t *out;       // all tiles' output slots
-    const uint8_t *bytes;     // bytes[batch][n < 81][kpad]
-    uint64_t bytes_batch;     // batch stride of bytes
-    uint32_t kpad;            // K padded to a multiple of KSTEP (the n stride of bytes)
-    uint32_t kb_per_split;    // K steps per split (blockIdx.z)
-    uint32_t kb_total;
+    const uint8_t *bytes;     // the input bytes, fragment order (byte_offset)
+    uint32_t kbs;             // K steps
     uint32_t a_kb_stride;     // bytes of A fragments per K step of a tile (A_STEP)
     uint32_t *y;              // output rows
-    uint64_t y_batch;         // words per batch in y
-    int accumulate;           // 1: atomicAdd (commit, several contributions), 0: store (fold)
-    uint32_t ntiles;          // blockIdx.x = batch * ntiles + tile
+    uint64_t y_batch;         // words per batch element in y
+    uint32_t batches;
+    uint32_t groups;          // ceil(batches / 3)
+    uint32_t ntiles;
+    int accumulate;           // 1: atomicAdd (several tiles add into one slot), 0: store
 };
 
 // Per-limb constants of the commit finish (decomposition) kernel.

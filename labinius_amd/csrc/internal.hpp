@@ -84,14 +84,14 @@ struct lbn_pp {
     // commit GEMM
     uint32_t ntiles = 0;
     int8_t *d_afrag = nullptr;
-    uint16_t *d_lut = nullptr, *d_epi = nullptr;
+    uint16_t *d_lut = nullptr, *d_ee = nullptr;
     int16_t *d_out = nullptr;
     lbn::TileDesc *d_tiles = nullptr;
     lbn::FinishLimb *d_finish = nullptr;
     uint16_t *d_slot648 = nullptr, *d_qcs = nullptr;
     // fold GEMM (base limb)
     uint32_t ntiles_f = 0;
-    uint16_t *d_lut_f = nullptr, *d_epi_f = nullptr, *d_expo_f = nullptr, *d_pw_f = nullptr;
+    uint16_t *d_lut_f = nullptr, *d_ee_f = nullptr, *d_expo_f = nullptr, *d_pw_f = nullptr;
     int16_t *d_out_f = nullptr;
     lbn::TileDesc *d_tiles_f = nullptr;
     uint32_t cond_f = 0, mult_f = 0;
@@ -127,5 +127,4 @@ struct lbn_prover {
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
     hipEvent_t ev[12] = {};
     float timings[11] = {};
-    uint32_t splits_c = 1;
 };

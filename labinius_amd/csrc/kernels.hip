@@ -25,16 +25,19 @@ namespace lbn {
 
 // The byte of ring element i of column c at position n < 81: bit t is coefficient 81 t + n, and
 // coefficient 4 m + k of a ring element is bit m of its F162 element k (f162.rs:1-16). Written
-// twice: bytes_c[c][n][lpad] (the commit's K = ring elements) and bytes_f[i][n][rpad] (the fold's
-// K = columns). A block covers 8 ring elements x 32 columns and stages its 2592 bytes in LDS so
-// both stores are runs of contiguous bytes.
-__global__ __launch_bounds__(256) void prepass_kernel(const uint64_t *__restrict__ w,
-                                                      uint32_t column_len, uint32_t L, uint32_t r,
-                                                      uint8_t *__restrict__ bytes_c, uint32_t lpad,
-                                                      uint8_t *__restrict__ bytes_f, uint32_t rpad) {
-    __shared__ __attribute__((aligned(16))) uint8_t st[32][8][84];  // [c][i][n]
-    const int ii = threadIdx.x >> 5, cc = threadIdx.x & 31;
-    const uint32_t i = blockIdx.y * 8 + ii, c = blockIdx.x * 32 + cc;
+// twice in fragment order (byte_offset): the commit's (batch c, K = i) and the fold's (batch i,
+// K = c). A block covers 16 ring elements x 32 columns, one thread per ring element, and stages
+// the bytes in LDS as st[n][c][i] so both stores are 16-byte runs.
+constexpr int PRE_I = 16, PRE_C = 32;
+
+__global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *__restrict__ w,
+                                                                uint32_t column_len, uint32_t L, uint32_t r,
+                                                                uint8_t *__restrict__ bytes_c, uint32_t kbs_c,
+                                                                uint8_t *__restrict__ bytes_f, uint32_t kbs_f) {
+    __shared__ __attribute__((aligned(16))) uint8_t st[TILE_USED][PRE_C][PRE_I];
+    const int il = threadIdx.x & (PRE_I - 1), cl = threadIdx.x / PRE_I;
+    const uint32_t i0 = blockIdx.y * PRE_I, c0 = blockIdx.x * PRE_C;
+    const uint32_t i = i0 + il, c = c0 + cl;
     uint32_t wd[4][6];
     if (i < L && c < r) {
         const uint64_t *e = w + 3 * ((size_t)c * column_len + 4 * (size_t)i);
@@ -42,7 +45,7 @@ __global__ __launch_bounds__(256) void prepass_kernel(const uint64_t *__restrict
         for (int k = 0; k < 4; k++)
 #pragma unroll
             for (int h = 0; h < 3; h++) {
-                uint64_t x = e[3 * k + h];
+                const uint64_t x = e[3 * k + h];
                 wd[k][2 * h] = (uint32_t)x;
                 wd[k][2 * h + 1] = (uint32_t)(x >> 32);
             }
@@ -53,42 +56,35 @@ __global__ __launch_bounds__(256) void prepass_kernel(const uint64_t *__restrict
             for (int h = 0; h < 6; h++) wd[k][h] = 0;
     }
 #pragma unroll
-    for (int n = 0; n < 81; n++) {
+    for (int n = 0; n < TILE_USED; n++) {
         uint32_t b = 0;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             const int m = 81 * t + n, k = m & 3, mp = m >> 2;
             b |= ((wd[k][mp >> 5] >> (mp & 31)) & 1u) << t;
         }
-        st[cc][ii][n] = (uint8_t)b;
+        st[n][cl][il] = (uint8_t)b;
     }
     __syncthreads();
-    // bytes_c: for each (c, n) the 8 bytes of ring elements i0..i0+7
-    const uint32_t i0 = blockIdx.y * 8, c0 = blockIdx.x * 32;
-    for (int u = threadIdx.x; u < 32 * 81; u += 256) {
-        const int cl = u / 81, n = u % 81;
-        const uint32_t cg = c0 + cl;
-        if (cg >= r || i0 >= L) continue;
-        uint64_t v = 0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v |= (uint64_t)st[cl][q][n] << (8 * q);
-        *(uint64_t *)(bytes_c + ((size_t)cg * 81 + n) * lpad + i0) = v;
+    // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16)
+    for (int u = threadIdx.x; u < PRE_C * TILE_USED; u += PRE_I * PRE_C) {
+        const int cc = u / TILE_USED, n = u % TILE_USED;
+        if (c0 + cc >= r) continue;
+        *(uint4 *)(bytes_c + byte_offset(c0 + cc, n, i0, kbs_c)) = *(const uint4 *)&st[n][cc][0];
     }
     if (!bytes_f) return;
-    // bytes_f: for each (i, n) the 32 bytes of columns c0..c0+31
-    for (int u = threadIdx.x; u < 8 * 81 * 2; u += 256) {
-        const int half = u & 1, rest = u >> 1, il = rest / 81, n = rest % 81;
-        const uint32_t ig = i0 + il;
-        if (ig >= L) continue;
+    // fold: for each (i, n) the 16 bytes of columns c0 + 16 h .. c0 + 16 h + 15
+    for (int u = threadIdx.x; u < PRE_I * TILE_USED * 2; u += PRE_I * PRE_C) {
+        const int h = u & 1, rest = u >> 1, ii = rest / TILE_USED, n = rest % TILE_USED;
+        if (i0 + ii >= L) continue;
         uint32_t v[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int cb = 16 * half + 4 * q;
-            v[q] = (uint32_t)st[cb][il][n] | (uint32_t)st[cb + 1][il][n] << 8 |
-                   (uint32_t)st[cb + 2][il][n] << 16 | (uint32_t)st[cb + 3][il][n] << 24;
+            const int cb = 16 * h + 4 * q;
+            v[q] = (uint32_t)st[n][cb][ii] | (uint32_t)st[n][cb + 1][ii] << 8 | (uint32_t)st[n][cb + 2][ii] << 16 |
+                   (uint32_t)st[n][cb + 3][ii] << 24;
         }
-        *(uint4 *)(bytes_f + ((size_t)ig * 81 + n) * rpad + c0 + 16 * half) =
-            make_uint4(v[0], v[1], v[2], v[3]);
+        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0 + 16 * h, kbs_f)) = make_uint4(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -96,170 +92,189 @@ __global__ __launch_bounds__(256) void prepass_kernel(const uint64_t *__restrict
 // the tile GEMM
 // =============================================================================================
 
-__device__ __forceinline__ long long shfl_xor64(long long v, int m) {
-    int lo = __shfl_xor((int)(unsigned long long)v, m, 64);
-    int hi = __shfl_xor((int)((unsigned long long)v >> 32), m, 64);
-    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+// One tile over one group of 3 batch elements: Z[m][j] = sum_k A[k][m] x_k[j] for the group's
+// 256 inputs j = 81 b + n (243 used) over all K, then y[b][out[m]] (+)= sum_n E[m][n] Z[m][81 b + n]
+// mod q.
+//
+// Eight waves, wave w owning inputs 32 w .. 32 w + 31. A (both digits, three 32-row blocks) is
+// staged per K step in LDS, double-buffered and shared by the eight waves. Each wave builds its
+// own B fragments in registers: one 16-byte load per lane gives the 16 bytes of its input and K
+// range, and the tile's LUT (in LDS) turns every byte into its two digits. Z is kept as three
+// int32 accumulators per element: ll = A_l x_l, mid = A_l x_h + A_h x_l, hh = A_h x_h.
+//
+// Blocks are ordered so that the 32 blocks an XCD runs at once are 8 tiles x 4 groups: each
+// A and each group's bytes then come from L2 for 4 and 8 blocks respectively.
+constexpr int GEMM_THREADS = 512;
+constexpr int GEMM_LDS = 1024 + GROUP_COLS * TILE * 4 + EE_TILE * 2;
+
+__host__ __device__ inline uint32_t gemm_blocks(uint32_t ntiles, uint32_t groups) {
+    return 256 * ((ntiles + 7) / 8) * ((groups + 31) / 32);
 }
 
-// One 96 x 96 tile of Z = sum_k A[k][m] x_k[n] for one batch element (a column for the commit,
-// a ring element for the fold) over the K steps of blockIdx.z, followed by the epilogue
-// y[out[m]] += sum_n E[m][n] Z[m][n] mod q.
-//
-// Operands are two signed 7-bit digits each (A = A_l + 2^7 A_h, x = x_l + 2^7 x_h), so Z is the
-// sum of four digit products. The 256 threads are four waves, one per digit product: wave w
-// multiplies A digit DA(w) by x digit DX(w) over all nine 32 x 32 blocks of the tile with
-// v_mfma_i32_32x32x32_i8 (waves 2 and 3 are the two cross terms, which the epilogue weights
-// by 2^7; wave 1's by 2^14). Every wave also takes a third of the work of turning the next K
-// step's bytes into x digits by table lookup, into the other half of a double-buffered LDS
-// tile; its A fragments come straight from global memory in MFMA lane order, one K step ahead.
-constexpr int GEMM_THREADS = 256;
+__device__ __forceinline__ uint32_t mod_q_i64(long long z, uint32_t q, double qinv) {
+    const long long t = (long long)floor((double)z * qinv);
+    long long m = z - t * (long long)q;
+    if (m < 0) m += q;
+    if (m >= (long long)q) m -= q;
+    return (uint32_t)m;
+}
 
-__global__ __launch_bounds__(GEMM_THREADS, 2) void tile_gemm_kernel(GemmArgs g) {
-    // tiles vary fastest, so the blocks of one batch element run together and share its bytes
-    const TileDesc td = g.tiles[blockIdx.x % g.ntiles];
-    const uint32_t batch = blockIdx.x / g.ntiles;
-    const uint32_t kb0 = blockIdx.z * g.kb_per_split;
-    const uint32_t kb1 = min(kb0 + g.kb_per_split, g.kb_total);
+__global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
+    const uint32_t xcd = blockIdx.x & 7, s = blockIdx.x >> 3;
+    const uint32_t round = s >> 5, within = s & 31, tgroups = (g.ntiles + 7) / 8;
+    const uint32_t tile = 8 * (round % tgroups) + (within & 7);
+    const uint32_t grp = 4 * (8 * (round / tgroups) + xcd) + (within >> 3);
+    if (tile >= g.ntiles || grp >= g.groups) return;
+    const TileDesc td = g.tiles[tile];
 
-    __shared__ uint16_t lut[512];
-    __shared__ __attribute__((aligned(16))) uint8_t bt[2][B_TILE];
-    __shared__ long long rowsum[TILE];
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    uint16_t *lut = (uint16_t *)smem;                                   // [2][256]
+    uint8_t *abuf = smem + 1024;                                        // [2][A_STEP]
+    uint32_t *zbuf = (uint32_t *)(smem + 1024);                         // [256][96], after the loop
+    uint16_t *ee = (uint16_t *)(smem + 1024 + GROUP_COLS * TILE * 4);   // [81][96]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int da = wave & 1, dx = (wave == 1 || wave == 2) ? 1 : 0;
     lut[tid] = g.lut[td.lut_off + tid];
-    lut[tid + 256] = g.lut[td.lut_off + tid + 256];
-    if (tid < TILE) rowsum[tid] = 0;
 
-    const uint8_t *bbase = g.bytes + (size_t)batch * g.bytes_batch;
+    const int jl = 32 * wave + (lane & 31);  // this lane's input within the group
+    const uint16_t *L = lut + ((((int)td.par + jl % TILE_USED) & 1) << 8);
+    const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + lane) * 16;
     const int8_t *abase = g.afrag + td.a_off;
 
-    // lookup unit u = (nb, kh, lane, half): 8 bytes of input column n = 32 nb + (lane & 31) at
-    // K = 64 kb + 32 kh + 16 (lane >> 5) + 8 half
-    uint2 raw[3];
-    auto load_bytes = [&](uint32_t kb) {
+    uint4 raw[2];
+    auto load_raw = [&](uint32_t kb) {
 #pragma unroll
-        for (int it = 0; it < 3; it++) {
-            const int u = tid + 256 * it;
-            const int nb = u >> 8, rem = u & 255, kh = rem >> 7, ln = (rem >> 1) & 63, h = rem & 1;
-            const int n = nb * 32 + (ln & 31);
-            raw[it] = make_uint2(0, 0);
-            if (n < TILE_USED)
-                raw[it] = *(const uint2 *)(bbase + (size_t)n * g.kpad + kb * KSTEP + 32 * kh + 16 * (ln >> 5) + 8 * h);
+        for (int kh = 0; kh < 2; kh++) raw[kh] = *(const uint4 *)(bp + ((uint64_t)kb * 2 + kh) * 1024);
+    };
+    v4i bl[2], bh[2];
+    auto build = [&]() {
+#pragma unroll
+        for (int kh = 0; kh < 2; kh++) {
+            const uint32_t w4[4] = {raw[kh].x, raw[kh].y, raw[kh].z, raw[kh].w};
+            int ol[4], oh[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const uint32_t r = w4[d];
+                const uint32_t v0 = L[r & 255], v1 = L[(r >> 8) & 255], v2 = L[(r >> 16) & 255], v3 = L[r >> 24];
+                // p01 = [v0.b0, v1.b0, v0.b1, v1.b1]; then the four low and the four high digits
+                const uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
+                const uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
+                ol[d] = (int)__builtin_amdgcn_perm(p23, p01, 0x05040100u);
+                oh[d] = (int)__builtin_amdgcn_perm(p23, p01, 0x07060302u);
+            }
+            bl[kh] = (v4i){ol[0], ol[1], ol[2], ol[3]};
+            bh[kh] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
-    auto build = [&](int buf) {
+    constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
+    v4i apre[AP];
+    auto load_a = [&](uint32_t kb) {
+        const int8_t *ak = abase + (size_t)kb * g.a_kb_stride;
 #pragma unroll
-        for (int it = 0; it < 3; it++) {
-            const int u = tid + 256 * it;
-            const int nb = u >> 8, rem = u & 255, kh = rem >> 7, ln = (rem >> 1) & 63, h = rem & 1;
-            const int n = nb * 32 + (ln & 31);
-            const uint16_t *L = lut + (((td.par + n) & 1) << 8);
-            const uint2 r = raw[it];
-            const uint32_t v0 = L[r.x & 255], v1 = L[(r.x >> 8) & 255];
-            const uint32_t v2 = L[(r.x >> 16) & 255], v3 = L[r.x >> 24];
-            const uint32_t v4 = L[r.y & 255], v5 = L[(r.y >> 8) & 255];
-            const uint32_t v6 = L[(r.y >> 16) & 255], v7 = L[r.y >> 24];
-            // p01 = [v0.b0, v1.b0, v0.b1, v1.b1], then split the low and high digits; a padding
-            // column (n >= 81) reads byte 0, whose table entry is 0
-            const uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
-            const uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
-            const uint32_t p45 = __builtin_amdgcn_perm(v5, v4, 0x05010400u);
-            const uint32_t p67 = __builtin_amdgcn_perm(v7, v6, 0x05010400u);
-            uint8_t *o = &bt[buf][((kh * 2) * 3 + nb) * FRAG + ln * 16 + 8 * h];
-            *(uint2 *)o = make_uint2(__builtin_amdgcn_perm(p23, p01, 0x05040100u),
-                                     __builtin_amdgcn_perm(p67, p45, 0x05040100u));
-            *(uint2 *)(o + 3 * FRAG) = make_uint2(__builtin_amdgcn_perm(p23, p01, 0x07060302u),
-                                                  __builtin_amdgcn_perm(p67, p45, 0x07060302u));
-        }
+        for (int i = 0; i < AP; i++)
+            if (tid + GEMM_THREADS * i < A_STEP / 16) apre[i] = *(const v4i *)(ak + (tid + GEMM_THREADS * i) * 16);
+    };
+    auto store_a = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < AP; i++)
+            if (tid + GEMM_THREADS * i < A_STEP / 16)
+                *(v4i *)(abuf + buf * A_STEP + (tid + GEMM_THREADS * i) * 16) = apre[i];
     };
 
     v16i acc[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) acc[a][b] = (v16i){};
-    // A fragments [kh][digit][mb]: lane holds row 32 mb + (lane & 31), K 32 kh + 16 (lane >> 5) + byte
-    v4i af[2][3];
-    auto load_a = [&](uint32_t kb) {
-        const int8_t *ak = abase + (size_t)kb * g.a_kb_stride;
+        for (int c = 0; c < 3; c++) acc[a][c] = (v16i){};
+
+    const uint32_t kbs = g.kbs;
+    load_a(0);
+    load_raw(0);
+    store_a(0);
+    __syncthreads();  // the LUT and A(0)
+    build();
+    if (1 < kbs) {
+        load_a(1);
+        load_raw(1);
+    }
+    for (uint32_t kb = 0; kb < kbs; kb++) {
+        const uint8_t *ab = abuf + (kb & 1) * A_STEP;
+        v4i cl[2], ch[2];
 #pragma unroll
-        for (int kh = 0; kh < 2; kh++)
+        for (int kh = 0; kh < 2; kh++) {
+            cl[kh] = bl[kh];
+            ch[kh] = bh[kh];
+        }
+        if (kb + 1 < kbs) build();  // the next step's B (its bytes are loaded)
+#pragma unroll
+        for (int kh = 0; kh < 2; kh++) {
+            v4i a0[3], a1[3];
+#pragma unroll
+            for (int mb = 0; mb < 3; mb++) {
+                a0[mb] = *(const v4i *)(ab + (((kh * 2 + 0) * 3 + mb) * 64 + lane) * 16);
+                a1[mb] = *(const v4i *)(ab + (((kh * 2 + 1) * 3 + mb) * 64 + lane) * 16);
+            }
+            // the two products that share an accumulator are three MFMAs apart
 #pragma unroll
             for (int mb = 0; mb < 3; mb++)
-                af[kh][mb] = *(const v4i *)(ak + (((kh * 2 + da) * 3 + mb) * 64 + lane) * 16);
-    };
-
-    __syncthreads();  // the LUT
-    if (kb0 < kb1) {
-        load_bytes(kb0);
-        load_a(kb0);
-        build(0);
-        if (kb0 + 1 < kb1) load_bytes(kb0 + 1);
-    }
-    __syncthreads();
-
-    for (uint32_t kb = kb0; kb < kb1; kb++) {
-        const int buf = (kb - kb0) & 1;
-        v4i a[2][3];
+                acc[mb][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mb], cl[kh], acc[mb][0], 0, 0, 0);
 #pragma unroll
-        for (int kh = 0; kh < 2; kh++)
+            for (int mb = 0; mb < 3; mb++)
+                acc[mb][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mb], ch[kh], acc[mb][1], 0, 0, 0);
 #pragma unroll
-            for (int mb = 0; mb < 3; mb++) a[kh][mb] = af[kh][mb];
-        if (kb + 1 < kb1) load_a(kb + 1);
+            for (int mb = 0; mb < 3; mb++)
+                acc[mb][2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mb], ch[kh], acc[mb][2], 0, 0, 0);
 #pragma unroll
-        for (int kh = 0; kh < 2; kh++)
-#pragma unroll
-            for (int nb = 0; nb < 3; nb++) {
-                const v4i x = *(const v4i *)(&bt[buf][((kh * 2 + dx) * 3 + nb) * FRAG + lane * 16]);
-#pragma unroll
-                for (int mb = 0; mb < 3; mb++)
-                    acc[mb][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[kh][mb], x, acc[mb][nb], 0, 0, 0);
+            for (int mb = 0; mb < 3; mb++)
+                acc[mb][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mb], cl[kh], acc[mb][1], 0, 0, 0);
+        }
+        if (kb + 1 < kbs) {
+            store_a((kb + 1) & 1);
+            if (kb + 2 < kbs) {
+                load_a(kb + 2);
+                load_raw(kb + 2);
             }
-        if (kb + 1 < kb1) {
-            build(buf ^ 1);
-            if (kb + 2 < kb1) load_bytes(kb + 2);
         }
         __syncthreads();
     }
 
-    // epilogue: the D fragment of a 32 x 32 block holds, in register r, row
-    // 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
-    {
-        const int cls = wave == 0 ? 0 : (wave == 1 ? 2 : 1);
-        const uint16_t *ep = g.epi + td.epi_off;
+    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zbuf[j][m]; the D fragment of a 32 x 32
+    // block holds, in register r, row 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
+    const uint32_t q = td.q;
+    const double qinv = 1.0 / (double)q;
+    for (int i = tid; i < EE_TILE; i += GEMM_THREADS) ee[i] = g.ee[td.ee_off + i];
 #pragma unroll
-        for (int mb = 0; mb < 3; mb++)
+    for (int mb = 0; mb < 3; mb++)
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                long long s = 0;
+        for (int r4 = 0; r4 < 4; r4++) {
+            uint32_t z4[4];
 #pragma unroll
-                for (int nb = 0; nb < 3; nb++)
-                    s += (long long)ep[(((mb * 3 + nb) * 64 + lane) * 16 + r) * 3 + cls] * acc[mb][nb][r];
-                s += shfl_xor64(s, 16);
-                s += shfl_xor64(s, 8);
-                s += shfl_xor64(s, 4);
-                s += shfl_xor64(s, 2);
-                s += shfl_xor64(s, 1);
-                if ((lane & 31) == 0)
-                    atomicAdd((unsigned long long *)&rowsum[32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)],
-                              (unsigned long long)s);
+            for (int r = 0; r < 4; r++) {
+                const int rr = 4 * r4 + r;
+                const long long z = (long long)acc[mb][0][rr] + ((long long)acc[mb][1][rr] << 7) +
+                                    ((long long)acc[mb][2][rr] << 14);
+                z4[r] = mod_q_i64(z, q, qinv);
             }
-    }
-    __syncthreads();
-    if (tid < TILE) {
-        const int slot = g.out[td.out_off + tid];
-        if (slot >= 0) {
-            const long long qq = td.q;
-            long long m = rowsum[tid] % qq;
-            if (m < 0) m += qq;
-            uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
-            if (g.accumulate)
-                atomicAdd(yp, (uint32_t)m);
-            else
-                *yp = (uint32_t)m;
+            const int m0 = 32 * mb + 8 * r4 + 4 * (lane >> 5);
+            *(uint4 *)&zbuf[jl * TILE + m0] = make_uint4(z4[0], z4[1], z4[2], z4[3]);
         }
+    __syncthreads();
+    // epilogue 2: y[b][out[m]] (+)= sum_n E[m][n] Z[81 b + n][m]
+    for (int it = tid; it < GROUP_BATCH * TILE; it += GEMM_THREADS) {
+        const int b = it / TILE, m = it % TILE;
+        const uint32_t batch = grp * GROUP_BATCH + b;
+        const int slot = g.out[td.out_off + m];
+        if (batch >= g.batches || slot < 0) continue;
+        unsigned long long sum = 0;
+        const uint32_t *zb = zbuf + (TILE_USED * b) * TILE + m;
+#pragma unroll 9
+        for (int n = 0; n < TILE_USED; n++) sum += (unsigned long long)ee[n * TILE + m] * zb[n * TILE];
+        const uint32_t v = (uint32_t)(sum % q);
+        uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
+        if (g.accumulate)
+            atomicAdd(yp, v);
+        else
+            *yp = v;
     }
 }
 
@@ -583,18 +598,20 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v) {
 }
 
 hipError_t launch_prepass(hipStream_t s, const uint64_t *w, uint32_t column_len, uint32_t L,
-                          uint32_t r, uint8_t *bytes_c, uint32_t lpad, uint8_t *bytes_f,
-                          uint32_t rpad) {
-    dim3 grid((r + 31) / 32, (L + 7) / 8);
-    LBN_LAUNCH(prepass_kernel, grid, dim3(256), 0, s, w, column_len, L, r, bytes_c, lpad, bytes_f, rpad);
+                          uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
+                          uint32_t kbs_f) {
+    dim3 grid((r + PRE_C - 1) / PRE_C, (L + PRE_I - 1) / PRE_I);
+    LBN_LAUNCH(prepass_kernel, grid, dim3(PRE_I * PRE_C), 0, s, w, column_len, L, r, bytes_c, kbs_c, bytes_f,
+               kbs_f);
     return hipSuccess;
 }
 
-hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t ntiles, uint32_t batches,
-                            uint32_t splits) {
-    GemmArgs a = g;
-    a.ntiles = ntiles;
-    LBN_LAUNCH(tile_gemm_kernel, dim3(ntiles * batches, 1, splits), dim3(GEMM_THREADS), 0, s, a);
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g) {
+    static const hipError_t attr = hipFuncSetAttribute((const void *)tile_gemm_kernel,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
+    if (attr != hipSuccess) return attr;
+    if (g.groups == 0 || g.ntiles == 0) return hipSuccess;
+    LBN_LAUNCH(tile_gemm_kernel, dim3(gemm_blocks(g.ntiles, g.groups)), dim3(GEMM_THREADS), GEMM_LDS, s, g);
     return hipSuccess;
 }
 

@@ -19,10 +19,9 @@ struct lbn_challenge_dev {
 hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 hipError_t launch_prepass(hipStream_t s, const uint64_t *w, uint32_t column_len, uint32_t L,
-                          uint32_t r, uint8_t *bytes_c, uint32_t lpad, uint8_t *bytes_f,
-                          uint32_t rpad);
-hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t ntiles, uint32_t batches,
-                            uint32_t splits);
+                          uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
+                          uint32_t kbs_f);
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g);
 hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t limbs,
                                 const FinishLimb *fl, const uint16_t *slot648, const uint16_t *qcs,
                                 uint32_t *raw, int16_t *comp, uint32_t cols);

@@ -126,20 +126,10 @@ static HostTile quad_fold_tile(const Prime &P, int k, int b) {
     return h;
 }
 
-// E, 2^7 E and 2^14 E in the D-fragment order of the MFMA: [mb][nb][lane][r][3], row
-// 32 mb + 8 (r >> 2) + 4 (lane >> 5) + (r & 3), column 32 nb + (lane & 31).
-static void pack_epi(const HostTile &h, uint32_t q, uint16_t *dst) {
-    for (int mb = 0; mb < NBLK; mb++)
-        for (int nb = 0; nb < NBLK; nb++)
-            for (int lane = 0; lane < 64; lane++)
-                for (int r = 0; r < 16; r++) {
-                    const int m = 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), n = 32 * nb + (lane & 31);
-                    const uint64_t e = h.E[m * TILE + n];
-                    uint16_t *o = dst + (((mb * NBLK + nb) * 64 + lane) * 16 + r) * 3;
-                    o[0] = (uint16_t)e;
-                    o[1] = (uint16_t)(e * 128 % q);
-                    o[2] = (uint16_t)(e * 16384 % q);
-                }
+// E transposed, [n < 81][m < 96] (mod q): the GEMM's epilogue reads it by input n.
+static void pack_ee(const HostTile &h, uint16_t *dst) {
+    for (int n = 0; n < TILE_USED; n++)
+        for (int m = 0; m < TILE; m++) dst[n * TILE + m] = (uint16_t)h.E[m * TILE + n];
 }
 
 // The A operand of a commit tile in MFMA lane order: [kb][kh][digit][mb][lane][16 bytes], lane
@@ -180,11 +170,11 @@ static void pp_free(lbn_pp *pp) {
 struct HostTables {
     uint32_t lpad = 0, rpad = 0, kb_c = 0, kb_f = 0;
     std::vector<int8_t> afrag;
-    std::vector<uint16_t> lut, epi;
+    std::vector<uint16_t> lut, ee;
     std::vector<int16_t> out;
     std::vector<TileDesc> desc;
     std::vector<FinishLimb> finish;
-    std::vector<uint16_t> lut_f, epi_f, expo_f, pw_f;
+    std::vector<uint16_t> lut_f, ee_f, expo_f, pw_f;
     std::vector<int16_t> out_f;
     std::vector<TileDesc> desc_f;
     uint32_t cond_f = 0, mult_f = 0;
@@ -219,10 +209,10 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
         }
     }
     const size_t a_tile = (size_t)H.kb_c * A_STEP;
-    const size_t epi_tile = (size_t)EPI_TILE;
+    const size_t ee_tile = (size_t)EE_TILE;
     H.afrag.assign(a_tile * tiles.size(), 0);
     H.lut.assign(512 * tiles.size(), 0);
-    H.epi.assign(epi_tile * tiles.size(), 0);
+    H.ee.assign(ee_tile * tiles.size(), 0);
     H.out.assign(TILE * tiles.size(), 0);
     H.desc.resize(tiles.size());
     for (size_t ti = 0; ti < tiles.size(); ti++) {
@@ -230,9 +220,9 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
         pack_key(tiles[ti], key.data() + (size_t)k * s.len_ring * N, s.len_ring, H.kb_c,
                  H.afrag.data() + ti * a_tile);
         std::memcpy(H.lut.data() + 512 * ti, tiles[ti].lut, 1024);
-        pack_epi(tiles[ti], q, H.epi.data() + epi_tile * ti);
+        pack_ee(tiles[ti], H.ee.data() + ee_tile * ti);
         for (int m = 0; m < TILE; m++) H.out[TILE * ti + m] = (int16_t)tiles[ti].out[m];
-        H.desc[ti] = TileDesc{ti * a_tile, (uint32_t)(512 * ti), (uint32_t)(epi_tile * ti), (uint32_t)(TILE * ti),
+        H.desc[ti] = TileDesc{ti * a_tile, (uint32_t)(512 * ti), (uint32_t)(ee_tile * ti), (uint32_t)(TILE * ti),
                               q, tiles[ti].par, (uint32_t)(k * N)};
     }
 
@@ -273,18 +263,18 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     }
     const size_t af_tile = (size_t)H.kb_f * A_STEP;
     H.lut_f.assign(512 * ft.size(), 0);
-    H.epi_f.assign(epi_tile * ft.size(), 0);
+    H.ee_f.assign(ee_tile * ft.size(), 0);
     H.expo_f.assign(TILE * ft.size(), 0);
     H.out_f.assign(TILE * ft.size(), 0);
     H.desc_f.resize(ft.size());
     for (size_t ti = 0; ti < ft.size(); ti++) {
         std::memcpy(H.lut_f.data() + 512 * ti, ft[ti].lut, 1024);
-        pack_epi(ft[ti], B.q, H.epi_f.data() + epi_tile * ti);
+        pack_ee(ft[ti], H.ee_f.data() + ee_tile * ti);
         for (int m = 0; m < TILE; m++) {
             H.out_f[TILE * ti + m] = (int16_t)ft[ti].out[m];
             H.expo_f[TILE * ti + m] = (uint16_t)ft[ti].expo[m];
         }
-        H.desc_f[ti] = TileDesc{ti * af_tile, (uint32_t)(512 * ti), (uint32_t)(epi_tile * ti),
+        H.desc_f[ti] = TileDesc{ti * af_tile, (uint32_t)(512 * ti), (uint32_t)(ee_tile * ti),
                                 (uint32_t)(TILE * ti), B.q, ft[ti].par, 0};
     }
     H.cond_f = B.quad ? CONDUCTOR_QUAD : CONDUCTOR;
@@ -324,14 +314,14 @@ static int build_device(lbn_pp *pp) {
     int rc;
     if ((rc = upload(pp, &pp->d_afrag, H.afrag.data(), H.afrag.size()))) return rc;
     if ((rc = upload(pp, &pp->d_lut, H.lut.data(), H.lut.size()))) return rc;
-    if ((rc = upload(pp, &pp->d_epi, H.epi.data(), H.epi.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_ee, H.ee.data(), H.ee.size()))) return rc;
     if ((rc = upload(pp, &pp->d_out, H.out.data(), H.out.size()))) return rc;
     if ((rc = upload(pp, &pp->d_tiles, H.desc.data(), H.desc.size()))) return rc;
     if ((rc = upload(pp, &pp->d_finish, H.finish.data(), H.finish.size()))) return rc;
     if ((rc = upload(pp, &pp->d_slot648, &t.slot_648[0][0], 4 * N162))) return rc;
     if ((rc = upload(pp, &pp->d_qcs, &t.quad_class_slot[0][0], 2 * N162))) return rc;
     if ((rc = upload(pp, &pp->d_lut_f, H.lut_f.data(), H.lut_f.size()))) return rc;
-    if ((rc = upload(pp, &pp->d_epi_f, H.epi_f.data(), H.epi_f.size()))) return rc;
+    if ((rc = upload(pp, &pp->d_ee_f, H.ee_f.data(), H.ee_f.size()))) return rc;
     if ((rc = upload(pp, &pp->d_out_f, H.out_f.data(), H.out_f.size()))) return rc;
     if ((rc = upload(pp, &pp->d_expo_f, H.expo_f.data(), H.expo_f.size()))) return rc;
     if ((rc = upload(pp, &pp->d_tiles_f, H.desc_f.data(), H.desc_f.size()))) return rc;
@@ -363,7 +353,7 @@ static void prover_free(lbn_prover *pr) {
     delete pr;
 }
 
-static size_t bytes_f_size(const lbn_pp *pp) { return (size_t)81 * pp->rpad * pp->shape.len_ring; }
+static size_t bytes_f_size(const lbn_pp *pp) { return bytes_size((uint32_t)pp->shape.len_ring, pp->kb_f); }
 
 static int take_bytes_f(lbn_prover *pr, uint8_t **out) {
     if (!pr->free_bytes_f.empty()) {
@@ -394,25 +384,25 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     hipStream_t st = pr->stream;
     HIP_TRY(hipEventRecord(pr->ev[0], st));
     HIP_TRY(launch_prepass(st, d_w, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r, pr->d_bytes_c,
-                           pp->lpad, bytes_f, pp->rpad));
+                           pp->kb_c, bytes_f, pp->kb_f));
     HIP_TRY(launch_fill_u32(st, pr->d_yacc, s.r * limbs * N, 0));
     HIP_TRY(hipEventRecord(pr->ev[1], st));
     GemmArgs g{};
     g.tiles = pp->d_tiles;
     g.afrag = pp->d_afrag;
     g.lut = pp->d_lut;
-    g.epi = pp->d_epi;
+    g.ee = pp->d_ee;
     g.out = pp->d_out;
     g.bytes = pr->d_bytes_c;
-    g.bytes_batch = (uint64_t)81 * pp->lpad;
-    g.kpad = pp->lpad;
-    g.kb_total = pp->kb_c;
-    g.kb_per_split = (pp->kb_c + pr->splits_c - 1) / pr->splits_c;
+    g.kbs = pp->kb_c;
     g.a_kb_stride = A_STEP;
     g.y = pr->d_yacc;
     g.y_batch = limbs * N;
+    g.batches = (uint32_t)s.r;
+    g.groups = (uint32_t)((s.r + GROUP_BATCH - 1) / GROUP_BATCH);
+    g.ntiles = pp->ntiles;
     g.accumulate = 1;
-    HIP_TRY(launch_tile_gemm(st, g, pp->ntiles, (uint32_t)s.r, pr->splits_c));
+    HIP_TRY(launch_tile_gemm(st, g));
     HIP_TRY(hipEventRecord(pr->ev[2], st));
     HIP_TRY(launch_commit_finish(st, pr->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, pr->d_raw,
                                  pr->d_comp, (uint32_t)s.r));
@@ -448,18 +438,18 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     g.tiles = pp->d_tiles_f;
     g.afrag = pr->d_afrag_f;
     g.lut = pp->d_lut_f;
-    g.epi = pp->d_epi_f;
+    g.ee = pp->d_ee_f;
     g.out = pp->d_out_f;
     g.bytes = op->d_bytes_f;
-    g.bytes_batch = (uint64_t)81 * pp->rpad;
-    g.kpad = pp->rpad;
-    g.kb_total = pp->kb_f;
-    g.kb_per_split = pp->kb_f;
+    g.kbs = pp->kb_f;
     g.a_kb_stride = A_STEP;
     g.y = pr->d_vhat;
     g.y_batch = N;
+    g.batches = (uint32_t)s.len_ring;
+    g.groups = (uint32_t)((s.len_ring + GROUP_BATCH - 1) / GROUP_BATCH);
+    g.ntiles = pp->ntiles_f;
     g.accumulate = 0;
-    HIP_TRY(launch_tile_gemm(st, g, pp->ntiles_f, (uint32_t)s.len_ring, 1));
+    HIP_TRY(launch_tile_gemm(st, g));
     HIP_TRY(hipEventRecord(pr->ev[6], st));
     HIP_TRY(launch_intt(st, pr->d_vhat, (uint32_t)s.len_ring, pp->intt, pp->d_twi, pr->d_v));
     HIP_TRY(hipEventRecord(pr->ev[7], st));
@@ -556,7 +546,7 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
                                                                                    : fail(LBN_ERR_HIP, "stream"));
     for (hipEvent_t &e : pr->ev)
         TRY_RC(hipEventCreate(&e) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "event"));
-    const size_t bc = (size_t)81 * pp->lpad * s.r;
+    const size_t bc = bytes_size((uint32_t)s.r, pp->kb_c);
     TRY_RC(dalloc(&pr->d_bytes_c, bc));
     TRY_RC(hipMemset(pr->d_bytes_c, 0, bc) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "memset"));
     TRY_RC(dalloc(&pr->d_yacc, s.r * limbs * N));
@@ -570,11 +560,6 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     TRY_RC(dalloc(&pr->d_eq, 3 * s.column_len));
     TRY_RC(dalloc(&pr->d_eqpart, 3 * 256 * (size_t)((s.rows + 7) / 8)));
     TRY_RC(dalloc(&pr->d_u, 3 * s.r));
-    // split K when there are too few tiles x columns to fill the device
-    const size_t wgs = (size_t)pp->ntiles * s.r;
-    uint32_t splits = 1;
-    while (wgs * splits < 2048 && pp->kb_c / (splits * 2) >= 4) splits *= 2;
-    pr->splits_c = splits;
 
     // warm-up (scheme/prover.rs:39-49): a zero witness through every kernel once
     {
@@ -747,11 +732,11 @@ int lbn_debug_tables(const lbn_params *params, const uint8_t seed[32], int which
     switch (which) {
         case 0: src = H.afrag.data(); n = H.afrag.size(); break;
         case 1: src = H.lut.data(); n = H.lut.size() * 2; break;
-        case 2: src = H.epi.data(); n = H.epi.size() * 2; break;
+        case 2: src = H.ee.data(); n = H.ee.size() * 2; break;
         case 3: src = H.out.data(); n = H.out.size() * 2; break;
         case 4: src = H.desc.data(); n = H.desc.size() * sizeof(TileDesc); break;
         case 5: src = H.lut_f.data(); n = H.lut_f.size() * 2; break;
-        case 6: src = H.epi_f.data(); n = H.epi_f.size() * 2; break;
+        case 6: src = H.ee_f.data(); n = H.ee_f.size() * 2; break;
         case 7: src = H.out_f.data(); n = H.out_f.size() * 2; break;
         case 8: src = H.expo_f.data(); n = H.expo_f.size() * 2; break;
         case 9: src = H.desc_f.data(); n = H.desc_f.size() * sizeof(TileDesc); break;

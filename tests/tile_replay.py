@@ -8,8 +8,8 @@ import numpy as np
 N = 648
 TILE, KSTEP, NBLK, FRAG = 96, 64, 3, 1024
 A_STEP = 2 * 2 * NBLK * FRAG
-EPI_TILE = NBLK * NBLK * 64 * 16 * 3
-DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("epi_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
+EE_TILE = 81 * TILE
+DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("ee_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
                  ("par", "<u4"), ("y_off", "<u4")])
 
 
@@ -53,17 +53,9 @@ def a_matrix(afrag, kbs):
     return A
 
 
-def epi_matrix(epi):
-    """[mb][nb][lane][r][3] -> E[3][m][n]: register r of lane holds row
-    32 mb + 8 (r >> 2) + 4 (lane >> 5) + (r & 3), column 32 nb + (lane & 31)"""
-    e = epi.reshape(NBLK, NBLK, 64, 16, 3).astype(np.int64)
-    E = np.zeros((3, TILE, TILE), np.int64)
-    for lane in range(64):
-        for r in range(16):
-            for mb in range(NBLK):
-                for nb in range(NBLK):
-                    E[:, 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), 32 * nb + (lane & 31)] = e[mb, nb, lane, r]
-    return E
+def ee_matrix(ee):
+    """[n 81][m 96] -> E[m][n] (mod q)"""
+    return ee.reshape(81, TILE).astype(np.int64).T.copy()
 
 
 def x_digits(bytes_kn, lut, par):
@@ -78,11 +70,10 @@ def x_digits(bytes_kn, lut, par):
 
 
 def tile_rows(A, X, E, q):
-    Zll = A[0].T @ X[0]
-    Zmid = A[0].T @ X[1] + A[1].T @ X[0]
-    Zhh = A[1].T @ X[1]
-    s = (E[0] * Zll + E[1] * Zmid + E[2] * Zhh).sum(axis=1)
-    return s % q
+    """y[m] = sum_n E[m][n] Z[m][n] mod q with Z = A^T X from the digit products"""
+    Z = (A[0].T @ X[0]) + 128 * (A[0].T @ X[1] + A[1].T @ X[0]) + 16384 * (A[1].T @ X[1])
+    Z = Z[:, :81] % q
+    return (E * Z).sum(axis=1) % q
 
 
 def replay_commit(lbn, params, seed, w):
@@ -92,7 +83,7 @@ def replay_commit(lbn, params, seed, w):
     kbs = lpad // KSTEP
     afrag = debug(lbn, p, seed, 0, np.int8)
     lut = debug(lbn, p, seed, 1, np.uint16)
-    epi = debug(lbn, p, seed, 2, np.uint16)
+    ee = debug(lbn, p, seed, 2, np.uint16)
     out = debug(lbn, p, seed, 3, np.int16)
     desc = debug(lbn, p, seed, 4, DESC)
     rb = ring_bytes(w, cols, L)
@@ -100,7 +91,7 @@ def replay_commit(lbn, params, seed, w):
     a_tile = kbs * A_STEP
     for d in desc:
         A = a_matrix(afrag[d["a_off"]:d["a_off"] + a_tile], kbs)
-        E = epi_matrix(epi[d["epi_off"]:d["epi_off"] + EPI_TILE])
+        E = ee_matrix(ee[d["ee_off"]:d["ee_off"] + EE_TILE])
         lt = lut[d["lut_off"]:d["lut_off"] + 512].astype(np.int64)
         o = out[d["out_off"]:d["out_off"] + TILE]
         for c in range(cols):
@@ -124,7 +115,7 @@ def replay_fold(lbn, params, seed, w, challenges):
     rpad = -(-cols // KSTEP) * KSTEP
     kbs = rpad // KSTEP
     lut = debug(lbn, p, seed, 5, np.uint16)
-    epi = debug(lbn, p, seed, 6, np.uint16)
+    ee = debug(lbn, p, seed, 6, np.uint16)
     out = debug(lbn, p, seed, 7, np.int16)
     expo = debug(lbn, p, seed, 8, np.uint16)
     desc = debug(lbn, p, seed, 9, DESC)
@@ -150,7 +141,7 @@ def replay_fold(lbn, params, seed, w, challenges):
                 x = x - q if x > (q - 1) // 2 else x
                 xl = ((x + 64) & 127) - 64
                 A[0, c, m], A[1, c, m] = xl, (x - xl) // 128
-        E = epi_matrix(epi[d["epi_off"]:d["epi_off"] + EPI_TILE])
+        E = ee_matrix(ee[d["ee_off"]:d["ee_off"] + EE_TILE])
         lt = lut[d["lut_off"]:d["lut_off"] + 512].astype(np.int64)
         o = out[d["out_off"]:d["out_off"] + TILE]
         for i in range(L):
