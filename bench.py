@@ -53,6 +53,21 @@ def algorithmic_bytes(params):
     return dict(commit=w + t + a, fold=t + chat + v, eval=w, total=(w + t + a) + (t + chat + v) + w)
 
 
+def profiled_traffic(suite):
+    """HBM bytes per launch of the commit GEMM, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (scaled by a plain-copy calibration); None for other workloads."""
+    if suite != "xl":
+        return None
+    path = os.path.join(ROOT, "profiles", "commit_fold_eval_xl_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch tile_gemm_kernel (commit) ="):
+                return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -220,8 +235,10 @@ def main():
         "hbm_floor_ms": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3, 4),
         "hbm_roofline_frac": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3 / ms, 4),
         "roofline": {"kernel": "tile_gemm_kernel (commit)", "bound": "mfma", "achieved": round(achieved, 1),
-                     "peak": PEAK_INT8_TOPS, "unit": "TOP/s int8", "frac": round(achieved / PEAK_INT8_TOPS, 4),
-                     "traffic": None, "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4)},
+                     "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
+                     "unit_note": "int8 MFMA operations (multiply and add counted separately), 10^12 per second",
+                     "frac": round(achieved / PEAK_INT8_TOPS, 4), "traffic": profiled_traffic(args.suite),
+                     "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4)},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline...")
