@@ -93,6 +93,10 @@ int lbn_params_check(const lbn_params *params);
  * to the current HIP device in the layout the commit kernel streams; refcounted. */
 int lbn_pp_create(const lbn_params *params, const uint8_t matrix_seed[32], lbn_pp **out);
 void lbn_pp_release(lbn_pp *pp);
+/* The same public parameters without any device state: the key and the shape, for a verifier on a
+ * host without a GPU (Verifier::new, scheme/verifier.rs:28-37, needs only these). A prover cannot
+ * be created on it (LBN_ERR_NO_DEVICE). */
+int lbn_pp_create_host(const lbn_params *params, const uint8_t matrix_seed[32], lbn_pp **out);
 /* CommitmentKey::row for every limb, key.rs:139-141 ([limb][i][648] int16, centered) */
 int lbn_pp_key(const lbn_pp *pp, int16_t *a, size_t len);
 
@@ -171,6 +175,27 @@ int lbn_verify_evaluation(const lbn_pp *pp, const lbn_f162 *p1, size_t p1_len,
 int lbn_verify_opening_clear(const lbn_pp *pp, const int16_t *folded_commitment, const int16_t *v,
                              size_t len_ring, const lbn_f162 *p0, size_t p0_len,
                              const lbn_f162 *folded_row_value);
+
+/* Bit-dropped commitments (Opening::BitDropped, scheme/params.rs:73,231-237), on the host.
+ * top is [c * 648 + i] u16, digits is [limb - 1][c * 648 + i] u16 (bd/mod.rs:7-14). */
+/* bd/digits.rs:121-182 (drop_bits, called by Prover::commit, scheme/prover.rs:69-72): from the raw
+ * rows ([limb][c][648] u32 as lbn_commit returns them) */
+int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, uint16_t *digits);
+/* wire/mod.rs:386-397 (pack_dropped); with out == NULL only *len is set */
+int lbn_pack_dropped(const lbn_params *params, const uint16_t *top, const uint16_t *digits, uint8_t *out,
+                     size_t cap, size_t *len);
+/* verifier.rs:61-100, the CommitmentValue::Dropped arm */
+int lbn_derive_evaluation_point_dropped(const lbn_pp *pp, lbn_transcript *t, const uint16_t *top,
+                                        const uint16_t *digits, lbn_f162 *p0, lbn_f162 *p1);
+/* bd/residual.rs:77-130: the squared CRT norm of A v - sum_j c_j col_j (as two u64 halves) */
+int lbn_bd_residual(const lbn_pp *pp, const uint16_t *top, const uint16_t *digits, const lbn_challenge *c,
+                    size_t r, const int16_t *v, size_t len_ring, uint64_t *normsq_lo, uint64_t *normsq_hi);
+/* Params::bd_cap, scheme/params.rs:250-252 and bd/mod.rs:71-74 (0 if the parameters drop no bits) */
+uint64_t lbn_bd_cap(const lbn_params *params);
+/* verifier.rs:232-275 (verify_folded_opening_bd): LBN_OK or LBN_REJECTED */
+int lbn_verify_opening_bit_dropped(const lbn_pp *pp, const uint16_t *top, const uint16_t *digits,
+                                   const lbn_challenge *c, size_t r, const int16_t *v, size_t len_ring,
+                                   const lbn_f162 *p0, size_t p0_len, const lbn_f162 *folded_row_value);
 
 /* BLAKE3 as the transcript and the seed expansions use it: mode 0 hash, 1 keyed (key = 32 bytes),
  * 2 derive_key (key = NUL-terminated context). */

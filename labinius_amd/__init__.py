@@ -127,6 +127,7 @@ def lib():
                                           ctypes.c_size_t]
         L.lbn_sample_short_challenge.restype = ctypes.c_uint64
         L.lbn_sample_short_challenge.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.c_void_p]
+        L.lbn_bd_cap.restype = ctypes.c_uint64
         L.lbn_dwitness_device_ptr.restype = ctypes.c_void_p
         L.lbn_dwitness_device_ptr.argtypes = [ctypes.c_void_p]
         for name in ("lbn_pp_release", "lbn_prover_release", "lbn_dwitness_release"):
@@ -270,6 +271,15 @@ class PublicParameters:
         _check(lib().lbn_pp_create(ctypes.byref(params.c()), seed, ctypes.byref(h)))
         return cls(params, matrix_seed, h.value)
 
+    @classmethod
+    def host_only(cls, params, matrix_seed):
+        """The same parameters without device state (the key and the shape): enough for a Verifier
+        on a host without a GPU; a Prover cannot be built on them."""
+        h = ctypes.c_void_p()
+        seed = (ctypes.c_uint8 * 32).from_buffer_copy(bytes(matrix_seed))
+        _check(lib().lbn_pp_create_host(ctypes.byref(params.c()), seed, ctypes.byref(h)))
+        return cls(params, matrix_seed, h.value)
+
     def key(self):
         """CommitmentKey::row for every limb: [limb][len_ring][648] int16, centered."""
         p = self.params
@@ -343,14 +353,55 @@ class ResidentWitness:
             self._h = None
 
 
-class Commitment:
-    """scheme/commitment.rs:24-118: the 4 x columns matrix per modulus, plus the raw rows."""
+class Dropped:
+    """bd/mod.rs:7-56: per coefficient of every column the base residue's top bits and the mixed-radix
+    digits over the other primes."""
 
-    def __init__(self, primes, columns, components, raw):
+    def __init__(self, params, top, digits):
+        self.params = params
+        self.top = np.ascontiguousarray(top, np.uint16)        # [columns * 648]
+        self.digits = np.ascontiguousarray(digits, np.uint16)  # [limbs - 1][columns * 648]
+
+    @classmethod
+    def of_raw(cls, params, raw):
+        """bd/digits.rs:121-182 from the raw rows [limb][column][648]."""
+        count = params.columns() * N
+        top = np.zeros(count, np.uint16)
+        digits = np.zeros((len(params.primes()) - 1, count), np.uint16)
+        raw = np.ascontiguousarray(raw, np.uint32)
+        _check(lib().lbn_drop_bits(ctypes.byref(params.c()), _p(raw), _p(top), _p(digits)))
+        return cls(params, top, digits)
+
+    def dropped_bits(self):
+        return self.params.dropped_bits()
+
+    def pack(self):
+        """wire::pack_dropped (wire/mod.rs:386-397)."""
+        n = ctypes.c_size_t()
+        _check(lib().lbn_pack_dropped(ctypes.byref(self.params.c()), _p(self.top), _p(self.digits), None,
+                                      ctypes.c_size_t(0), ctypes.byref(n)))
+        out = np.zeros(max(1, n.value), np.uint8)
+        _check(lib().lbn_pack_dropped(ctypes.byref(self.params.c()), _p(self.top), _p(self.digits), _p(out),
+                                      ctypes.c_size_t(out.size), None))
+        return bytes(out[:n.value])
+
+    def wire_bytes(self):
+        n = ctypes.c_size_t()
+        _check(lib().lbn_pack_dropped(ctypes.byref(self.params.c()), _p(self.top), _p(self.digits), None,
+                                      ctypes.c_size_t(0), ctypes.byref(n)))
+        return n.value
+
+
+class Commitment:
+    """scheme/commitment.rs:24-118: the 4 x columns matrix per modulus, plus the raw rows; in the
+    bit-dropped mode the value is the Dropped form (scheme/prover.rs:69-72)."""
+
+    def __init__(self, primes, columns, components, raw, dropped=None):
         self.primes = primes
         self.columns_ = columns
         self.components = components  # [column][row][limb][162] int16
         self.raw = raw                # [limb][column][648] uint32
+        self.dropped = dropped        # Dropped or None
 
     def rows(self):
         return 4
@@ -452,7 +503,12 @@ class Prover:
         e = np.ascontiguousarray(witness.elements, np.uint64)
         _check(lib().lbn_commit(ctypes.c_void_p(self._h), _p(e), ctypes.c_size_t(len(e)), _p(comp), _p(raw),
                                 ctypes.byref(op)))
-        return Commitment(self.params.primes(), self.params.columns(), comp, raw), CommitmentOpening(self, op.value)
+        return self._commitment(comp, raw), CommitmentOpening(self, op.value)
+
+    def _commitment(self, comp, raw):
+        p = self.params
+        dropped = Dropped.of_raw(p, raw) if p.dropped_bits() > 0 else None
+        return Commitment(p.primes(), p.columns(), comp, raw, dropped)
 
     def commit_resident(self, rw, want_outputs=True):
         comp, raw = self._buffers()
@@ -460,7 +516,9 @@ class Prover:
         _check(lib().lbn_commit_resident(ctypes.c_void_p(self._h), ctypes.c_void_p(rw._h),
                                          _p(comp) if want_outputs else None, _p(raw) if want_outputs else None,
                                          ctypes.byref(op)))
-        return Commitment(self.params.primes(), self.params.columns(), comp, raw), CommitmentOpening(self, op.value)
+        if not want_outputs:
+            return Commitment(self.params.primes(), self.params.columns(), comp, raw), CommitmentOpening(self, op.value)
+        return self._commitment(comp, raw), CommitmentOpening(self, op.value)
 
     def fold(self, opening, challenges, want_output=True):
         p = self.params
@@ -554,7 +612,7 @@ class Transcript:
 
 
 class Verifier:
-    """scheme/verifier.rs:21-337 (Clear mode)."""
+    """scheme/verifier.rs:21-337 (the Clear and bit-dropped modes)."""
 
     def __init__(self, pp):
         self.pp = pp
@@ -564,6 +622,12 @@ class Verifier:
         p = self.params
         p0 = np.zeros((p.row_log_len(), 3), np.uint64)
         p1 = np.zeros((p.column_log_len, 3), np.uint64)
+        if p.dropped_bits() > 0:
+            d = commitment.dropped
+            _check(lib().lbn_derive_evaluation_point_dropped(ctypes.c_void_p(self.pp._h),
+                                                             ctypes.c_void_p(transcript._h), _p(d.top),
+                                                             _p(d.digits), _p(p0), _p(p1)))
+            return EvaluationPoint(p0, p1)
         comp = np.ascontiguousarray(commitment.components, np.int16)
         _check(lib().lbn_derive_evaluation_point(ctypes.c_void_p(self.pp._h), ctypes.c_void_p(transcript._h),
                                                  _p(comp), _p(p0), _p(p1)))
@@ -602,7 +666,22 @@ class Verifier:
             raise VerificationError("Rejected")
         _check(rc)
 
+    def bd_residual(self, commitment, challenges, folded_witness):
+        """bd/residual.rs:77-130 as a Python int."""
+        d = commitment.dropped
+        v = np.ascontiguousarray(folded_witness.elements, np.int16)
+        arr = challenges.array()
+        lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().lbn_bd_residual(ctypes.c_void_p(self.pp._h), _p(d.top), _p(d.digits), arr,
+                                     ctypes.c_size_t(len(arr)), _p(v), ctypes.c_size_t(len(v)), ctypes.byref(lo),
+                                     ctypes.byref(hi)))
+        return lo.value | (hi.value << 64)
+
     def verify_opening(self, commitment, challenges, point, folded_commitment, folded_witness, folded_row_value):
+        """Verifier::verify_opening (verifier.rs:281-337): the Clear message carries the folded
+        commitment; the bit-dropped one does not (pass None)."""
+        if self.params.dropped_bits() > 0:
+            return self._verify_opening_bd(commitment, challenges, point, folded_witness, folded_row_value)
         fc = np.ascontiguousarray(folded_commitment.rows, np.int16)
         v = np.ascontiguousarray(folded_witness.elements, np.int16)
         p0 = np.ascontiguousarray(point.p0, np.uint64)
@@ -614,6 +693,27 @@ class Verifier:
         if rc == LBN_REJECTED:
             raise VerificationError("Rejected")
         _check(rc)
+
+
+    def _verify_opening_bd(self, commitment, challenges, point, folded_witness, folded_row_value):
+        d = commitment.dropped
+        if d is None:
+            raise VerificationError("Rejected")
+        v = np.ascontiguousarray(folded_witness.elements, np.int16)
+        p0 = np.ascontiguousarray(point.p0, np.uint64)
+        fr = np.ascontiguousarray(folded_row_value, np.uint64)
+        arr = challenges.array()
+        rc = lib().lbn_verify_opening_bit_dropped(ctypes.c_void_p(self.pp._h), _p(d.top), _p(d.digits), arr,
+                                                  ctypes.c_size_t(len(arr)), _p(v), ctypes.c_size_t(len(v)),
+                                                  _p(p0), ctypes.c_size_t(len(p0)), _p(fr))
+        if rc == LBN_REJECTED:
+            raise VerificationError("Rejected")
+        _check(rc)
+
+
+def bd_cap(params):
+    """Params::bd_cap (scheme/params.rs:250-252)."""
+    return int(lib().lbn_bd_cap(ctypes.byref(params.c())))
 
 
 def blake3(data, out_len=32, key=None, context=None):

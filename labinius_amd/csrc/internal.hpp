@@ -68,6 +68,14 @@ uint64_t sample_short_challenge(Transcript &t, uint32_t weight, double bound, lb
 
 }  // namespace lbn
 
+struct lbn_pp;
+namespace lbn {
+// verifier.rs:41-57
+void absorb_parameters(Transcript &t, const lbn_pp *pp);
+// verifier.rs:85-99
+void derive_point(Transcript &t, const Shape &s, lbn_f162 *p0, lbn_f162 *p1);
+}  // namespace lbn
+
 struct lbn_transcript {
     lbn::Transcript t;
 };
@@ -80,6 +88,7 @@ struct lbn_pp {
     uint8_t seed[32];
     std::vector<int16_t> key;  // [limb][len_ring][648], centered
     int device = 0;
+    bool host_only = false;  // the key and shape only (a verifier on a host without a GPU)
     uint32_t lpad = 0, rpad = 0, kb_c = 0, kb_f = 0;
     // commit GEMM
     uint32_t ntiles = 0;

@@ -200,7 +200,21 @@ uint64_t sample_short_challenge(Transcript &t, uint32_t weight, double bound, lb
 // the verifier's pieces
 // ---------------------------------------------------------------------------------------------
 
-static void absorb_parameters(Transcript &t, const lbn_pp *pp) {
+// verifier.rs:85-99: one F162 per variable from one read, limb 2 masked to 34 bits
+void derive_point(Transcript &t, const Shape &s, lbn_f162 *p0, lbn_f162 *p1) {
+    const size_t nv = s.wl;
+    std::vector<uint8_t> bytes(24 * nv);
+    t.fill("labinius/evaluation-point", 25, bytes.data(), bytes.size());
+    for (size_t e = 0; e < nv; e++) {
+        lbn_f162 x;
+        std::memcpy(x.w, bytes.data() + 24 * e, 24);
+        x.w[2] &= (1ull << 34) - 1;
+        if (e < s.rows) p0[e] = x;
+        else p1[e - s.rows] = x;
+    }
+}
+
+void absorb_parameters(Transcript &t, const lbn_pp *pp) {
     const Shape &s = pp->shape;
     t.absorb_bytes("labinius/parameters", 19);
     t.absorb_u64(s.wl);
@@ -279,7 +293,7 @@ int lbn_derive_evaluation_point(const lbn_pp *pp, lbn_transcript *tr, const int1
     if (!pp || !tr || !comp) return fail(LBN_ERR_ARG, "null argument");
     const Shape &s = pp->shape;
     if (s.dropped_bits > 0)
-        return fail(LBN_ERR_UNSUPPORTED, "bit-dropped commitments are not produced by this build");
+        return fail(LBN_ERR_SHAPE, "a bit-dropped key commits to (top, digits): lbn_derive_evaluation_point_dropped");
     Transcript &t = tr->t;
     absorb_parameters(t, pp);
     t.absorb_bytes("labinius/commitment", 19);
@@ -293,16 +307,7 @@ int lbn_derive_evaluation_point(const lbn_pp *pp, lbn_transcript *tr, const int1
             t.state.update(buf.data(), per);
         }
     }
-    const size_t nv = s.wl;
-    std::vector<uint8_t> bytes(24 * nv);
-    t.fill("labinius/evaluation-point", 25, bytes.data(), bytes.size());
-    for (size_t e = 0; e < nv; e++) {
-        lbn_f162 x;
-        std::memcpy(x.w, bytes.data() + 24 * e, 24);
-        x.w[2] &= (1ull << 34) - 1;
-        if (e < s.rows) p0[e] = x;
-        else p1[e - s.rows] = x;
-    }
+    derive_point(t, s, p0, p1);
     return LBN_OK;
 }
 

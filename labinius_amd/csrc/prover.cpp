@@ -513,6 +513,25 @@ int lbn_pp_create(const lbn_params *params, const uint8_t matrix_seed[32], lbn_p
     return LBN_OK;
 }
 
+int lbn_pp_create_host(const lbn_params *params, const uint8_t matrix_seed[32], lbn_pp **out) {
+    if (!out || !matrix_seed) return fail(LBN_ERR_ARG, "null argument");
+    *out = nullptr;
+    Shape s;
+    int rc = make_shape(params, &s);
+    if (rc) return rc;
+    if (s.opening == LBN_OPENING_RECURSIVE)
+        return fail(LBN_ERR_UNSUPPORTED, "Opening::Recursive (LaBRADOR) is outside this prover");
+    lbn_pp *pp = new lbn_pp();
+    pp->params = *params;
+    pp->shape = s;
+    std::memcpy(pp->seed, matrix_seed, 32);
+    pp->key = expand_key(s.primes, s.len_ring, matrix_seed);
+    pp->device = -1;
+    pp->host_only = true;
+    *out = pp;
+    return LBN_OK;
+}
+
 void lbn_pp_release(lbn_pp *pp) {
     if (pp && --pp->refs == 0) pp_free(pp);
 }
@@ -527,6 +546,7 @@ int lbn_pp_key(const lbn_pp *pp, int16_t *a, size_t len) {
 int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     if (!pp || !out) return fail(LBN_ERR_ARG, "null argument");
     *out = nullptr;
+    if (pp->host_only) return fail(LBN_ERR_NO_DEVICE, "public parameters created without a device");
     (void)hipSetDevice(pp->device);
     lbn_prover *pr = new lbn_prover();
     pr->pp = pp;

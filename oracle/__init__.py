@@ -60,6 +60,11 @@ def lib():
         _lib.orc_canonical_norm_sq.restype = ctypes.c_double
         _lib.orc_verify_folded_opening.restype = ctypes.c_int
         _lib.orc_is_quad.restype = ctypes.c_int
+        _lib.orc_residue_bits.restype = ctypes.c_uint32
+        _lib.orc_top_bits.restype = ctypes.c_uint32
+        _lib.orc_bd_cap.restype = ctypes.c_uint64
+        _lib.orc_pack_dropped.restype = ctypes.c_size_t
+        _lib.orc_verify_opening_bd.restype = ctypes.c_int
     return _lib
 
 
@@ -336,3 +341,71 @@ def verify_folded_opening(params, a, folded_comm, v, p0, u_folded):
     p0 = np.ascontiguousarray(p0, np.uint64)
     uf = np.ascontiguousarray(u_folded, np.uint64)
     return bool(lib().orc_verify_folded_opening(ctypes.byref(params), _p(a), _p(fc), _p(v), _p(p0), _p(uf)))
+
+
+# ---- bit-dropped commitments (bd/, wire/mod.rs:386-397, verifier.rs:232-275) ----
+
+def residue_bits(q):
+    return int(lib().orc_residue_bits(ctypes.c_uint32(q)))
+
+
+def top_bits(q0, d):
+    return int(lib().orc_top_bits(ctypes.c_uint32(q0), ctypes.c_uint32(d)))
+
+
+def bd_cap(cols, d):
+    return int(lib().orc_bd_cap(ctypes.c_uint32(cols), ctypes.c_uint32(d)))
+
+
+def drop_bits(primes, cols, raw, d):
+    """raw [limb][c][648] -> top [c*648] u16, digits [limb-1][c*648] u16"""
+    pr = np.array(primes, np.uint32)
+    raw = np.ascontiguousarray(raw, np.uint32)
+    top = np.zeros(cols * N, np.uint16)
+    digits = np.zeros((len(primes) - 1, cols * N), np.uint16)
+    lib().orc_drop_bits(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols), _p(raw), ctypes.c_uint32(d),
+                        _p(top), _p(digits))
+    return top, digits
+
+
+def pack_dropped(primes, cols, d, top, digits):
+    pr = np.array(primes, np.uint32)
+    top = np.ascontiguousarray(top, np.uint16)
+    digits = np.ascontiguousarray(digits, np.uint16)
+    bits = top_bits(primes[0], d) + sum(residue_bits(q) for q in primes[1:])
+    out = np.zeros((cols * N * bits + 7) // 8 + 8, np.uint8)
+    n = lib().orc_pack_dropped(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols), ctypes.c_uint32(d), _p(top),
+                               _p(digits), _p(out))
+    return bytes(out[:n])
+
+
+def bd_residual(primes, len_ring, cols, a, d, top, digits, chs, v):
+    pr = np.array(primes, np.uint32)
+    a = np.ascontiguousarray(a, np.int16)
+    top = np.ascontiguousarray(top, np.uint16)
+    digits = np.ascontiguousarray(digits, np.uint16)
+    v = np.ascontiguousarray(v, np.int16)
+    lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().orc_bd_residual(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(len_ring), ctypes.c_int(cols), _p(a),
+                          ctypes.c_uint32(d), _p(top), _p(digits), challenges_array(chs), _p(v), ctypes.byref(lo),
+                          ctypes.byref(hi))
+    return lo.value | (hi.value << 64)
+
+
+def derive_evaluation_point_dropped(t, params, packed):
+    pk = np.frombuffer(bytes(packed), np.uint8).copy()
+    pt = np.zeros((params.witness_log_len, 3), np.uint64)
+    lib().orc_derive_evaluation_point_dropped(t.buf, ctypes.byref(params), _p(pk), ctypes.c_size_t(len(pk)), _p(pt))
+    rows = params.witness_log_len - params.column_log_len
+    return pt[:rows].copy(), pt[rows:].copy()
+
+
+def verify_opening_bd(params, a, top, digits, chs, v, p0, u_folded):
+    a = np.ascontiguousarray(a, np.int16)
+    top = np.ascontiguousarray(top, np.uint16)
+    digits = np.ascontiguousarray(digits, np.uint16)
+    v = np.ascontiguousarray(v, np.int16)
+    p0 = np.ascontiguousarray(p0, np.uint64)
+    uf = np.ascontiguousarray(u_folded, np.uint64)
+    return bool(lib().orc_verify_opening_bd(ctypes.byref(params), _p(a), _p(top), _p(digits), challenges_array(chs),
+                                            _p(v), _p(p0), _p(uf)))

@@ -1399,3 +1399,227 @@ int orc_verify_folded_opening(const orc_params *p, const int16_t *a, const int16
     free(eq);
     return acc[0] == u_folded[0] && acc[1] == u_folded[1] && acc[2] == u_folded[2];
 }
+
+/* ============================================================================================
+ * bit-dropped commitments: bd/mod.rs:58-77, bd/digits.rs:121-182, simd/bd.rs:5-37,
+ * wire/mod.rs:77-79,386-397, bd/residual.rs:77-130, bd/digits.rs (Garner::normsq),
+ * verifier.rs:61-100 (Dropped arm), verifier.rs:232-275
+ * ========================================================================================== */
+
+/* wire/mod.rs:77-79 */
+uint32_t orc_residue_bits(uint32_t q) { return 32 - (uint32_t)__builtin_clz(q - 1); }
+
+/* bd/mod.rs:58-64 */
+uint32_t orc_top_bits(uint32_t q0, uint32_t d) {
+    uint32_t bound = ((q0 - 1) + (1u << (d - 1))) >> d;
+    return 32 - (uint32_t)__builtin_clz(bound);
+}
+
+/* bd/mod.rs:71-74, evaluated in f64 in the same order */
+uint64_t orc_bd_cap(uint32_t cols, uint32_t d) {
+    double spread = (double)((1ull << (2 * d)) - 1) / 12.0;
+    return (uint64_t)ceil(4.0 * (double)((uint64_t)N * cols * 28) * spread);
+}
+
+/* bd/digits.rs:121-182: per limb the column's coefficients (the inverse transform of its raw row,
+ * any representative mod q), then Garner's mixed radix over the primes: digits[k-1][i] = x_k, and
+ * the base residue rounded to its top bits. raw is [k][c][648]; top and each digit plane are
+ * [c * 648 + i]. */
+void orc_drop_bits(const uint32_t *primes, int limbs, int cols, const uint32_t *raw, uint32_t d,
+                   uint16_t *top, uint16_t *digits) {
+    init_tables();
+    size_t count = (size_t)cols * N;
+    uint32_t *res = (uint32_t *)malloc(sizeof(uint32_t) * count * limbs);
+    for (int k = 0; k < limbs; k++)
+        for (int c = 0; c < cols; c++)
+            orc_intt(primes[k], raw + ((size_t)k * cols + c) * N, res + (size_t)k * count + (size_t)c * N);
+    uint32_t round = 1u << (d - 1);
+    for (size_t i = 0; i < count; i++) {
+        uint64_t mixed[8];
+        uint32_t base = res[i] % primes[0];
+        top[i] = (uint16_t)((base + round) >> d);
+        mixed[0] = base;
+        for (int k = 1; k < limbs; k++) {
+            uint64_t q = primes[k];
+            uint64_t x = res[(size_t)k * count + i] % q;
+            for (int m = 0; m < k; m++) x = (x + q - mixed[m] % q) % q * inv_mod(primes[m] % q, q) % q;
+            mixed[k] = x;
+            digits[(size_t)(k - 1) * count + i] = (uint16_t)x;
+        }
+    }
+    free(res);
+}
+
+/* wire/mod.rs:386-397: per coefficient the top (top_bits) then each digit (residue_bits of its
+ * prime), least significant bit first, the last byte zero-padded. Returns the byte count. */
+size_t orc_pack_dropped(const uint32_t *primes, int limbs, int cols, uint32_t d, const uint16_t *top,
+                        const uint16_t *digits, uint8_t *out) {
+    size_t count = (size_t)cols * N, len = 0;
+    uint64_t acc = 0;
+    uint32_t n = 0, tb = orc_top_bits(primes[0], d);
+    for (size_t i = 0; i < count; i++) {
+        for (int k = 0; k < limbs; k++) {
+            uint32_t bits = k == 0 ? tb : orc_residue_bits(primes[k]);
+            uint64_t value = k == 0 ? top[i] : digits[(size_t)(k - 1) * count + i];
+            acc |= (value & ((1ull << bits) - 1)) << n;
+            n += bits;
+            while (n >= 8) {
+                out[len++] = (uint8_t)acc;
+                acc >>= 8;
+                n -= 8;
+            }
+        }
+    }
+    if (n > 0) out[len++] = (uint8_t)acc;
+    return len;
+}
+
+/* simd/bd.rs:5-37: (top << d) + q0 (x_1 + q1 (x_2 + ...)) mod q, centered */
+static int32_t bd_combine(const uint32_t *primes, int limbs, uint32_t d, const uint16_t *top,
+                          const uint16_t *digits, size_t count, size_t i, uint32_t q) {
+    uint64_t inner = digits[(size_t)(limbs - 2) * count + i];
+    for (int m = limbs - 3; m >= 0; m--) inner = (digits[(size_t)m * count + i] + inner * primes[m + 1]) % q;
+    uint64_t r = (((uint64_t)top[i] << d) + inner * primes[0]) % q;
+    return r > (q - 1) / 2 ? (int32_t)r - (int32_t)q : (int32_t)r;
+}
+
+/* slot-wise product of two transforms mod q: a product per slot (split) or per leaf (quadratic) */
+static void mul_slots(const prime_tab *t, const uint64_t *a, const uint64_t *b, uint64_t *c) {
+    if (t->quad)
+        mul_quad_slots(t, a, b, c);
+    else
+        for (int u = 0; u < N; u++) c[u] = a[u] * b[u] % t->q;
+}
+
+static void forward_mod(const prime_tab *t, const int64_t *x, uint64_t *v) {
+    for (int u = 0; u < N; u++) v[u] = (uint64_t)((x[u] % (int64_t)t->q + (int64_t)t->q) % (int64_t)t->q);
+    if (t->quad)
+        ntt_quad(t, v);
+    else
+        ntt_split(t, v);
+}
+
+/* bd/residual.rs:77-130 with Garner::normsq (bd/digits.rs): per limb k,
+ * z_k = A_k v - sum_j c_j(-X^4) col_j mod q_k in coefficients, col_j rebuilt from (top, digits);
+ * then sum over the 648 coefficients of the square of the CRT value centered mod prod q.
+ * Returns the norm as two u64 halves. */
+void orc_bd_residual(const uint32_t *primes, int limbs, int len_ring, int cols, const int16_t *a,
+                     uint32_t d, const uint16_t *top, const uint16_t *digits, const orc_challenge *ch,
+                     const int16_t *v, uint64_t *normsq_lo, uint64_t *normsq_hi) {
+    init_tables();
+    size_t count = (size_t)cols * N;
+    uint32_t (*cen)[N] = (uint32_t(*)[N])malloc(sizeof(uint32_t) * N * limbs);
+    for (int k = 0; k < limbs; k++) {
+        const prime_tab *t = prime(primes[k]);
+        uint64_t q = t->q, fc[N], y[N], pr[N], x[N], cv[N];
+        int64_t tmp[N];
+        memset(fc, 0, sizeof fc);
+        memset(y, 0, sizeof y);
+        for (int j = 0; j < cols; j++) {
+            int32_t e[N];
+            orc_embed(&ch[j], e);
+            for (int u = 0; u < N; u++) tmp[u] = e[u];
+            forward_mod(t, tmp, cv);
+            for (int u = 0; u < N; u++)
+                tmp[u] = bd_combine(primes, limbs, d, top, digits, count, (size_t)j * N + u, (uint32_t)q);
+            forward_mod(t, tmp, x);
+            mul_slots(t, cv, x, pr);
+            for (int u = 0; u < N; u++) fc[u] = (fc[u] + pr[u]) % q;
+        }
+        for (int i = 0; i < len_ring; i++) {
+            uint64_t av[N];
+            for (int u = 0; u < N; u++) {
+                tmp[u] = v[(size_t)i * N + u];
+                av[u] = (uint64_t)((a[((size_t)k * len_ring + i) * N + u] % (int64_t)q + (int64_t)q) % (int64_t)q);
+            }
+            forward_mod(t, tmp, x);
+            mul_slots(t, av, x, pr);
+            for (int u = 0; u < N; u++) y[u] = (y[u] + pr[u]) % q;
+        }
+        uint32_t z[N];
+        for (int u = 0; u < N; u++) z[u] = (uint32_t)((y[u] + q - fc[u]) % q);
+        orc_intt(primes[k], z, cen[k]);
+    }
+    unsigned __int128 modulus = 1, normsq = 0;
+    for (int k = 0; k < limbs; k++) modulus *= primes[k];
+    for (int i = 0; i < N; i++) {
+        uint64_t mixed[8];
+        for (int k = 0; k < limbs; k++) {
+            uint64_t q = primes[k], x = cen[k][i] % q;
+            for (int m = 0; m < k; m++) x = (x + q - mixed[m] % q) % q * inv_mod(primes[m] % q, q) % q;
+            mixed[k] = x;
+        }
+        unsigned __int128 value = 0;
+        for (int k = limbs - 1; k >= 0; k--) value = value * primes[k] + mixed[k];
+        unsigned __int128 mag = value > modulus / 2 ? modulus - value : value;
+        normsq += mag * mag;
+    }
+    free(cen);
+    *normsq_lo = (uint64_t)normsq;
+    *normsq_hi = (uint64_t)(normsq >> 64);
+}
+
+/* verifier.rs:61-100, the Dropped arm */
+void orc_derive_evaluation_point_dropped(orc_transcript *t, const orc_params *p, const uint8_t *packed,
+                                         size_t packed_len, uint64_t *point) {
+    absorb_parameters(t, p);
+    orc_absorb_bytes(t, (const uint8_t *)"labinius/commitment", 19);
+    orc_absorb_u64(t, (uint64_t)1 << p->column_log_len);
+    orc_absorb_bytes(t, (const uint8_t *)"labinius/dropped-commitment", 27);
+    orc_absorb_bytes(t, packed, packed_len);
+    int nv = (int)(p->witness_log_len);
+    uint8_t *bytes = (uint8_t *)malloc(24 * (size_t)nv);
+    orc_fill(t, "labinius/evaluation-point", bytes, 24 * (size_t)nv);
+    for (int e = 0; e < nv; e++) {
+        for (int k = 0; k < 3; k++) {
+            uint64_t x = 0;
+            for (int b = 7; b >= 0; b--) x = x << 8 | bytes[24 * e + 8 * k + b];
+            point[3 * e + k] = x;
+        }
+        point[3 * e + 2] &= (1ull << 34) - 1;
+    }
+    free(bytes);
+}
+
+/* verifier.rs:232-275: the norm checks on v, the residual against bd_cap, the binary check.
+ * Returns 1 on acceptance. */
+int orc_verify_opening_bd(const orc_params *p, const int16_t *a, const uint16_t *top, const uint16_t *digits,
+                          const orc_challenge *ch, const int16_t *v, const uint64_t *p0,
+                          const uint64_t *u_folded) {
+    init_tables();
+    int limbs = (int)p->n_primes;
+    int rows = (int)(p->witness_log_len - p->column_log_len);
+    int len_ring = (1 << rows) / 4;
+    int cols = 1 << p->column_log_len;
+    int32_t half = (int32_t)((p->primes[0] - 1) / 2);
+    uint64_t normsq = 0;
+    int32_t worst = 0;
+    for (size_t i = 0; i < (size_t)len_ring * N; i++) {
+        int32_t x = v[i];
+        int32_t ax = x < 0 ? -x : x;
+        if (ax > worst) worst = ax;
+        normsq += (uint64_t)((int64_t)x * x);
+    }
+    double capd = ceil(108.0 * (double)(((size_t)1 << p->witness_log_len) / 4 * N));
+    if (worst > half || normsq > (uint64_t)capd) return 0;
+    uint64_t lo, hi;
+    orc_bd_residual(p->primes, limbs, len_ring, cols, a, p->dropped_bits, top, digits, ch, v, &lo, &hi);
+    if (hi != 0 || lo > orc_bd_cap((uint32_t)cols, p->dropped_bits)) return 0;
+    size_t wdim = (size_t)1 << rows;
+    uint64_t *eq = (uint64_t *)malloc(sizeof(uint64_t) * 3 * wdim);
+    orc_eq_table(p0, rows, eq);
+    uint64_t acc[3] = {0, 0, 0};
+    for (int m = 0; m < len_ring; m++)
+        for (int k = 0; k < 4; k++) {
+            uint64_t e[3] = {0, 0, 0};
+            for (int b = 0; b < 162; b++)
+                if (v[(size_t)m * N + 4 * b + k] & 1) e[b / 64] |= 1ull << (b % 64);
+            uint64_t pr[3];
+            orc_f162_mul(eq + 3 * (4 * (size_t)m + k), e, pr);
+            acc[0] ^= pr[0];
+            acc[1] ^= pr[1];
+            acc[2] ^= pr[2];
+        }
+    free(eq);
+    return acc[0] == u_folded[0] && acc[1] == u_folded[1] && acc[2] == u_folded[2];
+}

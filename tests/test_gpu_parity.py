@@ -9,6 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+N = 648
 MATRIX_SEED = bytes([0x5A]) * 32
 WITNESS_SEED = bytes([0xC7]) * 32
 
@@ -187,3 +188,76 @@ def test_row_evaluation_many_rows(lbn, orc, wl, cl):
     row = prover.row_evaluate(w, point)
     assert np.array_equal(row.values, orc.row_evaluate(w.elements, point.p0))
     assert np.array_equal(prover.mle_evaluate(w, point), orc.claim(row.values, point.p1))
+
+
+@pytest.mark.parametrize("wl,cl,extra,d", [(12, 3, [2917, 4861], 8), (11, 2, [9721], 10), (10, 2, [2917, 12637], 7)])
+def test_bit_dropped_round_bit_exact(lbn, orc, wl, cl, extra, d):
+    # scheme/prover.rs:69-72 (drop_bits after the GPU commitment), verifier.rs:61-100 (the Dropped
+    # arm of the point), verifier.rs:232-275 (the residual check)
+    say(f"case: bit-dropped wl={wl} cl={cl} extra={extra} d={d}")
+    params = lbn.Params(wl, cl, [lbn.Modulus(q) for q in extra], opening=lbn.Opening.bit_dropped(d))
+    pp, prover, w, a = _setup(lbn, orc, params)
+    primes, L, cols = params.primes(), params.len_ring(), params.columns()
+    commitment, opening = prover.commit(w)
+    raw, comp = orc.commit(primes, L, cols, a, w.elements)
+    assert np.array_equal(commitment.raw, raw)
+    top, digits = orc.drop_bits(primes, cols, raw, d)
+    assert np.array_equal(commitment.dropped.top, top) and np.array_equal(commitment.dropped.digits, digits)
+    verifier = lbn.Verifier(pp)
+    t = lbn.Transcript(b"labinius/test/bd")
+    point = verifier.derive_evaluation_point(t, commitment)
+    ot = orc.Transcript(b"labinius/test/bd")
+    oparams = orc.make_params(wl, cl, primes, MATRIX_SEED, dropped_bits=d)
+    op0, op1 = orc.derive_evaluation_point_dropped(ot, oparams, orc.pack_dropped(primes, cols, d, top, digits))
+    assert np.array_equal(point.p0, op0) and np.array_equal(point.p1, op1)
+    row = prover.row_evaluate(w, point)
+    assert np.array_equal(row.values, orc.row_evaluate(w.elements, point.p0))
+    challenges = verifier.derive_folding_challenges(t, row)
+    och = orc.derive_folding_challenges(ot, row.values)
+    folded = prover.fold(opening, challenges)
+    assert np.array_equal(folded.elements, orc.fold(primes[0], L, cols, w.elements, och))
+    fr = verifier.fold_row_evaluation(row, challenges)
+    verifier.verify_opening(commitment, challenges, point, None, folded, fr)
+    assert orc.verify_opening_bd(oparams, a, top, digits, och, folded.elements, point.p0, fr)
+    assert verifier.bd_residual(commitment, challenges, folded) == orc.bd_residual(primes, L, cols, a, d, top, digits,
+                                                                                   och, folded.elements)
+
+
+@pytest.mark.parametrize("mode", ["clear", "bit_dropped"])
+def test_suite_l_round(lbn, orc, mode):
+    """Params::sized(SUITES[2], mode): 2^22 F162 in 512 columns, moduli 3889 / 2917 / 4861 (8 dropped
+    bits). Bit-exact against the oracle on sampled columns and rows (each is independent of the
+    others), and the verifier accepts the whole opening."""
+    say(f"case: suite l {mode}")
+    opening_mode = lbn.Opening.CLEAR if mode == "clear" else lbn.Opening.bit_dropped(8)
+    params = lbn.Params.sized(lbn.SUITES[2], opening_mode)
+    pp, prover, w, a = _setup(lbn, orc, params)
+    primes, L, cols, clen = params.primes(), params.len_ring(), params.columns(), params.column_len()
+    rw = prover.upload(w)
+    commitment, opening = prover.commit_resident(rw)
+    sample = [0, 1, 257, cols - 1]
+    for c in sample:
+        raw_c, comp_c = orc.commit(primes, L, 1, a, w.elements[c * clen:(c + 1) * clen])
+        assert np.array_equal(commitment.raw[:, c], raw_c[:, 0]), c
+        assert np.array_equal(commitment.components[c], comp_c[0]), c
+        if mode == "bit_dropped":
+            top, digits = orc.drop_bits(primes, 1, raw_c, 8)
+            assert np.array_equal(commitment.dropped.top[c * N:(c + 1) * N], top)
+            assert np.array_equal(commitment.dropped.digits[:, c * N:(c + 1) * N], digits)
+    verifier = lbn.Verifier(pp)
+    t = lbn.Transcript(b"labinius/reference")
+    point = verifier.derive_evaluation_point(t, commitment)
+    row = prover.row_evaluate(rw, point)
+    for c in sample:
+        assert np.array_equal(row.values[c], orc.row_evaluate(w.elements[c * clen:(c + 1) * clen], point.p0)[0])
+    claimed = prover.mle_evaluate(rw, point)
+    assert np.array_equal(claimed, orc.claim(row.values, point.p1))
+    challenges = verifier.derive_folding_challenges(t, row)
+    folded = prover.fold(opening, challenges)
+    rows = np.array([0, 1, 1000, L - 1])
+    assert np.array_equal(folded.elements[rows],
+                          orc.fold_rows(primes[0], L, cols, w.elements, challenges.challenges, rows))
+    fr = verifier.fold_row_evaluation(row, challenges)
+    verifier.verify_evaluation(point, claimed, row)
+    fc = None if mode == "bit_dropped" else verifier.fold_commitment(commitment, challenges)
+    verifier.verify_opening(commitment, challenges, point, fc, folded, fr)

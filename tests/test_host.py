@@ -99,3 +99,53 @@ def test_f162_matches_oracle(lbn, orc):
         x[2] &= np.uint64((1 << 34) - 1)
         y[2] &= np.uint64((1 << 34) - 1)
         assert np.array_equal(lbn.f162_mul(x, y), orc.f162_mul(x, y))
+
+
+@pytest.mark.parametrize("extra,d", [((2917, 4861), 8), ((9721,), 10), ((2917, 12637), 7)])
+def test_bit_dropped_host_side_matches_oracle(lbn, orc, extra, d):
+    # bd/digits.rs:121-182, wire/mod.rs:386-397, verifier.rs:61-100 and 232-275, bd/residual.rs,
+    # on a commitment the oracle computes; the verifier needs no device (host-only parameters)
+    M = lbn.Modulus
+    params = lbn.Params(10, 2, [M(q) for q in extra], opening=lbn.Opening.bit_dropped(d))
+    seed = bytes([0x5A]) * 32
+    pp = lbn.PublicParameters.host_only(params, seed)
+    with pytest.raises(lbn.LabiniusError):
+        lbn.Prover(pp)
+    primes, L, cols = params.primes(), params.len_ring(), params.columns()
+    a = pp.key()
+    assert np.array_equal(a, orc.key_expand(primes, L, orc.matrix_seed(seed)))
+    w = orc.witness_random(bytes([0x11]) * 32, params.witness_len())
+    raw, comp = orc.commit(primes, L, cols, a, w)
+    dropped = lbn.Dropped.of_raw(params, raw)
+    top, digits = orc.drop_bits(primes, cols, raw, d)
+    assert np.array_equal(dropped.top, top) and np.array_equal(dropped.digits, digits)
+    packed = orc.pack_dropped(primes, cols, d, top, digits)
+    assert dropped.pack() == packed and dropped.wire_bytes() == len(packed)
+    assert lbn.bd_cap(params) == orc.bd_cap(cols, d)
+
+    commitment = lbn.Commitment(primes, cols, comp, raw, dropped)
+    verifier = lbn.Verifier(pp)
+    t = lbn.Transcript(b"host/bd")
+    point = verifier.derive_evaluation_point(t, commitment)
+    ot = orc.Transcript(b"host/bd")
+    p0, p1 = orc.derive_evaluation_point_dropped(ot, orc.make_params(10, 2, primes, seed, dropped_bits=d), packed)
+    assert np.array_equal(point.p0, p0) and np.array_equal(point.p1, p1)
+    u = orc.row_evaluate(w, p0)
+    challenges = verifier.derive_folding_challenges(t, lbn.RowEvaluation(u))
+    och = orc.derive_folding_challenges(ot, u)
+    for c, o in zip(challenges.challenges, och):
+        assert list(c.positions) == list(o.positions) and c.signs == o.signs and c.weight == o.weight
+    v = orc.fold(primes[0], L, cols, w, och)
+    folded = lbn.FoldedWitness(v)
+    assert verifier.bd_residual(commitment, challenges, folded) == orc.bd_residual(primes, L, cols, a, d, top,
+                                                                                   digits, och, v)
+    uf = orc.fold_binary(u, och)
+    verifier.verify_opening(commitment, challenges, point, None, folded, uf)
+    bad = lbn.Dropped(params, top, digits.copy())
+    bad.digits[-1, 5] = (int(bad.digits[-1, 5]) + primes[-1] // 2) % primes[-1]  # far over the cap
+    with pytest.raises(lbn.VerificationError):
+        verifier.verify_opening(lbn.Commitment(primes, cols, comp, raw, bad), challenges, point, None, folded, uf)
+    v2 = v.copy()
+    v2[0, 0] += 1
+    with pytest.raises(lbn.VerificationError):
+        verifier.verify_opening(commitment, challenges, point, None, lbn.FoldedWitness(v2), uf)

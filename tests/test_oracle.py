@@ -334,3 +334,155 @@ def test_honest_round_verifies_and_tamper_rejects(orc):
     bad_u = uf.copy()
     bad_u[0] ^= np.uint64(1)
     assert not orc.verify_folded_opening(params, a, fc, v, p0, bad_u)
+
+
+# ---- bit-dropped commitments --------------------------------------------------------------------
+
+def _bd_setup(orc, primes, d, wl=9, cl=2, seed=61):
+    L, cols = (1 << (wl - cl)) // 4, 1 << cl
+    a = orc.key_expand(primes, L, 4242 + seed)
+    w = orc.witness_random(bytes([seed]) * 32, 4 * L * cols)
+    raw, comp = orc.commit(primes, L, cols, a, w)
+    return L, cols, a, w, raw
+
+
+def _mixed_radix(x, primes):
+    out = []
+    for q in primes:
+        out.append(x % q)
+        x //= q
+    return out
+
+
+@pytest.mark.parametrize("primes,d", [([3889, 2917, 4861], 8), ([3889, 9721], 10), ([2917, 4861, 12637], 7)])
+def test_drop_bits_is_the_mixed_radix_of_the_crt_value(orc, primes, d):
+    # bd/digits.rs:121-182: per coefficient, the CRT value of the column's coefficients over all
+    # primes written in mixed radix (digit k < q_k); the base digit keeps its top bits, rounded
+    L, cols, a, w, raw = _bd_setup(orc, primes, d)
+    top, digits = orc.drop_bits(primes, cols, raw, d)
+    M = int(np.prod([int(q) for q in primes], dtype=object))
+    for c in range(cols):
+        coeffs = [orc.intt(q, raw[k, c]).astype(np.int64) for k, q in enumerate(primes)]
+        for i in range(0, N, 7):
+            # CRT by search over the base residue class (independent of Garner's formula)
+            x = 0
+            mod = 1
+            for k, q in enumerate(primes):
+                r = int(coeffs[k][i]) % q
+                while x % q != r:
+                    x += mod
+                mod *= q
+            assert x < M
+            digs = _mixed_radix(x, primes)
+            assert top[c * N + i] == (digs[0] + (1 << (d - 1))) >> d
+            for k in range(1, len(primes)):
+                assert digits[k - 1, c * N + i] == digs[k]
+
+
+def test_pack_dropped_bit_layout(orc):
+    # wire/mod.rs:386-397: top (top_bits) then each digit (residue_bits), LSB first
+    primes, d = [3889, 2917, 4861], 8
+    L, cols, a, w, raw = _bd_setup(orc, primes, d)
+    top, digits = orc.drop_bits(primes, cols, raw, d)
+    tb = orc.top_bits(primes[0], d)
+    # bd/mod.rs:58-64 and wire/mod.rs:77-79 by their definitions
+    assert tb == ((primes[0] - 1 + (1 << (d - 1))) >> d).bit_length()
+    assert [orc.residue_bits(q) for q in primes[1:]] == [(q - 1).bit_length() for q in primes[1:]]
+    bits = []
+    for i in range(cols * N):
+        fields = [(int(top[i]), tb)] + [(int(digits[k - 1, i]), (primes[k] - 1).bit_length())
+                                        for k in range(1, len(primes))]
+        for v, n in fields:
+            bits += [(v >> b) & 1 for b in range(n)]
+    want = bytearray((len(bits) + 7) // 8)
+    for j, b in enumerate(bits):
+        want[j // 8] |= b << (j % 8)
+    assert orc.pack_dropped(primes, cols, d, top, digits) == bytes(want)
+
+
+def test_bd_cap_formula(orc):
+    # bd/mod.rs:71-74 evaluated by hand for two shapes: BD_CAP * N * columns * 28 * (4^d - 1) / 12
+    assert orc.bd_cap(4, 8) == 24192 * 65535        # 4 * 72576 / 12 = 24192, exact
+    assert orc.bd_cap(512, 8) == 3096576 * 65535    # 4 * 9289728 / 12 = 3096576, exact
+
+
+@pytest.mark.parametrize("primes,d", [([3889, 2917, 4861], 8), ([3889, 9721], 6)])
+def test_bd_residual_is_the_coefficient_domain_norm(orc, primes, d):
+    # bd/residual.rs:77-130: the columns rebuilt from (top, digits), folded with the challenges,
+    # subtracted from A v per prime; computed here in the coefficient domain with mul_mod_phi and a
+    # big-integer CRT, then centered and squared
+    L, cols, a, w, raw = _bd_setup(orc, primes, d)
+    top, digits = orc.drop_bits(primes, cols, raw, d)
+    ch = _challenges(orc, cols, b"oracle/bd")
+    v = orc.fold(primes[0], L, cols, w, ch)
+    got = orc.bd_residual(primes, L, cols, a, d, top, digits, ch, v)
+    M = int(np.prod([int(q) for q in primes], dtype=object))
+    cnt = cols * N
+    rebuilt = []
+    for j in range(cols):
+        col = []
+        for i in range(N):
+            inner = 0
+            for k in range(len(primes) - 1, 0, -1):
+                inner = inner * primes[k] + int(digits[k - 1, j * N + i])
+            col.append((int(top[j * N + i]) << d) + primes[0] * inner)
+        rebuilt.append(col)
+    z = []
+    for k, q in enumerate(primes):
+        acc = np.zeros(N, np.int64)
+        for i in range(L):
+            ai = orc.intt(q, (a[k, i].astype(np.int64) % q).astype(np.uint32))
+            acc = (acc + orc.mul_mod_phi(ai, (v[i].astype(np.int64) % q).astype(np.uint32), q)) % q
+        for j in range(cols):
+            e = orc.embed(ch[j])
+            cj = (np.asarray(e, np.int64) % q).astype(np.uint32)
+            colq = np.array([x % q for x in rebuilt[j]], np.uint32)
+            acc = (acc - orc.mul_mod_phi(cj, colq, q).astype(np.int64)) % q
+        z.append(acc)
+    want = 0
+    for i in range(N):
+        x, mod = 0, 1
+        for k, q in enumerate(primes):
+            r = int(z[k][i]) % q
+            while x % q != r:
+                x += mod
+            mod *= q
+        mag = M - x if x > M // 2 else x
+        want += mag * mag
+    assert got == want
+    # honest: the residual is the rounding noise sum_j c_j e_j, far under the cap
+    assert got <= orc.bd_cap(cols, d)
+
+
+def test_bd_round_verifies_and_tamper_rejects(orc):
+    # scheme/prover.rs:69-72 and verifier.rs:232-275 in miniature
+    wl, cl, primes, d = 10, 2, [3889, 2917, 4861], 8
+    seed = bytes([71]) * 32
+    params = orc.make_params(wl, cl, primes, seed, dropped_bits=d)
+    L, cols = (1 << (wl - cl)) // 4, 1 << cl
+    a = orc.key_expand(primes, L, orc.matrix_seed(seed))
+    w = orc.witness_random(bytes([73]) * 32, 1 << wl)
+    raw, comp = orc.commit(primes, L, cols, a, w)
+    top, digits = orc.drop_bits(primes, cols, raw, d)
+    t = orc.Transcript(b"oracle/bd-round")
+    p0, p1 = orc.derive_evaluation_point_dropped(t, params, orc.pack_dropped(primes, cols, d, top, digits))
+    # the dropped arm absorbs different bytes than the matrix arm
+    t2 = orc.Transcript(b"oracle/bd-round")
+    q0, _ = orc.derive_evaluation_point(t2, params, comp)
+    assert not np.array_equal(p0, q0)
+    u = orc.row_evaluate(w, p0)
+    ch = orc.derive_folding_challenges(t, u)
+    v = orc.fold(primes[0], L, cols, w, ch)
+    uf = orc.fold_binary(u, ch)
+    assert orc.verify_opening_bd(params, a, top, digits, ch, v, p0, uf)
+    bad = v.copy()
+    bad[1, 5] += 1
+    assert not orc.verify_opening_bd(params, a, top, digits, ch, bad, p0, uf)
+    # one coefficient off by q0 q1 (the last digit): far over the cap. (A single top bit is a change
+    # of 2^d, inside the noise the cap tolerates by design.)
+    bad_digits = digits.copy()
+    bad_digits[1, 17] = (int(bad_digits[1, 17]) + 1) % primes[2]
+    assert not orc.verify_opening_bd(params, a, top, bad_digits, ch, v, p0, uf)
+    uf_bad = uf.copy()
+    uf_bad[1] ^= np.uint64(4)
+    assert not orc.verify_opening_bd(params, a, top, digits, ch, v, p0, uf_bad)
