@@ -119,6 +119,67 @@ def cpu_baseline(params):
                       f"host {cpu_model()}"}
 
 
+def batched(lbn, reps):
+    """The batched commitment of 64 uneven witnesses (2^16 .. 2^22 F162, a quarter of them lifted B128
+    traces, one key) against the same 64 commitments one at a time on the GPU. Both leave their
+    outputs in HBM. Latency is from the start of the workload to each commitment's completion."""
+    import ctypes
+    import numpy as np
+    import torch
+    from labinius_amd import workloads as wl
+    key = wl.key_params()
+    pp = lbn.PublicParameters.from_seed(key, MATRIX_SEED)
+    prover = lbn.Prover(pp)
+    specs = wl.batch_specs()
+    t0 = time.time()
+    ws = [wl.batch_witness(i, e, lifted) for i, (e, lifted) in enumerate(specs)]
+    rws = [prover.upload_params(w) for w in ws]
+    log(f"batch: {len(ws)} witnesses, {sum(w.params.witness_len() for w in ws) / 1e6:.1f} M F162, "
+        f"{sum(w.params.columns() for w in ws)} columns, set up in {time.time() - t0:.1f} s")
+    prover.commit_batch(rws, want_outputs=False)  # autotune and capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prover.commit_batch(rws, want_outputs=False)
+    torch.cuda.synchronize()
+    batch_ms = 1e3 * (time.perf_counter() - t0) / reps
+    info = prover.batch_info()
+    # one at a time: one prover per size (the same key), each commitment on its own
+    L = lbn.lib()
+    singles, res = {}, []
+    for w in ws:
+        e = w.params.witness_log_len
+        if e not in singles:
+            singles[e] = lbn.Prover(lbn.PublicParameters.from_seed(w.params, MATRIX_SEED))
+        res.append((singles[e], singles[e].upload(w)))
+    for pr, rw in res[:4]:  # warm
+        lbn._check(L.lbn_commit_resident(ctypes.c_void_p(pr._h), ctypes.c_void_p(rw._h), None, None, None))
+    seq = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = []
+        for pr, rw in res:
+            lbn._check(L.lbn_commit_resident(ctypes.c_void_p(pr._h), ctypes.c_void_p(rw._h), None, None, None))
+            done.append(1e3 * (time.perf_counter() - t0))
+        seq.append(done)
+    one_ms = float(np.mean([d[-1] for d in seq]))
+    lat = np.array(seq).mean(axis=0)
+    return {"witnesses": len(ws), "f162": int(sum(w.params.witness_len() for w in ws)),
+            "columns": int(sum(w.params.columns() for w in ws)), "lifted_b128": int(sum(l for _, l in specs)),
+            "batched_ms": round(batch_ms, 3), "batched_device_ms": round(info["ms"], 3),
+            "batched_commits_per_s": round(len(ws) / batch_ms * 1e3, 1),
+            "batched_p50_ms": round(batch_ms, 3), "batched_p99_ms": round(batch_ms, 3),
+            "one_at_a_time_ms": round(one_ms, 3),
+            "one_at_a_time_commits_per_s": round(len(ws) / one_ms * 1e3, 1),
+            "one_at_a_time_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "one_at_a_time_p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "speedup": round(one_ms / batch_ms, 3),
+            "autotuned": {"tiles_per_xcd_round": info["tiles_per_xcd_round"],
+                          "persistent_blocks": info["persistent_blocks"]},
+            "graph": "one hipGraph launch per batch (layout, clear, GEMM, finish)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--suite", default="xl", choices=["s", "m", "l", "xl"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-commit comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) five times, to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -240,6 +302,9 @@ def main():
                      "frac": round(achieved / PEAK_INT8_TOPS, 4), "traffic": profiled_traffic(args.suite),
                      "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4)},
     }
+    if not args.no_batched:
+        log("batched commit...")
+        line["batched"] = batched(lbn, 3)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
         line["cpu_baseline"] = cpu_baseline(params)

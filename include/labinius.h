@@ -144,6 +144,19 @@ int lbn_mle_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *
  *   [8] eval total    [9] eq table            [10] row reduce */
 int lbn_prover_timings(const lbn_prover *prover, float *ms, size_t n);
 
+/* Batched commitment (no counterpart in the reference, which commits one witness per call; each
+ * result equals Prover::commit, scheme/prover.rs:55, of that witness alone). A witness of other
+ * Params that share this prover's key (the same row_log_len and moduli, so the same A, key.rs:48)
+ * is uploaded with lbn_witness_upload_params; lbn_commit_batch commits `count` of them in one
+ * captured HIP graph: components[i] ([cols_i][4][limb][162]) and raw[i] ([limb][cols_i][648])
+ * may be NULL. The GEMM's block schedule is autotuned per total column count. */
+int lbn_witness_upload_params(lbn_prover *prover, const lbn_params *params, const lbn_f162 *w, size_t n,
+                              lbn_dwitness **out);
+int lbn_commit_batch(lbn_prover *prover, const lbn_dwitness *const *ws, size_t count, int16_t *const *components,
+                     uint32_t *const *raw);
+/* The last batch's schedule (tiles per XCD round, persistent blocks or 0) and its device ms. */
+int lbn_batch_info(const lbn_prover *prover, uint32_t *sched_t, uint32_t *persistent_blocks, float *ms);
+
 /* Transcript, challenge.rs:66-160 */
 lbn_transcript *lbn_transcript_new(const uint8_t *domain, size_t len);
 lbn_transcript *lbn_transcript_clone(const lbn_transcript *t);

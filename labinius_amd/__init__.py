@@ -486,6 +486,45 @@ class Prover:
         _check(lib().lbn_witness_upload(ctypes.c_void_p(self._h), _p(e), ctypes.c_size_t(len(e)), ctypes.byref(h)))
         return ResidentWitness(self, witness, h.value)
 
+    def upload_params(self, witness):
+        """A witness of other Params that share this prover's key (row_log_len and moduli)."""
+        h = ctypes.c_void_p()
+        e = np.ascontiguousarray(witness.elements, np.uint64)
+        _check(lib().lbn_witness_upload_params(ctypes.c_void_p(self._h), ctypes.byref(witness.params.c()), _p(e),
+                                               ctypes.c_size_t(len(e)), ctypes.byref(h)))
+        return ResidentWitness(self, witness, h.value)
+
+    def commit_batch(self, resident, want_outputs=True):
+        """Every witness of the list committed in one launch; one Commitment per witness, each equal
+        to that witness's own Prover::commit."""
+        n = len(resident)
+        hs = (ctypes.c_void_p * n)(*[r._h for r in resident])
+        if not want_outputs:
+            _check(lib().lbn_commit_batch(ctypes.c_void_p(self._h), hs, ctypes.c_size_t(n), None, None))
+            return None
+        comps, raws = [], []
+        cptr = (ctypes.c_void_p * n)()
+        rptr = (ctypes.c_void_p * n)()
+        for i, r in enumerate(resident):
+            p = r.params
+            limbs = len(p.primes())
+            comps.append(np.zeros((p.columns(), 4, limbs, N162), np.int16))
+            raws.append(np.zeros((limbs, p.columns(), N), np.uint32))
+            cptr[i] = comps[i].ctypes.data
+            rptr[i] = raws[i].ctypes.data
+        _check(lib().lbn_commit_batch(ctypes.c_void_p(self._h), hs, ctypes.c_size_t(n), cptr, rptr))
+        out = []
+        for r, comp, raw in zip(resident, comps, raws):
+            p = r.params
+            dropped = Dropped.of_raw(p, raw) if p.dropped_bits() > 0 else None
+            out.append(Commitment(p.primes(), p.columns(), comp, raw, dropped))
+        return out
+
+    def batch_info(self):
+        t, pb, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
+        _check(lib().lbn_batch_info(ctypes.c_void_p(self._h), ctypes.byref(t), ctypes.byref(pb), ctypes.byref(ms)))
+        return {"tiles_per_xcd_round": t.value, "persistent_blocks": pb.value, "ms": ms.value}
+
     def _buffers(self):
         p = self.params
         limbs = len(p.primes())

@@ -81,6 +81,8 @@ struct GemmArgs {
     uint32_t groups;          // ceil(batches / 3)
     uint32_t ntiles;
     int accumulate;           // 1: atomicAdd (several tiles add into one slot), 0: store
+    uint32_t sched_t;         // tiles per XCD round (1 .. 32, a power of two); the rest are groups
+    uint32_t items;           // work items (set by the launcher)
 };
 
 // Per-limb constants of the commit finish (decomposition) kernel.

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -19,6 +20,28 @@ namespace lbn {
 
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
+
+#define HIP_TRY(x)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? LBN_ERR_OOM : LBN_ERR_HIP,            \
+                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+    HIP_TRY(hipMalloc((void **)p, (count > 0 ? count : 1) * sizeof(T)));
+    return LBN_OK;
+}
+
+inline float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
 
 // Params as the host uses them (scheme/params.rs:62-68, 254-287).
 struct Shape {
@@ -113,12 +136,35 @@ struct lbn_dwitness {
     lbn_prover *owner = nullptr;
     uint64_t *d_w = nullptr;
     size_t n = 0;
+    size_t cols = 0;  // its own Params::columns(); the key (row length, moduli) is the prover's
+};
+
+// The batched commit's workspace (grow-only), its captured graph and the autotuned schedules.
+struct lbn_batch_ws {
+    size_t cap_cols = 0;
+    uint8_t *d_bytes = nullptr;
+    uint32_t *d_yacc = nullptr, *d_raw = nullptr;
+    int16_t *d_comp = nullptr;
+    const uint64_t **d_cols = nullptr;
+    const uint64_t **h_cols = nullptr;  // pinned
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    size_t graph_cols = 0;
+    uint32_t graph_t = 0, graph_persist = 0;
+    std::vector<std::array<uint32_t, 3>> tuned;  // (columns, sched_t, persistent blocks)
+    hipEvent_t ev[2] = {};
+    float ms = 0;
+    uint32_t sched_t = 8, persist = 0;
 };
 
 struct lbn_opening {
     lbn_prover *owner = nullptr;
     uint8_t *d_bytes_f = nullptr;  // the witness in the fold's byte layout
 };
+
+namespace lbn {
+void batch_free(lbn_batch_ws *b);
+}
 
 struct lbn_prover {
     lbn_pp *pp = nullptr;
@@ -136,4 +182,5 @@ struct lbn_prover {
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
     hipEvent_t ev[12] = {};
     float timings[11] = {};
+    lbn_batch_ws *batch = nullptr;
 };

@@ -31,6 +31,7 @@ namespace lbn {
 constexpr int PRE_I = 16, PRE_C = 32;
 
 __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *__restrict__ w,
+                                                                const uint64_t *const *__restrict__ cols,
                                                                 uint32_t column_len, uint32_t L, uint32_t r,
                                                                 uint8_t *__restrict__ bytes_c, uint32_t kbs_c,
                                                                 uint8_t *__restrict__ bytes_f, uint32_t kbs_f) {
@@ -40,7 +41,8 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
     const uint32_t i = i0 + il, c = c0 + cl;
     uint32_t wd[4][6];
     if (i < L && c < r) {
-        const uint64_t *e = w + 3 * ((size_t)c * column_len + 4 * (size_t)i);
+        // column c is at w + c * column_len elements, or wherever the column table says
+        const uint64_t *e = (cols ? cols[c] : w + 3 * (size_t)c * column_len) + 12 * (size_t)i;
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
@@ -67,6 +69,7 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
     }
     __syncthreads();
     // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16)
+    if (bytes_c)
     for (int u = threadIdx.x; u < PRE_C * TILE_USED; u += PRE_I * PRE_C) {
         const int cc = u / TILE_USED, n = u % TILE_USED;
         if (c0 + cc >= r) continue;
@@ -107,8 +110,8 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 constexpr int GEMM_THREADS = 512;
 constexpr int GEMM_LDS = 1024 + GROUP_COLS * TILE * 4 + EE_TILE * 2;
 
-__host__ __device__ inline uint32_t gemm_blocks(uint32_t ntiles, uint32_t groups) {
-    return 256 * ((ntiles + 7) / 8) * ((groups + 31) / 32);
+__host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
+    return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
 }
 
 __device__ __forceinline__ uint32_t mod_q_i64(long long z, uint32_t q, double qinv) {
@@ -119,15 +122,32 @@ __device__ __forceinline__ uint32_t mod_q_i64(long long z, uint32_t q, double qi
     return (uint32_t)m;
 }
 
-__global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
-    const uint32_t xcd = blockIdx.x & 7, s = blockIdx.x >> 3;
-    const uint32_t round = s >> 5, within = s & 31, tgroups = (g.ntiles + 7) / 8;
-    const uint32_t tile = 8 * (round % tgroups) + (within & 7);
-    const uint32_t grp = 4 * (8 * (round / tgroups) + xcd) + (within >> 3);
-    if (tile >= g.ntiles || grp >= g.groups) return;
-    const TileDesc td = g.tiles[tile];
+// The block order: item -> (tile, group). The 32 items an XCD runs at once (item & 7 is the XCD a
+// block lands on when the grid is a multiple of 8) are t tiles x 32/t groups.
+__device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, uint32_t &tile, uint32_t &grp) {
+    const uint32_t t = g.sched_t, xcd = item & 7, s = item >> 3;
+    const uint32_t round = s >> 5, within = s & 31, tgroups = (g.ntiles + t - 1) / t;
+    tile = t * (round % tgroups) + within % t;
+    grp = (32 / t) * (8 * (round / tgroups) + xcd) + within / t;
+    return tile < g.ntiles && grp < g.groups;
+}
 
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem);
+
+__global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides
+    for (uint32_t item = blockIdx.x; item < g.items; item += gridDim.x) {
+        uint32_t tile, grp;
+        if (!gemm_item_of(g, item, tile, grp)) continue;
+        gemm_item(g, tile, grp, smem);
+        __syncthreads();  // the next item overwrites the LDS
+    }
+}
+
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem) {
+ const TileDesc td = g.tiles[tile];
+
     uint16_t *lut = (uint16_t *)smem;                                   // [2][256]
     uint8_t *abuf = smem + 1024;                                        // [2][A_STEP]
     uint32_t *zbuf = (uint32_t *)(smem + 1024);                         // [256][96], after the loop
@@ -597,21 +617,26 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v) {
     return hipSuccess;
 }
 
-hipError_t launch_prepass(hipStream_t s, const uint64_t *w, uint32_t column_len, uint32_t L,
-                          uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
+hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *const *cols, uint32_t column_len,
+                          uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
                           uint32_t kbs_f) {
     dim3 grid((r + PRE_C - 1) / PRE_C, (L + PRE_I - 1) / PRE_I);
-    LBN_LAUNCH(prepass_kernel, grid, dim3(PRE_I * PRE_C), 0, s, w, column_len, L, r, bytes_c, kbs_c, bytes_f,
+    LBN_LAUNCH(prepass_kernel, grid, dim3(PRE_I * PRE_C), 0, s, w, cols, column_len, L, r, bytes_c, kbs_c, bytes_f,
                kbs_f);
     return hipSuccess;
 }
 
-hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g) {
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks) {
     static const hipError_t attr = hipFuncSetAttribute((const void *)tile_gemm_kernel,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
     if (attr != hipSuccess) return attr;
     if (g.groups == 0 || g.ntiles == 0) return hipSuccess;
-    LBN_LAUNCH(tile_gemm_kernel, dim3(gemm_blocks(g.ntiles, g.groups)), dim3(GEMM_THREADS), GEMM_LDS, s, g);
+    GemmArgs a = g;
+    if (a.sched_t == 0 || a.sched_t > 32 || (a.sched_t & (a.sched_t - 1))) a.sched_t = 8;
+    a.items = gemm_items(a.ntiles, a.groups, a.sched_t);
+    uint32_t grid = a.items;
+    if (persistent_blocks && persistent_blocks < grid) grid = (persistent_blocks + 7) / 8 * 8;
+    LBN_LAUNCH(tile_gemm_kernel, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
     return hipSuccess;
 }
 

@@ -18,10 +18,12 @@ struct lbn_challenge_dev {
 
 hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
-hipError_t launch_prepass(hipStream_t s, const uint64_t *w, uint32_t column_len, uint32_t L,
-                          uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
+// w + c * column_len (cols == NULL) or cols[c] is column c; either output may be NULL
+hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *const *cols, uint32_t column_len,
+                          uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
                           uint32_t kbs_f);
-hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g);
+// persistent_blocks == 0: one block per work item; otherwise that many blocks (rounded up to 8) loop
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks);
 hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t limbs,
                                 const FinishLimb *fl, const uint16_t *slot648, const uint16_t *qcs,
                                 uint32_t *raw, int16_t *comp, uint32_t cols);

@@ -8,15 +8,6 @@
 
 namespace lbn {
 
-#define HIP_TRY(x)                                                                        \
-    do {                                                                                  \
-        hipError_t e_ = (x);                                                              \
-        if (e_ != hipSuccess)                                                             \
-            return fail(e_ == hipErrorOutOfMemory ? LBN_ERR_OOM : LBN_ERR_HIP,            \
-                        std::string(#x) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
-
-static size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
 // ---------------------------------------------------------------------------------------------
 // the tables of one GEMM tile (see device.hpp)
@@ -334,11 +325,6 @@ static int build_device(lbn_pp *pp) {
 // the prover
 // ---------------------------------------------------------------------------------------------
 
-template <typename T>
-static int dalloc(T **p, size_t count) {
-    HIP_TRY(hipMalloc((void **)p, std::max<size_t>(1, count) * sizeof(T)));
-    return LBN_OK;
-}
 
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
@@ -346,6 +332,7 @@ static void prover_free(lbn_prover *pr) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
+    batch_free(pr->batch);
     for (hipEvent_t &e : pr->ev)
         if (e) (void)hipEventDestroy(e);
     if (pr->stream) (void)hipStreamDestroy(pr->stream);
@@ -368,23 +355,20 @@ static int take_bytes_f(lbn_prover *pr, uint8_t **out) {
     return LBN_OK;
 }
 
-static float elapsed(hipEvent_t a, hipEvent_t b) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
-    return ms;
-}
 
 static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
     const size_t limbs = s.primes.size();
-    uint8_t *bytes_f = nullptr;
-    int rc = take_bytes_f(pr, &bytes_f);
-    if (rc) return rc;
+    uint8_t *bytes_f = nullptr;  // the fold's layout, only when an opening is asked for
+    if (opening) {
+        int rc = take_bytes_f(pr, &bytes_f);
+        if (rc) return rc;
+    }
     hipStream_t st = pr->stream;
     HIP_TRY(hipEventRecord(pr->ev[0], st));
-    HIP_TRY(launch_prepass(st, d_w, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r, pr->d_bytes_c,
-                           pp->kb_c, bytes_f, pp->kb_f));
+    HIP_TRY(launch_prepass(st, d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r,
+                           pr->d_bytes_c, pp->kb_c, bytes_f, pp->kb_f));
     HIP_TRY(launch_fill_u32(st, pr->d_yacc, s.r * limbs * N, 0));
     HIP_TRY(hipEventRecord(pr->ev[1], st));
     GemmArgs g{};
@@ -402,7 +386,8 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     g.groups = (uint32_t)((s.r + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
     g.accumulate = 1;
-    HIP_TRY(launch_tile_gemm(st, g));
+    g.sched_t = 8;
+    HIP_TRY(launch_tile_gemm(st, g, 0));
     HIP_TRY(hipEventRecord(pr->ev[2], st));
     HIP_TRY(launch_commit_finish(st, pr->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, pr->d_raw,
                                  pr->d_comp, (uint32_t)s.r));
@@ -419,8 +404,6 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
         op->owner = pr;
         op->d_bytes_f = bytes_f;
         *opening = op;
-    } else {
-        pr->free_bytes_f.push_back(bytes_f);
     }
     return LBN_OK;
 }
@@ -449,7 +432,8 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     g.groups = (uint32_t)((s.len_ring + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles_f;
     g.accumulate = 0;
-    HIP_TRY(launch_tile_gemm(st, g));
+    g.sched_t = 8;
+    HIP_TRY(launch_tile_gemm(st, g, 0));
     HIP_TRY(hipEventRecord(pr->ev[6], st));
     HIP_TRY(launch_intt(st, pr->d_vhat, (uint32_t)s.len_ring, pp->intt, pp->d_twi, pr->d_v));
     HIP_TRY(hipEventRecord(pr->ev[7], st));
@@ -619,6 +603,7 @@ int lbn_witness_upload(lbn_prover *pr, const lbn_f162 *w, size_t n, lbn_dwitness
     lbn_dwitness *d = new lbn_dwitness();
     d->owner = pr;
     d->n = n;
+    d->cols = pr->pp->shape.r;
     if (hipMalloc((void **)&d->d_w, n * sizeof(lbn_f162)) != hipSuccess) {
         delete d;
         return fail(LBN_ERR_OOM, "witness allocation");
