@@ -180,6 +180,47 @@ def batched(lbn, reps):
             "graph": "one hipGraph launch per batch (layout, clear, GEMM, finish)"}
 
 
+def serving_bench(lbn, rate, count):
+    """Poisson arrivals of commit -> evaluate -> fold requests (sizes 2^16 40 %, 2^18 30 %, 2^20 20 %,
+    2^22 8 %, 2^24 2 %; Clear or bit-dropped at random) through the continuous-batching server,
+    then the same stream one request at a time. Latency is arrival to folded witness."""
+    import numpy as np
+    from labinius_amd import serving
+    from labinius_amd import workloads as wl
+    spec = wl.serving_requests(rate, count)
+    rng = np.random.default_rng(1)
+    reqs = []
+    for t, e, d, i in spec:
+        x = rng.integers(0, 2**63, (1 << e, 3), dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, (1 << e, 3),
+                                                                                                  dtype=np.uint64)
+        x[:, 2] &= np.uint64((1 << 34) - 1)
+        reqs.append((i, e, d, lbn.Witness.from_elements(wl.witness_params(e), x), t))
+    server = serving.Server(MATRIX_SEED)
+    warm = [(10**6 + k, e, 8 * (k % 2), lbn.Witness.random(wl.witness_params(e), bytes([k]) * 32), 0.0)
+            for k, e in enumerate(server.workers)]
+    serving.run_stream(server, warm)
+    server.batches.clear()
+    t0 = time.time()
+    mixed = serving.run_stream(server, reqs)
+    server.close()
+    log(f"served {len(mixed)} in {time.time() - t0:.1f} s, {len(server.batches)} batched commitments")
+    unb = serving.serve_unbatched(MATRIX_SEED, reqs)
+    sizes = {}
+    for _, e, _, _, _ in reqs:
+        sizes[e] = sizes.get(e, 0) + 1
+    stage = lambda rs, a, b: round(1e3 * float(np.mean([getattr(r, b) - getattr(r, a) for r in rs])), 3)
+    return {"rate_per_s": rate, "sizes": {f"2^{e}": c for e, c in sorted(sizes.items())},
+            "bit_dropped": sum(1 for r in reqs if r[2]),
+            "served": dict(serving.summarize(mixed), batches=len(server.batches),
+                           mean_requests_per_batch=round(len(mixed) / max(1, len(server.batches)), 2),
+                           queue_ms=stage(mixed, "arrival", "admitted"),
+                           commit_ms=stage(mixed, "admitted", "committed"), round_ms=stage(mixed, "committed", "done"),
+                           peak_hbm_gb=round(server.peak_reserved / 1e9, 3)),
+            "unbatched": dict(serving.summarize(unb), queue_ms=stage(unb, "arrival", "admitted"),
+                              commit_ms=stage(unb, "admitted", "committed"), round_ms=stage(unb, "committed", "done")),
+            "hbm_budget_gb": 200}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +229,9 @@ def main():
     ap.add_argument("--suite", default="xl", choices=["s", "m", "l", "xl"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-commit comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
+    ap.add_argument("--serving-rate", type=float, default=100.0, help="requests per second")
+    ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) five times, to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -305,6 +349,9 @@ def main():
     if not args.no_batched:
         log("batched commit...")
         line["batched"] = batched(lbn, 3)
+    if not args.no_serving:
+        log("serving...")
+        line["serving"] = serving_bench(lbn, args.serving_rate, args.serving_requests)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
         line["cpu_baseline"] = cpu_baseline(params)

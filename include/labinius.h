@@ -129,11 +129,18 @@ int lbn_fold(lbn_prover *prover, lbn_opening *opening, const lbn_challenge *chal
              int16_t *v);
 /* Prover::recycle, scheme/prover.rs:116-118 */
 int lbn_recycle(lbn_prover *prover, lbn_opening *opening);
+/* Prover::fold (scheme/prover.rs:120-131) of a witness that stayed resident in HBM since its
+ * commitment, without an opening: the fold's layout is rebuilt from the witness (what a batched
+ * commitment, which keeps none, needs). Same result as lbn_fold. */
+int lbn_fold_resident(lbn_prover *prover, const lbn_dwitness *w, const lbn_challenge *challenges, size_t r,
+                      int16_t *v);
 
 /* Witness::row_evaluate / mle_evaluate, scheme/witness.rs:94-105: u has 2^column_log_len F162;
  * p0 has row_log_len coordinates, p1 column_log_len. */
 int lbn_row_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len,
                      lbn_f162 *u);
+/* eval.rs:185-193 (claim): t = sum_j eq(p1)_j u_j from the row values, on the host */
+int lbn_claim(const lbn_f162 *u, size_t r, const lbn_f162 *p1, size_t p1_len, lbn_f162 *t);
 int lbn_mle_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len,
                      const lbn_f162 *p1, size_t p1_len, lbn_f162 *t);
 

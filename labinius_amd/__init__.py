@@ -570,6 +570,15 @@ class Prover:
                               _p(v) if want_output else None))
         return FoldedWitness(v)
 
+    def fold_resident(self, rw, challenges, want_output=True):
+        """The fold of a witness resident since its commitment, without an opening (lbn_fold_resident)."""
+        p = self.params
+        v = np.zeros((p.len_ring(), N), np.int16)
+        arr = challenges.array()
+        _check(lib().lbn_fold_resident(ctypes.c_void_p(self._h), ctypes.c_void_p(rw._h), arr,
+                                       ctypes.c_size_t(len(arr)), _p(v) if want_output else None))
+        return FoldedWitness(v)
+
     def recycle(self, opening):
         h, opening._h = opening._h, None
         _check(lib().lbn_recycle(ctypes.c_void_p(self._h), ctypes.c_void_p(h)))
@@ -748,6 +757,15 @@ class Verifier:
         if rc == LBN_REJECTED:
             raise VerificationError("Rejected")
         _check(rc)
+
+
+def claim(u, p1):
+    """eval.rs:185-193: sum_j eq(p1)_j u_j."""
+    u = np.ascontiguousarray(u, np.uint64).reshape(-1, 3)
+    p1 = np.ascontiguousarray(p1, np.uint64).reshape(-1, 3)
+    t = np.zeros(3, np.uint64)
+    _check(lib().lbn_claim(_p(u), ctypes.c_size_t(len(u)), _p(p1), ctypes.c_size_t(len(p1)), _p(t)))
+    return t
 
 
 def bd_cap(params):

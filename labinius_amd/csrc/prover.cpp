@@ -657,6 +657,24 @@ int lbn_fold(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch, size_t r,
     return rc;
 }
 
+int lbn_fold_resident(lbn_prover *pr, const lbn_dwitness *w, const lbn_challenge *ch, size_t r, int16_t *v) {
+    if (!pr || !w || !ch) return fail(LBN_ERR_ARG, "null argument");
+    const Shape &s = pr->pp->shape;
+    if (w->n != s.n) return fail(LBN_ERR_SHAPE, "the witness was built for other parameters");
+    if (r != s.r) return fail(LBN_ERR_SHAPE, "one challenge per column");
+    for (size_t j = 0; j < r; j++)
+        if (ch[j].weight > 32) return fail(LBN_ERR_ARG, "challenge weight above 32");
+    lbn_opening op;
+    op.owner = pr;
+    int rc = take_bytes_f(pr, &op.d_bytes_f);
+    if (rc) return rc;
+    const hipError_t e = launch_prepass(pr->stream, w->d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring,
+                                        (uint32_t)s.r, nullptr, 0, op.d_bytes_f, pr->pp->kb_f);
+    rc = e == hipSuccess ? fold_device(pr, &op, ch, r, v) : fail(LBN_ERR_HIP, hipGetErrorString(e));
+    pr->free_bytes_f.push_back(op.d_bytes_f);
+    return rc;
+}
+
 int lbn_recycle(lbn_prover *pr, lbn_opening *op) {
     if (!pr || !op) return fail(LBN_ERR_ARG, "null argument");
     if (op->owner != pr) return fail(LBN_ERR_SHAPE, "the opening belongs to another prover");
@@ -670,6 +688,22 @@ int lbn_row_evaluate(lbn_prover *pr, const lbn_dwitness *w, const lbn_f162 *p0, 
     if (w->n != pr->pp->shape.n || p0_len != pr->pp->shape.rows)
         return fail(LBN_ERR_SHAPE, "the evaluation point does not match the witness");
     return row_eval_device(pr, w->d_w, p0, u);
+}
+
+int lbn_claim(const lbn_f162 *u, size_t r, const lbn_f162 *p1, size_t p1_len, lbn_f162 *t) {
+    if (!u || !p1 || !t) return fail(LBN_ERR_ARG, "null argument");
+    if (r != ((size_t)1 << p1_len)) return fail(LBN_ERR_SHAPE, "2^len(p1) row values");
+    std::vector<F162> ps(p1_len);
+    for (size_t k = 0; k < p1_len; k++) std::memcpy(ps[k].w, p1[k].w, 24);
+    const std::vector<F162> eq = eq_table(ps.data(), (int)p1_len);
+    F162 acc{{0, 0, 0}};
+    for (size_t j = 0; j < r; j++) {
+        F162 x;
+        std::memcpy(x.w, u[j].w, 24);
+        acc = f162_add(acc, f162_mul(eq[j], x));
+    }
+    std::memcpy(t->w, acc.w, 24);
+    return LBN_OK;
 }
 
 int lbn_mle_evaluate(lbn_prover *pr, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len, const lbn_f162 *p1,
