@@ -516,57 +516,153 @@ __global__ void eq_kernel(const uint64_t *__restrict__ part, uint32_t rows, uint
     eq[3 * (size_t)b + 2] = acc[2];
 }
 
-// u_j += sum over a chunk of i of eq_i W[i + wdim j] (eval.rs:171-182): thread b < 162 collects
-// S_b = XOR of eq_i over the i whose element has bit b set; the block then forms
-// sum_b x^b S_b mod x^162 + x^81 + 1 and XORs it into u_j.
-constexpr int EVAL_CHUNK = 256;
-__global__ __launch_bounds__(192) void row_eval_kernel(const uint64_t *__restrict__ w,
-                                                       const uint64_t *__restrict__ eq,
-                                                       uint32_t wdim, uint64_t *__restrict__ u) {
-    __shared__ uint64_t ws[EVAL_CHUNK * 3], es[EVAL_CHUNK * 3];
-    __shared__ uint64_t red[6];
-    const uint32_t j = blockIdx.x, i0 = blockIdx.y * EVAL_CHUNK;
-    const int b = threadIdx.x;
-    const uint32_t cnt = min((uint32_t)EVAL_CHUNK, wdim - i0);
-    for (uint32_t x = threadIdx.x; x < cnt * 3; x += blockDim.x) {
-        ws[x] = w[3 * ((size_t)j * wdim + i0) + x];
-        es[x] = eq[3 * (size_t)i0 + x];
-    }
-    if (threadIdx.x < 6) red[threadIdx.x] = 0;
-    __syncthreads();
-    uint64_t s0 = 0, s1 = 0, s2 = 0;
-    if (b < 162) {
-        const int wi = b >> 6, bi = b & 63;
-        for (uint32_t i = 0; i < cnt; i++) {
-            const uint64_t mask = 0ull - ((ws[3 * i + wi] >> bi) & 1ull);
-            s0 ^= es[3 * i] & mask;
-            s1 ^= es[3 * i + 1] & mask;
-            s2 ^= es[3 * i + 2] & mask;
-        }
-        // x^b S_b as a 324-bit value in 6 words: word w takes S[w - ws] << bs and
-        // S[w - ws - 1] >> (64 - bs)
-        const int wsh = b >> 6, bs = b & 63;
-        auto pick = [&](int k) -> uint64_t { return k == 0 ? s0 : k == 1 ? s1 : k == 2 ? s2 : 0ull; };
+// u_j = sum_i eq_i W[i + wdim j] (eval.rs:171-182) by bit positions: with S_b the XOR of eq_i over
+// the i whose element has bit b set, u_j = sum_b x^b S_b mod x^162 + x^81 + 1. S_b is gathered
+// eight elements at a time (the method of the four Russians): per group of 8 consecutive i a
+// 256-entry table T[m] = XOR of eq_{8g+k} over the bits k of m, shared by every column; then each
+// bit position b of a column costs one lookup per group, at the byte "bits b of the 8 elements".
+//
+// A block takes 8 columns (two per wave) and 16 chunks of 64 elements: per chunk it builds the 8
+// tables in LDS, each wave transposes its column's chunk into per-bit-position bytes (8 x 8 bit
+// transposes) and looks them up; a lane owns bit positions lane, lane + 64 and lane + 128.
+constexpr int RE_CHUNK = 64, RE_COLS = 8, RE_SPAN = 16;
+
+__device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
+    // byte t, bit p -> byte p, bit t
+    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+    x ^= t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+    x ^= t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+    x ^= t ^ (t << 28);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m, 64), hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Visibility of a wave's own LDS writes to its other lanes, without a block barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restrict__ w,
+                                                       const uint64_t *__restrict__ eq, uint32_t wdim,
+                                                       uint32_t cols, uint64_t *__restrict__ u) {
+    __shared__ uint64_t T[8][256][3];
+    __shared__ uint64_t NT[8][2][16][3];                  // nibble tables: low and high four elements
+    __shared__ __attribute__((aligned(16))) uint64_t ws[4][3][RE_CHUNK];  // [wave][word][element]
+    __shared__ uint64_t by[4][8][21];  // [wave][group][bit position / 8]: byte b is bits b of the group
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t j0 = blockIdx.x * RE_COLS, nch = wdim / RE_CHUNK;
+    const uint32_t ch0 = blockIdx.y * RE_SPAN, ch1 = min(ch0 + RE_SPAN, nch);
+    uint64_t S[2][3][3];
 #pragma unroll
-        for (int wd = 0; wd < 6; wd++) {
-            uint64_t x = pick(wd - wsh) << bs;
-            if (bs) x ^= pick(wd - wsh - 1) >> (64 - bs);
-            if (x) atomicXor((unsigned long long *)&red[wd], (unsigned long long)x);
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S[a][b][0] = S[a][b][1] = S[a][b][2] = 0;
+
+    for (uint32_t ch = ch0; ch < ch1; ch++) {
+        const uint32_t i0 = ch * RE_CHUNK;
+        __syncthreads();  // the previous chunk's tables are no longer read
+        {
+            // nibble table (g, half h, x): XOR of eq_{8g + 4h + k} over the bits k of x
+            const int g = tid >> 5, h = (tid >> 4) & 1, x = tid & 15;
+            const uint64_t *e = eq + 3 * ((size_t)i0 + 8 * g + 4 * h);
+            uint64_t a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t msk = 0ull - (uint64_t)((x >> k) & 1);
+                a0 ^= e[3 * k] & msk;
+                a1 ^= e[3 * k + 1] & msk;
+                a2 ^= e[3 * k + 2] & msk;
+            }
+            NT[g][h][x][0] = a0;
+            NT[g][h][x][1] = a1;
+            NT[g][h][x][2] = a2;
+        }
+        __syncthreads();
+        for (int e = tid; e < 8 * 256; e += 256) {
+            const int g = e >> 8, m = e & 255;
+            const uint64_t *lo = NT[g][0][m & 15], *hi = NT[g][1][m >> 4];
+            T[g][m][0] = lo[0] ^ hi[0];
+            T[g][m][1] = lo[1] ^ hi[1];
+            T[g][m][2] = lo[2] ^ hi[2];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int cw = 0; cw < 2; cw++) {
+            const uint32_t j = j0 + wave + 4 * cw;
+            if (j >= cols) continue;  // wave-uniform; nothing below waits on other waves
+            // the chunk, word-major: ws[k][i] = word k of element i0 + i
+            const uint64_t *src = w + 3 * ((size_t)j * wdim + i0);
+            for (int x = lane; x < RE_CHUNK * 3; x += 64) ws[wave][x % 3][x / 3] = src[x];
+            wave_sync();
+            // item (g, word k, byte c): bits 64 k + 8 c .. + 7 of the group's 8 elements
+            for (int it = lane; it < 8 * 24; it += 64) {
+                const int g = it / 24, k = (it >> 3) % 3, c = it & 7;
+                if (8 * k + c > 20) continue;  // bit positions past 167
+                const uint64_t *e8 = &ws[wave][k][8 * g];
+                uint64_t x = 0;
+#pragma unroll
+                for (int t = 0; t < 8; t++) x |= ((e8[t] >> (8 * c)) & 255ull) << (8 * t);
+                by[wave][g][8 * k + c] = transpose8x8(x);
+            }
+            wave_sync();
+            const uint8_t *bb = (const uint8_t *)by[wave];
+#pragma unroll
+            for (int sl = 0; sl < 3; sl++) {
+                const int b = lane + 64 * sl;
+                if (b < 162) {
+#pragma unroll
+                    for (int g = 0; g < 8; g++) {
+                        const uint64_t *t = T[g][bb[g * 168 + b]];
+                        S[cw][sl][0] ^= t[0];
+                        S[cw][sl][1] ^= t[1];
+                        S[cw][sl][2] ^= t[2];
+                    }
+                }
+            }
+            wave_sync();  // ws and by are rewritten by the next column
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t p[6] = {red[0], red[1], red[2], red[3], red[4], red[5]};
-        uint64_t H0 = (p[2] >> 34) | (p[3] << 30), H1 = (p[3] >> 34) | (p[4] << 30);
-        uint64_t H2w = (p[4] >> 34) | (p[5] << 30);
-        uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
-        uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
-        uint64_t o0 = p[0] ^ H0 ^ G0;
-        uint64_t o1 = p[1] ^ H1 ^ G1 ^ (S0 << 17);
-        uint64_t o2 = (p[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
-        atomicXor((unsigned long long *)&u[3 * j], (unsigned long long)o0);
-        atomicXor((unsigned long long *)&u[3 * j + 1], (unsigned long long)o1);
-        atomicXor((unsigned long long *)&u[3 * j + 2], (unsigned long long)o2);
+    // u_j ^= sum_b x^b S_b mod x^162 + x^81 + 1
+#pragma unroll
+    for (int cw = 0; cw < 2; cw++) {
+        const uint32_t j = j0 + wave + 4 * cw;
+        if (j >= cols) continue;  // wave-uniform
+        uint64_t p[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int sl = 0; sl < 3; sl++) {
+            const int b = lane + 64 * sl;
+            if (b >= 162) continue;
+            // x^b S as 6 words: S lands at word sl with the shift lane (b = 64 sl + lane)
+            const uint64_t *sv = S[cw][sl];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                p[sl + q] ^= sv[q] << lane;
+                if (lane) p[sl + q + 1] ^= sv[q] >> (64 - lane);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) p[q] ^= shfl_xor_u64(p[q], m);
+        if (lane == 0) {
+            const uint64_t H0 = (p[2] >> 34) | (p[3] << 30), H1 = (p[3] >> 34) | (p[4] << 30);
+            const uint64_t H2w = (p[4] >> 34) | (p[5] << 30);
+            const uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+            const uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
+            const uint64_t o0 = p[0] ^ H0 ^ G0;
+            const uint64_t o1 = p[1] ^ H1 ^ G1 ^ (S0 << 17);
+            const uint64_t o2 = (p[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
+            atomicXor((unsigned long long *)&u[3 * j], (unsigned long long)o0);
+            atomicXor((unsigned long long *)&u[3 * j + 1], (unsigned long long)o1);
+            atomicXor((unsigned long long *)&u[3 * j + 2], (unsigned long long)o2);
+        }
     }
 }
 
@@ -672,8 +768,10 @@ hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t 
 
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u) {
-    LBN_LAUNCH(row_eval_kernel, dim3(cols, (wdim + EVAL_CHUNK - 1) / EVAL_CHUNK), dim3(192), 0, s, w, eq, wdim,
-               u);
+    // the kernel's chunks are 64 elements: wdim (2^row_log_len >= 128) is a multiple
+    const uint32_t nch = wdim / RE_CHUNK;
+    LBN_LAUNCH(row_eval_kernel, dim3((cols + RE_COLS - 1) / RE_COLS, (nch + RE_SPAN - 1) / RE_SPAN), dim3(256), 0, s,
+               w, eq, wdim, cols, u);
     return hipSuccess;
 }
 
