@@ -213,10 +213,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     store_a(0);
     __syncthreads();  // the LUT and A(0)
     build();
-    if (1 < kbs) {
-        load_a(1);
-        load_raw(1);
-    }
+    load_a(kbs > 1 ? 1 : 0);
+    load_raw(kbs > 1 ? 1 : 0);
+    // One basic block per step, so that the next step's lookups and loads are scheduled among
+    // this step's MFMAs: past the last step the prefetches repeat the last one and the
+    // B fragments they build are not used.
     for (uint32_t kb = 0; kb < kbs; kb++) {
         const uint8_t *ab = abuf + (kb & 1) * A_STEP;
         v4i cl[2], ch[2];
@@ -225,7 +226,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             cl[kh] = bl[kh];
             ch[kh] = bh[kh];
         }
-        if (kb + 1 < kbs) build();  // the next step's B (its bytes are loaded)
+        build();  // step kb + 1 (its bytes are loaded)
 #pragma unroll
         for (int kh = 0; kh < 2; kh++) {
             v4i a0[3], a1[3];
@@ -248,13 +249,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             for (int mb = 0; mb < 3; mb++)
                 acc[mb][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mb], cl[kh], acc[mb][1], 0, 0, 0);
         }
-        if (kb + 1 < kbs) {
-            store_a((kb + 1) & 1);
-            if (kb + 2 < kbs) {
-                load_a(kb + 2);
-                load_raw(kb + 2);
-            }
-        }
+        store_a((kb + 1) & 1);  // A(kb + 1); buffer (kb + 1) & 1 was last read at step kb - 1
+        load_a(min(kb + 2, kbs - 1));
+        load_raw(min(kb + 2, kbs - 1));
         __syncthreads();
     }
 
