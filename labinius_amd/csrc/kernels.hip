@@ -26,16 +26,19 @@ namespace lbn {
 // The byte of ring element i of column c at position n < 81: bit t is coefficient 81 t + n, and
 // coefficient 4 m + k of a ring element is bit m of its F162 element k (f162.rs:1-16). Written
 // twice in fragment order (byte_offset): the commit's (batch c, K = i) and the fold's (batch i,
-// K = c). A block covers 16 ring elements x 32 columns, one thread per ring element, and stages
-// the bytes in LDS as st[n][c][i] so both stores are 16-byte runs.
-constexpr int PRE_I = 16, PRE_C = 32;
+// K = c). A block covers 16 ring elements x 16 columns, one thread per ring element, and stages
+// the bytes in LDS twice, ring elements fastest and columns fastest, so that each store gathers
+// its 16 bytes with one aligned 16-byte read. Rows are padded to 272 bytes, which puts 16
+// consecutive n on distinct banks.
+constexpr int PRE_I = 16, PRE_C = 16, PRE_ROW = PRE_I * PRE_C + 16;
 
 __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *__restrict__ w,
                                                                 const uint64_t *const *__restrict__ cols,
                                                                 uint32_t column_len, uint32_t L, uint32_t r,
                                                                 uint8_t *__restrict__ bytes_c, uint32_t kbs_c,
                                                                 uint8_t *__restrict__ bytes_f, uint32_t kbs_f) {
-    __shared__ __attribute__((aligned(16))) uint8_t st[TILE_USED][PRE_C][PRE_I];
+    __shared__ __attribute__((aligned(16))) uint8_t stc[TILE_USED][PRE_ROW];  // [n][c * 16 + i]
+    __shared__ __attribute__((aligned(16))) uint8_t stf[TILE_USED][PRE_ROW];  // [n][i * 16 + c]
     const int il = threadIdx.x & (PRE_I - 1), cl = threadIdx.x / PRE_I;
     const uint32_t i0 = blockIdx.y * PRE_I, c0 = blockIdx.x * PRE_C;
     const uint32_t i = i0 + il, c = c0 + cl;
@@ -65,29 +68,23 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
             const int m = 81 * t + n, k = m & 3, mp = m >> 2;
             b |= ((wd[k][mp >> 5] >> (mp & 31)) & 1u) << t;
         }
-        st[n][cl][il] = (uint8_t)b;
+        stc[n][cl * PRE_I + il] = (uint8_t)b;
+        stf[n][il * PRE_C + cl] = (uint8_t)b;
     }
     __syncthreads();
     // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16)
     if (bytes_c)
-    for (int u = threadIdx.x; u < PRE_C * TILE_USED; u += PRE_I * PRE_C) {
-        const int cc = u / TILE_USED, n = u % TILE_USED;
-        if (c0 + cc >= r) continue;
-        *(uint4 *)(bytes_c + byte_offset(c0 + cc, n, i0, kbs_c)) = *(const uint4 *)&st[n][cc][0];
-    }
-    if (!bytes_f) return;
-    // fold: for each (i, n) the 16 bytes of columns c0 + 16 h .. c0 + 16 h + 15
-    for (int u = threadIdx.x; u < PRE_I * TILE_USED * 2; u += PRE_I * PRE_C) {
-        const int h = u & 1, rest = u >> 1, ii = rest / TILE_USED, n = rest % TILE_USED;
-        if (i0 + ii >= L) continue;
-        uint32_t v[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int cb = 16 * h + 4 * q;
-            v[q] = (uint32_t)st[n][cb][ii] | (uint32_t)st[n][cb + 1][ii] << 8 | (uint32_t)st[n][cb + 2][ii] << 16 |
-                   (uint32_t)st[n][cb + 3][ii] << 24;
+        for (int u = threadIdx.x; u < PRE_C * TILE_USED; u += PRE_I * PRE_C) {
+            const int cc = u / TILE_USED, n = u % TILE_USED;
+            if (c0 + cc >= r) continue;
+            *(uint4 *)(bytes_c + byte_offset(c0 + cc, n, i0, kbs_c)) = *(const uint4 *)&stc[n][cc * PRE_I];
         }
-        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0 + 16 * h, kbs_f)) = make_uint4(v[0], v[1], v[2], v[3]);
+    if (!bytes_f) return;
+    // fold: for each (i, n) the 16 bytes of columns c0 .. c0 + 15 (c0 is a multiple of 16)
+    for (int u = threadIdx.x; u < PRE_I * TILE_USED; u += PRE_I * PRE_C) {
+        const int ii = u / TILE_USED, n = u % TILE_USED;
+        if (i0 + ii >= L) continue;
+        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0, kbs_f)) = *(const uint4 *)&stf[n][ii * PRE_C];
     }
 }
 
