@@ -438,28 +438,26 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
 }
 
 // =============================================================================================
-// F162 on the device: a carry-less 162 x 162 product by shifted XOR over the bits of a, then the
+// F162 on the device: a carry-less 162 x 162 product (the comb method: for each bit position j
+// of a word, b << j is formed once and XORed in at every word of a with bit j set), then the
 // reduction x^162 = x^81 + 1 (the same two-step fold as the host's f162_mul).
 // =============================================================================================
 
 __device__ void f162_mul_dev(const uint64_t *a, const uint64_t *b, uint64_t *out) {
     uint64_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
-    uint64_t s0 = b[0], s1 = b[1], s2 = b[2], s3 = 0, s4 = 0, s5 = 0;  // b << i
-    for (int i = 0; i < 162; i++) {
-        const uint64_t aw = i < 64 ? a[0] : (i < 128 ? a[1] : a[2]);
-        const uint64_t m = 0ull - ((aw >> (i & 63)) & 1ull);
-        p0 ^= s0 & m;
-        p1 ^= s1 & m;
-        p2 ^= s2 & m;
-        p3 ^= s3 & m;
-        p4 ^= s4 & m;
-        p5 ^= s5 & m;
-        s5 = (s5 << 1) | (s4 >> 63);
-        s4 = (s4 << 1) | (s3 >> 63);
-        s3 = (s3 << 1) | (s2 >> 63);
-        s2 = (s2 << 1) | (s1 >> 63);
-        s1 = (s1 << 1) | (s0 >> 63);
-        s0 <<= 1;
+    const uint64_t a0 = a[0], a1 = a[1], a2 = a[2], b0 = b[0], b1 = b[1], b2 = b[2];
+    for (int j = 0; j < 64; j++) {
+        const uint64_t s0 = b0 << j;
+        const uint64_t s1 = (b1 << j) | (j ? b0 >> (64 - j) : 0);
+        const uint64_t s2 = (b2 << j) | (j ? b1 >> (64 - j) : 0);
+        const uint64_t s3 = j ? b2 >> (64 - j) : 0;
+        const uint64_t m0 = 0ull - ((a0 >> j) & 1), m1 = 0ull - ((a1 >> j) & 1), m2 = 0ull - ((a2 >> j) & 1);
+        p0 ^= s0 & m0;
+        p1 ^= (s1 & m0) ^ (s0 & m1);
+        p2 ^= (s2 & m0) ^ (s1 & m1) ^ (s0 & m2);
+        p3 ^= (s3 & m0) ^ (s2 & m1) ^ (s1 & m2);
+        p4 ^= (s3 & m1) ^ (s2 & m2);
+        p5 ^= s3 & m2;
     }
     const uint64_t H0 = (p2 >> 34) | (p3 << 30), H1 = (p3 >> 34) | (p4 << 30);
     const uint64_t H2w = (p4 >> 34) | (p5 << 30);
@@ -471,21 +469,41 @@ __device__ void f162_mul_dev(const uint64_t *a, const uint64_t *b, uint64_t *out
 }
 
 // eq(p0)[b] = prod_k (bit k of b ? p_k : 1 + p_k) (eval.rs:51-63), as a product of sub-tables over
-// groups of 8 coordinates: part[c][x] for the coordinates 8c .. 8c + 7 (x < 256)
-__global__ void eq_part_kernel(const uint64_t *__restrict__ p0, uint32_t rows, uint64_t *__restrict__ part) {
+// groups of 8 coordinates: part[c][x] for the coordinates 8c .. 8c + 7 (x < 256), built as a
+// product tree of depth 3 (pairs of coordinates, quads, then the 256 entries).
+__global__ __launch_bounds__(256) void eq_part_kernel(const uint64_t *__restrict__ p0, uint32_t rows,
+                                                      uint64_t *__restrict__ part) {
+    __shared__ uint64_t pr[4][4][3];   // pair j (coordinates 2j, 2j + 1), 2 bits
+    __shared__ uint64_t qd[2][16][3];  // quad j (coordinates 4j .. 4j + 3), 4 bits
     const uint32_t c = blockIdx.x, x = threadIdx.x;
     const uint32_t k0 = 8 * c, nk = min(8u, rows - k0);
-    if (x >= (1u << nk)) return;
-    uint64_t acc[3] = {1, 0, 0};
-    for (uint32_t k = 0; k < nk; k++) {
-        uint64_t f[3] = {p0[3 * (k0 + k)], p0[3 * (k0 + k) + 1], p0[3 * (k0 + k) + 2]};
-        if (!((x >> k) & 1)) f[0] ^= 1;
-        uint64_t t[3];
-        f162_mul_dev(acc, f, t);
-        acc[0] = t[0];
-        acc[1] = t[1];
-        acc[2] = t[2];
+    // a coordinate past the last one contributes the factor 1 (its bit is then always 0)
+    auto factor = [&](uint32_t k, uint32_t bit, uint64_t *f) {
+        if (k >= nk) {
+            f[0] = bit ? 0 : 1;
+            f[1] = f[2] = 0;
+            return;
+        }
+        f[0] = p0[3 * (k0 + k)] ^ (bit ? 0 : 1);
+        f[1] = p0[3 * (k0 + k) + 1];
+        f[2] = p0[3 * (k0 + k) + 2];
+    };
+    if (x < 16) {
+        const uint32_t j = x >> 2, v = x & 3;
+        uint64_t f[3], g2[3];
+        factor(2 * j, v & 1, f);
+        factor(2 * j + 1, v >> 1, g2);
+        f162_mul_dev(f, g2, pr[j][v]);
     }
+    __syncthreads();
+    if (x < 32) {
+        const uint32_t j = x >> 4, v = x & 15;
+        f162_mul_dev(pr[2 * j][v & 3], pr[2 * j + 1][v >> 2], qd[j][v]);
+    }
+    __syncthreads();
+    if (x >= (1u << nk)) return;
+    uint64_t acc[3];
+    f162_mul_dev(qd[0][x & 15], qd[1][x >> 4], acc);
     uint64_t *o = part + 3 * ((size_t)c * 256 + x);
     o[0] = acc[0];
     o[1] = acc[1];
