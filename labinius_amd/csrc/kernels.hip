@@ -565,12 +565,15 @@ __global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restric
                                                        const uint64_t *__restrict__ eq, uint32_t wdim,
                                                        uint32_t cols, uint64_t *__restrict__ u) {
     __shared__ uint64_t T[8][256][3];
-    __shared__ uint64_t NT[8][2][16][3];                  // nibble tables: low and high four elements
-    __shared__ __attribute__((aligned(16))) uint64_t ws[4][3][RE_CHUNK];  // [wave][word][element]
-    __shared__ uint64_t by[4][8][21];  // [wave][group][bit position / 8]: byte b is bits b of the group
+    __shared__ uint64_t NT[8][2][16][3];  // nibble tables: low and high four elements
+    __shared__ __attribute__((aligned(16))) uint32_t ws[4][2][3][2 * RE_CHUNK];  // [wave][col][word][elem, half]
+    __shared__ uint64_t by[4][2][8][21];  // [wave][col][group][bit position / 8]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t j0 = blockIdx.x * RE_COLS, nch = wdim / RE_CHUNK;
     const uint32_t ch0 = blockIdx.y * RE_SPAN, ch1 = min(ch0 + RE_SPAN, nch);
+    bool live[2];
+#pragma unroll
+    for (int cw = 0; cw < 2; cw++) live[cw] = j0 + wave + 4 * cw < cols;
     uint64_t S[2][3][3];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -596,6 +599,17 @@ __global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restric
             NT[g][h][x][1] = a1;
             NT[g][h][x][2] = a2;
         }
+        // both columns' chunks, word-major as dwords: ws[c][k][2 i + half] = half of word k of element i
+#pragma unroll
+        for (int cw = 0; cw < 2; cw++)
+            if (live[cw]) {
+                const uint64_t *src = w + 3 * ((size_t)(j0 + wave + 4 * cw) * wdim + i0);
+                for (int x = lane; x < RE_CHUNK * 3; x += 64) {
+                    const uint64_t v = src[x];
+                    ws[wave][cw][x % 3][2 * (x / 3)] = (uint32_t)v;
+                    ws[wave][cw][x % 3][2 * (x / 3) + 1] = (uint32_t)(v >> 32);
+                }
+            }
         __syncthreads();
         for (int e = tid; e < 8 * 256; e += 256) {
             const int g = e >> 8, m = e & 255;
@@ -604,42 +618,47 @@ __global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restric
             T[g][m][1] = lo[1] ^ hi[1];
             T[g][m][2] = lo[2] ^ hi[2];
         }
-        __syncthreads();
+        // item (g, word k, byte c): bits 64 k + 8 c .. + 7 of the group's 8 elements, gathered
+        // with byte permutes: byte c of the 8 words, then an 8 x 8 bit transpose
 #pragma unroll
-        for (int cw = 0; cw < 2; cw++) {
-            const uint32_t j = j0 + wave + 4 * cw;
-            if (j >= cols) continue;  // wave-uniform; nothing below waits on other waves
-            // the chunk, word-major: ws[k][i] = word k of element i0 + i
-            const uint64_t *src = w + 3 * ((size_t)j * wdim + i0);
-            for (int x = lane; x < RE_CHUNK * 3; x += 64) ws[wave][x % 3][x / 3] = src[x];
-            wave_sync();
-            // item (g, word k, byte c): bits 64 k + 8 c .. + 7 of the group's 8 elements
-            for (int it = lane; it < 8 * 24; it += 64) {
-                const int g = it / 24, k = (it >> 3) % 3, c = it & 7;
-                if (8 * k + c > 20) continue;  // bit positions past 167
-                const uint64_t *e8 = &ws[wave][k][8 * g];
-                uint64_t x = 0;
+        for (int cw = 0; cw < 2; cw++)
+            if (live[cw])
+                for (int it = lane; it < 8 * 24; it += 64) {
+                    const int g = it / 24, k = (it >> 3) % 3, c = it & 7;
+                    if (8 * k + c > 20) continue;  // bit positions past 167
+                    const uint4 *q = (const uint4 *)&ws[wave][cw][k][16 * g];
+                    const uint4 d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
+                    // element t's dword (c < 4: low half, else high half)
+                    const bool hiw = c >= 4;
+                    const uint32_t e0 = hiw ? d0.y : d0.x, e1 = hiw ? d0.w : d0.z, e2 = hiw ? d1.y : d1.x,
+                                   e3 = hiw ? d1.w : d1.z, e4 = hiw ? d2.y : d2.x, e5 = hiw ? d2.w : d2.z,
+                                   e6 = hiw ? d3.y : d3.x, e7 = hiw ? d3.w : d3.z;
+                    const uint32_t cb = c & 3, sel2 = cb | ((cb + 4) << 8);
+                    const uint32_t x01 = __builtin_amdgcn_perm(e1, e0, sel2), x23 = __builtin_amdgcn_perm(e3, e2, sel2);
+                    const uint32_t x45 = __builtin_amdgcn_perm(e5, e4, sel2), x67 = __builtin_amdgcn_perm(e7, e6, sel2);
+                    const uint32_t lo = __builtin_amdgcn_perm(x23, x01, 0x05040100u);
+                    const uint32_t hi = __builtin_amdgcn_perm(x67, x45, 0x05040100u);
+                    by[wave][cw][g][8 * k + c] = transpose8x8(((uint64_t)hi << 32) | lo);
+                }
+        __syncthreads();  // the tables and both columns' bytes
 #pragma unroll
-                for (int t = 0; t < 8; t++) x |= ((e8[t] >> (8 * c)) & 255ull) << (8 * t);
-                by[wave][g][8 * k + c] = transpose8x8(x);
-            }
-            wave_sync();
-            const uint8_t *bb = (const uint8_t *)by[wave];
+        for (int cw = 0; cw < 2; cw++)
+            if (live[cw]) {
+                const uint8_t *bb = (const uint8_t *)by[wave][cw];
 #pragma unroll
-            for (int sl = 0; sl < 3; sl++) {
-                const int b = lane + 64 * sl;
-                if (b < 162) {
+                for (int sl = 0; sl < 3; sl++) {
+                    const int b = lane + 64 * sl;
+                    if (b < 162) {
 #pragma unroll
-                    for (int g = 0; g < 8; g++) {
-                        const uint64_t *t = T[g][bb[g * 168 + b]];
-                        S[cw][sl][0] ^= t[0];
-                        S[cw][sl][1] ^= t[1];
-                        S[cw][sl][2] ^= t[2];
+                        for (int g = 0; g < 8; g++) {
+                            const uint64_t *t = T[g][bb[g * 168 + b]];
+                            S[cw][sl][0] ^= t[0];
+                            S[cw][sl][1] ^= t[1];
+                            S[cw][sl][2] ^= t[2];
+                        }
                     }
                 }
             }
-            wave_sync();  // ws and by are rewritten by the next column
-        }
     }
     // u_j ^= sum_b x^b S_b mod x^162 + x^81 + 1
 #pragma unroll

@@ -30,6 +30,19 @@ def test_blake3_matches_oracle(lbn, orc):
         assert lbn.blake3(data, 40, context="ctx") == orc.blake3(data, 40, context="ctx")
 
 
+def test_blake3_matches_spec_vectors(lbn):
+    import json
+    import os
+    v = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "blake3_spec_vectors.json")))
+    for case in v["cases"]:
+        data = bytes(i % 251 for i in range(case["input_len"]))
+        assert lbn.blake3(data).hex() == case["hash"]
+        if "keyed_hash" in case:
+            assert lbn.blake3(data, key=v["key"].encode()).hex() == case["keyed_hash"]
+        if "derive_key" in case:
+            assert lbn.blake3(data, context=v["context"]).hex() == case["derive_key"]
+
+
 def test_params_errors(lbn):
     # scheme/params.rs:179-195 and tests/api.rs:46-80
     P, M = lbn.Params, lbn.Modulus
