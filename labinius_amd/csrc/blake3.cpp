@@ -9,6 +9,9 @@
 #include "blake3.hpp"
 
 #include <cstring>
+#include <array>
+#include <thread>
+#include <vector>
 
 namespace lbn {
 
@@ -80,7 +83,72 @@ void load_words(const uint8_t *b, uint32_t *w, int n) {
     }
 }
 
+// The chaining value of one whole (non-root) chunk of 1024 bytes.
+void chunk_cv(const uint32_t key[8], uint32_t flags, const uint8_t *p, uint64_t counter, uint32_t out[8]) {
+    uint32_t cv[8], m[16], o[16];
+    std::memcpy(cv, key, 32);
+    for (int b = 0; b < 16; b++) {
+        load_words(p + 64 * b, m, 16);
+        compress(cv, m, counter, 64, flags | (b == 0 ? CHUNK_START : 0) | (b == 15 ? CHUNK_END : 0), o);
+        std::memcpy(cv, o, 32);
+    }
+    std::memcpy(out, cv, 32);
+}
+
+void parent_cv(const uint32_t key[8], uint32_t flags, const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
+    uint32_t m[16], o[16];
+    std::memcpy(m, l, 32);
+    std::memcpy(m + 8, r, 32);
+    compress(key, m, 0, 64, flags | PARENT, o);
+    std::memcpy(out, o, 32);
+}
+
+// The chaining value of a complete (non-root) subtree of n = 2^k chunks starting at chunk
+// `counter`; the spec's tree splits it into two halves of 2^(k-1) chunks.
+void subtree_cv(const uint32_t key[8], uint32_t flags, const uint8_t *p, uint64_t counter, uint64_t n,
+                uint32_t out[8]) {
+    if (n == 1) return chunk_cv(key, flags, p, counter, out);
+    uint32_t l[8], r[8];
+    subtree_cv(key, flags, p, counter, n / 2, l);
+    subtree_cv(key, flags, p + n / 2 * 1024, counter + n / 2, n / 2, r);
+    parent_cv(key, flags, l, r, out);
+}
+
+// The same over up to 16 threads: 16 (or fewer) equal subtrees, then their parents.
+void subtree_cv_threads(const uint32_t key[8], uint32_t flags, const uint8_t *p, uint64_t counter, uint64_t n,
+                        uint32_t out[8]) {
+    uint64_t parts = 16;
+    while (parts > n) parts /= 2;
+    const uint64_t per = n / parts;
+    std::vector<std::array<uint32_t, 8>> cvs(parts);
+    std::vector<std::thread> th;
+    for (uint64_t t = 1; t < parts; t++)
+        th.emplace_back([&, t] { subtree_cv(key, flags, p + t * per * 1024, counter + t * per, per, cvs[t].data()); });
+    subtree_cv(key, flags, p, counter, per, cvs[0].data());
+    for (auto &x : th) x.join();
+    for (uint64_t w = parts; w > 1; w /= 2)
+        for (uint64_t i = 0; i < w / 2; i++) parent_cv(key, flags, cvs[2 * i].data(), cvs[2 * i + 1].data(), cvs[i].data());
+    std::memcpy(out, cvs[0].data(), 32);
+}
+
+constexpr uint64_t SUBTREE_MIN = 16;    // chunks hashed as one subtree when the input allows
+constexpr uint64_t THREADS_MIN = 1024;  // chunks (1 MiB) from which the subtree uses threads
+
 }  // namespace
+
+// A complete subtree of 2^level chunks enters the binary counter at `level`: chunk_ is a multiple of
+// 2^level, so every lower level is empty, and the carry proceeds as when a chunk closes.
+void Blake3::push_subtree(const uint32_t cv_in[8], int level) {
+    uint32_t cv[8];
+    std::memcpy(cv, cv_in, 32);
+    while (has_ >> level & 1) {
+        parent_cv(key_, flags_, subtree_[level], cv, cv);
+        has_ &= ~(1ull << level);
+        level++;
+    }
+    std::memcpy(subtree_[level], cv, 32);
+    has_ |= 1ull << level;
+}
 
 Blake3::Blake3(const uint32_t key[8], uint32_t flags) {
     std::memcpy(key_, key, sizeof key_);
@@ -149,6 +217,27 @@ void Blake3::update(const void *data, size_t len) {
                 std::memcpy(cv_, o, sizeof cv_);
                 blocks_++;
                 buf_len_ = 0;
+            }
+        }
+        // whole aligned subtrees straight from the input, while more input follows them (the last
+        // chunk stays pending: finalize may have to make it the root)
+        if (buf_len_ == 0 && blocks_ == 0 && len > SUBTREE_MIN * 1024) {
+            uint64_t n = 1;
+            while (n * 2 * 1024 < len) n *= 2;
+            while (n > 1 && (chunk_ & (n - 1))) n /= 2;
+            if (n >= SUBTREE_MIN) {
+                uint32_t cv[8];
+                if (n >= THREADS_MIN)
+                    subtree_cv_threads(key_, flags_, p, chunk_, n, cv);
+                else
+                    subtree_cv(key_, flags_, p, chunk_, n, cv);
+                int level = 0;
+                while ((1ull << level) < n) level++;
+                push_subtree(cv, level);
+                chunk_ += n;
+                p += n * 1024;
+                len -= n * 1024;
+                continue;
             }
         }
         size_t take = 64 - buf_len_;

@@ -33,6 +33,7 @@ class Blake3 {
    private:
     Blake3(const uint32_t key[8], uint32_t flags);
     void close_chunk();
+    void push_subtree(const uint32_t cv[8], int level);
     uint32_t key_[8];
     uint32_t flags_;
     // the chunk being filled

@@ -43,6 +43,21 @@ def test_blake3_matches_spec_vectors(lbn):
             assert lbn.blake3(data, context=v["context"]).hex() == case["derive_key"]
 
 
+def test_blake3_subtree_path_matches_oracle(lbn, orc):
+    # large inputs take whole aligned subtrees (threaded from 1 MiB); a prefix shifts the alignment
+    for n in (16 * 1024, 16 * 1024 + 1, 17 * 1024, 3_000_000, (1 << 20) + 5):
+        d = bytes((i * 131 + 7) % 256 for i in range(n))
+        assert lbn.blake3(d, 40) == orc.blake3(d, 40), n
+    for pre in (1, 1024, 2047, 5000):
+        t, o = lbn.Transcript(b"x"), orc.Transcript(b"x")
+        t.absorb_bytes(bytes(pre))
+        o.absorb_bytes(bytes(pre))
+        big = bytes((i * 7) % 256 for i in range(2_500_000 + pre))
+        t.absorb_bytes(big)
+        o.absorb_bytes(big)
+        assert t.fill(b"label", 64) == o.fill(b"label", 64), pre
+
+
 def test_params_errors(lbn):
     # scheme/params.rs:179-195 and tests/api.rs:46-80
     P, M = lbn.Params, lbn.Modulus
