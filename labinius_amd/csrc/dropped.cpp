@@ -1,7 +1,9 @@
 // Bit-dropped commitments on the host: the prover's (top, digits) from the raw rows, their wire
 // packing (what the transcript absorbs), and the verifier's residual check.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -93,27 +95,39 @@ int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, 
     if (rc) return rc;
     if ((rc = check_dropped(s))) return rc;
     const size_t limbs = s.primes.size(), count = s.r * N;
-    // bd/digits.rs:121-132: the columns' coefficients per limb, any representative mod q
-    std::vector<uint32_t> res(limbs * count);
-    std::vector<uint64_t> v(N);
-    for (size_t k = 0; k < limbs; k++) {
-        const Prime &P = prime(s.primes[k]);
-        for (size_t c = 0; c < s.r; c++) {
-            const uint32_t *y = raw + (k * s.r + c) * N;
-            for (int u = 0; u < N; u++) v[u] = y[u];
-            intt(P, v.data());
-            for (int u = 0; u < N; u++) res[k * count + c * N + u] = (uint32_t)(v[u] % P.q);
-        }
-    }
-    // bd/digits.rs:134-182
     const Garner g(s.primes);
     const uint32_t d = s.dropped_bits, round = 1u << (d - 1);
-    std::vector<uint64_t> r(limbs), m(limbs);
-    for (size_t i = 0; i < count; i++) {
-        for (size_t k = 0; k < limbs; k++) r[k] = res[k * count + i];
-        g.digits(r.data(), m.data());
-        top[i] = (uint16_t)((m[0] + round) >> d);
-        for (size_t k = 1; k < limbs; k++) digits[(k - 1) * count + i] = (uint16_t)m[k];
+    std::vector<const Prime *> P(limbs);
+    for (size_t k = 0; k < limbs; k++) P[k] = &prime(s.primes[k]);
+    // columns are independent: a share of them per thread
+    auto work = [&](size_t c0, size_t c1) {
+        std::vector<uint64_t> v(N), r(limbs), m(limbs);
+        std::vector<uint32_t> res(limbs * N);
+        for (size_t c = c0; c < c1; c++) {
+            // bd/digits.rs:121-132: the column's coefficients per limb, any representative mod q
+            for (size_t k = 0; k < limbs; k++) {
+                const uint32_t *y = raw + (k * s.r + c) * N;
+                for (int u = 0; u < N; u++) v[u] = y[u];
+                intt(*P[k], v.data());
+                for (int u = 0; u < N; u++) res[k * N + u] = (uint32_t)(v[u] % P[k]->q);
+            }
+            // bd/digits.rs:134-182
+            for (int u = 0; u < N; u++) {
+                for (size_t k = 0; k < limbs; k++) r[k] = res[k * N + u];
+                g.digits(r.data(), m.data());
+                const size_t i = c * N + u;
+                top[i] = (uint16_t)((m[0] + round) >> d);
+                for (size_t k = 1; k < limbs; k++) digits[(k - 1) * count + i] = (uint16_t)m[k];
+            }
+        }
+    };
+    const size_t nt = std::min<size_t>(std::max<size_t>(1, s.r / 64), std::max(1u, std::thread::hardware_concurrency()));
+    if (nt <= 1) {
+        work(0, s.r);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < nt; t++) th.emplace_back(work, s.r * t / nt, s.r * (t + 1) / nt);
+        for (auto &x : th) x.join();
     }
     return LBN_OK;
 }
