@@ -129,19 +129,24 @@ __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, u
     return tile < g.ntiles && grp < g.groups;
 }
 
+template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem);
 
+// REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
+// half-wave read distinct banks; worth its set-up for long K (the commit), not for the fold's.
+template <bool REP>
 __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides
     for (uint32_t item = blockIdx.x; item < g.items; item += gridDim.x) {
         uint32_t tile, grp;
         if (!gemm_item_of(g, item, tile, grp)) continue;
-        gemm_item(g, tile, grp, smem);
+        gemm_item<REP>(g, tile, grp, smem);
         __syncthreads();  // the next item overwrites the LDS
     }
 }
 
+template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem) {
  const TileDesc td = g.tiles[tile];
 
@@ -154,7 +159,13 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     lut[tid] = g.lut[td.lut_off + tid];
 
     const int jl = 32 * wave + (lane & 31);  // this lane's input within the group
-    const uint16_t *L = lut + ((((int)td.par + jl % TILE_USED) & 1) << 8);
+    const int par = ((int)td.par + jl % TILE_USED) & 1;
+    const uint16_t *L = lut + (par << 8);
+    // the replicated LUT [parity 2][byte 256][copy 32] sits in the epilogue's area, unused until then
+    uint32_t *lutr = (uint32_t *)(smem + 1024 + 2 * A_STEP);
+    if (REP)
+        for (int i = tid; i < 512 * 32; i += GEMM_THREADS) lutr[i] = g.lut[td.lut_off + (i >> 5)];
+    const uint32_t *LR = lutr + (par << 13) + (lane & 31);
     const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + lane) * 16;
     const int8_t *abase = g.afrag + td.a_off;
 
@@ -172,7 +183,18 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
             for (int d = 0; d < 4; d++) {
                 const uint32_t r = w4[d];
-                const uint32_t v0 = L[r & 255], v1 = L[(r >> 8) & 255], v2 = L[(r >> 16) & 255], v3 = L[r >> 24];
+                uint32_t v0, v1, v2, v3;
+                if (REP) {
+                    v0 = LR[(r & 255) << 5];
+                    v1 = LR[((r >> 8) & 255) << 5];
+                    v2 = LR[((r >> 16) & 255) << 5];
+                    v3 = LR[(r >> 24) << 5];
+                } else {
+                    v0 = L[r & 255];
+                    v1 = L[(r >> 8) & 255];
+                    v2 = L[(r >> 16) & 255];
+                    v3 = L[r >> 24];
+                }
                 // p01 = [v0.b0, v1.b0, v0.b1, v1.b1]; then the four low and the four high digits
                 const uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
                 const uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
@@ -754,16 +776,22 @@ hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *cons
 }
 
 hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks) {
-    static const hipError_t attr = hipFuncSetAttribute((const void *)tile_gemm_kernel,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
-    if (attr != hipSuccess) return attr;
+    static const hipError_t attr0 = hipFuncSetAttribute((const void *)tile_gemm_kernel<false>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
+    static const hipError_t attr1 = hipFuncSetAttribute((const void *)tile_gemm_kernel<true>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
+    if (attr0 != hipSuccess) return attr0;
+    if (attr1 != hipSuccess) return attr1;
     if (g.groups == 0 || g.ntiles == 0) return hipSuccess;
     GemmArgs a = g;
     if (a.sched_t == 0 || a.sched_t > 32 || (a.sched_t & (a.sched_t - 1))) a.sched_t = 8;
     a.items = gemm_items(a.ntiles, a.groups, a.sched_t);
     uint32_t grid = a.items;
     if (persistent_blocks && persistent_blocks < grid) grid = (persistent_blocks + 7) / 8 * 8;
-    LBN_LAUNCH(tile_gemm_kernel, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    if (a.kbs >= 32)
+        LBN_LAUNCH(tile_gemm_kernel<true>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    else
+        LBN_LAUNCH(tile_gemm_kernel<false>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
     return hipSuccess;
 }
 
