@@ -133,7 +133,7 @@ template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem);
 
 // REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
-// half-wave read distinct banks; worth its set-up for long K (the commit), not for the fold's.
+// half-wave read distinct banks; its set-up (eight 16-byte stores a thread) pays from 8 K steps.
 template <bool REP>
 __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
@@ -163,8 +163,13 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     const uint16_t *L = lut + (par << 8);
     // the replicated LUT [parity 2][byte 256][copy 32] sits in the epilogue's area, unused until then
     uint32_t *lutr = (uint32_t *)(smem + 1024 + 2 * A_STEP);
-    if (REP)
-        for (int i = tid; i < 512 * 32; i += GEMM_THREADS) lutr[i] = g.lut[td.lut_off + (i >> 5)];
+    if (REP) {
+        // thread t writes entry t's 32 copies: eight 16-byte stores of one value
+        const uint32_t v = g.lut[td.lut_off + tid];
+        const uint4 v4 = make_uint4(v, v, v, v);
+#pragma unroll
+        for (int c = 0; c < 8; c++) *(uint4 *)&lutr[tid * 32 + 4 * c] = v4;
+    }
     const uint32_t *LR = lutr + (par << 13) + (lane & 31);
     const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + lane) * 16;
     const int8_t *abase = g.afrag + td.a_off;
@@ -788,7 +793,7 @@ hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persisten
     a.items = gemm_items(a.ntiles, a.groups, a.sched_t);
     uint32_t grid = a.items;
     if (persistent_blocks && persistent_blocks < grid) grid = (persistent_blocks + 7) / 8 * 8;
-    if (a.kbs >= 32)
+    if (a.kbs >= 8)
         LBN_LAUNCH(tile_gemm_kernel<true>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
     else
         LBN_LAUNCH(tile_gemm_kernel<false>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
