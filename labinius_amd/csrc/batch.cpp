@@ -57,7 +57,7 @@ static int batch_reserve(lbn_prover *pr, size_t cols) {
     int rc;
     if ((rc = dalloc(&b->d_bytes, nb))) return rc;
     HIP_TRY(hipMemset(b->d_bytes, 0, nb));  // padding inputs are never written and must read as zero
-    if ((rc = dalloc(&b->d_yacc, cap * limbs * N))) return rc;
+    if ((rc = dalloc(&b->d_yacc, cap * YACC_PLANES * limbs * N))) return rc;
     if ((rc = dalloc(&b->d_raw, cap * limbs * N))) return rc;
     if ((rc = dalloc(&b->d_comp, cap * 4 * limbs * N162))) return rc;
     if ((rc = dalloc(&b->d_cols, cap))) return rc;
@@ -76,7 +76,6 @@ static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t per
     if (with_prepass)
         HIP_TRY(launch_prepass(st, nullptr, b->d_cols, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)R,
                                b->d_bytes, pp->kb_c, nullptr, 0));
-    HIP_TRY(launch_fill_u32(st, b->d_yacc, R * limbs * N, 0));
     GemmArgs g{};
     g.tiles = pp->d_tiles;
     g.afrag = pp->d_afrag;
@@ -87,11 +86,10 @@ static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t per
     g.kbs = pp->kb_c;
     g.a_kb_stride = A_STEP;
     g.y = b->d_yacc;
-    g.y_batch = limbs * N;
+    g.y_batch = YACC_PLANES * limbs * N;
     g.batches = (uint32_t)R;
     g.groups = (uint32_t)((R + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
-    g.accumulate = 1;
     g.sched_t = sched_t;
     HIP_TRY(launch_tile_gemm(st, g, persist));
     HIP_TRY(launch_commit_finish(st, b->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, b->d_raw,

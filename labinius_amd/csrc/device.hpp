@@ -75,15 +75,20 @@ struct GemmArgs {
     const uint8_t *bytes;     // the input bytes, fragment order (byte_offset)
     uint32_t kbs;             // K steps
     uint32_t a_kb_stride;     // bytes of A fragments per K step of a tile (A_STEP)
-    uint32_t *y;              // output rows
+    uint32_t *y;              // output rows; every (batch element, tile output) has its own word
     uint64_t y_batch;         // words per batch element in y
     uint32_t batches;
     uint32_t groups;          // ceil(batches / 3)
     uint32_t ntiles;
-    int accumulate;           // 1: atomicAdd (several tiles add into one slot), 0: store
     uint32_t sched_t;         // tiles per XCD round (1 .. 32, a power of two); the rest are groups
     uint32_t items;           // work items (set by the launcher)
 };
+
+// The commit GEMM's output is y[c][limb][plane][648]. A splitting limb's 8 tiles write each of
+// the 648 slots once, into plane 0. A quadratic limb's slot 2j + t is written by the two tiles of
+// input parity b = 0 and 1, each into plane b; the finish adds the planes. Every word then has
+// one writer, so the GEMM stores without atomics and y needs no clearing.
+constexpr int YACC_PLANES = 2;
 
 // Per-limb constants of the commit finish (decomposition) kernel.
 struct FinishLimb {

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // =============================================================================================
 
 // One tile over one group of 3 batch elements: Z[m][j] = sum_k A[k][m] x_k[j] for the group's
-// 256 inputs j = 81 b + n (243 used) over all K, then y[b][out[m]] (+)= sum_n E[m][n] Z[m][81 b + n]
+// 256 inputs j = 81 b + n (243 used) over all K, then y[b][out[m]] = sum_n E[m][n] Z[m][81 b + n]
 // mod q.
 //
 // Eight waves, wave w owning inputs 32 w .. 32 w + 31. A (both digits, three 32-row blocks) is
@@ -105,7 +105,15 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // Blocks are ordered so that the 32 blocks an XCD runs at once are 8 tiles x 4 groups: each
 // A and each group's bytes then come from L2 for 4 and 8 blocks respectively.
 constexpr int GEMM_THREADS = 512;
-constexpr int GEMM_LDS = 1024 + GROUP_COLS * TILE * 4 + EE_TILE * 2;
+// Z rows in LDS are padded from 96 to 100 words: the 32 inputs a half-wave stores then start on 16
+// distinct groups of 4 banks instead of 2.
+constexpr int ZROW = TILE + 4;
+constexpr int GEMM_LDS = 1024 + GROUP_COLS * ZROW * 4 + EE_TILE * 2;
+// A is staged per K step in an LDS buffer of AP 16-byte words per thread (A_STEP bytes used).
+constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
+constexpr int A_LDS = AP * GEMM_THREADS * 16;
+// During the K loop: LUT (1 KB), two A buffers, the replicated LUT (64 KB); all below E's area.
+static_assert(1024 + 2 * A_LDS + 2 * 256 * 32 * 4 <= 1024 + GROUP_COLS * ZROW * 4, "K-loop LDS overlaps E");
 
 __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
@@ -151,9 +159,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
  const TileDesc td = g.tiles[tile];
 
     uint16_t *lut = (uint16_t *)smem;                                   // [2][256]
-    uint8_t *abuf = smem + 1024;                                        // [2][A_STEP]
-    uint32_t *zbuf = (uint32_t *)(smem + 1024);                         // [256][96], after the loop
-    uint16_t *ee = (uint16_t *)(smem + 1024 + GROUP_COLS * TILE * 4);   // [81][96]
+    uint8_t *abuf = smem + 1024;                                        // [2][A_LDS]
+    uint32_t *zbuf = (uint32_t *)(smem + 1024);                         // [256][ZROW], after the loop
+    uint16_t *ee = (uint16_t *)(smem + 1024 + GROUP_COLS * ZROW * 4);   // [81][96]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     lut[tid] = g.lut[td.lut_off + tid];
@@ -162,7 +170,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     const int par = ((int)td.par + jl % TILE_USED) & 1;
     const uint16_t *L = lut + (par << 8);
     // the replicated LUT [parity 2][byte 256][copy 32] sits in the epilogue's area, unused until then
-    uint32_t *lutr = (uint32_t *)(smem + 1024 + 2 * A_STEP);
+    uint32_t *lutr = (uint32_t *)(smem + 1024 + 2 * A_LDS);
     if (REP) {
         // thread t writes entry t's 32 copies: eight 16-byte stores of one value
         const uint32_t v = g.lut[td.lut_off + tid];
@@ -210,19 +218,19 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             bh[kh] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
-    constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
+    // Every thread loads and stores AP 16-byte words of A a step, with no branch: the compiler can
+    // then count the loads in flight and wait for the input bytes alone. Words past the step's
+    // 768 repeat the last one and land in the buffer's padding.
     v4i apre[AP];
     auto load_a = [&](uint32_t kb) {
         const int8_t *ak = abase + (size_t)kb * g.a_kb_stride;
 #pragma unroll
         for (int i = 0; i < AP; i++)
-            if (tid + GEMM_THREADS * i < A_STEP / 16) apre[i] = *(const v4i *)(ak + (tid + GEMM_THREADS * i) * 16);
+            apre[i] = *(const v4i *)(ak + min(tid + GEMM_THREADS * i, A_STEP / 16 - 1) * 16);
     };
     auto store_a = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < AP; i++)
-            if (tid + GEMM_THREADS * i < A_STEP / 16)
-                *(v4i *)(abuf + buf * A_STEP + (tid + GEMM_THREADS * i) * 16) = apre[i];
+        for (int i = 0; i < AP; i++) *(v4i *)(abuf + buf * A_LDS + (tid + GEMM_THREADS * i) * 16) = apre[i];
     };
 
     v16i acc[3][3];
@@ -231,11 +239,23 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
         for (int c = 0; c < 3; c++) acc[a][c] = (v16i){};
 
+    // E and the output slots are loaded now, behind the K loop; E's LDS area is not used before
+    // the epilogue. 81 x 96 x 2 bytes are 972 16-byte words, two per thread.
+    static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
+    static_assert(GROUP_BATCH * TILE <= GEMM_THREADS, "one epilogue output a thread");
+    constexpr int EE_V = EE_TILE * 2 / 16;
+    const uint4 *eesrc = (const uint4 *)(g.ee + td.ee_off);
+    uint4 ee0 = eesrc[tid], ee1 = make_uint4(0, 0, 0, 0);
+    if (tid + GEMM_THREADS < EE_V) ee1 = eesrc[tid + GEMM_THREADS];
+    const int slot = tid < GROUP_BATCH * TILE ? g.out[td.out_off + tid % TILE] : -1;
+
     const uint32_t kbs = g.kbs;
     load_a(0);
     load_raw(0);
+    ((uint4 *)ee)[tid] = ee0;
+    if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
     store_a(0);
-    __syncthreads();  // the LUT and A(0)
+    __syncthreads();  // the LUT, E and A(0)
     build();
     load_a(kbs > 1 ? 1 : 0);
     load_raw(kbs > 1 ? 1 : 0);
@@ -243,14 +263,15 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // this step's MFMAs: past the last step the prefetches repeat the last one and the
     // B fragments they build are not used.
     for (uint32_t kb = 0; kb < kbs; kb++) {
-        const uint8_t *ab = abuf + (kb & 1) * A_STEP;
+        const uint8_t *ab = abuf + (kb & 1) * A_LDS;
         v4i cl[2], ch[2];
 #pragma unroll
         for (int kh = 0; kh < 2; kh++) {
             cl[kh] = bl[kh];
             ch[kh] = bh[kh];
         }
-        build();  // step kb + 1 (its bytes are loaded)
+        build();                          // step kb + 1 (its bytes are loaded)
+        load_raw(min(kb + 2, kbs - 1));   // a whole step ahead, and older than the A loads below
 #pragma unroll
         for (int kh = 0; kh < 2; kh++) {
             v4i a0[3], a1[3];
@@ -275,7 +296,6 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         }
         store_a((kb + 1) & 1);  // A(kb + 1); buffer (kb + 1) & 1 was last read at step kb - 1
         load_a(min(kb + 2, kbs - 1));
-        load_raw(min(kb + 2, kbs - 1));
         __syncthreads();
     }
 
@@ -283,7 +303,6 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // block holds, in register r, row 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
     const uint32_t q = td.q;
     const double qinv = 1.0 / (double)q;
-    for (int i = tid; i < EE_TILE; i += GEMM_THREADS) ee[i] = g.ee[td.ee_off + i];
 #pragma unroll
     for (int mb = 0; mb < 3; mb++)
 #pragma unroll
@@ -297,25 +316,20 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
                 z4[r] = mod_q_i64(z, q, qinv);
             }
             const int m0 = 32 * mb + 8 * r4 + 4 * (lane >> 5);
-            *(uint4 *)&zbuf[jl * TILE + m0] = make_uint4(z4[0], z4[1], z4[2], z4[3]);
+            *(uint4 *)&zbuf[jl * ZROW + m0] = make_uint4(z4[0], z4[1], z4[2], z4[3]);
         }
     __syncthreads();
-    // epilogue 2: y[b][out[m]] (+)= sum_n E[m][n] Z[81 b + n][m]
-    for (int it = tid; it < GROUP_BATCH * TILE; it += GEMM_THREADS) {
-        const int b = it / TILE, m = it % TILE;
+    // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m]
+    {
+        const int b = tid / TILE, m = tid % TILE;
         const uint32_t batch = grp * GROUP_BATCH + b;
-        const int slot = g.out[td.out_off + m];
-        if (batch >= g.batches || slot < 0) continue;
+        if (slot < 0 || batch >= g.batches) return;  // slot < 0 for tid >= 288 too
         unsigned long long sum = 0;
-        const uint32_t *zb = zbuf + (TILE_USED * b) * TILE + m;
+        const uint32_t *zb = zbuf + (TILE_USED * b) * ZROW + m;
 #pragma unroll 9
-        for (int n = 0; n < TILE_USED; n++) sum += (unsigned long long)ee[n * TILE + m] * zb[n * TILE];
+        for (int n = 0; n < TILE_USED; n++) sum += (unsigned long long)ee[n * TILE + m] * zb[n * ZROW];
         const uint32_t v = (uint32_t)(sum % q);
-        uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
-        if (g.accumulate)
-            atomicAdd(yp, v);
-        else
-            *yp = v;
+        g.y[(size_t)batch * g.y_batch + td.y_off + slot] = v;
     }
 }
 
@@ -341,7 +355,7 @@ __global__ void commit_finish_kernel(const uint32_t *__restrict__ yacc, uint32_t
     if (s >= 162) return;
     const FinishLimb &f = fl[k];
     const uint32_t q = f.q;
-    const uint32_t *y = yacc + ((size_t)c * limbs + k) * 648;
+    const uint32_t *y = yacc + ((size_t)c * limbs + k) * (YACC_PLANES * 648);
     uint32_t *rw = raw + ((size_t)k * cols + c) * 648;
     int16_t *out = comp + (size_t)c * 4 * limbs * 162;  // [t][limb][s]
     if (!f.quad) {
@@ -362,7 +376,8 @@ __global__ void commit_finish_kernel(const uint32_t *__restrict__ yacc, uint32_t
         const uint32_t jp = qcs[s], jm = qcs[162 + s];
 #pragma unroll
         for (int kk = 0; kk < 2; kk++) {
-            const uint32_t ep = y[2 * jp + kk] % q, em = y[2 * jm + kk] % q;
+            const uint32_t ep = (y[2 * jp + kk] + y[648 + 2 * jp + kk]) % q;
+            const uint32_t em = (y[2 * jm + kk] + y[648 + 2 * jm + kk]) % q;
             rw[2 * jp + kk] = ep;
             rw[2 * jm + kk] = em;
             out[((size_t)kk * limbs + k) * 162 + s] = centered(mulmod(ep + em, f.inv2, q), q);

@@ -214,7 +214,7 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
         pack_ee(tiles[ti], H.ee.data() + ee_tile * ti);
         for (int m = 0; m < TILE; m++) H.out[TILE * ti + m] = (int16_t)tiles[ti].out[m];
         H.desc[ti] = TileDesc{ti * a_tile, (uint32_t)(512 * ti), (uint32_t)(ee_tile * ti), (uint32_t)(TILE * ti),
-                              q, tiles[ti].par, (uint32_t)(k * N)};
+                              q, tiles[ti].par, (uint32_t)((YACC_PLANES * k + tiles[ti].par) * N)};
     }
 
     // the commitment's finish
@@ -369,7 +369,6 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     HIP_TRY(hipEventRecord(pr->ev[0], st));
     HIP_TRY(launch_prepass(st, d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r,
                            pr->d_bytes_c, pp->kb_c, bytes_f, pp->kb_f));
-    HIP_TRY(launch_fill_u32(st, pr->d_yacc, s.r * limbs * N, 0));
     HIP_TRY(hipEventRecord(pr->ev[1], st));
     GemmArgs g{};
     g.tiles = pp->d_tiles;
@@ -381,11 +380,10 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     g.kbs = pp->kb_c;
     g.a_kb_stride = A_STEP;
     g.y = pr->d_yacc;
-    g.y_batch = limbs * N;
+    g.y_batch = YACC_PLANES * limbs * N;
     g.batches = (uint32_t)s.r;
     g.groups = (uint32_t)((s.r + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
-    g.accumulate = 1;
     g.sched_t = 8;
     HIP_TRY(launch_tile_gemm(st, g, 0));
     HIP_TRY(hipEventRecord(pr->ev[2], st));
@@ -431,7 +429,6 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     g.batches = (uint32_t)s.len_ring;
     g.groups = (uint32_t)((s.len_ring + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles_f;
-    g.accumulate = 0;
     g.sched_t = 8;
     HIP_TRY(launch_tile_gemm(st, g, 0));
     HIP_TRY(hipEventRecord(pr->ev[6], st));
@@ -553,7 +550,7 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     const size_t bc = bytes_size((uint32_t)s.r, pp->kb_c);
     TRY_RC(dalloc(&pr->d_bytes_c, bc));
     TRY_RC(hipMemset(pr->d_bytes_c, 0, bc) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "memset"));
-    TRY_RC(dalloc(&pr->d_yacc, s.r * limbs * N));
+    TRY_RC(dalloc(&pr->d_yacc, s.r * YACC_PLANES * limbs * N));
     TRY_RC(dalloc(&pr->d_raw, s.r * limbs * N));
     TRY_RC(dalloc(&pr->d_comp, s.r * 4 * limbs * N162));
     TRY_RC(dalloc(&pr->d_ch, s.r));

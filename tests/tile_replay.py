@@ -9,6 +9,7 @@ N = 648
 TILE, KSTEP, NBLK, FRAG = 96, 64, 3, 1024
 A_STEP = 2 * 2 * NBLK * FRAG
 EE_TILE = 81 * TILE
+YACC_PLANES = 2  # commit output planes: one per input parity of a quadratic limb
 DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("ee_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
                  ("par", "<u4"), ("y_off", "<u4")])
 
@@ -87,7 +88,8 @@ def replay_commit(lbn, params, seed, w):
     out = debug(lbn, p, seed, 3, np.int16)
     desc = debug(lbn, p, seed, 4, DESC)
     rb = ring_bytes(w, cols, L)
-    y = np.zeros((cols, limbs * N), np.int64)
+    y = np.zeros((cols, YACC_PLANES * limbs * N), np.int64)
+    written = np.zeros(y.shape, bool)
     a_tile = kbs * A_STEP
     for d in desc:
         A = a_matrix(afrag[d["a_off"]:d["a_off"] + a_tile], kbs)
@@ -101,10 +103,14 @@ def replay_commit(lbn, params, seed, w):
             rows = tile_rows(A, X, E, int(d["q"]))
             for m in range(TILE):
                 if o[m] >= 0:
-                    y[c, d["y_off"] + o[m]] += rows[m]
+                    # the GEMM stores without atomics: every word has exactly one writer
+                    assert not written[c, d["y_off"] + o[m]]
+                    written[c, d["y_off"] + o[m]] = True
+                    y[c, d["y_off"] + o[m]] = rows[m]
     raw = np.zeros((limbs, cols, N), np.uint32)
     for k, q in enumerate(p.primes()):
-        raw[k] = (y[:, k * N:(k + 1) * N] % q).astype(np.uint32)
+        planes = y[:, YACC_PLANES * k * N:YACC_PLANES * (k + 1) * N].reshape(cols, YACC_PLANES, N)
+        raw[k] = (planes.sum(axis=1) % q).astype(np.uint32)
     return raw
 
 
