@@ -108,12 +108,19 @@ constexpr int GEMM_THREADS = 512;
 // Z rows in LDS are padded from 96 to 100 words: the 32 inputs a half-wave stores then start on 16
 // distinct groups of 4 banks instead of 2.
 constexpr int ZROW = TILE + 4;
-constexpr int GEMM_LDS = 1024 + GROUP_COLS * ZROW * 4 + EE_TILE * 2;
 // A is staged per K step in an LDS buffer of AP 16-byte words per thread (A_STEP bytes used).
 constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
 constexpr int A_LDS = AP * GEMM_THREADS * 16;
-// During the K loop: LUT (1 KB), two A buffers, the replicated LUT (64 KB); all below E's area.
-static_assert(1024 + 2 * A_LDS + 2 * 256 * 32 * 4 <= 1024 + GROUP_COLS * ZROW * 4, "K-loop LDS overlaps E");
+// LDS (the kernel's only LDS is the dynamic block, at address 0):
+//   K loop:   [0, 64 K) replicated LUT | plain LUT (1 KB) | two A buffers
+//   epilogue: [0, ...) Z [256][ZROW] words, overwriting the above
+//   E, loaded at item start and read in the epilogue, above both.
+// The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
+// address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
+constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
+constexpr int EE_OFF = GROUP_COLS * ZROW * 4;
+constexpr int GEMM_LDS = EE_OFF + EE_TILE * 2;
+static_assert(ABUF_OFF + 2 * A_LDS <= EE_OFF, "K-loop LDS overlaps E");
 
 __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
@@ -154,31 +161,38 @@ __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) 
     }
 }
 
+// The LDS word at byte address a. The GEMM's dynamic LDS block is the kernel's only LDS, so it
+// starts at address 0 and the LUT offsets above are LDS addresses.
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+__device__ __forceinline__ lds_u32 *lds_word(uint32_t a) { return (lds_u32 *)(uintptr_t)(a + LUTR_OFF); }
+
 template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem) {
  const TileDesc td = g.tiles[tile];
 
-    uint16_t *lut = (uint16_t *)smem;                                   // [2][256]
-    uint8_t *abuf = smem + 1024;                                        // [2][A_LDS]
-    uint32_t *zbuf = (uint32_t *)(smem + 1024);                         // [256][ZROW], after the loop
-    uint16_t *ee = (uint16_t *)(smem + 1024 + GROUP_COLS * ZROW * 4);   // [81][96]
+    uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
+    uint8_t *abuf = smem + ABUF_OFF;                  // [2][A_LDS]
+    uint32_t *zbuf = (uint32_t *)smem;                // [256][ZROW], after the loop
+    uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [81][96]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    lut[tid] = g.lut[td.lut_off + tid];
+    const uint32_t lv = g.lut[td.lut_off + tid];     // entry (parity tid >> 8, byte tid & 255)
+    if (REP) {
+        // its 32 copies: eight 16-byte stores of one value
+        const uint4 v4 = make_uint4(lv, lv, lv, lv);
+        uint8_t *row = smem + LUTR_OFF + (tid & 255) * 256 + (tid >> 8) * 128;
+#pragma unroll
+        for (int c = 0; c < 8; c++) *(uint4 *)(row + 16 * c) = v4;
+    } else {
+        lut[tid] = (uint16_t)lv;
+    }
 
     const int jl = 32 * wave + (lane & 31);  // this lane's input within the group
     const int par = ((int)td.par + jl % TILE_USED) & 1;
     const uint16_t *L = lut + (par << 8);
-    // the replicated LUT [parity 2][byte 256][copy 32] sits in the epilogue's area, unused until then
-    uint32_t *lutr = (uint32_t *)(smem + 1024 + 2 * A_LDS);
-    if (REP) {
-        // thread t writes entry t's 32 copies: eight 16-byte stores of one value
-        const uint32_t v = g.lut[td.lut_off + tid];
-        const uint4 v4 = make_uint4(v, v, v, v);
-#pragma unroll
-        for (int c = 0; c < 8; c++) *(uint4 *)&lutr[tid * 32 + 4 * c] = v4;
-    }
-    const uint32_t *LR = lutr + (par << 13) + (lane & 31);
+    // byte 0 of a lookup address: this lane's parity and copy; byte 1 comes from the input word
+    const uint32_t lsel = (uint32_t)(par << 7) | (uint32_t)((lane & 31) << 2);
+
     const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + lane) * 16;
     const int8_t *abase = g.afrag + td.a_off;
 
@@ -198,10 +212,10 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
                 const uint32_t r = w4[d];
                 uint32_t v0, v1, v2, v3;
                 if (REP) {
-                    v0 = LR[(r & 255) << 5];
-                    v1 = LR[((r >> 8) & 255) << 5];
-                    v2 = LR[((r >> 16) & 255) << 5];
-                    v3 = LR[(r >> 24) << 5];
+                    v0 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0400u));
+                    v1 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0500u));
+                    v2 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0600u));
+                    v3 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0700u));
                 } else {
                     v0 = L[r & 255];
                     v1 = L[(r >> 8) & 255];
