@@ -201,6 +201,15 @@ int lbn_verify_opening_clear(const lbn_pp *pp, const int16_t *folded_commitment,
 /* bd/digits.rs:121-182 (drop_bits, called by Prover::commit, scheme/prover.rs:69-72): from the raw
  * rows ([limb][c][648] u32 as lbn_commit returns them) */
 int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, uint16_t *digits);
+/* The same drop_bits (bd/digits.rs:121-182) on the device: every limb's inverse NTT, then the
+ * Garner digits, on the prover's stream. raw is host memory as for lbn_drop_bits; params must
+ * have the prover's moduli. The outputs equal lbn_drop_bits's. */
+int lbn_drop_bits_device(lbn_prover *pr, const lbn_params *params, const uint32_t *raw, uint16_t *top,
+                         uint16_t *digits);
+/* Prover::commit in bit-dropped mode (scheme/prover.rs:69-72): drop_bits on the device over the raw
+ * rows of the prover's last lbn_commit / lbn_commit_resident, which are still in HBM (the prover's
+ * parameters drop bits; LBN_ERR_ARG before any commitment). */
+int lbn_commit_drop_bits(lbn_prover *pr, uint16_t *top, uint16_t *digits);
 /* wire/mod.rs:386-397 (pack_dropped); with out == NULL only *len is set */
 int lbn_pack_dropped(const lbn_params *params, const uint16_t *top, const uint16_t *digits, uint8_t *out,
                      size_t cap, size_t *len);

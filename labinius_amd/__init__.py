@@ -372,6 +372,27 @@ class Dropped:
         _check(lib().lbn_drop_bits(ctypes.byref(params.c()), _p(raw), _p(top), _p(digits)))
         return cls(params, top, digits)
 
+    @classmethod
+    def _empty(cls, params):
+        count = params.columns() * N
+        return np.zeros(count, np.uint16), np.zeros((len(params.primes()) - 1, count), np.uint16)
+
+    @classmethod
+    def of_device(cls, prover, params, raw):
+        """The same digits computed on the GPU by prover (whose moduli params shares)."""
+        top, digits = cls._empty(params)
+        raw = np.ascontiguousarray(raw, np.uint32)
+        _check(lib().lbn_drop_bits_device(ctypes.c_void_p(prover._h), ctypes.byref(params.c()), _p(raw), _p(top),
+                                          _p(digits)))
+        return cls(params, top, digits)
+
+    @classmethod
+    def of_last_commit(cls, prover):
+        """The digits of prover's last commitment, from its raw rows still on the GPU."""
+        top, digits = cls._empty(prover.params)
+        _check(lib().lbn_commit_drop_bits(ctypes.c_void_p(prover._h), _p(top), _p(digits)))
+        return cls(prover.params, top, digits)
+
     def dropped_bits(self):
         return self.params.dropped_bits()
 
@@ -516,7 +537,7 @@ class Prover:
         out = []
         for r, comp, raw in zip(resident, comps, raws):
             p = r.params
-            dropped = Dropped.of_raw(p, raw) if p.dropped_bits() > 0 else None
+            dropped = Dropped.of_device(self, p, raw) if p.dropped_bits() > 0 else None
             out.append(Commitment(p.primes(), p.columns(), comp, raw, dropped))
         return out
 
@@ -546,7 +567,7 @@ class Prover:
 
     def _commitment(self, comp, raw):
         p = self.params
-        dropped = Dropped.of_raw(p, raw) if p.dropped_bits() > 0 else None
+        dropped = Dropped.of_last_commit(self) if p.dropped_bits() > 0 else None
         return Commitment(p.primes(), p.columns(), comp, raw, dropped)
 
     def commit_resident(self, rw, want_outputs=True):

@@ -99,12 +99,23 @@ struct FinishLimb {
     uint16_t twist[4][162]; // splitting: 4^-1 psi^{-v_s k}; quadratic: [0] = 2^-1 psi'^{-v_s}
 };
 
-// Constants of the base limb's inverse transform (the fold's last step).
+// Constants of one limb's inverse transform (the fold's last step on the base limb; every limb's
+// for the bit-dropped digits).
 struct InttConsts {
     uint32_t q;
     uint32_t quad;
     uint32_t w, w2, z6, inv2, inv3, det;
     uint32_t twi_off[7];    // offset of level l's inverse twiddles in the twi array
+};
+
+// Mixed-radix (Garner) digits of the bit-dropped commitment (bd/digits.rs:134-182): x = m_0 +
+// q_0 (m_1 + q_1 (m_2 + ...)) from the residues r_k, with inv[k][j] = q_j^-1 mod q_k.
+constexpr int GARNER_MAX = 8;
+struct GarnerConsts {
+    uint32_t limbs;
+    uint32_t d, round;  // dropped bits, 2^(d - 1)
+    uint32_t q[GARNER_MAX];
+    uint32_t inv[GARNER_MAX][GARNER_MAX];
 };
 
 }  // namespace lbn

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "launch.hpp"
 
 namespace lbn {
 namespace {
@@ -81,6 +82,53 @@ std::vector<uint8_t> pack(const Shape &s, const uint16_t *top, const uint16_t *d
     return out;
 }
 
+// The prover's bit-dropped scratch for n = limbs x coefficients (grow-only).
+int bd_reserve(lbn_prover *pr, size_t n) {
+    if (n <= pr->bd_cap) return LBN_OK;
+    void *old[] = {pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    pr->d_bd_raw = nullptr;
+    pr->d_bd_cen = nullptr;
+    pr->d_bd_out = nullptr;
+    pr->bd_cap = 0;
+    int rc;
+    if ((rc = dalloc(&pr->d_bd_raw, n))) return rc;
+    if ((rc = dalloc(&pr->d_bd_cen, n))) return rc;
+    if ((rc = dalloc(&pr->d_bd_out, n))) return rc;
+    pr->bd_cap = n;
+    return LBN_OK;
+}
+
+// bd/digits.rs:121-182 on the device: the raw rows d_raw ([limb][c][648], on the device) of a
+// commitment of shape s -> every limb's inverse transform -> Garner -> top and digits (host).
+int drop_bits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw, uint16_t *top, uint16_t *digits) {
+    const lbn_pp *pp = pr->pp;
+    const size_t limbs = s.primes.size(), count = s.r * N;
+    if (limbs > (size_t)GARNER_MAX) return fail(LBN_ERR_SHAPE, "too many moduli for the device digits");
+    if (pp->intt_limb.size() != limbs) return fail(LBN_ERR_SHAPE, "the moduli differ from the prover's");
+    int rc = bd_reserve(pr, limbs * count);
+    if (rc) return rc;
+    hipStream_t st = pr->stream;
+    for (size_t k = 0; k < limbs; k++)
+        HIP_TRY(launch_intt(st, d_raw + k * count, (uint32_t)s.r, pp->intt_limb[k], pp->d_twi_limb[k],
+                            pr->d_bd_cen + k * count));
+    GarnerConsts gc{};
+    gc.limbs = (uint32_t)limbs;
+    gc.d = s.dropped_bits;
+    gc.round = 1u << (s.dropped_bits - 1);
+    for (size_t k = 0; k < limbs; k++) {
+        gc.q[k] = s.primes[k];
+        for (size_t j = 0; j < k; j++) gc.inv[k][j] = (uint32_t)inv_mod(s.primes[j] % s.primes[k], s.primes[k]);
+    }
+    HIP_TRY(launch_garner(st, pr->d_bd_cen, gc, (uint32_t)count, pr->d_bd_out, pr->d_bd_out + count));
+    HIP_TRY(hipMemcpyAsync(top, pr->d_bd_out, count * sizeof(uint16_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(digits, pr->d_bd_out + count, (limbs - 1) * count * sizeof(uint16_t),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return LBN_OK;
+}
+
 }  // namespace
 }  // namespace lbn
 
@@ -130,6 +178,30 @@ int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, 
         for (auto &x : th) x.join();
     }
     return LBN_OK;
+}
+
+int lbn_drop_bits_device(lbn_prover *pr, const lbn_params *params, const uint32_t *raw, uint16_t *top,
+                         uint16_t *digits) {
+    if (!pr || !params || !raw || !top || !digits) return fail(LBN_ERR_ARG, "null argument");
+    Shape s;
+    int rc = make_shape(params, &s);
+    if (rc) return rc;
+    if ((rc = check_dropped(s))) return rc;
+    if (s.primes != pr->pp->shape.primes) return fail(LBN_ERR_SHAPE, "the moduli differ from the prover's");
+    const size_t limbs = s.primes.size(), count = s.r * N;
+    if (limbs > (size_t)GARNER_MAX) return fail(LBN_ERR_SHAPE, "too many moduli for the device digits");
+    if ((rc = bd_reserve(pr, limbs * count))) return rc;
+    HIP_TRY(hipMemcpyAsync(pr->d_bd_raw, raw, limbs * count * sizeof(uint32_t), hipMemcpyHostToDevice, pr->stream));
+    return drop_bits_device(pr, s, pr->d_bd_raw, top, digits);
+}
+
+int lbn_commit_drop_bits(lbn_prover *pr, uint16_t *top, uint16_t *digits) {
+    if (!pr || !top || !digits) return fail(LBN_ERR_ARG, "null argument");
+    const Shape &s = pr->pp->shape;
+    int rc = check_dropped(s);
+    if (rc) return rc;
+    if (!pr->committed) return fail(LBN_ERR_ARG, "the prover has made no commitment");
+    return drop_bits_device(pr, s, pr->d_raw, top, digits);
 }
 
 int lbn_pack_dropped(const lbn_params *params, const uint16_t *top, const uint16_t *digits, uint8_t *out,

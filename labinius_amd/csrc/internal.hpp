@@ -129,6 +129,9 @@ struct lbn_pp {
     uint32_t cond_f = 0, mult_f = 0;
     lbn::InttConsts intt{};
     uint32_t *d_twi = nullptr;
+    // every limb's inverse transform (the bit-dropped digits)
+    std::vector<lbn::InttConsts> intt_limb;
+    std::vector<uint32_t *> d_twi_limb;
     std::vector<void *> allocations;
 };
 
@@ -180,6 +183,12 @@ struct lbn_prover {
     int16_t *d_v = nullptr;
     uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_eqpart = nullptr, *d_u = nullptr;
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
+    // the bit-dropped digits (grow-only): raw rows from the host, centered coefficients, top + digits
+    uint32_t *d_bd_raw = nullptr;
+    int16_t *d_bd_cen = nullptr;
+    uint16_t *d_bd_out = nullptr;
+    size_t bd_cap = 0;       // coefficients x limbs
+    bool committed = false;  // d_raw holds the raw rows of a commitment
     hipEvent_t ev[12] = {};
     float timings[11] = {};
     lbn_batch_ws *batch = nullptr;

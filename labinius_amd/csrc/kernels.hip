@@ -4,7 +4,8 @@
 //   tile_gemm_kernel      the int8-MFMA tile GEMM behind both the commitment and the fold
 //   commit_finish_kernel  y mod q -> raw rows and the 648 -> 4 x 162 decomposition
 //   chat_kernel           the challenges' transform c_j(-X^4) at every slot, as MFMA A fragments
-//   intt_kernel           the base prime's inverse transform -> centered folded witness
+//   intt_kernel           a limb's inverse transform -> centered coefficients (the folded witness)
+//   garner_kernel         the bit-dropped commitment's mixed-radix digits and top values
 //   eq_kernel             the eq tensor over F162
 //   row_eval_kernel       u_j = sum_i eq_i W[i + wdim j] over F162
 //
@@ -753,6 +754,33 @@ __global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restric
     }
 }
 
+// =============================================================================================
+// the bit-dropped digits
+// =============================================================================================
+
+// bd/digits.rs:134-182, one thread per coefficient i of count: the limbs' centered coefficients
+// cen[k][i] -> residues r_k in [0, q_k) -> mixed-radix digits m_k, top = (m_0 + 2^(d-1)) >> d.
+__global__ void garner_kernel(const int16_t *__restrict__ cen, GarnerConsts gc, uint32_t count,
+                              uint16_t *__restrict__ top, uint16_t *__restrict__ digits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t m[GARNER_MAX];
+#pragma unroll
+    for (int k = 0; k < GARNER_MAX; k++) {
+        if (k >= (int)gc.limbs) break;
+        const uint32_t qk = gc.q[k];
+        const int c = cen[(size_t)k * count + i];
+        uint32_t x = c < 0 ? (uint32_t)(c + (int)qk) : (uint32_t)c;
+#pragma unroll
+        for (int j = 0; j < k; j++) x = (x + qk - m[j] % qk) % qk * gc.inv[k][j] % qk;
+        m[k] = x;
+    }
+    top[i] = (uint16_t)((m[0] + gc.round) >> gc.d);
+#pragma unroll
+    for (int k = 1; k < GARNER_MAX; k++)
+        if (k < (int)gc.limbs) digits[(size_t)(k - 1) * count + i] = (uint16_t)m[k];
+}
+
 // One v_mfma_i32_16x16x64_i8 with the operand and result lane layouts the tile GEMM assumes:
 // A[m][k] row-major 16 x 64, B[k][n] row-major 64 x 16, D[m][n] 16 x 16. One wave.
 __global__ void mfma_probe_kernel(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
@@ -849,6 +877,13 @@ hipError_t launch_chat(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, u
 hipError_t launch_intt(hipStream_t s, const uint32_t *vhat, uint32_t L, const InttConsts &ic,
                        const uint32_t *twi, int16_t *v) {
     LBN_LAUNCH(intt_kernel, dim3(L), dim3(256), 0, s, vhat, ic, twi, v);
+    return hipSuccess;
+}
+
+hipError_t launch_garner(hipStream_t s, const int16_t *cen, const GarnerConsts &gc, uint32_t count,
+                         uint16_t *top, uint16_t *digits) {
+    if (count == 0) return hipSuccess;
+    LBN_LAUNCH(garner_kernel, dim3((count + 255) / 256), dim3(256), 0, s, cen, gc, count, top, digits);
     return hipSuccess;
 }
 

@@ -32,6 +32,8 @@ hipError_t launch_chat(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, u
                        uint32_t q, int8_t *afrag, uint32_t ntiles);
 hipError_t launch_intt(hipStream_t s, const uint32_t *vhat, uint32_t L, const InttConsts &ic,
                        const uint32_t *twi, int16_t *v);
+hipError_t launch_garner(hipStream_t s, const int16_t *cen, const GarnerConsts &gc, uint32_t count,
+                         uint16_t *top, uint16_t *digits);
 hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *part, uint64_t *eq);
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u);

@@ -171,7 +171,26 @@ struct HostTables {
     uint32_t cond_f = 0, mult_f = 0;
     InttConsts intt{};
     std::vector<uint32_t> twi;
+    std::vector<InttConsts> intt_limb;
+    std::vector<std::vector<uint32_t>> twi_limb;
 };
+
+// One limb's inverse transform constants and its twiddles (7 levels, concatenated).
+static void intt_tables(const Prime &P, InttConsts &ic, std::vector<uint32_t> &twi) {
+    ic.q = P.q;
+    ic.quad = P.quad;
+    ic.w = (uint32_t)P.w;
+    ic.w2 = (uint32_t)P.w2;
+    ic.z6 = (uint32_t)P.z6;
+    ic.inv2 = (uint32_t)P.inv2;
+    ic.inv3 = (uint32_t)P.inv3;
+    ic.det = (uint32_t)P.det;
+    twi.clear();
+    for (int l = 0; l < 7; l++) {
+        ic.twi_off[l] = (uint32_t)twi.size();
+        twi.insert(twi.end(), P.twi[l].begin(), P.twi[l].end());
+    }
+}
 
 static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, HostTables &H) {
     const size_t limbs = s.primes.size();
@@ -272,21 +291,12 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     H.mult_f = B.quad ? 2 : 4;
     H.pw_f.assign(B.pw.begin(), B.pw.end());
 
-    // the base prime's inverse transform
-    InttConsts &ic = H.intt;
-    ic.q = B.q;
-    ic.quad = B.quad;
-    ic.w = (uint32_t)B.w;
-    ic.w2 = (uint32_t)B.w2;
-    ic.z6 = (uint32_t)B.z6;
-    ic.inv2 = (uint32_t)B.inv2;
-    ic.inv3 = (uint32_t)B.inv3;
-    ic.det = (uint32_t)B.det;
-    H.twi.clear();
-    for (int l = 0; l < 7; l++) {
-        ic.twi_off[l] = (uint32_t)H.twi.size();
-        H.twi.insert(H.twi.end(), B.twi[l].begin(), B.twi[l].end());
-    }
+    // every limb's inverse transform (the base limb's is the fold's)
+    H.intt_limb.resize(limbs);
+    H.twi_limb.resize(limbs);
+    for (size_t k = 0; k < limbs; k++) intt_tables(prime(s.primes[k]), H.intt_limb[k], H.twi_limb[k]);
+    H.intt = H.intt_limb[0];
+    H.twi = H.twi_limb[0];
 }
 
 static int build_device(lbn_pp *pp) {
@@ -318,6 +328,10 @@ static int build_device(lbn_pp *pp) {
     if ((rc = upload(pp, &pp->d_tiles_f, H.desc_f.data(), H.desc_f.size()))) return rc;
     if ((rc = upload(pp, &pp->d_pw_f, H.pw_f.data(), H.pw_f.size()))) return rc;
     if ((rc = upload(pp, &pp->d_twi, H.twi.data(), H.twi.size()))) return rc;
+    pp->intt_limb = H.intt_limb;
+    pp->d_twi_limb.assign(H.twi_limb.size(), nullptr);
+    for (size_t k = 0; k < H.twi_limb.size(); k++)
+        if ((rc = upload(pp, &pp->d_twi_limb[k], H.twi_limb[k].data(), H.twi_limb[k].size()))) return rc;
     return LBN_OK;
 }
 
@@ -328,7 +342,8 @@ static int build_device(lbn_pp *pp) {
 
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
-                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_u, pr->d_wscratch};
+                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_u, pr->d_wscratch,
+                    pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
@@ -393,6 +408,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     if (comp) HIP_TRY(hipMemcpyAsync(comp, pr->d_comp, s.r * 4 * limbs * N162 * sizeof(int16_t), hipMemcpyDeviceToHost, st));
     if (raw) HIP_TRY(hipMemcpyAsync(raw, pr->d_raw, limbs * s.r * N * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    pr->committed = true;
     pr->timings[0] = elapsed(pr->ev[0], pr->ev[3]);
     pr->timings[1] = elapsed(pr->ev[0], pr->ev[1]);
     pr->timings[2] = elapsed(pr->ev[1], pr->ev[2]);
@@ -570,6 +586,11 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
         TRY_RC(lbn_witness_upload(pr, zero.data(), zero.size(), &dw));
         lbn_opening *op = nullptr;
         rc = lbn_commit_resident(pr, dw, nullptr, nullptr, &op);
+        if (!rc && s.dropped_bits > 0) {
+            std::vector<uint16_t> top(s.r * N), digits((s.primes.size() - 1) * s.r * N);
+            rc = lbn_commit_drop_bits(pr, top.data(), digits.data());
+        }
+        pr->committed = false;  // the warm-up's commitment is not the caller's
         std::vector<lbn_f162> p0(s.rows), u(s.r);
         std::memset(p0.data(), 0, p0.size() * sizeof(lbn_f162));
         if (!rc) rc = lbn_row_evaluate(pr, dw, p0.data(), p0.size(), u.data());

@@ -82,9 +82,9 @@ def round_for(prover, verifier, rw, params, commitment, rid, opening=None):
     """Everything after the commitment, as one request's round (scheme/verifier.rs:61-111 for the
     transcript steps, Witness::row_evaluate, eval.rs claim, Prover::fold: of the opening when there
     is one, else of the resident witness)."""
-    if params.dropped_bits() > 0:
+    if params.dropped_bits() > 0 and commitment.dropped is None:
         commitment = Commitment(params.primes(), params.columns(), commitment.components, commitment.raw,
-                                Dropped.of_raw(params, commitment.raw))
+                                Dropped.of_device(prover, params, commitment.raw))
     t = Transcript(b"labinius/serving/" + int(rid).to_bytes(8, "little"))
     point = verifier.derive_evaluation_point(t, commitment)
     row = prover.row_evaluate(rw, point)

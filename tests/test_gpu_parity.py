@@ -190,6 +190,32 @@ def test_row_evaluation_many_rows(lbn, orc, wl, cl):
     assert np.array_equal(prover.mle_evaluate(w, point), orc.claim(row.values, point.p1))
 
 
+@pytest.mark.parametrize("base,extra,d", [(3889, [9721], 10), (3889, [2917, 4861], 8),
+                                          (9721, [17497, 19441, 2917, 12637], 12), (2917, [3889], 11)])
+def test_drop_bits_device_matches_host(lbn, base, extra, d):
+    """bd/digits.rs:121-182 on the GPU (every limb's inverse NTT, then Garner) against the host
+    restatement, which the oracle tests pin: random raw rows, all-zero and all-(q - 1) rows, and a
+    quadratic base limb. Then the resident form over the prover's own commitment."""
+    say(f"case: drop_bits device base={base} extra={extra} d={d}")
+    params = lbn.Params(13, 4, [lbn.Modulus(q) for q in extra], opening=lbn.Opening.bit_dropped(d),
+                        base=lbn.Modulus(base))
+    prover = lbn.Prover(lbn.PublicParameters.from_seed(params, MATRIX_SEED))
+    with pytest.raises(lbn.LabiniusError):
+        lbn.Dropped.of_last_commit(prover)  # no commitment yet
+    primes, cols = params.primes(), params.columns()
+    rng = np.random.default_rng(d)
+    cases = [np.stack([rng.integers(0, q, (cols, N), dtype=np.uint32) for q in primes]),
+             np.zeros((len(primes), cols, N), np.uint32),
+             np.stack([np.full((cols, N), q - 1, np.uint32) for q in primes])]
+    for raw in cases:
+        host = lbn.Dropped.of_raw(params, raw)
+        dev = lbn.Dropped.of_device(prover, params, raw)
+        assert np.array_equal(dev.top, host.top) and np.array_equal(dev.digits, host.digits)
+    commitment, _ = prover.commit(lbn.Witness.random(params, WITNESS_SEED))
+    host = lbn.Dropped.of_raw(params, commitment.raw)
+    assert np.array_equal(commitment.dropped.top, host.top) and np.array_equal(commitment.dropped.digits, host.digits)
+
+
 @pytest.mark.parametrize("wl,cl,extra,d", [(12, 3, [2917, 4861], 8), (11, 2, [9721], 10), (10, 2, [2917, 12637], 7)])
 def test_bit_dropped_round_bit_exact(lbn, orc, wl, cl, extra, d):
     # scheme/prover.rs:69-72 (drop_bits after the GPU commitment), verifier.rs:61-100 (the Dropped
