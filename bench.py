@@ -41,6 +41,25 @@ def tile_ops(params):
     return 2 * 4 * per_ring * (params.witness_len() // 4)
 
 
+def executed_ops(params):
+    """int8 MFMA ops the commit GEMM executes, padding included: per (tile, group of 3 columns) and
+    K step of 64 ring elements, 8 waves x 24 v_mfma_i32_32x32x32_i8 of 2 x 32^3 ops (a tile is 96
+    rows of which 81 are used, a group 256 inputs of which 243)."""
+    ntiles = sum(16 if (q - 1) % 972 == 0 and (q - 1) % 1944 != 0 else 8 for q in params.primes())
+    groups = -(-params.columns() // 3)
+    kbs = -(-params.len_ring() // 64)
+    return ntiles * groups * kbs * 8 * 24 * 2 * 32 ** 3
+
+
+def mfma_rate(L, random_operands):
+    """The device's int8 MFMA rate (TOP/s), measured by lbn_debug_mfma_rate."""
+    t = ctypes.c_double()
+    rc = L.lbn_debug_mfma_rate(ctypes.c_int(random_operands), ctypes.byref(t))
+    if rc != 0:
+        raise RuntimeError(L.lbn_last_error().decode())
+    return round(t.value, 1)
+
+
 def algorithmic_bytes(params):
     """Algorithmic HBM bytes of one commit + fold + eval unit (the reference's kept transform
     counted at 2 B per slot): W read by commit and eval, T written by commit and read by fold, A."""
@@ -177,7 +196,7 @@ def batched(lbn, reps):
             "speedup": round(one_ms / batch_ms, 3),
             "autotuned": {"tiles_per_xcd_round": info["tiles_per_xcd_round"],
                           "persistent_blocks": info["persistent_blocks"]},
-            "graph": "one hipGraph launch per batch (layout, clear, GEMM, finish)"}
+            "graph": "one hipGraph launch per batch (layout, GEMM, finish)"}
 
 
 def serving_bench(lbn, rate, count):
@@ -318,6 +337,8 @@ def main():
     gemm_ms = stage["commit_gemm"]
     ops = tile_ops(params)
     achieved = ops / (gemm_ms * 1e-3) / 1e12
+    xops = executed_ops(params)
+    rate_c, rate_r = mfma_rate(L, 0), mfma_rate(L, 1)
     ab = algorithmic_bytes(params)
     line = {
         "metric": "commit+fold+eval ms @2^24 bit-exact",
@@ -344,7 +365,11 @@ def main():
                      "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                      "unit_note": "int8 MFMA operations (multiply and add counted separately), 10^12 per second",
                      "frac": round(achieved / PEAK_INT8_TOPS, 4), "traffic": profiled_traffic(args.suite),
-                     "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4)},
+                     "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4),
+                     "executed_ops_per_launch": xops,
+                     "measured_mfma_rate": {"constant_operands": rate_c, "random_operands": rate_r,
+                                            "unit": "TOP/s",
+                                            "executed_frac_of_random_rate": round(xops / (gemm_ms * 1e-3) / 1e12 / rate_r, 4)}},
     }
     if not args.no_batched:
         log("batched commit...")

@@ -4,7 +4,7 @@
 // CommitmentKey::random, key.rs:48, depends on nothing else) is a list of independent columns, so
 // a batch is one commitment over the union of their columns: the layout pass reads column c
 // through a table of column addresses, and the GEMM, the finish and the outputs run over all
-// columns at once. The sequence (layout, clear, GEMM, finish) is captured once per batch width in
+// columns at once. The sequence (layout, GEMM, finish) is captured once per batch width in
 // a HIP graph and replayed; the GEMM's block schedule is chosen per width by timing the
 // candidates on the device the first time that width is seen.
 #include <algorithm>

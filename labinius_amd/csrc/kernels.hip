@@ -781,24 +781,60 @@ __global__ void garner_kernel(const int16_t *__restrict__ cen, GarnerConsts gc, 
         if (k < (int)gc.limbs) digits[(size_t)(k - 1) * count + i] = (uint16_t)m[k];
 }
 
-// One v_mfma_i32_16x16x64_i8 with the operand and result lane layouts the tile GEMM assumes:
-// A[m][k] row-major 16 x 64, B[k][n] row-major 64 x 16, D[m][n] 16 x 16. One wave.
+// One v_mfma_i32_32x32x32_i8 with the operand and result lane layouts the tile GEMM assumes:
+// A[m][k] and B[k][n] row-major 32 x 32; lane l holds A row l & 31 and B column l & 31 at
+// K 16 (l >> 5) .. + 15; D register r is row 8 (r >> 2) + 4 (l >> 5) + (r & 3), column l & 31.
 __global__ void mfma_probe_kernel(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                                   int32_t *__restrict__ D) {
     const int lane = threadIdx.x;
     int8_t ab[16], bb[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-        ab[j] = A[(lane & 15) * 64 + 16 * (lane >> 4) + j];
-        bb[j] = B[(16 * (lane >> 4) + j) * 16 + (lane & 15)];
+        ab[j] = A[(lane & 31) * 32 + 16 * (lane >> 5) + j];
+        bb[j] = B[(16 * (lane >> 5) + j) * 32 + (lane & 31)];
     }
     v4i a, b;
     __builtin_memcpy(&a, ab, 16);
     __builtin_memcpy(&b, bb, 16);
-    v4i acc = {0, 0, 0, 0};
-    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+    v16i acc = {};
+    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 4; r++) D[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[r];
+    for (int r = 0; r < 16; r++) D[(8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 32 + (lane & 31)] = acc[r];
+}
+
+// The int8 MFMA rate: each wave issues iters x 8 independent v_mfma_i32_32x32x32_i8, on operands
+// that are constant, or random per lane and alternating between two sets. Random operands toggle
+// more bits, draw more power and run at a lower clock.
+__device__ __forceinline__ int rate_hash(uint32_t x) {
+    x *= 2654435761u;
+    x ^= x >> 15;
+    x *= 0x5bd1e995u;
+    x ^= x >> 13;
+    return (int)x;
+}
+__global__ __launch_bounds__(256) void mfma_rate_kernel(int random, uint32_t iters, int32_t *__restrict__ sink) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    v4i a0 = {0x01010101, 0x01010101, 0x01010101, 0x01010101}, b0 = a0, a1 = a0, b1 = a0;
+    if (random) {
+        a0 = (v4i){rate_hash(16 * t), rate_hash(16 * t + 1), rate_hash(16 * t + 2), rate_hash(16 * t + 3)};
+        b0 = (v4i){rate_hash(16 * t + 4), rate_hash(16 * t + 5), rate_hash(16 * t + 6), rate_hash(16 * t + 7)};
+        a1 = (v4i){rate_hash(16 * t + 8), rate_hash(16 * t + 9), rate_hash(16 * t + 10), rate_hash(16 * t + 11)};
+        b1 = (v4i){rate_hash(16 * t + 12), rate_hash(16 * t + 13), rate_hash(16 * t + 14), rate_hash(16 * t + 15)};
+    }
+    v16i acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = (v16i){};
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            acc[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8((i & 1) ? a1 : a0, (i & 1) ? b1 : b0, acc[i], 0, 0, 0);
+    }
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) s ^= acc[i][r];
+    sink[t] = s;
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
@@ -816,6 +852,11 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
         hipError_t e_ = hipGetLastError();                                  \
         if (e_ != hipSuccess) return e_;                                    \
     } while (0)
+
+hipError_t launch_mfma_rate(hipStream_t s, int random, uint32_t blocks, uint32_t iters, int32_t *sink) {
+    LBN_LAUNCH(mfma_rate_kernel, dim3(blocks), dim3(256), 0, s, random, iters, sink);
+    return hipSuccess;
+}
 
 hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D) {
     LBN_LAUNCH(mfma_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D);

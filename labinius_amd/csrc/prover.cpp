@@ -754,17 +754,42 @@ int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d) {
     int32_t *dd = nullptr;
     int rc = LBN_OK;
     if (hipMalloc((void **)&da, 1024) != hipSuccess || hipMalloc((void **)&db, 1024) != hipSuccess ||
-        hipMalloc((void **)&dd, 1024) != hipSuccess) {
+        hipMalloc((void **)&dd, 4096) != hipSuccess) {
         rc = fail(LBN_ERR_OOM, "probe buffers");
     } else if (hipMemcpy(da, a, 1024, hipMemcpyHostToDevice) != hipSuccess ||
                hipMemcpy(db, b, 1024, hipMemcpyHostToDevice) != hipSuccess ||
                launch_mfma_probe(nullptr, da, db, dd) != hipSuccess ||
-               hipMemcpy(d, dd, 1024, hipMemcpyDeviceToHost) != hipSuccess) {
+               hipMemcpy(d, dd, 4096, hipMemcpyDeviceToHost) != hipSuccess) {
         rc = fail(LBN_ERR_HIP, "probe launch");
     }
     if (da) (void)hipFree(da);
     if (db) (void)hipFree(db);
     if (dd) (void)hipFree(dd);
+    return rc;
+}
+
+// Measurement (include/labinius_debug.h): the int8 MFMA rate in TOP/s on constant or random operands.
+int lbn_debug_mfma_rate(int random_operands, double *tops) {
+    if (!tops) return fail(LBN_ERR_ARG, "null argument");
+    const uint32_t blocks = 1024, iters = 4000;  // 4 waves a block, 4 blocks a CU
+    int32_t *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LBN_OK;
+    float ms = 0;
+    if (hipMalloc((void **)&sink, (size_t)blocks * 256 * sizeof(int32_t)) != hipSuccess) return fail(LBN_ERR_OOM, "rate sink");
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        launch_mfma_rate(nullptr, random_operands, blocks, 16, sink) != hipSuccess ||  // warm-up
+        hipEventRecord(e0, nullptr) != hipSuccess ||
+        launch_mfma_rate(nullptr, random_operands, blocks, iters, sink) != hipSuccess ||
+        hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(LBN_ERR_HIP, "rate launch");
+    } else {
+        *tops = (double)blocks * 4 * iters * 8 * (32.0 * 32 * 32 * 2) / (ms * 1e-3) / 1e12;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(sink);
     return rc;
 }
 
