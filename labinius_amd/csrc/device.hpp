@@ -73,7 +73,9 @@ struct GemmArgs {
     const uint16_t *ee;       // all tiles' E (mod q)
     const int16_t *out;       // all tiles' output slots
     const uint8_t *bytes;     // the input bytes, fragment order (byte_offset)
-    uint32_t kbs;             // K steps
+    uint32_t kbs;             // K steps of the layouts
+    uint32_t kb0, kbn;        // this launch's steps: kb0 .. kb0 + kbn - 1 (set by the launcher)
+    uint32_t add;             // 1: y = (y + result) mod q (a later K chunk), 0: y = result
     uint32_t a_kb_stride;     // bytes of A fragments per K step of a tile (A_STEP)
     uint32_t *y;              // output rows; every (batch element, tile output) has its own word
     uint64_t y_batch;         // words per batch element in y
@@ -83,6 +85,12 @@ struct GemmArgs {
     uint32_t sched_t;         // tiles per XCD round (1 .. 32, a power of two); the rest are groups
     uint32_t items;           // work items (set by the launcher)
 };
+
+// The int32 accumulators are exact while K x 9728 < 2^31: a digit is in [-64, 63] (low) or
+// [-76, 76] (high, q < 2^15), and the largest class, mid = A_l x_h + A_h x_l, adds at most
+// 2 x 64 x 76 = 9728 per K index. The launcher splits longer K into chunks of KB_EXACT steps and
+// the later chunks add their outputs mod q.
+constexpr uint32_t KB_EXACT = 2147483647u / (KSTEP * 9728u);  // 3449 steps, K = 220 736
 
 // The commit GEMM's output is y[c][limb][plane][648]. A splitting limb's 8 tiles write each of
 // the 648 slots once, into plane 0. A quadratic limb's slot 2j + t is written by the two tiles of

@@ -12,6 +12,8 @@
 // The algebra of the tile GEMM is in device.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -200,7 +202,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     uint4 raw[2];
     auto load_raw = [&](uint32_t kb) {
 #pragma unroll
-        for (int kh = 0; kh < 2; kh++) raw[kh] = *(const uint4 *)(bp + ((uint64_t)kb * 2 + kh) * 1024);
+        for (int kh = 0; kh < 2; kh++) raw[kh] = *(const uint4 *)(bp + ((uint64_t)(g.kb0 + kb) * 2 + kh) * 1024);
     };
     v4i bl[2], bh[2];
     auto build = [&]() {
@@ -238,7 +240,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // 768 repeat the last one and land in the buffer's padding.
     v4i apre[AP];
     auto load_a = [&](uint32_t kb) {
-        const int8_t *ak = abase + (size_t)kb * g.a_kb_stride;
+        const int8_t *ak = abase + (size_t)(g.kb0 + kb) * g.a_kb_stride;
 #pragma unroll
         for (int i = 0; i < AP; i++)
             apre[i] = *(const v4i *)(ak + min(tid + GEMM_THREADS * i, A_STEP / 16 - 1) * 16);
@@ -264,7 +266,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     if (tid + GEMM_THREADS < EE_V) ee1 = eesrc[tid + GEMM_THREADS];
     const int slot = tid < GROUP_BATCH * TILE ? g.out[td.out_off + tid % TILE] : -1;
 
-    const uint32_t kbs = g.kbs;
+    const uint32_t kbs = g.kbn;  // steps of this launch, counted from kb0
     load_a(0);
     load_raw(0);
     ((uint4 *)ee)[tid] = ee0;
@@ -344,7 +346,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll 9
         for (int n = 0; n < TILE_USED; n++) sum += (unsigned long long)ee[n * TILE + m] * zb[n * ZROW];
         const uint32_t v = (uint32_t)(sum % q);
-        g.y[(size_t)batch * g.y_batch + td.y_off + slot] = v;
+        uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
+        *yp = g.add ? (*yp + v) % q : v;
     }
 }
 
@@ -878,6 +881,8 @@ hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *cons
     return hipSuccess;
 }
 
+uint32_t gemm_kb_chunk = 0;
+
 hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks) {
     static const hipError_t attr0 = hipFuncSetAttribute((const void *)tile_gemm_kernel<false>,
                                                         hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
@@ -891,10 +896,16 @@ hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persisten
     a.items = gemm_items(a.ntiles, a.groups, a.sched_t);
     uint32_t grid = a.items;
     if (persistent_blocks && persistent_blocks < grid) grid = (persistent_blocks + 7) / 8 * 8;
-    if (a.kbs >= 8)
-        LBN_LAUNCH(tile_gemm_kernel<true>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
-    else
-        LBN_LAUNCH(tile_gemm_kernel<false>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    // K in chunks the int32 accumulators hold exactly; the later chunks add mod q
+    const uint32_t chunk = gemm_kb_chunk ? gemm_kb_chunk : KB_EXACT;
+    for (a.kb0 = 0; a.kb0 < g.kbs; a.kb0 += chunk) {
+        a.kbn = std::min(chunk, g.kbs - a.kb0);
+        a.add = a.kb0 > 0;
+        if (a.kbn >= 8)
+            LBN_LAUNCH(tile_gemm_kernel<true>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+        else
+            LBN_LAUNCH(tile_gemm_kernel<false>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    }
     return hipSuccess;
 }
 

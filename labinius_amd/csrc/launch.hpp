@@ -23,8 +23,10 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *const *cols, uint32_t column_len,
                           uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
                           uint32_t kbs_f);
-// persistent_blocks == 0: one block per work item; otherwise that many blocks (rounded up to 8) loop
+// persistent_blocks == 0: one block per work item; otherwise that many blocks (rounded up to 8) loop.
+// K runs in chunks of gemm_kb_chunk steps (0: KB_EXACT), one launch each.
 hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks);
+extern uint32_t gemm_kb_chunk;  // tests only (lbn_debug_gemm_chunk)
 hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t limbs,
                                 const FinishLimb *fl, const uint16_t *slot648, const uint16_t *qcs,
                                 uint32_t *raw, int16_t *comp, uint32_t cols);

@@ -768,6 +768,13 @@ int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d) {
     return rc;
 }
 
+// Test-only (include/labinius_debug.h): the GEMM's K chunk in steps, 0 for the exact-range default.
+int lbn_debug_gemm_chunk(uint32_t kb) {
+    if (kb > KB_EXACT) return fail(LBN_ERR_ARG, "a K chunk beyond the accumulators' exact range");
+    gemm_kb_chunk = kb;
+    return LBN_OK;
+}
+
 // Measurement (include/labinius_debug.h): the int8 MFMA rate in TOP/s on constant or random operands.
 int lbn_debug_mfma_rate(int random_operands, double *tops) {
     if (!tops) return fail(LBN_ERR_ARG, "null argument");

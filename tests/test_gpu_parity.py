@@ -89,6 +89,34 @@ def test_commit_fold_eval_parity(lbn, orc, wl, cl, base, extra):
                                      folded.elements, point.p0, fr)
 
 
+@pytest.mark.parametrize("chunk,wl,cl,extra", [(1, 13, 3, [2917]), (3, 16, 8, [9721]), (8, 18, 2, [2917])])
+def test_gemm_k_chunks(lbn, orc, chunk, wl, cl, extra):
+    """K run as several launches whose later ones add mod q (the path for K past the int32
+    accumulators' exact range, 220 736 ring elements or columns), here with chunks of 1, 3 and 8
+    steps: commitments and folds stay bit-exact. (16, 8) splits the fold's K; (18, 2) the commit's
+    64 steps into 8-step launches on the replicated-LUT path."""
+    say(f"case: K chunk {chunk} wl={wl} cl={cl} extra={extra}")
+    L = lbn.lib()
+    assert L.lbn_debug_gemm_chunk(ctypes.c_uint32(chunk)) == 0
+    try:
+        params = lbn.Params(wl, cl, [lbn.Modulus(q) for q in extra])
+        pp, prover, w, a = _setup(lbn, orc, params)
+        commitment, opening = prover.commit(w)
+        raw, comp = orc.commit(params.primes(), params.len_ring(), params.columns(), a, w.elements)
+        assert np.array_equal(commitment.raw, raw) and np.array_equal(commitment.components, comp)
+        t = lbn.Transcript(b"labinius/test/chunks")
+        verifier = lbn.Verifier(pp)
+        point = verifier.derive_evaluation_point(t, commitment)
+        row = prover.row_evaluate(w, point)
+        challenges = verifier.derive_folding_challenges(t, row)
+        folded = prover.fold(opening, challenges)
+        assert np.array_equal(folded.elements, orc.fold(params.primes()[0], params.len_ring(), params.columns(),
+                                                        w.elements, challenges.challenges))
+    finally:
+        assert L.lbn_debug_gemm_chunk(ctypes.c_uint32(0)) == 0
+    assert L.lbn_debug_gemm_chunk(ctypes.c_uint32(3450)) != 0  # past the exact range
+
+
 def test_basic_round_bit_exact(lbn, orc):
     """Params::basic() (2^18 F162, 128 columns, {3889, 9721}) with the reference bench's seeds."""
     say("case: basic round")

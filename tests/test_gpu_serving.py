@@ -30,7 +30,9 @@ def _stream(lbn, n, sizes, seed=7):
         e = sizes[rng.below(len(sizes))]
         d = 8 if rng.below(2) else 0
         w = lbn.Witness.random(wl.witness_params(e), wl.witness_seed(b"labinius/test/serving", i))
-        out.append((i, e, d, w, 0.0005 * i))
+        # a burst of 8, then one every 0.5 ms: the burst's later requests arrive while the first
+        # one's batch is on the GPU and join the next batch
+        out.append((i, e, d, w, 0.0 if i < 8 else 0.0005 * i))
     return out
 
 
