@@ -594,10 +594,11 @@ __global__ void eq_kernel(const uint64_t *__restrict__ part, uint32_t rows, uint
 // 256-entry table T[m] = XOR of eq_{8g+k} over the bits k of m, shared by every column; then each
 // bit position b of a column costs one lookup per group, at the byte "bits b of the 8 elements".
 //
-// A block takes 8 columns (two per wave) and 16 chunks of 64 elements: per chunk it builds the 8
-// tables in LDS, each wave transposes its column's chunk into per-bit-position bytes (8 x 8 bit
-// transposes) and looks them up; a lane owns bit positions lane, lane + 64 and lane + 128.
-constexpr int RE_CHUNK = 64, RE_COLS = 8, RE_SPAN = 16;
+// A block takes 32 columns (four per wave) and a span of chunks of 64 elements: per chunk it
+// builds the 8 tables in LDS once for all 32 columns; each wave then takes its columns two at a
+// time, transposes their chunk into per-bit-position bytes (8 x 8 bit transposes) in its own LDS
+// area and looks them up; a lane owns bit positions lane, lane + 64 and lane + 128.
+constexpr int RE_CHUNK = 64, RE_WAVES = 8, RE_CPW = 4, RE_COLS = RE_WAVES * RE_CPW;
 
 __device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
     // byte t, bit p -> byte p, bit t
@@ -621,29 +622,31 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restrict__ w,
-                                                       const uint64_t *__restrict__ eq, uint32_t wdim,
-                                                       uint32_t cols, uint64_t *__restrict__ u) {
+__global__ __launch_bounds__(64 * RE_WAVES) void row_eval_kernel(const uint64_t *__restrict__ w,
+                                                                const uint64_t *__restrict__ eq, uint32_t wdim,
+                                                                uint32_t cols, uint32_t span,
+                                                                uint64_t *__restrict__ u) {
     __shared__ uint64_t T[8][256][3];
     __shared__ uint64_t NT[8][2][16][3];  // nibble tables: low and high four elements
-    __shared__ __attribute__((aligned(16))) uint32_t ws[4][2][3][2 * RE_CHUNK];  // [wave][col][word][elem, half]
-    __shared__ uint64_t by[4][2][8][21];  // [wave][col][group][bit position / 8]
+    __shared__ __attribute__((aligned(16))) uint32_t ws[RE_WAVES][2][3][2 * RE_CHUNK];  // [wave][col][word][elem, half]
+    __shared__ uint64_t by[RE_WAVES][2][8][21];  // [wave][col][group][bit position / 8]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t j0 = blockIdx.x * RE_COLS, nch = wdim / RE_CHUNK;
-    const uint32_t ch0 = blockIdx.y * RE_SPAN, ch1 = min(ch0 + RE_SPAN, nch);
-    bool live[2];
+    const uint32_t ch0 = blockIdx.y * span, ch1 = min(ch0 + span, nch);
+    // this wave's column c is j0 + wave + RE_WAVES c
+    bool live[RE_CPW];
 #pragma unroll
-    for (int cw = 0; cw < 2; cw++) live[cw] = j0 + wave + 4 * cw < cols;
-    uint64_t S[2][3][3];
+    for (int c = 0; c < RE_CPW; c++) live[c] = j0 + wave + RE_WAVES * c < cols;
+    uint64_t S[RE_CPW][3][3];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RE_CPW; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) S[a][b][0] = S[a][b][1] = S[a][b][2] = 0;
 
     for (uint32_t ch = ch0; ch < ch1; ch++) {
         const uint32_t i0 = ch * RE_CHUNK;
         __syncthreads();  // the previous chunk's tables are no longer read
-        {
+        if (tid < 256) {
             // nibble table (g, half h, x): XOR of eq_{8g + 4h + k} over the bits k of x
             const int g = tid >> 5, h = (tid >> 4) & 1, x = tid & 15;
             const uint64_t *e = eq + 3 * ((size_t)i0 + 8 * g + 4 * h);
@@ -659,71 +662,77 @@ __global__ __launch_bounds__(256) void row_eval_kernel(const uint64_t *__restric
             NT[g][h][x][1] = a1;
             NT[g][h][x][2] = a2;
         }
-        // both columns' chunks, word-major as dwords: ws[c][k][2 i + half] = half of word k of element i
-#pragma unroll
-        for (int cw = 0; cw < 2; cw++)
-            if (live[cw]) {
-                const uint64_t *src = w + 3 * ((size_t)(j0 + wave + 4 * cw) * wdim + i0);
-                for (int x = lane; x < RE_CHUNK * 3; x += 64) {
-                    const uint64_t v = src[x];
-                    ws[wave][cw][x % 3][2 * (x / 3)] = (uint32_t)v;
-                    ws[wave][cw][x % 3][2 * (x / 3) + 1] = (uint32_t)(v >> 32);
-                }
-            }
         __syncthreads();
-        for (int e = tid; e < 8 * 256; e += 256) {
+        for (int e = tid; e < 8 * 256; e += 64 * RE_WAVES) {
             const int g = e >> 8, m = e & 255;
             const uint64_t *lo = NT[g][0][m & 15], *hi = NT[g][1][m >> 4];
             T[g][m][0] = lo[0] ^ hi[0];
             T[g][m][1] = lo[1] ^ hi[1];
             T[g][m][2] = lo[2] ^ hi[2];
         }
-        // item (g, word k, byte c): bits 64 k + 8 c .. + 7 of the group's 8 elements, gathered
-        // with byte permutes: byte c of the 8 words, then an 8 x 8 bit transpose
+        __syncthreads();  // the tables
 #pragma unroll
-        for (int cw = 0; cw < 2; cw++)
-            if (live[cw])
-                for (int it = lane; it < 8 * 24; it += 64) {
-                    const int g = it / 24, k = (it >> 3) % 3, c = it & 7;
-                    if (8 * k + c > 20) continue;  // bit positions past 167
-                    const uint4 *q = (const uint4 *)&ws[wave][cw][k][16 * g];
-                    const uint4 d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
-                    // element t's dword (c < 4: low half, else high half)
-                    const bool hiw = c >= 4;
-                    const uint32_t e0 = hiw ? d0.y : d0.x, e1 = hiw ? d0.w : d0.z, e2 = hiw ? d1.y : d1.x,
-                                   e3 = hiw ? d1.w : d1.z, e4 = hiw ? d2.y : d2.x, e5 = hiw ? d2.w : d2.z,
-                                   e6 = hiw ? d3.y : d3.x, e7 = hiw ? d3.w : d3.z;
-                    const uint32_t cb = c & 3, sel2 = cb | ((cb + 4) << 8);
-                    const uint32_t x01 = __builtin_amdgcn_perm(e1, e0, sel2), x23 = __builtin_amdgcn_perm(e3, e2, sel2);
-                    const uint32_t x45 = __builtin_amdgcn_perm(e5, e4, sel2), x67 = __builtin_amdgcn_perm(e7, e6, sel2);
-                    const uint32_t lo = __builtin_amdgcn_perm(x23, x01, 0x05040100u);
-                    const uint32_t hi = __builtin_amdgcn_perm(x67, x45, 0x05040100u);
-                    by[wave][cw][g][8 * k + c] = transpose8x8(((uint64_t)hi << 32) | lo);
+        for (int pr = 0; pr < RE_CPW / 2; pr++) {
+            wave_sync();  // the previous pair's words and bytes are no longer read
+            // both columns' chunks, word-major as dwords: ws[c][k][2 i + half] = half of word k of element i
+#pragma unroll
+            for (int cw = 0; cw < 2; cw++)
+                if (live[2 * pr + cw]) {
+                    const uint64_t *src = w + 3 * ((size_t)(j0 + wave + RE_WAVES * (2 * pr + cw)) * wdim + i0);
+                    for (int x = lane; x < RE_CHUNK * 3; x += 64) {
+                        const uint64_t v = src[x];
+                        ws[wave][cw][x % 3][2 * (x / 3)] = (uint32_t)v;
+                        ws[wave][cw][x % 3][2 * (x / 3) + 1] = (uint32_t)(v >> 32);
+                    }
                 }
-        __syncthreads();  // the tables and both columns' bytes
+            wave_sync();
+            // item (g, word k, byte c): bits 64 k + 8 c .. + 7 of the group's 8 elements, gathered
+            // with byte permutes: byte c of the 8 words, then an 8 x 8 bit transpose
 #pragma unroll
-        for (int cw = 0; cw < 2; cw++)
-            if (live[cw]) {
-                const uint8_t *bb = (const uint8_t *)by[wave][cw];
+            for (int cw = 0; cw < 2; cw++)
+                if (live[2 * pr + cw])
+                    for (int it = lane; it < 8 * 24; it += 64) {
+                        const int g = it / 24, k = (it >> 3) % 3, c = it & 7;
+                        if (8 * k + c > 20) continue;  // bit positions past 167
+                        const uint4 *q = (const uint4 *)&ws[wave][cw][k][16 * g];
+                        const uint4 d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
+                        // element t's dword (c < 4: low half, else high half)
+                        const bool hiw = c >= 4;
+                        const uint32_t e0 = hiw ? d0.y : d0.x, e1 = hiw ? d0.w : d0.z, e2 = hiw ? d1.y : d1.x,
+                                       e3 = hiw ? d1.w : d1.z, e4 = hiw ? d2.y : d2.x, e5 = hiw ? d2.w : d2.z,
+                                       e6 = hiw ? d3.y : d3.x, e7 = hiw ? d3.w : d3.z;
+                        const uint32_t cb = c & 3, sel2 = cb | ((cb + 4) << 8);
+                        const uint32_t x01 = __builtin_amdgcn_perm(e1, e0, sel2), x23 = __builtin_amdgcn_perm(e3, e2, sel2);
+                        const uint32_t x45 = __builtin_amdgcn_perm(e5, e4, sel2), x67 = __builtin_amdgcn_perm(e7, e6, sel2);
+                        const uint32_t lo = __builtin_amdgcn_perm(x23, x01, 0x05040100u);
+                        const uint32_t hi = __builtin_amdgcn_perm(x67, x45, 0x05040100u);
+                        by[wave][cw][g][8 * k + c] = transpose8x8(((uint64_t)hi << 32) | lo);
+                    }
+            wave_sync();  // the pair's bytes
 #pragma unroll
-                for (int sl = 0; sl < 3; sl++) {
-                    const int b = lane + 64 * sl;
-                    if (b < 162) {
+            for (int cw = 0; cw < 2; cw++)
+                if (live[2 * pr + cw]) {
+                    const uint8_t *bb = (const uint8_t *)by[wave][cw];
 #pragma unroll
-                        for (int g = 0; g < 8; g++) {
-                            const uint64_t *t = T[g][bb[g * 168 + b]];
-                            S[cw][sl][0] ^= t[0];
-                            S[cw][sl][1] ^= t[1];
-                            S[cw][sl][2] ^= t[2];
+                    for (int sl = 0; sl < 3; sl++) {
+                        const int b = lane + 64 * sl;
+                        if (b < 162) {
+#pragma unroll
+                            for (int g = 0; g < 8; g++) {
+                                const uint64_t *t = T[g][bb[g * 168 + b]];
+                                S[2 * pr + cw][sl][0] ^= t[0];
+                                S[2 * pr + cw][sl][1] ^= t[1];
+                                S[2 * pr + cw][sl][2] ^= t[2];
+                            }
                         }
                     }
                 }
-            }
+        }
     }
     // u_j ^= sum_b x^b S_b mod x^162 + x^81 + 1
 #pragma unroll
-    for (int cw = 0; cw < 2; cw++) {
-        const uint32_t j = j0 + wave + 4 * cw;
+    for (int cw = 0; cw < RE_CPW; cw++) {
+        const uint32_t j = j0 + wave + RE_WAVES * cw;
         if (j >= cols) continue;  // wave-uniform
         uint64_t p[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -948,10 +957,12 @@ hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t 
 
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u) {
-    // the kernel's chunks are 64 elements: wdim (2^row_log_len >= 128) is a multiple
-    const uint32_t nch = wdim / RE_CHUNK;
-    LBN_LAUNCH(row_eval_kernel, dim3((cols + RE_COLS - 1) / RE_COLS, (nch + RE_SPAN - 1) / RE_SPAN), dim3(256), 0, s,
-               w, eq, wdim, cols, u);
+    // the kernel's chunks are 64 elements: wdim (2^row_log_len >= 128) is a multiple. Chunks are
+    // spread over the grid's y so that there are about two blocks per CU.
+    const uint32_t nch = wdim / RE_CHUNK, gx = (cols + RE_COLS - 1) / RE_COLS;
+    const uint32_t gy_want = std::max(1u, std::min(nch, (512 + gx - 1) / gx));
+    const uint32_t span = (nch + gy_want - 1) / gy_want, gy = (nch + span - 1) / span;
+    LBN_LAUNCH(row_eval_kernel, dim3(gx, gy), dim3(64 * RE_WAVES), 0, s, w, eq, wdim, cols, span, u);
     return hipSuccess;
 }
 
