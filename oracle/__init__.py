@@ -409,3 +409,111 @@ def verify_opening_bd(params, a, top, digits, chs, v, p0, u_folded):
     uf = np.ascontiguousarray(u_folded, np.uint64)
     return bool(lib().orc_verify_opening_bd(ctypes.byref(params), _p(a), _p(top), _p(digits), challenges_array(chs),
                                             _p(v), _p(p0), _p(uf)))
+
+
+# --------------------------------------------------------------------------------------------
+# the cross-field switch (switch.rs:40-209, fields/crossfield.rs, fields/sumcheck.rs). B128 values
+# are uint64 pairs (low word first), F162 values uint64 triples.
+# --------------------------------------------------------------------------------------------
+
+def _u64(a, shape):
+    return np.ascontiguousarray(np.asarray(a, np.uint64).reshape(shape))
+
+
+def b128_mul(a, b):
+    a, b = _u64(a, (2,)), _u64(b, (2,))
+    out = np.zeros(2, np.uint64)
+    lib().orc_b128_mul(_p(a), _p(b), _p(out))
+    return out
+
+
+def eq_b128(r):
+    r = _u64(r, (-1, 2))
+    t = np.zeros((1 << len(r), 2), np.uint64)
+    lib().orc_eq_b128(_p(r), ctypes.c_int(len(r)), _p(t))
+    return t
+
+
+def eq_f162_msb(r):
+    r = _u64(r, (-1, 3))
+    t = np.zeros((1 << len(r), 3), np.uint64)
+    lib().orc_eq_f162_msb(_p(r), ctypes.c_int(len(r)), _p(t))
+    return t
+
+
+def b128_mle(trace, r):
+    trace, r = _u64(trace, (-1, 2)), _u64(r, (-1, 2))
+    out = np.zeros(2, np.uint64)
+    lib().orc_b128_mle(_p(trace), ctypes.c_int(len(r)), _p(r), _p(out))
+    return out
+
+
+def bit_mle(trace, r_lo, r_hi):
+    trace, r_lo, r_hi = _u64(trace, (-1, 2)), _u64(r_lo, (7, 2)), _u64(r_hi, (-1, 2))
+    out = np.zeros(2, np.uint64)
+    lib().orc_bit_mle(_p(trace), ctypes.c_int(len(r_hi)), _p(r_lo), _p(r_hi), _p(out))
+    return out
+
+
+def transparent_coeff(r_hi, r_pp, batch):
+    r_hi, r_pp, batch = _u64(r_hi, (-1, 2)), _u64(r_pp, (-1, 3)), _u64(batch, (128, 3))
+    out = np.zeros(3, np.uint64)
+    lib().orc_transparent_coeff(_p(r_hi), _p(r_pp), ctypes.c_int(len(r_hi)), _p(batch), _p(out))
+    return out
+
+
+def _absorb_b128(self, xs):
+    xs = _u64(xs, (-1, 2))
+    lib().orc_absorb_b128(self.buf, _p(xs), ctypes.c_size_t(len(xs)))
+
+
+def _absorb_f162(self, xs):
+    xs = _u64(xs, (-1, 3))
+    lib().orc_absorb_f162(self.buf, _p(xs), ctypes.c_size_t(len(xs)))
+
+
+def _sample_f162(self, label, n):
+    out = np.zeros((n, 3), np.uint64)
+    lib().orc_sample_f162(self.buf, ctypes.c_char_p(bytes(label)), ctypes.c_size_t(n), _p(out))
+    return out
+
+
+def _sample_b128(self, label, n):
+    out = np.zeros((n, 2), np.uint64)
+    lib().orc_sample_b128(self.buf, ctypes.c_char_p(bytes(label)), ctypes.c_size_t(n), _p(out))
+    return out
+
+
+Transcript.absorb_b128 = _absorb_b128
+Transcript.absorb_f162 = _absorb_f162
+Transcript.sample_f162 = _sample_f162
+Transcript.sample_b128 = _sample_b128
+
+
+def switch_prove(trace, cl, r_lo, r_hi, claim, transcript):
+    """switch.rs:137-164: (partials [128][2], rounds [l][2][3], opened [3], p0, p1)."""
+    trace, r_hi = _u64(trace, (-1, 2)), _u64(r_hi, (-1, 2))
+    r_lo = _u64(r_lo if r_lo is not None else np.zeros((0, 2)), (-1, 2))
+    claim = _u64(claim, (2,))
+    l = len(r_hi)
+    partials = np.zeros((128, 2), np.uint64)
+    rounds = np.zeros((l, 2, 3), np.uint64)
+    opened = np.zeros(3, np.uint64)
+    p0 = np.zeros((l - cl, 3), np.uint64)
+    p1 = np.zeros((cl, 3), np.uint64)
+    lib().orc_switch_prove(_p(trace), ctypes.c_int(l), ctypes.c_int(cl), _p(r_lo), ctypes.c_int(len(r_lo)), _p(r_hi),
+                           _p(claim), transcript.buf, _p(partials), _p(rounds), _p(opened), _p(p0), _p(p1))
+    return partials, rounds, opened, p0, p1
+
+
+def switch_verify(l, cl, r_lo, r_hi, claim, partials, rounds, opened, transcript):
+    """switch.rs:166-209: (accepted, p0, p1)."""
+    r_hi = _u64(r_hi, (-1, 2))
+    r_lo = _u64(r_lo if r_lo is not None else np.zeros((0, 2)), (-1, 2))
+    claim, partials = _u64(claim, (2,)), _u64(partials, (128, 2))
+    rounds, opened = _u64(rounds, (l, 2, 3)), _u64(opened, (3,))
+    p0 = np.zeros((l - cl, 3), np.uint64)
+    p1 = np.zeros((cl, 3), np.uint64)
+    ok = lib().orc_switch_verify(ctypes.c_int(l), ctypes.c_int(cl), _p(r_lo), ctypes.c_int(len(r_lo)), _p(r_hi),
+                                 _p(claim), _p(partials), _p(rounds), _p(opened), transcript.buf, _p(p0), _p(p1))
+    return bool(ok), p0, p1

@@ -1623,3 +1623,319 @@ int orc_verify_opening_bd(const orc_params *p, const int16_t *a, const uint16_t 
     free(eq);
     return acc[0] == u_folded[0] && acc[1] == u_folded[1] && acc[2] == u_folded[2];
 }
+
+/* ============================================================================================
+ * the cross-field switch: switch.rs:40-209, fields/crossfield.rs:9-115 and 327-560,
+ * fields/sumcheck.rs:100-145. B128 = GF(2)[x] / (x^128 + x^7 + x^2 + x + 1) (scalar.rs:4, 41-63)
+ * as two u64, bit i the coefficient of x^i; F162 as in orc_f162_mul.
+ * ========================================================================================== */
+
+/* bit by bit: the 255-bit product, then x^128 = x^7 + x^2 + x + 1 from the top down */
+void orc_b128_mul(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    uint64_t p[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 128; i++) {
+        if (!((a[i / 64] >> (i % 64)) & 1)) continue;
+        int ws = i / 64, bs = i % 64;
+        for (int k = 0; k < 2; k++) {
+            p[k + ws] ^= b[k] << bs;
+            if (bs) p[k + ws + 1] ^= b[k] >> (64 - bs);
+        }
+    }
+    for (int j = 254; j >= 128; j--)
+        if ((p[j / 64] >> (j % 64)) & 1) {
+            const int t[5] = {j, j - 128, j - 127, j - 126, j - 121};
+            for (int u = 0; u < 5; u++) p[t[u] / 64] ^= 1ull << (t[u] % 64);
+        }
+    out[0] = p[0];
+    out[1] = p[1];
+}
+
+/* crossfield.rs:85-99: eq(r, j) with coordinate 0 the most significant bit of j, doubled in place
+ * from the back */
+void orc_eq_b128(const uint64_t *r, int n, uint64_t *t) {
+    t[0] = 1;
+    t[1] = 0;
+    for (int round = 0; round < n; round++)
+        for (long i = (1L << round) - 1; i >= 0; i--) {
+            uint64_t hi[2];
+            orc_b128_mul(t + 2 * i, r + 2 * round, hi);
+            const uint64_t x0 = t[2 * i], x1 = t[2 * i + 1];
+            t[2 * (2 * i + 1)] = hi[0];
+            t[2 * (2 * i + 1) + 1] = hi[1];
+            t[2 * (2 * i)] = x0 ^ hi[0];
+            t[2 * (2 * i) + 1] = x1 ^ hi[1];
+        }
+}
+
+/* crossfield.rs:66-78: the same order over F162 */
+void orc_eq_f162_msb(const uint64_t *r, int n, uint64_t *t) {
+    t[0] = 1;
+    t[1] = t[2] = 0;
+    for (int round = 0; round < n; round++)
+        for (long i = (1L << round) - 1; i >= 0; i--) {
+            uint64_t hi[3], x[3] = {t[3 * i], t[3 * i + 1], t[3 * i + 2]};
+            orc_f162_mul(x, r + 3 * round, hi);
+            for (int k = 0; k < 3; k++) {
+                t[3 * (2 * i + 1) + k] = hi[k];
+                t[3 * (2 * i) + k] = x[k] ^ hi[k];
+            }
+        }
+}
+
+/* switch.rs:40-46: sum_j trace[j] eq(r, j) */
+void orc_b128_mle(const uint64_t *trace, int l, const uint64_t *r, uint64_t *out) {
+    const size_t n = (size_t)1 << l;
+    uint64_t *eq = (uint64_t *)malloc(16 * n);
+    orc_eq_b128(r, l, eq);
+    uint64_t acc[2] = {0, 0};
+    for (size_t j = 0; j < n; j++) {
+        uint64_t pr[2];
+        orc_b128_mul(trace + 2 * j, eq + 2 * j, pr);
+        acc[0] ^= pr[0];
+        acc[1] ^= pr[1];
+    }
+    free(eq);
+    out[0] = acc[0];
+    out[1] = acc[1];
+}
+
+/* switch.rs:49-62: bit k of word j at index k + 128 j */
+void orc_bit_mle(const uint64_t *trace, int l, const uint64_t *r_lo, const uint64_t *r_hi, uint64_t *out) {
+    const size_t n = (size_t)1 << l;
+    uint64_t eq_lo[2 * 128];
+    uint64_t *eq_hi = (uint64_t *)malloc(16 * n);
+    orc_eq_b128(r_lo, 7, eq_lo);
+    orc_eq_b128(r_hi, l, eq_hi);
+    uint64_t acc[2] = {0, 0};
+    for (size_t j = 0; j < n; j++) {
+        uint64_t bits[2] = {0, 0}, pr[2];
+        for (int k = 0; k < 128; k++)
+            if ((trace[2 * j + k / 64] >> (k % 64)) & 1) {
+                bits[0] ^= eq_lo[2 * k];
+                bits[1] ^= eq_lo[2 * k + 1];
+            }
+        orc_b128_mul(bits, eq_hi + 2 * j, pr);
+        acc[0] ^= pr[0];
+        acc[1] ^= pr[1];
+    }
+    free(eq_hi);
+    out[0] = acc[0];
+    out[1] = acc[1];
+}
+
+/* crossfield.rs:9-62,106-115: a 128 x F162 tensor (horizontal index = the B128 basis x^i) */
+static void tensor_shift_x(uint64_t c[128][3]) {
+    uint64_t top[3] = {c[127][0], c[127][1], c[127][2]};
+    memmove(c[1], c[0], 127 * 24);
+    c[0][0] = c[0][1] = c[0][2] = 0;
+    const int taps[4] = {0, 1, 2, 7};
+    for (int u = 0; u < 4; u++)
+        for (int w = 0; w < 3; w++) c[taps[u]][w] ^= top[w];
+}
+
+void orc_transparent_coeff(const uint64_t *r_hi, const uint64_t *r_pp, int l, const uint64_t *batch, uint64_t *out) {
+    static uint64_t t[128][3], vs[128][3], hs[128][3];
+    memset(t, 0, sizeof t);
+    t[0][0] = 1;
+    for (int round = 0; round < l; round++) {
+        const uint64_t *v = r_pp + 3 * round, *h = r_hi + 2 * round;
+        for (int i = 0; i < 128; i++) orc_f162_mul(t[i], v, vs[i]);
+        memset(hs, 0, sizeof hs);
+        for (int k = 127; k >= 0; k--) {
+            tensor_shift_x(hs);
+            if ((h[k / 64] >> (k % 64)) & 1)
+                for (int i = 0; i < 128; i++)
+                    for (int w = 0; w < 3; w++) hs[i][w] ^= t[i][w];
+        }
+        for (int i = 0; i < 128; i++)
+            for (int w = 0; w < 3; w++) t[i][w] ^= vs[i][w] ^ hs[i][w];
+    }
+    uint64_t acc[3] = {0, 0, 0};
+    for (int i = 0; i < 128; i++) {
+        uint64_t pr[3];
+        orc_f162_mul(t[i], batch + 3 * i, pr);
+        for (int w = 0; w < 3; w++) acc[w] ^= pr[w];
+    }
+    memcpy(out, acc, 24);
+}
+
+/* challenge.rs:113-149: length-prefixed absorbs; F162 samples cut to 34 top bits */
+void orc_absorb_b128(orc_transcript *t, const uint64_t *xs, size_t n) {
+    t_u64(t, (uint64_t)n);
+    for (size_t i = 0; i < 2 * n; i++) t_u64(t, xs[i]);
+}
+
+void orc_absorb_f162(orc_transcript *t, const uint64_t *xs, size_t n) {
+    t_u64(t, (uint64_t)n);
+    for (size_t i = 0; i < 3 * n; i++) t_u64(t, xs[i]);
+}
+
+void orc_sample_f162(orc_transcript *t, const char *label, size_t n, uint64_t *out) {
+    uint8_t *b = (uint8_t *)malloc(24 * n + 1);
+    orc_fill(t, label, b, 24 * n);
+    for (size_t i = 0; i < 3 * n; i++) {
+        uint64_t x = 0;
+        for (int k = 0; k < 8; k++) x |= (uint64_t)b[8 * i + k] << (8 * k);
+        out[i] = (i % 3 == 2) ? (x & ((1ull << 34) - 1)) : x;
+    }
+    free(b);
+}
+
+void orc_sample_b128(orc_transcript *t, const char *label, size_t n, uint64_t *out) {
+    uint8_t *b = (uint8_t *)malloc(16 * n + 1);
+    orc_fill(t, label, b, 16 * n);
+    for (size_t i = 0; i < 2 * n; i++) {
+        uint64_t x = 0;
+        for (int k = 0; k < 8; k++) x |= (uint64_t)b[8 * i + k] << (8 * k);
+        out[i] = x;
+    }
+    free(b);
+}
+
+/* switch.rs:129-134 */
+static void switch_statement(orc_transcript *t, const uint64_t *r_lo, int r_lo_len, const uint64_t *r_hi, int l,
+                             const uint64_t *claim) {
+    orc_absorb_bytes(t, (const uint8_t *)"labinius/switch", 15);
+    orc_absorb_b128(t, r_lo, (size_t)r_lo_len);
+    orc_absorb_b128(t, r_hi, (size_t)l);
+    orc_absorb_b128(t, claim, 1);
+}
+
+/* EvaluationPoint::msb_first (commitment.rs:163-174): the first cl coordinates reversed are p1,
+ * the rest reversed p0 */
+static void msb_first(const uint64_t *r_pp, int l, int cl, uint64_t *p0, uint64_t *p1) {
+    for (int i = 0; i < cl; i++) memcpy(p1 + 3 * i, r_pp + 3 * (cl - 1 - i), 24);
+    for (int i = 0; i < l - cl; i++) memcpy(p0 + 3 * i, r_pp + 3 * (l - 1 - i), 24);
+}
+
+/* switch.rs:137-164 with crossfield.rs:410-476 and sumcheck.rs:100-145, by plain loops. trace is
+ * 2^l B128; r_lo_len is 0 (a claim on the words) or 7 (on their bits, weighed in the verifier);
+ * partials 128 B128, rounds l x [e0, einf], opened and the point's p0 (l - cl) and p1 (cl) F162. */
+void orc_switch_prove(const uint64_t *trace, int l, int cl, const uint64_t *r_lo, int r_lo_len, const uint64_t *r_hi,
+                      const uint64_t *claim, orc_transcript *t, uint64_t *partials, uint64_t *rounds,
+                      uint64_t *opened, uint64_t *p0, uint64_t *p1) {
+    const size_t n = (size_t)1 << l;
+    switch_statement(t, r_lo, r_lo_len, r_hi, l, claim);
+    uint64_t *eq = (uint64_t *)malloc(16 * n);
+    orc_eq_b128(r_hi, l, eq);
+    /* crossfield.rs:154-158: v[k] = sum_j bit_k(trace[j]) eq_hi[j] */
+    memset(partials, 0, 16 * 128);
+    for (size_t j = 0; j < n; j++)
+        for (int k = 0; k < 128; k++)
+            if ((trace[2 * j + k / 64] >> (k % 64)) & 1) {
+                partials[2 * k] ^= eq[2 * j];
+                partials[2 * k + 1] ^= eq[2 * j + 1];
+            }
+    orc_absorb_b128(t, partials, 128);
+    uint64_t r7[3 * 7], batch[3 * 128];
+    orc_sample_f162(t, "labinius/switch-batch", 7, r7);
+    orc_eq_f162_msb(r7, 7, batch);
+    /* a[j] = sum_k bit_k(eq_hi[j]) batch[k] (crossfield.rs:285-325); p[j] = from_b128(trace[j]) */
+    uint64_t *a = (uint64_t *)calloc(3 * n, 8), *p = (uint64_t *)calloc(3 * n, 8);
+    for (size_t j = 0; j < n; j++) {
+        for (int k = 0; k < 128; k++)
+            if ((eq[2 * j + k / 64] >> (k % 64)) & 1)
+                for (int w = 0; w < 3; w++) a[3 * j + w] ^= batch[3 * k + w];
+        p[3 * j] = trace[2 * j];
+        p[3 * j + 1] = trace[2 * j + 1];
+    }
+    uint64_t *r_pp = (uint64_t *)malloc(24 * (size_t)(l > 0 ? l : 1));
+    size_t half = n;
+    for (int round = 0; round < l; round++) {
+        half /= 2;
+        uint64_t e0[3] = {0, 0, 0}, einf[3] = {0, 0, 0};
+        for (size_t j = 0; j < half; j++) {
+            uint64_t pr[3], da[3], dp[3];
+            orc_f162_mul(a + 3 * j, p + 3 * j, pr);
+            for (int w = 0; w < 3; w++) {
+                e0[w] ^= pr[w];
+                da[w] = a[3 * j + w] ^ a[3 * (j + half) + w];
+                dp[w] = p[3 * j + w] ^ p[3 * (j + half) + w];
+            }
+            orc_f162_mul(da, dp, pr);
+            for (int w = 0; w < 3; w++) einf[w] ^= pr[w];
+        }
+        uint64_t msg[6] = {e0[0], e0[1], e0[2], einf[0], einf[1], einf[2]};
+        memcpy(rounds + 6 * round, msg, 48);
+        orc_absorb_f162(t, msg, 2);
+        uint64_t *r = r_pp + 3 * round;
+        orc_sample_f162(t, "labinius/switch-round", 1, r);
+        for (size_t j = 0; j < half; j++) {
+            uint64_t da[3], dp[3], ma[3], mp[3];
+            for (int w = 0; w < 3; w++) {
+                da[w] = a[3 * j + w] ^ a[3 * (j + half) + w];
+                dp[w] = p[3 * j + w] ^ p[3 * (j + half) + w];
+            }
+            orc_f162_mul(r, da, ma);
+            orc_f162_mul(r, dp, mp);
+            for (int w = 0; w < 3; w++) {
+                a[3 * j + w] ^= ma[w];
+                p[3 * j + w] ^= mp[w];
+            }
+        }
+    }
+    memcpy(opened, p, 24);
+    msb_first(r_pp, l, cl, p0, p1);
+    free(eq);
+    free(a);
+    free(p);
+    free(r_pp);
+}
+
+/* switch.rs:166-209, crossfield.rs:477-541: 1 accepts (and fills the point), 0 rejects */
+int orc_switch_verify(int l, int cl, const uint64_t *r_lo, int r_lo_len, const uint64_t *r_hi, const uint64_t *claim,
+                      const uint64_t *partials, const uint64_t *rounds, const uint64_t *opened, orc_transcript *t,
+                      uint64_t *p0, uint64_t *p1) {
+    switch_statement(t, r_lo, r_lo_len, r_hi, l, claim);
+    /* the weights: x^k for a claim on words, eq(r_lo, k) for one on bits (switch.rs:114-121) */
+    uint64_t wts[2 * 128];
+    if (r_lo_len == 0) {
+        memset(wts, 0, sizeof wts);
+        for (int k = 0; k < 128; k++) wts[2 * k + k / 64] = 1ull << (k % 64);
+    } else {
+        orc_eq_b128(r_lo, 7, wts);
+    }
+    uint64_t acc[2] = {0, 0};
+    for (int k = 0; k < 128; k++) {
+        uint64_t pr[2];
+        orc_b128_mul(partials + 2 * k, wts + 2 * k, pr);
+        acc[0] ^= pr[0];
+        acc[1] ^= pr[1];
+    }
+    if (acc[0] != claim[0] || acc[1] != claim[1]) return 0;
+    orc_absorb_b128(t, partials, 128);
+    uint64_t r7[3 * 7], batch[3 * 128];
+    orc_sample_f162(t, "labinius/switch-batch", 7, r7);
+    orc_eq_f162_msb(r7, 7, batch);
+    /* slice_sum (crossfield.rs:478-490): u_k = sum_i bit_k(v_i) x^i, s = sum_k from_b128(u_k) batch[k] */
+    uint64_t s[3] = {0, 0, 0};
+    for (int k = 0; k < 128; k++) {
+        uint64_t u[3] = {0, 0, 0}, pr[3];
+        for (int i = 0; i < 128; i++)
+            if ((partials[2 * i + k / 64] >> (k % 64)) & 1) u[i / 64] |= 1ull << (i % 64);
+        orc_f162_mul(u, batch + 3 * k, pr);
+        for (int w = 0; w < 3; w++) s[w] ^= pr[w];
+    }
+    uint64_t *r_pp = (uint64_t *)malloc(24 * (size_t)(l > 0 ? l : 1));
+    for (int round = 0; round < l; round++) {
+        const uint64_t *e0 = rounds + 6 * round, *einf = rounds + 6 * round + 3;
+        orc_absorb_f162(t, rounds + 6 * round, 2);
+        uint64_t *r = r_pp + 3 * round;
+        orc_sample_f162(t, "labinius/switch-round", 1, r);
+        /* s = e0 + r (e0 + e1 + einf) + r^2 einf with e1 = s + e0 */
+        uint64_t sum[3], t1[3], r2[3], t2[3];
+        for (int w = 0; w < 3; w++) sum[w] = e0[w] ^ (s[w] ^ e0[w]) ^ einf[w];
+        orc_f162_mul(r, sum, t1);
+        orc_f162_mul(r, r, r2);
+        orc_f162_mul(r2, einf, t2);
+        for (int w = 0; w < 3; w++) s[w] = e0[w] ^ t1[w] ^ t2[w];
+    }
+    uint64_t d[3], dv[3];
+    orc_transparent_coeff(r_hi, r_pp, l, batch, d);
+    int ok = (d[0] | d[1] | d[2]) != 0;
+    orc_f162_mul(d, opened, dv);
+    ok = ok && dv[0] == s[0] && dv[1] == s[1] && dv[2] == s[2];
+    if (ok) msb_first(r_pp, l, cl, p0, p1);
+    free(r_pp);
+    return ok;
+}
