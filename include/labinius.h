@@ -219,6 +219,21 @@ int lbn_derive_evaluation_point_dropped(const lbn_pp *pp, lbn_transcript *t, con
 /* bd/residual.rs:77-130: the squared CRT norm of A v - sum_j c_j col_j (as two u64 halves) */
 int lbn_bd_residual(const lbn_pp *pp, const uint16_t *top, const uint16_t *digits, const lbn_challenge *c,
                     size_t r, const int16_t *v, size_t len_ring, uint64_t *normsq_lo, uint64_t *normsq_hi);
+/* The cross-field switch (switch.rs). B128 values are two u64, the low word first.
+ * switch::prove / prove_bits (switch.rs:63-100, 137-164) on the device: w is the lifted trace
+ * (Witness::lifted, elements [lo, hi, 0]; the third word is read as zero) resident on pr, r_hi has
+ * witness_log_len coordinates, r_lo none (a claim on the words) or 7 (on their bits). Out: the 128
+ * partial evaluations, the witness_log_len sumcheck rounds [e0, einf], the opened value, and the
+ * point (EvaluationPoint::msb_first) at which the commitment to w must open to it. t advances as
+ * the reference's transcript does. */
+int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *w, const uint64_t *r_lo, size_t r_lo_len,
+                     const uint64_t *r_hi, size_t r_hi_len, const uint64_t claim[2], lbn_transcript *t,
+                     uint64_t *partials, lbn_f162 *rounds, lbn_f162 *opened, lbn_f162 *p0, lbn_f162 *p1);
+/* switch::verify / verify_bits (switch.rs:79-86, 110-121, 166-209) on the host: LBN_OK with the
+ * point, or LBN_REJECTED. */
+int lbn_switch_verify(const lbn_params *params, const uint64_t *r_lo, size_t r_lo_len, const uint64_t *r_hi,
+                      size_t r_hi_len, const uint64_t claim[2], const uint64_t *partials, const lbn_f162 *rounds,
+                      const lbn_f162 *opened, lbn_transcript *t, lbn_f162 *p0, lbn_f162 *p1);
 /* Params::bd_cap, scheme/params.rs:250-252 and bd/mod.rs:71-74 (0 if the parameters drop no bits) */
 uint64_t lbn_bd_cap(const lbn_params *params);
 /* verifier.rs:232-275 (verify_folded_opening_bd): LBN_OK or LBN_REJECTED */

@@ -126,4 +126,30 @@ struct GarnerConsts {
     uint32_t inv[GARNER_MAX][GARNER_MAX];
 };
 
+// B128 = GF(2)[x] / (x^128 + x^7 + x^2 + x + 1) (fields/scalar.rs:4, 41-63), low word first: the
+// 256-bit carry-less product by a comb over a's bits, then the high half H folded as
+// H (1 + x + x^2 + x^7) twice (its top 7 bits spill once more).
+__host__ __device__ inline void b128_mul(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1, uint64_t &o0,
+                                         uint64_t &o1) {
+    uint64_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    for (int j = 0; j < 64; j++) {
+        const uint64_t s0 = b0 << j, s1 = (b1 << j) | (j ? b0 >> (64 - j) : 0), s2 = j ? b1 >> (64 - j) : 0;
+        const uint64_t m0 = 0ull - ((a0 >> j) & 1), m1 = 0ull - ((a1 >> j) & 1);
+        p0 ^= s0 & m0;
+        p1 ^= (s1 & m0) ^ (s0 & m1);
+        p2 ^= (s2 & m0) ^ (s1 & m1);
+        p3 ^= s2 & m1;
+    }
+    const uint64_t w0 = p2 ^ (p2 << 1) ^ (p2 << 2) ^ (p2 << 7);
+    const uint64_t w1 = p3 ^ ((p3 << 1) | (p2 >> 63)) ^ ((p3 << 2) | (p2 >> 62)) ^ ((p3 << 7) | (p2 >> 57));
+    const uint64_t th = (p3 >> 63) ^ (p3 >> 62) ^ (p3 >> 57);
+    o0 = p0 ^ w0 ^ th ^ (th << 1) ^ (th << 2) ^ (th << 7);
+    o1 = p1 ^ w1;
+}
+
+// The F162 value passed by value to the sumcheck's fold.
+struct F162v {
+    uint64_t w[3];
+};
+
 }  // namespace lbn

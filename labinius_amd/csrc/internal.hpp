@@ -160,6 +160,13 @@ struct lbn_batch_ws {
     uint32_t sched_t = 8, persist = 0;
 };
 
+// The cross-field switch's workspace (grow-only): eq(r_hi), the batched eq image a, the folded
+// trace, and the small tables and outputs.
+struct lbn_switch_ws {
+    size_t cap_n = 0;
+    uint64_t *d_eq = nullptr, *d_a = nullptr, *d_p = nullptr, *d_small = nullptr;
+};
+
 struct lbn_opening {
     lbn_prover *owner = nullptr;
     uint8_t *d_bytes_f = nullptr;  // the witness in the fold's byte layout
@@ -192,4 +199,5 @@ struct lbn_prover {
     hipEvent_t ev[12] = {};
     float timings[11] = {};
     lbn_batch_ws *batch = nullptr;
+    lbn_switch_ws sw;
 };

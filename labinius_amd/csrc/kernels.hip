@@ -767,6 +767,176 @@ __global__ __launch_bounds__(64 * RE_WAVES) void row_eval_kernel(const uint64_t 
 }
 
 // =============================================================================================
+// the cross-field switch (switch.rs:137-164, fields/crossfield.rs:149-476, fields/sumcheck.rs:100-145)
+// =============================================================================================
+
+// eq(r_hi)[j] over B128 as the product of byte-indexed part tables: parts[c][m] covers bits
+// 8c .. 8c + 7 of j (the host builds them from the coordinates in the order they take the bits).
+__global__ void switch_eq_kernel(const uint64_t *__restrict__ parts, uint32_t np, uint32_t n, uint64_t *__restrict__ eq) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint64_t e0 = parts[2 * (j & 255)], e1 = parts[2 * (j & 255) + 1];
+    for (uint32_t c = 1; c < np; c++) {
+        const uint64_t *f = parts + 2 * ((size_t)c * 256 + ((j >> (8 * c)) & 255));
+        b128_mul(e0, e1, f[0], f[1], e0, e1);
+    }
+    eq[2 * (size_t)j] = e0;
+    eq[2 * (size_t)j + 1] = e1;
+}
+
+// v[k] = sum_j bit_k(trace[j]) eq[j] (crossfield.rs:149-203), the trace read from a lifted witness
+// (F162 [lo, hi, 0]). Per chunk of 64 j: the 256-entry XOR tables of each group of 8 eq values
+// (from two 16-entry nibble tables), the 8 x 128 bit matrix of each group's trace words transposed
+// into per-bit bytes, then one lookup per (k, group). Thread t owns k = t & 127 for half the groups.
+constexpr int SW_CHUNK = 64;
+__global__ __launch_bounds__(256) void switch_partials_kernel(const uint64_t *__restrict__ w,
+                                                              const uint64_t *__restrict__ eq, uint32_t n,
+                                                              uint32_t span, uint64_t *__restrict__ v) {
+    __shared__ uint64_t T[8][256][2];
+    __shared__ uint64_t NT[8][2][16][2];
+    __shared__ uint64_t ws[SW_CHUNK][2];
+    __shared__ uint8_t by[8][128];
+    __shared__ uint64_t red[128][2];
+    const int t = threadIdx.x;
+    const uint32_t nch = n / SW_CHUNK, c0 = blockIdx.x * span, c1 = min(c0 + span, nch);
+    uint64_t acc0 = 0, acc1 = 0;
+    for (uint32_t ch = c0; ch < c1; ch++) {
+        const size_t i0 = (size_t)ch * SW_CHUNK;
+        __syncthreads();  // the previous chunk's tables and bytes are no longer read
+        if (t < 2 * SW_CHUNK) ws[t >> 1][t & 1] = w[3 * (i0 + (t >> 1)) + (t & 1)];
+        {
+            const int g = t >> 5, h = (t >> 4) & 1, x = t & 15;
+            const uint64_t *e = eq + 2 * (i0 + 8 * g + 4 * h);
+            uint64_t a0 = 0, a1 = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t msk = 0ull - (uint64_t)((x >> k) & 1);
+                a0 ^= e[2 * k] & msk;
+                a1 ^= e[2 * k + 1] & msk;
+            }
+            NT[g][h][x][0] = a0;
+            NT[g][h][x][1] = a1;
+        }
+        __syncthreads();
+        for (int e = t; e < 8 * 256; e += 256) {
+            const int g = e >> 8, m = e & 255;
+            T[g][m][0] = NT[g][0][m & 15][0] ^ NT[g][1][m >> 4][0];
+            T[g][m][1] = NT[g][0][m & 15][1] ^ NT[g][1][m >> 4][1];
+        }
+        if (t < 128) {
+            // group g, byte column c: byte c of the group's 8 words, then byte p = bits 8c + p
+            const int g = t >> 4, c = t & 15, wd = c >> 3, sh = 8 * (c & 7);
+            uint64_t x = 0;
+#pragma unroll
+            for (int s = 0; s < 8; s++) x |= ((ws[8 * g + s][wd] >> sh) & 255) << (8 * s);
+            x = transpose8x8(x);
+#pragma unroll
+            for (int p = 0; p < 8; p++) by[g][8 * c + p] = (uint8_t)(x >> (8 * p));
+        }
+        __syncthreads();
+        const int k = t & 127, gh = t >> 7;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int g = 4 * gh + q;
+            const uint64_t *e = T[g][by[g][k]];
+            acc0 ^= e[0];
+            acc1 ^= e[1];
+        }
+    }
+    __syncthreads();
+    if (t >= 128) {
+        red[t - 128][0] = acc0;
+        red[t - 128][1] = acc1;
+    }
+    __syncthreads();
+    if (t < 128) {
+        atomicXor((unsigned long long *)&v[2 * t], (unsigned long long)(acc0 ^ red[t][0]));
+        atomicXor((unsigned long long *)&v[2 * t + 1], (unsigned long long)(acc1 ^ red[t][1]));
+    }
+}
+
+// a[j] = sum_k bit_k(eq[j]) batch[k] (crossfield.rs:285-325): 16 byte tables of 256 F162
+// (psi[c][m] = sum over the bits b of m of batch[8c + b]), staged in LDS, one lookup per byte.
+__global__ __launch_bounds__(256) void switch_psi_kernel(const uint64_t *__restrict__ eq, uint32_t n,
+                                                         const uint64_t *__restrict__ psi, uint64_t *__restrict__ a) {
+    extern __shared__ __attribute__((aligned(16))) uint64_t tab[];  // [16][256][3]
+    for (int i = threadIdx.x; i < 16 * 256 * 3; i += blockDim.x) tab[i] = psi[i];
+    __syncthreads();
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint64_t x[2] = {eq[2 * (size_t)j], eq[2 * (size_t)j + 1]};
+        uint64_t s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            const uint64_t *e = tab + 3 * (c * 256 + ((x[c >> 3] >> (8 * (c & 7))) & 255));
+            s0 ^= e[0];
+            s1 ^= e[1];
+            s2 ^= e[2];
+        }
+        a[3 * (size_t)j] = s0;
+        a[3 * (size_t)j + 1] = s1;
+        a[3 * (size_t)j + 2] = s2;
+    }
+}
+
+// One sumcheck message (sumcheck.rs:100-123): e0 = sum_{j < half} a[j] p[j] and
+// einf = sum (a[j] + a[j + half]) (p[j] + p[j + half]). top = 0 reads p's third word as zero (the
+// lifted trace). Per block an XOR reduction, then one atomic XOR per word into out[6].
+__global__ __launch_bounds__(256) void switch_msg_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ p,
+                                                         uint32_t half, int top, uint64_t *__restrict__ out) {
+    __shared__ uint64_t red[256][6];
+    uint64_t e[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < half; j += gridDim.x * blockDim.x) {
+        const uint64_t *a0 = a + 3 * (size_t)j, *a1 = a + 3 * ((size_t)j + half);
+        const uint64_t *q0 = p + 3 * (size_t)j, *q1 = p + 3 * ((size_t)j + half);
+        const uint64_t x0[3] = {a0[0], a0[1], a0[2]}, y0[3] = {q0[0], q0[1], top ? q0[2] : 0};
+        const uint64_t dx[3] = {a0[0] ^ a1[0], a0[1] ^ a1[1], a0[2] ^ a1[2]};
+        const uint64_t dy[3] = {q0[0] ^ q1[0], q0[1] ^ q1[1], top ? q0[2] ^ q1[2] : 0};
+        uint64_t m0[3], m1[3];
+        f162_mul_dev(x0, y0, m0);
+        f162_mul_dev(dx, dy, m1);
+#pragma unroll
+        for (int w = 0; w < 3; w++) {
+            e[w] ^= m0[w];
+            e[3 + w] ^= m1[w];
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < 6; w++) red[threadIdx.x][w] = e[w];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int w = 0; w < 6; w++) red[threadIdx.x][w] ^= red[threadIdx.x + s][w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) atomicXor((unsigned long long *)&out[threadIdx.x], (unsigned long long)red[0][threadIdx.x]);
+}
+
+// The fold (sumcheck.rs:125-145): a[j] += r (a[j] + a[j + half]) in place, p_out[j] = p[j] +
+// r (p[j] + p[j + half]) for j < half (p_out may be p: only j < half is written).
+__global__ __launch_bounds__(256) void switch_fold_kernel(uint64_t *__restrict__ a, const uint64_t *p, uint64_t *p_out,
+                                                          uint32_t half, F162v r, int top) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= half) return;
+    uint64_t *a0 = a + 3 * (size_t)j;
+    const uint64_t *a1 = a + 3 * ((size_t)j + half);
+    const uint64_t *q0 = p + 3 * (size_t)j, *q1 = p + 3 * ((size_t)j + half);
+    const uint64_t y0[3] = {q0[0], q0[1], top ? q0[2] : 0};
+    const uint64_t dx[3] = {a0[0] ^ a1[0], a0[1] ^ a1[1], a0[2] ^ a1[2]};
+    const uint64_t dy[3] = {y0[0] ^ q1[0], y0[1] ^ q1[1], top ? q0[2] ^ q1[2] : 0};
+    uint64_t m0[3], m1[3];
+    f162_mul_dev(r.w, dx, m0);
+    f162_mul_dev(r.w, dy, m1);
+    const uint64_t na[3] = {a0[0] ^ m0[0], a0[1] ^ m0[1], a0[2] ^ m0[2]};
+    uint64_t *o = p_out + 3 * (size_t)j;
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        a0[w] = na[w];
+        o[w] = y0[w] ^ m1[w];
+    }
+}
+
+// =============================================================================================
 // the bit-dropped digits
 // =============================================================================================
 
@@ -945,6 +1115,40 @@ hipError_t launch_garner(hipStream_t s, const int16_t *cen, const GarnerConsts &
                          uint16_t *top, uint16_t *digits) {
     if (count == 0) return hipSuccess;
     LBN_LAUNCH(garner_kernel, dim3((count + 255) / 256), dim3(256), 0, s, cen, gc, count, top, digits);
+    return hipSuccess;
+}
+
+hipError_t launch_switch_eq(hipStream_t s, const uint64_t *parts, uint32_t np, uint32_t n, uint64_t *eq) {
+    LBN_LAUNCH(switch_eq_kernel, dim3((n + 255) / 256), dim3(256), 0, s, parts, np, n, eq);
+    return hipSuccess;
+}
+
+hipError_t launch_switch_partials(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t n, uint64_t *v) {
+    // n is a multiple of 64 (a trace of 2^l words, l >= 8); about 2 blocks per CU
+    const uint32_t nch = n / SW_CHUNK, span = std::max(1u, (nch + 511) / 512), blocks = (nch + span - 1) / span;
+    LBN_LAUNCH(switch_partials_kernel, dim3(blocks), dim3(256), 0, s, w, eq, n, span, v);
+    return hipSuccess;
+}
+
+hipError_t launch_switch_psi(hipStream_t s, const uint64_t *eq, uint32_t n, const uint64_t *psi, uint64_t *a) {
+    static const hipError_t attr = hipFuncSetAttribute((const void *)switch_psi_kernel,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, 16 * 256 * 24);
+    if (attr != hipSuccess) return attr;
+    const uint32_t blocks = std::min(256u, (n + 255) / 256);
+    LBN_LAUNCH(switch_psi_kernel, dim3(blocks), dim3(256), 16 * 256 * 24, s, eq, n, psi, a);
+    return hipSuccess;
+}
+
+hipError_t launch_switch_msg(hipStream_t s, const uint64_t *a, const uint64_t *p, uint32_t half, int top, uint64_t *out) {
+    const uint32_t blocks = std::min(1024u, (half + 255) / 256);
+    LBN_LAUNCH(switch_msg_kernel, dim3(blocks), dim3(256), 0, s, a, p, half, top, out);
+    return hipSuccess;
+}
+
+hipError_t launch_switch_fold(hipStream_t s, uint64_t *a, const uint64_t *p, uint64_t *p_out, uint32_t half,
+                              const uint64_t r[3], int top) {
+    F162v rv{{r[0], r[1], r[2]}};
+    LBN_LAUNCH(switch_fold_kernel, dim3((half + 255) / 256), dim3(256), 0, s, a, p, p_out, half, rv, top);
     return hipSuccess;
 }
 

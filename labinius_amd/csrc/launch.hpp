@@ -38,6 +38,13 @@ hipError_t launch_intt(hipStream_t s, const uint32_t *vhat, uint32_t L, const In
 hipError_t launch_garner(hipStream_t s, const int16_t *cen, const GarnerConsts &gc, uint32_t count,
                          uint16_t *top, uint16_t *digits);
 hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *part, uint64_t *eq);
+// the cross-field switch's passes (B128 as 2 u64, F162 as 3 u64)
+hipError_t launch_switch_eq(hipStream_t s, const uint64_t *parts, uint32_t np, uint32_t n, uint64_t *eq);
+hipError_t launch_switch_partials(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t n, uint64_t *v);
+hipError_t launch_switch_psi(hipStream_t s, const uint64_t *eq, uint32_t n, const uint64_t *psi, uint64_t *a);
+hipError_t launch_switch_msg(hipStream_t s, const uint64_t *a, const uint64_t *p, uint32_t half, int top, uint64_t *out);
+hipError_t launch_switch_fold(hipStream_t s, uint64_t *a, const uint64_t *p, uint64_t *p_out, uint32_t half,
+                              const uint64_t r[3], int top);
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u);
 

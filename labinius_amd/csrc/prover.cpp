@@ -343,7 +343,8 @@ static int build_device(lbn_pp *pp) {
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
                     pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_u, pr->d_wscratch,
-                    pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out};
+                    pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out, pr->sw.d_eq, pr->sw.d_a, pr->sw.d_p,
+                    pr->sw.d_small};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);

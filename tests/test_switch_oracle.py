@@ -121,3 +121,30 @@ def test_switch_round_trip_and_rejection(orc, bits):
     # another transcript domain derives other challenges and another point
     _, _, _, o0, _ = orc.switch_prove(trace, cl, r_lo, r_hi, claim, orc.Transcript(b"u"))
     assert not np.array_equal(o0, p0)
+
+
+@pytest.mark.parametrize("bits", [False, True])
+def test_host_verifier_accepts_oracle_proofs(orc, lbn, bits):
+    """The library's host verifier (lbn_switch_verify) on proofs the oracle made: the same point,
+    and the same rejections."""
+    from labinius_amd import switch
+    l, cl = 10, 3
+    params = lbn.Params(l, cl, [])
+    rng = np.random.default_rng(7 + bits)
+    trace = rand_b128(rng, 1 << l)
+    r_hi = rand_b128(rng, l)
+    r_lo = rand_b128(rng, 7) if bits else None
+    claim = orc.bit_mle(trace, r_lo, r_hi) if bits else orc.b128_mle(trace, r_hi)
+    partials, rounds, opened, p0, p1 = orc.switch_prove(trace, cl, r_lo, r_hi, claim, orc.Transcript(b"t"))
+    proof = switch.SwitchProof(partials, rounds, opened)
+    verify = (lambda c, pr: switch.verify_bits(params, r_lo, r_hi, c, pr, lbn.Transcript(b"t"))) if bits else \
+        (lambda c, pr: switch.verify(params, r_hi, c, pr, lbn.Transcript(b"t")))
+    point, value = verify(claim, proof)
+    assert np.array_equal(point.p0, p0) and np.array_equal(point.p1, p1) and np.array_equal(value, opened)
+    wrong = claim.copy()
+    wrong[1] ^= np.uint64(1 << 40)
+    with pytest.raises(switch.SwitchError):
+        verify(wrong, proof)
+    bad = switch.SwitchProof(partials, rounds, opened ^ np.array([0, 1, 0], np.uint64))
+    with pytest.raises(switch.SwitchError):
+        verify(claim, bad)
