@@ -199,6 +199,50 @@ def batched(lbn, reps):
             "graph": "one hipGraph launch per batch (layout, GEMM, finish)"}
 
 
+def switch_bench(lbn, reps=5):
+    """The cross-field switch (switch.rs) on lifted B128 traces of 2^20 and 2^22 words, the batched
+    workload's largest, under its key (row length 2^11, moduli 3889 / 9721): the prover on the GPU
+    over the resident lifted trace, the verifier on the host. The word claim is the one the partial
+    evaluations give (sum_k v_k x^k), so the timed proof verifies."""
+    import numpy as np
+    from labinius_amd import switch
+    from labinius_amd import workloads as wl
+
+    def times_x(v, k):  # v x^k in B128 = GF(2)[x] / (x^128 + x^7 + x^2 + x + 1)
+        v <<= k
+        for i in range(v.bit_length() - 1, 127, -1):
+            if (v >> i) & 1:
+                v ^= (1 << i) | (0x87 << (i - 128))
+        return v
+
+    out = {}
+    rng = np.random.default_rng(0x5317)
+    for e in (20, 22):
+        params = wl.witness_params(e)
+        prover = lbn.Prover(lbn.PublicParameters.from_seed(params, MATRIX_SEED))
+        trace = rng.integers(0, 2**63, (1 << e, 2), dtype=np.uint64) * np.uint64(2)
+        rw = prover.upload(lbn.Witness.lifted(params, trace))
+        r = rng.integers(0, 2**63, (e, 2), dtype=np.uint64)
+        first, _ = switch.prove(prover, rw, r, np.zeros(2, np.uint64), lbn.Transcript(b"labinius/bench/switch"))
+        c = 0
+        for k, v in enumerate(first.partials):
+            c ^= times_x(int(v[0]) | (int(v[1]) << 64), k)
+        claim = np.array([c & (2**64 - 1), c >> 64], np.uint64)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            proof, _ = switch.prove(prover, rw, r, claim, lbn.Transcript(b"labinius/bench/switch"))
+            ts.append(1e3 * (time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        switch.verify(params, r, claim, proof, lbn.Transcript(b"labinius/bench/switch"))
+        verify_ms = 1e3 * (time.perf_counter() - t0)
+        out[f"2^{e}"] = {"prove_ms": round(float(np.median(ts)), 3), "verify_ms_host": round(verify_ms, 3),
+                         "rounds": e}
+    out["note"] = ("prove = GPU passes plus one host transcript step per sumcheck round (one device "
+                   "synchronisation each); the trace is resident as Witness::lifted")
+    return out
+
+
 def serving_bench(lbn, rate, count):
     """Poisson arrivals of commit -> evaluate -> fold requests (sizes 2^16 40 %, 2^18 30 %, 2^20 20 %,
     2^22 8 %, 2^24 2 %; Clear or bit-dropped at random) through the continuous-batching server,
@@ -249,6 +293,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-commit comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
+    ap.add_argument("--no-switch", action="store_true", help="skip the cross-field switch timing")
     ap.add_argument("--serving-rate", type=float, default=100.0, help="requests per second")
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -377,6 +422,9 @@ def main():
     if not args.no_serving:
         log("serving...")
         line["serving"] = serving_bench(lbn, args.serving_rate, args.serving_requests)
+    if not args.no_switch:
+        log("switch...")
+        line["switch"] = switch_bench(lbn)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
         line["cpu_baseline"] = cpu_baseline(params)
