@@ -881,25 +881,21 @@ __global__ __launch_bounds__(256) void switch_psi_kernel(const uint64_t *__restr
 // One sumcheck message (sumcheck.rs:100-123): e0 = sum_{j < half} a[j] p[j] and
 // einf = sum (a[j] + a[j + half]) (p[j] + p[j + half]). top = 0 reads p's third word as zero (the
 // lifted trace). Per block an XOR reduction, then one atomic XOR per word into out[6].
-__global__ __launch_bounds__(256) void switch_msg_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ p,
-                                                         uint32_t half, int top, uint64_t *__restrict__ out) {
-    __shared__ uint64_t red[256][6];
-    uint64_t e[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < half; j += gridDim.x * blockDim.x) {
-        const uint64_t *a0 = a + 3 * (size_t)j, *a1 = a + 3 * ((size_t)j + half);
-        const uint64_t *q0 = p + 3 * (size_t)j, *q1 = p + 3 * ((size_t)j + half);
-        const uint64_t x0[3] = {a0[0], a0[1], a0[2]}, y0[3] = {q0[0], q0[1], top ? q0[2] : 0};
-        const uint64_t dx[3] = {a0[0] ^ a1[0], a0[1] ^ a1[1], a0[2] ^ a1[2]};
-        const uint64_t dy[3] = {q0[0] ^ q1[0], q0[1] ^ q1[1], top ? q0[2] ^ q1[2] : 0};
-        uint64_t m0[3], m1[3];
-        f162_mul_dev(x0, y0, m0);
-        f162_mul_dev(dx, dy, m1);
+__device__ __forceinline__ void msg_terms(const uint64_t *x0, const uint64_t *x1, const uint64_t *y0, const uint64_t *y1,
+                                          uint64_t *e) {
+    const uint64_t dx[3] = {x0[0] ^ x1[0], x0[1] ^ x1[1], x0[2] ^ x1[2]};
+    const uint64_t dy[3] = {y0[0] ^ y1[0], y0[1] ^ y1[1], y0[2] ^ y1[2]};
+    uint64_t m0[3], m1[3];
+    f162_mul_dev(x0, y0, m0);
+    f162_mul_dev(dx, dy, m1);
 #pragma unroll
-        for (int w = 0; w < 3; w++) {
-            e[w] ^= m0[w];
-            e[3 + w] ^= m1[w];
-        }
+    for (int w = 0; w < 3; w++) {
+        e[w] ^= m0[w];
+        e[3 + w] ^= m1[w];
     }
+}
+
+__device__ __forceinline__ void block_xor6(uint64_t *e, uint64_t (*red)[6], uint64_t *out) {
 #pragma unroll
     for (int w = 0; w < 6; w++) red[threadIdx.x][w] = e[w];
     __syncthreads();
@@ -912,28 +908,116 @@ __global__ __launch_bounds__(256) void switch_msg_kernel(const uint64_t *__restr
     if (threadIdx.x < 6) atomicXor((unsigned long long *)&out[threadIdx.x], (unsigned long long)red[0][threadIdx.x]);
 }
 
-// The fold (sumcheck.rs:125-145): a[j] += r (a[j] + a[j + half]) in place, p_out[j] = p[j] +
-// r (p[j] + p[j + half]) for j < half (p_out may be p: only j < half is written).
-__global__ __launch_bounds__(256) void switch_fold_kernel(uint64_t *__restrict__ a, const uint64_t *p, uint64_t *p_out,
-                                                          uint32_t half, F162v r, int top) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= half) return;
-    uint64_t *a0 = a + 3 * (size_t)j;
-    const uint64_t *a1 = a + 3 * ((size_t)j + half);
-    const uint64_t *q0 = p + 3 * (size_t)j, *q1 = p + 3 * ((size_t)j + half);
-    const uint64_t y0[3] = {q0[0], q0[1], top ? q0[2] : 0};
-    const uint64_t dx[3] = {a0[0] ^ a1[0], a0[1] ^ a1[1], a0[2] ^ a1[2]};
-    const uint64_t dy[3] = {y0[0] ^ q1[0], y0[1] ^ q1[1], top ? q0[2] ^ q1[2] : 0};
-    uint64_t m0[3], m1[3];
-    f162_mul_dev(r.w, dx, m0);
-    f162_mul_dev(r.w, dy, m1);
-    const uint64_t na[3] = {a0[0] ^ m0[0], a0[1] ^ m0[1], a0[2] ^ m0[2]};
-    uint64_t *o = p_out + 3 * (size_t)j;
-#pragma unroll
-    for (int w = 0; w < 3; w++) {
-        a0[w] = na[w];
-        o[w] = y0[w] ^ m1[w];
+__global__ __launch_bounds__(256) void switch_msg_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ p,
+                                                         uint32_t half, int top, uint64_t *__restrict__ out) {
+    __shared__ uint64_t red[256][6];
+    uint64_t e[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < half; j += gridDim.x * blockDim.x) {
+        const uint64_t *q0 = p + 3 * (size_t)j, *q1 = p + 3 * ((size_t)j + half);
+        const uint64_t y0[3] = {q0[0], q0[1], top ? q0[2] : 0}, y1[3] = {q1[0], q1[1], top ? q1[2] : 0};
+        msg_terms(a + 3 * (size_t)j, a + 3 * ((size_t)j + half), y0, y1, e);
     }
+    block_xor6(e, red, out);
+}
+
+// The fold by the round's challenge r (sumcheck.rs:125-145) fused with the next round's message.
+// Multiplying by the fixed r is F2-linear: r d = sum_b T_b[byte b of d] over the 21 bytes of d,
+// T_b[m] = r (m x^{8b}), built per block in LDS from the basis r x^i (i < 162). With h = half / 2,
+// thread j folds the pairs (j, j + half) and (j + h, j + h + half) into j and j + h (in place;
+// p is read from pin, the lifted trace in the first round, and written to pout), then adds the
+// next message's terms for the pair (j, j + h). The grid is at most one block per CU, striding.
+constexpr int RT_BYTES = 21, RT_LDS = RT_BYTES * 256 * 24;
+constexpr int RT_TAIL = 256 * 6 * 8;  // after T: the basis (168 x 24 B), then the reduction (256 x 48 B)
+static_assert(168 * 24 <= RT_TAIL && RT_LDS + RT_TAIL <= 160 * 1024, "fold tables in LDS");
+__device__ __forceinline__ void mul_r(const uint64_t (*T)[3], const uint64_t *d, uint64_t *o) {
+    uint64_t s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int b = 0; b < RT_BYTES; b++) {
+        const uint64_t *e = T[b * 256 + ((d[b >> 3] >> (8 * (b & 7))) & 255)];
+        s0 ^= e[0];
+        s1 ^= e[1];
+        s2 ^= e[2];
+    }
+    o[0] = s0;
+    o[1] = s1;
+    o[2] = s2;
+}
+
+__global__ __launch_bounds__(256) void switch_fold_msg_kernel(uint64_t *__restrict__ a, const uint64_t *pin,
+                                                              uint64_t *pout, uint32_t half, F162v r, int top,
+                                                              uint64_t *__restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) uint64_t lds[];
+    uint64_t(*T)[3] = (uint64_t(*)[3])lds;                    // [21 * 256][3]
+    uint64_t(*B)[3] = (uint64_t(*)[3])(lds + RT_BYTES * 256 * 3);  // [168][3], then the reduction
+    uint64_t(*red)[6] = (uint64_t(*)[6])(lds + RT_BYTES * 256 * 3);
+    const int t = threadIdx.x;
+    if (t < 168) {
+        // r x^t: r shifted by t, reduced by x^162 = x^81 + 1 (t >= 162: zero, past the field)
+        uint64_t p6[6] = {0, 0, 0, 0, 0, 0};
+        if (t < 162) {
+            const int ws = t >> 6, bs = t & 63;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                p6[k + ws] ^= r.w[k] << bs;
+                if (bs) p6[k + ws + 1] ^= r.w[k] >> (64 - bs);
+            }
+        }
+        const uint64_t H0 = (p6[2] >> 34) | (p6[3] << 30), H1 = (p6[3] >> 34) | (p6[4] << 30);
+        const uint64_t H2w = (p6[4] >> 34) | (p6[5] << 30);
+        const uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+        const uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
+        B[t][0] = p6[0] ^ H0 ^ G0;
+        B[t][1] = p6[1] ^ H1 ^ G1 ^ (S0 << 17);
+        B[t][2] = (p6[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
+    }
+    __syncthreads();
+    for (int e = t; e < RT_BYTES * 256; e += 256) {
+        const int b = e >> 8, m = e & 255;
+        uint64_t s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint64_t msk = 0ull - (uint64_t)((m >> k) & 1);
+            s0 ^= B[8 * b + k][0] & msk;
+            s1 ^= B[8 * b + k][1] & msk;
+            s2 ^= B[8 * b + k][2] & msk;
+        }
+        T[e][0] = s0;
+        T[e][1] = s1;
+        T[e][2] = s2;
+    }
+    __syncthreads();  // T is built and B read: B's area becomes the reduction buffer
+    const uint32_t h = half / 2;
+    uint64_t e6[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t j = blockIdx.x * blockDim.x + t; j < h; j += gridDim.x * blockDim.x) {
+        uint64_t fa[2][3], fp[2][3];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const size_t i0 = (size_t)j + u * h, i1 = i0 + half;
+            const uint64_t x0[3] = {a[3 * i0], a[3 * i0 + 1], a[3 * i0 + 2]};
+            const uint64_t da[3] = {x0[0] ^ a[3 * i1], x0[1] ^ a[3 * i1 + 1], x0[2] ^ a[3 * i1 + 2]};
+            const uint64_t y0[3] = {pin[3 * i0], pin[3 * i0 + 1], top ? pin[3 * i0 + 2] : 0};
+            const uint64_t dp[3] = {y0[0] ^ pin[3 * i1], y0[1] ^ pin[3 * i1 + 1], top ? y0[2] ^ pin[3 * i1 + 2] : 0};
+            uint64_t ma[3], mp[3];
+            mul_r(T, da, ma);
+            mul_r(T, dp, mp);
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                fa[u][w] = x0[w] ^ ma[w];
+                fp[u][w] = y0[w] ^ mp[w];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const size_t i0 = (size_t)j + u * h;
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                a[3 * i0 + w] = fa[u][w];
+                pout[3 * i0 + w] = fp[u][w];
+            }
+        }
+        msg_terms(fa[0], fa[1], fp[0], fp[1], e6);
+    }
+    block_xor6(e6, red, out);
 }
 
 // =============================================================================================
@@ -1145,10 +1229,14 @@ hipError_t launch_switch_msg(hipStream_t s, const uint64_t *a, const uint64_t *p
     return hipSuccess;
 }
 
-hipError_t launch_switch_fold(hipStream_t s, uint64_t *a, const uint64_t *p, uint64_t *p_out, uint32_t half,
-                              const uint64_t r[3], int top) {
+hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pin, uint64_t *pout, uint32_t half,
+                                  const uint64_t r[3], int top, uint64_t *out) {
+    static const hipError_t attr = hipFuncSetAttribute((const void *)switch_fold_msg_kernel,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, RT_LDS + RT_TAIL);
+    if (attr != hipSuccess) return attr;
     F162v rv{{r[0], r[1], r[2]}};
-    LBN_LAUNCH(switch_fold_kernel, dim3((half + 255) / 256), dim3(256), 0, s, a, p, p_out, half, rv, top);
+    const uint32_t h = half / 2, blocks = std::max(1u, std::min(256u, (h + 255) / 256));
+    LBN_LAUNCH(switch_fold_msg_kernel, dim3(blocks), dim3(256), RT_LDS + RT_TAIL, s, a, pin, pout, half, rv, top, out);
     return hipSuccess;
 }
 

@@ -43,8 +43,9 @@ hipError_t launch_switch_eq(hipStream_t s, const uint64_t *parts, uint32_t np, u
 hipError_t launch_switch_partials(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t n, uint64_t *v);
 hipError_t launch_switch_psi(hipStream_t s, const uint64_t *eq, uint32_t n, const uint64_t *psi, uint64_t *a);
 hipError_t launch_switch_msg(hipStream_t s, const uint64_t *a, const uint64_t *p, uint32_t half, int top, uint64_t *out);
-hipError_t launch_switch_fold(hipStream_t s, uint64_t *a, const uint64_t *p, uint64_t *p_out, uint32_t half,
-                              const uint64_t r[3], int top);
+// round i's fold by r (arrays of 2 half -> half) and round i + 1's message (half >= 2)
+hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pin, uint64_t *pout, uint32_t half,
+                                  const uint64_t r[3], int top, uint64_t *out);
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u);
 

@@ -150,8 +150,8 @@ int sw_reserve(lbn_prover *pr, size_t n, size_t np) {
         w.cap_n = n;
     }
     if (!w.d_small) {
-        // parts (up to 4 x 256 B128), psi (16 x 256 F162), v (128 B128), msg (6 words)
-        if ((rc = dalloc(&w.d_small, (size_t)4 * 256 * 2 + 16 * 256 * 3 + 256 + 8))) return rc;
+        // parts (up to 4 x 256 B128), psi (16 x 256 F162), v (128 B128), one message per round
+        if ((rc = dalloc(&w.d_small, (size_t)4 * 256 * 2 + 16 * 256 * 3 + 256 + 6 * 32))) return rc;
     }
     (void)np;
     return LBN_OK;
@@ -218,26 +218,33 @@ int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *wt, const uint64_t *r_l
         }
     HIP_TRY(hipMemcpyAsync(d_psi, psi.data(), psi.size() * 8, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_switch_psi(st, ws.d_eq, (uint32_t)n, d_psi, ws.d_a));
-    // the sumcheck: a against the lifted trace (its third word read as zero), the folded trace in d_p
+    // the sumcheck: a against the lifted trace (its third word read as zero), the folded trace in
+    // d_p. Round i's fold runs fused with round i + 1's message; the last fold (two elements) is
+    // the host's.
     std::vector<F162> r_pp(l);
-    size_t half = n;
+    HIP_TRY(launch_fill_u32(st, (uint32_t *)d_msg, 12 * l, 0));
+    HIP_TRY(launch_switch_msg(st, ws.d_a, wt->d_w, (uint32_t)(n / 2), 0, d_msg));
+    size_t half = n / 2;
     for (size_t round = 0; round < l; round++) {
-        half /= 2;
-        const uint64_t *pin = round == 0 ? wt->d_w : ws.d_p;
-        const int top = round == 0 ? 0 : 1;
         uint64_t msg[6];
-        HIP_TRY(launch_fill_u32(st, (uint32_t *)d_msg, 12, 0));
-        HIP_TRY(launch_switch_msg(st, ws.d_a, pin, (uint32_t)half, top, d_msg));
-        HIP_TRY(hipMemcpyAsync(msg, d_msg, 48, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(msg, d_msg + 6 * round, 48, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         std::memcpy(rounds[2 * round].w, msg, 24);
         std::memcpy(rounds[2 * round + 1].w, msg + 3, 24);
         absorb_f162(t, (const F162 *)msg, 2);
         r_pp[round] = sample_f162(t, "labinius/switch-round", 1)[0];
-        HIP_TRY(launch_switch_fold(st, ws.d_a, pin, ws.d_p, (uint32_t)half, r_pp[round].w, top));
+        if (round + 1 < l) {
+            HIP_TRY(launch_switch_fold_msg(st, ws.d_a, round == 0 ? wt->d_w : ws.d_p, ws.d_p, (uint32_t)half,
+                                           r_pp[round].w, round == 0 ? 0 : 1, d_msg + 6 * (round + 1)));
+            half /= 2;
+        }
     }
-    HIP_TRY(hipMemcpyAsync(opened->w, ws.d_p, 24, hipMemcpyDeviceToHost, st));
+    // the last fold: p[0] + r (p[0] + p[1])
+    F162 pl[2];
+    HIP_TRY(hipMemcpyAsync(pl, ws.d_p, 48, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const F162 op = f162_add(pl[0], f162_mul(r_pp[l - 1], f162_add(pl[0], pl[1])));
+    std::memcpy(opened->w, op.w, 24);
     msb_first(r_pp, cl, p0, p1);
     return LBN_OK;
 }
