@@ -97,7 +97,7 @@ def cpu_model():
     return platform.machine()
 
 
-def cpu_baseline(params):
+def cpu_baseline(params, folded=None):
     """The oracle (a single-threaded plain-C restatement of the reference algorithm) on a sample of
     the same workload, scaled to the whole step: commit and row evaluation of a few columns (each
     column is independent), and the fold of a few output ring elements (each depends on every
@@ -131,11 +131,48 @@ def cpu_baseline(params):
     orc.fold_rows(params.primes()[0], nrows, cols, wf, ch, np.arange(nrows))
     t_fold = (time.perf_counter() - t0) / nrows * L
     total_ms = 1e3 * (t_commit + t_eval + t_fold)
-    return {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+    extra = {}
+    if folded is not None:  # the wire encode of the step's folded witness (rans.rs:18-99), whole
+        t0 = time.perf_counter()
+        orc.rans_encode(folded, params.primes()[0])
+        extra["wire_encode_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
+    return {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "port", **extra,
             "sample": f"oracle (plain C, one pinned thread) on {ncols} of {cols} columns for commit and row "
                       f"evaluation and {nrows} of {L} output ring elements for the fold, scaled linearly; "
                       f"commit {1e3 * t_commit:.0f} ms, eval {1e3 * t_eval:.0f} ms, fold {1e3 * t_fold:.0f} ms; "
                       f"host {cpu_model()}"}
+
+
+def wire_bench(lbn, prover, params, folded, reps=5):
+    """The wire forms of the step's outputs (wire/mod.rs, wire/rans.rs), on the GPU over what the
+    step left in HBM: the commitment bit-packed at ceil(log2 q) bits a slot, and the folded witness
+    rANS-coded (the min / max and counts on the device, the table quantised on the host, one
+    wavefront for the 64 interleaved coders). Times are wall clock with the bytes copied back."""
+    import numpy as np
+    from labinius_amd import wire
+
+    def med(f):
+        ts, out = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = f()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return out, round(float(np.median(ts)), 3)
+
+    packed, pack_ms = med(lambda: wire.pack_commitment(prover))
+    coded, encode_ms = med(lambda: wire.encode(prover, params.primes()[0]))
+    total, stats, kernel = wire.timings(prover)
+    assert np.array_equal(wire.decode(coded).elements, folded.elements)
+    coefs = folded.elements.size
+    entropy = wire.entropy_bytes(folded.elements)
+    return {"pack_commitment_ms": pack_ms, "commitment_bytes": len(packed),
+            "commitment_bytes_at_i16": 2 * 4 * params.columns() * len(params.primes()) * 162,
+            "encode_ms": encode_ms, "encode_device_ms": {"total": round(total, 4), "min_max_counts": round(stats, 4),
+                                                         "rans_kernel": round(kernel, 4)},
+            "folded_coefficients": coefs, "encoded_bytes": len(coded),
+            "bits_per_coefficient": round(8 * len(coded) / coefs, 4),
+            "entropy_bits_per_coefficient": round(8 * entropy / coefs, 4),
+            "sigma": round(float(np.sqrt(np.mean(folded.elements.astype(np.float64) ** 2))), 2)}
 
 
 def batched(lbn, reps):
@@ -376,7 +413,7 @@ def main():
     t0 = time.perf_counter()
     c2, o2 = prover.commit_resident(rw)
     prover.mle_evaluate(rw, point)
-    prover.fold(o2, challenges)
+    folded = prover.fold(o2, challenges)
     d2h_ms = 1e3 * (time.perf_counter() - t0)
 
     gemm_ms = stage["commit_gemm"]
@@ -416,6 +453,8 @@ def main():
                                             "unit": "TOP/s",
                                             "executed_frac_of_random_rate": round(xops / (gemm_ms * 1e-3) / 1e12 / rate_r, 4)}},
     }
+    log("wire...")
+    line["wire"] = wire_bench(lbn, prover, params, folded)
     if not args.no_batched:
         log("batched commit...")
         line["batched"] = batched(lbn, 3)
@@ -427,7 +466,7 @@ def main():
         line["switch"] = switch_bench(lbn)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
-        line["cpu_baseline"] = cpu_baseline(params)
+        line["cpu_baseline"] = cpu_baseline(params, folded.elements)
     print(json.dumps(line), flush=True)
 
 

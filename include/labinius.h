@@ -53,7 +53,10 @@ enum {
     LBN_ERR_ARG = 12,
     /* VerificationError::Rejected, scheme/commitment.rs:449-451 */
     LBN_REJECTED = 13,
-    LBN_ERR_NO_DEVICE = 14
+    LBN_ERR_NO_DEVICE = 14,
+    /* WireError, wire/mod.rs:67-74 */
+    LBN_WIRE_TRUNCATED = 15,         /* the byte string ran out before the object did */
+    LBN_WIRE_MALFORMED = 16          /* not a coding of any object of the expected shape */
 };
 
 enum { LBN_OPENING_CLEAR = 0, LBN_OPENING_BIT_DROPPED = 1, LBN_OPENING_RECURSIVE = 2 };
@@ -240,6 +243,43 @@ uint64_t lbn_bd_cap(const lbn_params *params);
 int lbn_verify_opening_bit_dropped(const lbn_pp *pp, const uint16_t *top, const uint16_t *digits,
                                    const lbn_challenge *c, size_t r, const int16_t *v, size_t len_ring,
                                    const lbn_f162 *p0, size_t p0_len, const lbn_f162 *folded_row_value);
+
+/* Wire forms (wire/mod.rs, wire/rans.rs). Packers and the encoder write `out` (capacity `cap`)
+ * and set *len; with out == NULL only *len is set. Unpackers and the decoder return LBN_OK,
+ * LBN_WIRE_TRUNCATED or LBN_WIRE_MALFORMED exactly where the reference returns Ok,
+ * WireError::Truncated or WireError::Malformed. */
+/* Commitment::wire_bytes: commitment_bytes (wire/mod.rs:473-476) or bd::bytes (bd/mod.rs:70-72) */
+int lbn_commitment_wire_bytes(const lbn_params *params, size_t *len);
+/* pack_commitment (wire/mod.rs:226-243) of the prover's last commitment, packed on the device from
+ * the matrix (or, with bit-dropped parameters, from the digits, wire/mod.rs:386-397, computed
+ * there if lbn_commit_drop_bits has not) that the commit left in HBM */
+int lbn_commit_pack(lbn_prover *prover, uint8_t *out, size_t cap, size_t *len);
+/* pack_commitment (wire/mod.rs:226-243) of host components, Clear parameters */
+int lbn_pack_commitment(const lbn_params *params, const int16_t *components, uint8_t *out, size_t cap,
+                        size_t *len);
+/* unpack_commitment (wire/mod.rs:246-301), the matrix arm; components as lbn_commit returns them */
+int lbn_unpack_commitment(const lbn_params *params, const uint8_t *bytes, size_t len, int16_t *components);
+/* unpack_dropped (wire/mod.rs:399-431) */
+int lbn_unpack_dropped(const lbn_params *params, const uint8_t *bytes, size_t len, uint16_t *top,
+                       uint16_t *digits);
+/* pack_f162 / unpack_f162 (wire/mod.rs:445-470): 162 bits an element (the row evaluation,
+ * pack_row_evaluation / unpack_row_evaluation, wire/mod.rs:434-442) */
+int lbn_pack_f162(const lbn_f162 *x, size_t n, uint8_t *out, size_t cap, size_t *len);
+int lbn_unpack_f162(const uint8_t *bytes, size_t len, size_t n, lbn_f162 *x);
+/* An upper bound on the encoded size of `count` ring elements. */
+size_t lbn_wire_encode_bound(size_t count);
+/* encode (rans.rs:18-99) of the prover's last folded witness (Prover::fold's output, coefficient
+ * representation), on the device over the copy the fold left in HBM */
+int lbn_fold_encode(lbn_prover *prover, uint16_t base_q, uint8_t *out, size_t cap, size_t *len);
+/* encode (rans.rs:18-99) of a host message of `count` ring elements ([count][648] int16), on the
+ * device of the prover; ntt is the header's representation flag */
+int lbn_wire_encode_device(lbn_prover *prover, const int16_t *v, size_t count, int ntt, uint16_t base_q,
+                           uint8_t *out, size_t cap, size_t *len);
+/* decode (rans.rs:101-196), on the host: v holds cap_count ring elements; *count and *ntt are set
+ * from the header (LBN_ERR_SHAPE if count > cap_count) */
+int lbn_wire_decode(const uint8_t *bytes, size_t len, int16_t *v, size_t cap_count, uint32_t *count, int *ntt);
+/* Device time of the last encode, in ms: [0] total [1] min / max / counts [2] the rANS kernel */
+int lbn_wire_timings(const lbn_prover *prover, float *ms, size_t n);
 
 /* BLAKE3 as the transcript and the seed expansions use it: mode 0 hash, 1 keyed (key = 32 bytes),
  * 2 derive_key (key = NUL-terminated context). */

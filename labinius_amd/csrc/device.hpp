@@ -152,4 +152,37 @@ struct F162v {
     uint64_t w[3];
 };
 
+// A bit-packed wire object (wire/mod.rs:226-243, 386-397): a sequence of records, each the same
+// run of segments, each `count` fields of `bits` bits, least significant bit first with nothing
+// between them. Field i of segment l of record r is the u16 at src[off + r rec_stride + i stride]
+// (a centered int16 taken mod q when q != 0, else the value's low `bits` bits).
+constexpr int PACK_SEGS = 8;
+struct PackSeg {
+    uint32_t count, bits, q, bit0;  // bit0: the segment's first bit inside a record
+    uint64_t off, rec_stride;
+    uint32_t stride, pad;
+};
+struct PackDesc {
+    uint32_t nseg, rec_bits;
+    uint64_t total_bits;
+    PackSeg seg[PACK_SEGS];
+};
+
+// A symbol of the rANS encoder (the step of rans.rs:306-317). With f its frequency and s its
+// start, the state x < 2^32 first sheds its low 16 bits when x > fm = f 2^20 - 1 (never for
+// f = 4096), then becomes x + q (4096 - f) + s with q = x / f: floor(x * rcp) in f64, where rcp is
+// 1/f rounded up (exact for a power of two), exact for x < 2^32 since x rcp overshoots x / f by
+// less than 2^-30 and the product rounds monotonically. cs = (4096 - f) | s << 16; an
+// unoccupied symbol (f = 0, folded into the escape) has cs = 4096.
+struct alignas(16) RansSym {
+    double rcp;
+    uint32_t fm, cs;
+};
+constexpr uint32_t RANS_LANES = 64;       // rans.rs:12
+constexpr uint32_t RANS_LOW = 1u << 16;   // wire/mod.rs:60
+constexpr uint32_t RANS_LDS_SYMS = 8192;  // symbol tables up to this size are read from LDS
+constexpr uint32_t RANS_ESCAPED = 4096;   // cs of an unoccupied symbol
+// the encoder's status words: [0] min, [1] max (int), [2..3] words pushed (u64), [4..68) states
+constexpr int RANS_STAT = 4 + RANS_LANES;
+
 }  // namespace lbn

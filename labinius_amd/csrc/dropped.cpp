@@ -100,9 +100,12 @@ int bd_reserve(lbn_prover *pr, size_t n) {
     return LBN_OK;
 }
 
+}  // namespace
+
 // bd/digits.rs:121-182 on the device: the raw rows d_raw ([limb][c][648], on the device) of a
-// commitment of shape s -> every limb's inverse transform -> Garner -> top and digits (host).
-int drop_bits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw, uint16_t *top, uint16_t *digits) {
+// commitment of shape s -> every limb's inverse transform -> Garner -> top and digits in
+// pr->d_bd_out ([c * 648 + i] top, then [limb - 1][c * 648 + i] digits), on the prover's stream.
+int bd_digits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw) {
     const lbn_pp *pp = pr->pp;
     const size_t limbs = s.primes.size(), count = s.r * N;
     if (limbs > (size_t)GARNER_MAX) return fail(LBN_ERR_SHAPE, "too many moduli for the device digits");
@@ -122,6 +125,17 @@ int drop_bits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw, uint
         for (size_t j = 0; j < k; j++) gc.inv[k][j] = (uint32_t)inv_mod(s.primes[j] % s.primes[k], s.primes[k]);
     }
     HIP_TRY(launch_garner(st, pr->d_bd_cen, gc, (uint32_t)count, pr->d_bd_out, pr->d_bd_out + count));
+    return LBN_OK;
+}
+
+namespace {
+
+// bd_digits_device, then top and digits to the host
+int drop_bits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw, uint16_t *top, uint16_t *digits) {
+    int rc = bd_digits_device(pr, s, d_raw);
+    if (rc) return rc;
+    const size_t limbs = s.primes.size(), count = s.r * N;
+    hipStream_t st = pr->stream;
     HIP_TRY(hipMemcpyAsync(top, pr->d_bd_out, count * sizeof(uint16_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(digits, pr->d_bd_out + count, (limbs - 1) * count * sizeof(uint16_t),
                            hipMemcpyDeviceToHost, st));
@@ -192,6 +206,7 @@ int lbn_drop_bits_device(lbn_prover *pr, const lbn_params *params, const uint32_
     if (limbs > (size_t)GARNER_MAX) return fail(LBN_ERR_SHAPE, "too many moduli for the device digits");
     if ((rc = bd_reserve(pr, limbs * count))) return rc;
     HIP_TRY(hipMemcpyAsync(pr->d_bd_raw, raw, limbs * count * sizeof(uint32_t), hipMemcpyHostToDevice, pr->stream));
+    pr->bd_ready = false;  // d_bd_out is about to hold the digits of these rows
     return drop_bits_device(pr, s, pr->d_bd_raw, top, digits);
 }
 
@@ -201,7 +216,9 @@ int lbn_commit_drop_bits(lbn_prover *pr, uint16_t *top, uint16_t *digits) {
     int rc = check_dropped(s);
     if (rc) return rc;
     if (!pr->committed) return fail(LBN_ERR_ARG, "the prover has made no commitment");
-    return drop_bits_device(pr, s, pr->d_raw, top, digits);
+    if ((rc = drop_bits_device(pr, s, pr->d_raw, top, digits))) return rc;
+    pr->bd_ready = true;
+    return LBN_OK;
 }
 
 int lbn_pack_dropped(const lbn_params *params, const uint16_t *top, const uint16_t *digits, uint8_t *out,

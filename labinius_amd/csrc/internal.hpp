@@ -167,6 +167,26 @@ struct lbn_switch_ws {
     uint64_t *d_eq = nullptr, *d_a = nullptr, *d_p = nullptr, *d_small = nullptr;
 };
 
+// The wire encoder's device scratch (grow-only, wire.cpp).
+struct lbn_wire_ws {
+    int16_t *d_in = nullptr;  // a host message uploaded for encoding
+    size_t cap_in = 0;        // coefficients
+    uint32_t *d_staged = nullptr;  // the encoder's words, a u32 per coefficient ([group][lane])
+    uint16_t *d_escw = nullptr;    // escapes' raw-bits words, [group][lane]
+    uint32_t *d_cnt = nullptr;     // words each group pushed
+    uint64_t *d_offs = nullptr;    // words pushed before each group
+    size_t cap_groups = 0;
+    uint16_t *d_stream = nullptr;
+    size_t cap_words = 0;
+    uint32_t *d_pack = nullptr;  // a bit-packed object
+    size_t cap_pack = 0;         // u32 words
+    uint32_t *d_stat = nullptr;  // RANS_STAT words
+    unsigned long long *d_hist = nullptr;  // 65537 counts
+    lbn::RansSym *d_sym = nullptr;         // 65537 symbols
+    hipEvent_t ev[5] = {};
+    float ms[3] = {};  // the last encode: total, statistics, the rANS kernel
+};
+
 struct lbn_opening {
     lbn_prover *owner = nullptr;
     uint8_t *d_bytes_f = nullptr;  // the witness in the fold's byte layout
@@ -174,6 +194,9 @@ struct lbn_opening {
 
 namespace lbn {
 void batch_free(lbn_batch_ws *b);
+// bd/digits.rs:121-182 on the device into pr->d_bd_out (dropped.cpp)
+int bd_digits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw);
+void wire_free(lbn_wire_ws &w);
 }
 
 struct lbn_prover {
@@ -196,8 +219,11 @@ struct lbn_prover {
     uint16_t *d_bd_out = nullptr;
     size_t bd_cap = 0;       // coefficients x limbs
     bool committed = false;  // d_raw holds the raw rows of a commitment
+    bool bd_ready = false;   // d_bd_out holds the top and digits of that commitment
+    bool folded = false;     // d_v holds the folded witness of a fold
     hipEvent_t ev[12] = {};
     float timings[11] = {};
     lbn_batch_ws *batch = nullptr;
     lbn_switch_ws sw;
+    lbn_wire_ws wire;
 };

@@ -1109,6 +1109,282 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
 }
 
 // =============================================================================================
+// wire formats (wire/mod.rs, wire/rans.rs)
+// =============================================================================================
+
+// Output word w of a bit-packed object: the fields that overlap bits [32 w, 32 w + 32), found from
+// the record / segment / field that holds bit 32 w and walked forward. Bits past the object's end
+// are zero (the writer's padding of the last byte).
+__global__ __launch_bounds__(256) void wire_pack_kernel(PackDesc d, const uint16_t *__restrict__ src,
+                                                        uint32_t *__restrict__ out, uint64_t words) {
+    const uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (w >= words) return;
+    const uint64_t p0 = 32 * w, p1 = min(p0 + 32, d.total_bits);
+    uint64_t rec = p0 / d.rec_bits;
+    const uint32_t off = (uint32_t)(p0 - rec * d.rec_bits);
+    uint32_t l = 0;
+    while (l + 1 < d.nseg && off >= d.seg[l + 1].bit0) l++;
+    uint32_t idx = (off - d.seg[l].bit0) / d.seg[l].bits;
+    uint64_t fs = rec * d.rec_bits + d.seg[l].bit0 + (uint64_t)idx * d.seg[l].bits;
+    uint64_t acc = 0;
+    while (fs < p1) {
+        const PackSeg &sg = d.seg[l];
+        const uint32_t raw = src[sg.off + rec * sg.rec_stride + (uint64_t)idx * sg.stride];
+        uint32_t v;
+        if (sg.q) {
+            int r = (int)(int16_t)raw % (int)sg.q;  // rem_euclid
+            v = (uint32_t)(r < 0 ? r + (int)sg.q : r);
+        } else {
+            v = raw & ((1u << sg.bits) - 1);
+        }
+        acc |= fs >= p0 ? (uint64_t)v << (fs - p0) : (uint64_t)v >> (p0 - fs);
+        fs += sg.bits;
+        if (++idx == sg.count) {
+            idx = 0;
+            if (++l == d.nseg) {
+                l = 0;
+                rec++;
+            }
+        }
+    }
+    out[w] = (uint32_t)acc;
+}
+
+constexpr int WIRE_THREADS = 256;
+constexpr uint32_t HIST_LDS = 12288;  // histograms up to this many bins are gathered in LDS
+
+__global__ void wire_init_kernel(int *stat, unsigned long long *hist, uint32_t nhist) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = id; i < nhist; i += gridDim.x * blockDim.x) hist[i] = 0;
+    if (id == 0) {
+        stat[0] = 32767;
+        stat[1] = -32768;
+        stat[2] = 0;
+        stat[3] = 0;
+    }
+}
+
+// min and max of the message (rans.rs:36-43), 8 coefficients a 16-byte load
+__global__ __launch_bounds__(WIRE_THREADS) void wire_minmax_kernel(const int16_t *__restrict__ v, uint64_t nvec,
+                                                                   int *stat) {
+    int lo = 32767, hi = -32768;
+    for (uint64_t i = (uint64_t)blockIdx.x * WIRE_THREADS + threadIdx.x; i < nvec;
+         i += (uint64_t)gridDim.x * WIRE_THREADS) {
+        const uint4 q = ((const uint4 *)v)[i];
+        const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const int a = (int16_t)(w4[d] & 0xffff), b = (int16_t)(w4[d] >> 16);
+            lo = min(lo, min(a, b));
+            hi = max(hi, max(a, b));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    __shared__ int sl[WIRE_THREADS / 64], sh[WIRE_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) {
+        sl[threadIdx.x >> 6] = lo;
+        sh[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < WIRE_THREADS / 64; i++) {
+            lo = min(lo, sl[i]);
+            hi = max(hi, sh[i]);
+        }
+        atomicMin(stat, lo);
+        atomicMax(stat + 1, hi);
+    }
+}
+
+// counts[c - low] (rans.rs:48-53): per block in LDS when the occupied range fits, else straight
+// into the global counts
+__global__ __launch_bounds__(WIRE_THREADS) void wire_hist_kernel(const int16_t *__restrict__ v, uint64_t nvec,
+                                                                 const int *stat, unsigned long long *hist) {
+    __shared__ uint32_t h[HIST_LDS];
+    const int low = stat[0];
+    const uint32_t length = (uint32_t)(stat[1] - low + 1);
+    const bool lds = length <= HIST_LDS;
+    if (lds) {
+        for (uint32_t i = threadIdx.x; i < length; i += WIRE_THREADS) h[i] = 0;
+        __syncthreads();
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * WIRE_THREADS + threadIdx.x; i < nvec;
+         i += (uint64_t)gridDim.x * WIRE_THREADS) {
+        const uint4 q = ((const uint4 *)v)[i];
+        const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const uint32_t bin = (uint32_t)((int)(int16_t)(w4[d >> 1] >> (16 * (d & 1))) - low);
+            if (lds)
+                atomicAdd(&h[bin], 1u);
+            else
+                atomicAdd(&hist[bin], 1ull);
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < length; i += WIRE_THREADS)
+            if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
+    }
+}
+
+// The interleaved rANS encoder (rans.rs:67-98) as one wavefront: lane k is rANS lane k and keeps
+// its state in a register. The groups run backwards, and all that stays on the wave's serial path
+// is each lane's state step: what a group emits is staged at a fixed place, for the scan and the
+// scatter below to put in stream order. Lane k's staged word of group g carries the low 16 bits
+// the lane may shed for its symbol (bit 16: it did); bit 17 flags an escape's raw-bits word, kept
+// in escw. Groups go in blocks of 32, each loaded a block ahead; a block with no escaped symbol
+// (the honest case) takes the short step.
+template <bool B>
+struct step_flag {
+    static constexpr bool value = B;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(64) void rans_encode_kernel(const int16_t *__restrict__ v, uint64_t coefs,
+                                                         const RansSym *__restrict__ sym, uint32_t nsym, int low,
+                                                         uint32_t raw, uint32_t *__restrict__ staged,
+                                                         uint16_t *__restrict__ escw, uint32_t *__restrict__ stat) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    const uint32_t k = threadIdx.x;
+    const RansSym *T = sym;
+    if (LDS) {
+        RansSym *ls = (RansSym *)smem;
+        for (uint32_t i = k; i < nsym; i += 64) ls[i] = sym[i];
+        __syncthreads();
+        T = ls;
+    }
+    const RansSym esc = T[nsym - 1];
+    constexpr int U = 32;
+    const uint64_t G = (coefs + 63) / 64, B = (G + U - 1) / U;
+    int cur[U], nxt[U];  // a register each (int16 pairs packed in one would wait for both loads)
+    // block b's coefficients; every block below the last one is whole (the last holds group G - 1)
+    auto load = [&](uint64_t b, int *dst, bool last) {
+        const int16_t *p = v + b * U * 64 + k;
+#pragma unroll
+        for (int u = 0; u < U; u++) dst[u] = last ? v[min((b * U + u) * 64 + k, coefs - 1)] : p[u * 64];
+    };
+    uint32_t x = RANS_LOW;
+    // one group's step; ESC: some lane may hold an escaped symbol; PART: some lanes may be past
+    // the message's end
+    auto step = [&](auto esc_t, auto part_t, uint64_t g, int c, RansSym e) {
+        constexpr bool ESC = decltype(esc_t)::value, PART = decltype(part_t)::value;
+        const bool act = !PART || g * 64 + k < coefs;
+        bool n1 = false;
+        if (ESC) {
+            if (act && (e.cs & 0xffff) == RANS_ESCAPED) {  // put_bits (rans.rs:320-328)
+                n1 = (x >> (32 - raw)) != 0;
+                escw[g * 64 + k] = (uint16_t)x;
+                if (n1) x >>= 16;
+                x = (x << raw) | (uint32_t)(c - low);
+                e = esc;
+            }
+        }
+        const bool n2 = act && x > e.fm;  // put_symbol (rans.rs:306-317)
+        const uint32_t w = (x & 0xffff) | (n2 ? 1u << 16 : 0u) | (n1 ? 1u << 17 : 0u);
+        const uint32_t xr = n2 ? x >> 16 : x;
+        const uint32_t q = (uint32_t)((double)xr * e.rcp);
+        const uint32_t xn = __umul24(q, e.cs & 0xffff) + xr + (e.cs >> 16);
+        x = act ? xn : x;
+        if (!PART || g < G) staged[g * 64 + k] = w;
+    };
+    using yes = step_flag<true>;
+    using no = step_flag<false>;
+    load(B - 1, nxt, true);
+    for (uint64_t b = B; b-- > 0;) {
+#pragma unroll
+        for (int u = 0; u < U; u++) cur[u] = nxt[u];
+        if (b > 0) load(b - 1, nxt, false);
+        // branch-free lookups (a lane past the end holds the last coefficient, a valid symbol)
+        RansSym sy[U];
+        bool escaped = false;
+#pragma unroll
+        for (int u = 0; u < U; u++) sy[u] = T[(uint32_t)(cur[u] - low)];
+#pragma unroll
+        for (int u = 0; u < U; u++) escaped |= (sy[u].cs & 0xffff) == RANS_ESCAPED;
+        if (b == B - 1) {  // the last groups: lanes past the end, and groups past it
+#pragma unroll
+            for (int u = U - 1; u >= 0; u--) step(yes{}, yes{}, b * U + u, cur[u], sy[u]);
+        } else if (__builtin_amdgcn_ballot_w64(escaped)) {
+#pragma unroll
+            for (int u = U - 1; u >= 0; u--) step(yes{}, no{}, b * U + u, cur[u], sy[u]);
+        } else {
+#pragma unroll
+            for (int u = U - 1; u >= 0; u--) step(no{}, no{}, b * U + u, cur[u], sy[u]);
+        }
+    }
+    stat[4 + k] = x;
+}
+
+// The words each group pushed: one wave a group.
+__global__ __launch_bounds__(256) void rans_count_kernel(const uint32_t *__restrict__ staged, uint64_t G,
+                                                         uint32_t *__restrict__ cnt) {
+    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= G) return;
+    const uint32_t w = staged[g * 64 + (threadIdx.x & 63)];
+    const uint32_t c = __popcll(__builtin_amdgcn_ballot_w64((w >> 16) & 1)) +
+                       __popcll(__builtin_amdgcn_ballot_w64((w >> 17) & 1));
+    if ((threadIdx.x & 63) == 0) cnt[g] = c;
+}
+
+// offs[g] = the words pushed before group g, i.e. by the groups after it (they run first), and
+// the total (stat[2..3]); one block: each thread sums a run of groups in push order, the block
+// scans the sums, and each thread writes its run's offsets.
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void rans_scan_kernel(const uint32_t *__restrict__ cnt, uint64_t G,
+                                                                 uint64_t *__restrict__ offs,
+                                                                 uint32_t *__restrict__ stat) {
+    __shared__ uint64_t part[SCAN_THREADS];
+    const uint32_t t = threadIdx.x;
+    const uint64_t span = (G + SCAN_THREADS - 1) / SCAN_THREADS, r0 = min((uint64_t)t * span, G),
+                   r1 = min(r0 + span, G);
+    uint64_t sum = 0;
+    for (uint64_t r = r0; r < r1; r++) sum += cnt[G - 1 - r];  // r counts groups in push order
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < SCAN_THREADS; d <<= 1) {  // inclusive scan (Hillis-Steele)
+        const uint64_t add = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint64_t run = part[t] - sum;
+    for (uint64_t r = r0; r < r1; r++) {
+        const uint64_t g = G - 1 - r;
+        offs[g] = run;
+        run += cnt[g];
+    }
+    if (t == SCAN_THREADS - 1) {
+        stat[2] = (uint32_t)part[t];
+        stat[3] = (uint32_t)(part[t] >> 32);
+    }
+}
+
+// The stream (rans.rs:93-97): the words in push order reversed. A wave takes one group: the words
+// of lanes above k were pushed before lane k's, an escape's raw-bits word before its symbol's.
+__global__ __launch_bounds__(256) void rans_scatter_kernel(const uint32_t *__restrict__ staged,
+                                                           const uint16_t *__restrict__ escw,
+                                                           const uint64_t *__restrict__ offs,
+                                                           const uint32_t *__restrict__ stat, uint64_t G,
+                                                           uint16_t *__restrict__ stream) {
+    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= G) return;
+    const uint32_t k = threadIdx.x & 63;
+    const uint32_t w = staged[g * 64 + k];
+    const bool n2 = (w >> 16) & 1, n1 = (w >> 17) & 1;
+    const uint64_t m1 = __builtin_amdgcn_ballot_w64(n1), m2 = __builtin_amdgcn_ballot_w64(n2);
+    const uint64_t above = k == 63 ? 0ull : ~0ull << (k + 1);
+    const uint64_t top = (uint64_t)stat[2] | (uint64_t)stat[3] << 32;
+    const uint64_t p = offs[g] + __popcll(m1 & above) + __popcll(m2 & above);
+    if (n1) stream[top - 1 - p] = escw[g * 64 + k];
+    if (n2) stream[top - 1 - (p + (n1 ? 1 : 0))] = (uint16_t)w;
+}
+
+// =============================================================================================
 // launchers
 // =============================================================================================
 
@@ -1255,6 +1531,51 @@ hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq,
     const uint32_t gy_want = std::max(1u, std::min(nch, (512 + gx - 1) / gx));
     const uint32_t span = (nch + gy_want - 1) / gy_want, gy = (nch + span - 1) / span;
     LBN_LAUNCH(row_eval_kernel, dim3(gx, gy), dim3(64 * RE_WAVES), 0, s, w, eq, wdim, cols, span, u);
+    return hipSuccess;
+}
+
+hipError_t launch_wire_pack(hipStream_t s, const PackDesc &d, const uint16_t *src, uint32_t *out) {
+    const uint64_t words = (d.total_bits + 31) / 32;
+    if (words == 0) return hipSuccess;
+    LBN_LAUNCH(wire_pack_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, s, d, src, out, words);
+    return hipSuccess;
+}
+
+hipError_t launch_wire_stats(hipStream_t s, const int16_t *v, uint64_t coefs, int *stat, unsigned long long *hist,
+                             uint32_t nhist) {
+    // coefs is a multiple of 648, so of 8: whole 16-byte vectors
+    const uint64_t nvec = coefs / 8;
+    LBN_LAUNCH(wire_init_kernel, dim3(64), dim3(256), 0, s, stat, hist, nhist);
+    if (nvec == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(512, (nvec + 4 * WIRE_THREADS - 1) / (4 * WIRE_THREADS));
+    LBN_LAUNCH(wire_minmax_kernel, dim3(blocks), dim3(WIRE_THREADS), 0, s, v, nvec, stat);
+    LBN_LAUNCH(wire_hist_kernel, dim3(blocks), dim3(WIRE_THREADS), 0, s, v, nvec, (const int *)stat, hist);
+    return hipSuccess;
+}
+
+hipError_t launch_rans_encode(hipStream_t s, const int16_t *v, uint64_t coefs, const RansSym *sym, uint32_t nsym,
+                              int low, uint32_t raw, uint32_t *staged, uint16_t *escw, uint32_t *stat) {
+    if (nsym <= RANS_LDS_SYMS) {
+        static const hipError_t attr = hipFuncSetAttribute((const void *)rans_encode_kernel<true>,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           RANS_LDS_SYMS * sizeof(RansSym));
+        if (attr != hipSuccess) return attr;
+        LBN_LAUNCH(rans_encode_kernel<true>, dim3(1), dim3(64), nsym * sizeof(RansSym), s, v, coefs, sym, nsym, low,
+                   raw, staged, escw, stat);
+    } else {
+        LBN_LAUNCH(rans_encode_kernel<false>, dim3(1), dim3(64), 0, s, v, coefs, sym, nsym, low, raw, staged, escw,
+                   stat);
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_rans_scatter(hipStream_t s, const uint32_t *staged, const uint16_t *escw, uint32_t *cnt,
+                               uint64_t *offs, uint32_t *stat, uint64_t groups, uint16_t *stream) {
+    if (groups == 0) return hipSuccess;
+    const dim3 grid((uint32_t)((groups + 3) / 4));
+    LBN_LAUNCH(rans_count_kernel, grid, dim3(256), 0, s, staged, groups, cnt);
+    LBN_LAUNCH(rans_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, cnt, groups, offs, stat);
+    LBN_LAUNCH(rans_scatter_kernel, grid, dim3(256), 0, s, staged, escw, offs, stat, groups, stream);
     return hipSuccess;
 }
 

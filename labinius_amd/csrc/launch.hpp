@@ -48,5 +48,16 @@ hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pi
                                   const uint64_t r[3], int top, uint64_t *out);
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u);
+// wire formats: a bit-packed object into ceil(total_bits / 32) words; the encoder's statistics
+// (stat: min, max; hist: nhist u64 counts, cleared first); the one-wave rANS encoder, which stages
+// each group's words at fixed places; their scatter into the stream
+hipError_t launch_wire_pack(hipStream_t s, const PackDesc &d, const uint16_t *src, uint32_t *out);
+hipError_t launch_wire_stats(hipStream_t s, const int16_t *v, uint64_t coefs, int *stat, unsigned long long *hist,
+                             uint32_t nhist);
+hipError_t launch_rans_encode(hipStream_t s, const int16_t *v, uint64_t coefs, const RansSym *sym, uint32_t nsym,
+                              int low, uint32_t raw, uint32_t *staged, uint16_t *escw, uint32_t *stat);
+// the staged words' counts (cnt, a u32 a group), their scan (offs) and the stream
+hipError_t launch_rans_scatter(hipStream_t s, const uint32_t *staged, const uint16_t *escw, uint32_t *cnt,
+                               uint64_t *offs, uint32_t *stat, uint64_t groups, uint16_t *stream);
 
 }  // namespace lbn

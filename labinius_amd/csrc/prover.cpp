@@ -349,6 +349,7 @@ static void prover_free(lbn_prover *pr) {
         if (b) (void)hipFree(b);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
     batch_free(pr->batch);
+    wire_free(pr->wire);
     for (hipEvent_t &e : pr->ev)
         if (e) (void)hipEventDestroy(e);
     if (pr->stream) (void)hipStreamDestroy(pr->stream);
@@ -410,6 +411,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     if (raw) HIP_TRY(hipMemcpyAsync(raw, pr->d_raw, limbs * s.r * N * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     pr->committed = true;
+    pr->bd_ready = false;
     pr->timings[0] = elapsed(pr->ev[0], pr->ev[3]);
     pr->timings[1] = elapsed(pr->ev[0], pr->ev[1]);
     pr->timings[2] = elapsed(pr->ev[1], pr->ev[2]);
@@ -453,6 +455,7 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     HIP_TRY(hipEventRecord(pr->ev[7], st));
     if (v) HIP_TRY(hipMemcpyAsync(v, pr->d_v, s.len_ring * N * sizeof(int16_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    pr->folded = true;
     pr->timings[4] = elapsed(pr->ev[4], pr->ev[7]);
     pr->timings[5] = elapsed(pr->ev[4], pr->ev[5]);
     pr->timings[6] = elapsed(pr->ev[5], pr->ev[6]);
@@ -601,6 +604,13 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
         lbn_transcript_free(t);
         if (!rc) rc = lbn_fold(pr, op, ch.data(), ch.size(), nullptr);
         else if (op) lbn_recycle(pr, op);
+        if (!rc) {  // the wire encoder's kernels (a fold of zeros)
+            size_t len = 0;
+            rc = lbn_fold_encode(pr, (uint16_t)s.primes[0], nullptr, 0, &len);
+        }
+        pr->committed = false;
+        pr->bd_ready = false;
+        pr->folded = false;
         lbn_dwitness_release(dw);
         if (rc) {
             prover_free(pr);

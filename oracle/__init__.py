@@ -65,6 +65,11 @@ def lib():
         _lib.orc_bd_cap.restype = ctypes.c_uint64
         _lib.orc_pack_dropped.restype = ctypes.c_size_t
         _lib.orc_verify_opening_bd.restype = ctypes.c_int
+        for name in ("orc_commitment_bytes", "orc_pack_commitment", "orc_pack_f162", "orc_rans_bound",
+                     "orc_rans_encode"):
+            getattr(_lib, name).restype = ctypes.c_size_t
+        _lib.orc_rans_count.restype = ctypes.c_uint32
+        _lib.orc_entropy_bytes.restype = ctypes.c_double
     return _lib
 
 
@@ -517,3 +522,99 @@ def switch_verify(l, cl, r_lo, r_hi, claim, partials, rounds, opened, transcript
     ok = lib().orc_switch_verify(ctypes.c_int(l), ctypes.c_int(cl), _p(r_lo), ctypes.c_int(len(r_lo)), _p(r_hi),
                                  _p(claim), _p(partials), _p(rounds), _p(opened), transcript.buf, _p(p0), _p(p1))
     return bool(ok), p0, p1
+
+
+# =============================================================================================
+# wire formats (wire/mod.rs, wire/rans.rs)
+# =============================================================================================
+
+WIRE_OK, WIRE_TRUNCATED, WIRE_MALFORMED = 0, 1, 2
+LANES = 64
+
+
+def commitment_bytes(primes, cols):
+    pr = np.array(primes, np.uint32)
+    return lib().orc_commitment_bytes(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols))
+
+
+def pack_commitment(primes, cols, comp):
+    """wire/mod.rs:226-243: comp [c][4][limb][162] int16 centered -> bytes"""
+    pr = np.array(primes, np.uint32)
+    comp = np.ascontiguousarray(comp, np.int16)
+    assert comp.size == cols * 4 * len(primes) * N162
+    out = np.zeros(commitment_bytes(primes, cols) + 8, np.uint8)
+    n = lib().orc_pack_commitment(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols), _p(comp), _p(out))
+    return bytes(out[:n])
+
+
+def unpack_commitment(primes, cols, data):
+    """wire/mod.rs:246-301: (error, comp [c][4][limb][162])"""
+    pr = np.array(primes, np.uint32)
+    buf = np.frombuffer(bytes(data) + b"\0", np.uint8)
+    comp = np.zeros((cols, 4, len(primes), N162), np.int16)
+    err = lib().orc_unpack_commitment(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols), _p(buf),
+                                      ctypes.c_size_t(len(data)), _p(comp))
+    return err, comp
+
+
+def unpack_dropped(primes, cols, d, data):
+    """wire/mod.rs:399-431: (error, top, digits)"""
+    pr = np.array(primes, np.uint32)
+    buf = np.frombuffer(bytes(data) + b"\0", np.uint8)
+    top = np.zeros(cols * N, np.uint16)
+    digits = np.zeros((len(primes) - 1, cols * N), np.uint16)
+    err = lib().orc_unpack_dropped(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(cols), ctypes.c_uint32(d),
+                                   _p(buf), ctypes.c_size_t(len(data)), _p(top), _p(digits))
+    return err, top, digits
+
+
+def pack_f162(values):
+    """wire/mod.rs:445-453"""
+    v = np.ascontiguousarray(np.asarray(values, np.uint64).reshape(-1, 3))
+    out = np.zeros((len(v) * 162 + 7) // 8 + 8, np.uint8)
+    n = lib().orc_pack_f162(_p(v), ctypes.c_size_t(len(v)), _p(out))
+    return bytes(out[:n])
+
+
+def unpack_f162(data, count):
+    """wire/mod.rs:456-470: (error, values [count][3])"""
+    buf = np.frombuffer(bytes(data) + b"\0", np.uint8)
+    v = np.zeros((count, 3), np.uint64)
+    err = lib().orc_unpack_f162(_p(buf), ctypes.c_size_t(len(data)), ctypes.c_size_t(count), _p(v))
+    return err, v
+
+
+def rans_quantize(counts):
+    """rans.rs:236-271: the last entry is the escape symbol"""
+    c = np.ascontiguousarray(counts, np.uint64)
+    f = np.zeros(len(c), np.uint32)
+    lib().orc_rans_quantize(_p(c), ctypes.c_size_t(len(c)), _p(f))
+    return f
+
+
+def rans_encode(v, base_q, ntt=False):
+    """rans.rs:18-99: v [count][648] int16 -> bytes"""
+    v = np.ascontiguousarray(np.asarray(v, np.int16).reshape(-1, N))
+    out = np.zeros(lib().orc_rans_bound(ctypes.c_size_t(len(v))), np.uint8)
+    n = lib().orc_rans_encode(_p(v), ctypes.c_size_t(len(v)), ctypes.c_int(int(ntt)), ctypes.c_uint16(base_q),
+                              _p(out))
+    return bytes(out[:n])
+
+
+def rans_decode(data):
+    """rans.rs:101-196: (error, v [count][648] int16, ntt)"""
+    data = bytes(data)
+    buf = np.frombuffer(data + b"\0", np.uint8)
+    count = lib().orc_rans_count(_p(buf), ctypes.c_size_t(len(data)))
+    cap = min(count, 1 << 16)
+    v = np.zeros((max(cap, 1), N), np.int16)
+    cnt, ntt = ctypes.c_uint32(), ctypes.c_int()
+    err = lib().orc_rans_decode(_p(buf), ctypes.c_size_t(len(data)), _p(v), ctypes.c_size_t(cap), ctypes.byref(cnt),
+                                ctypes.byref(ntt))
+    return err, v[:cnt.value] if err == WIRE_OK else None, bool(ntt.value)
+
+
+def entropy_bytes(v):
+    """rans.rs:500-524"""
+    v = np.ascontiguousarray(v, np.int16).ravel()
+    return lib().orc_entropy_bytes(_p(v), ctypes.c_size_t(v.size))

@@ -28,7 +28,13 @@
  *     empty input) plus agreement with the product's independently structured implementation at
  *     every input length around the chunk and tree boundaries;
  *   - A expansion, Witness::random, the transcript framing and the challenge sampler: parity
- *     unpinned (no reference execution is possible in this container); restated byte by byte.
+ *     unpinned (no reference execution is possible in this container); restated byte by byte;
+ *   - wire formats: the reference's own wire tests (tests/wire.rs) restated case for case (exact
+ *     sizes, the one-word-per-lane stream of 16 + 5 + 4 * 64 + 2 bytes, round trips on honest and
+ *     adversarial messages, which error each broken stream gets), known answers worked from the
+ *     format's description (bit order, a hand-computed largest-remainder quantisation), and a
+ *     decoder written from the rANS definition in the tests (slot search, no tables). The rANS
+ *     step here uses the exact quotient x / f, the device the reference's reciprocal form.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1938,4 +1944,441 @@ int orc_switch_verify(int l, int cl, const uint64_t *r_lo, int r_lo_len, const u
     if (ok) msb_first(r_pp, l, cl, p0, p1);
     free(r_pp);
     return ok;
+}
+
+/* ============================================================================================
+ * wire formats (wire/mod.rs, wire/rans.rs): the bit-packed commitment and row evaluation, the
+ * rANS-coded folded witness. Errors: 0 ok, 1 Truncated, 2 Malformed (wire/mod.rs:67-74).
+ * ========================================================================================= */
+
+enum { W_OK = 0, W_TRUNCATED = 1, W_MALFORMED = 2, W_CAPACITY = 3 };
+#define W_SCALE 4096u      /* wire/mod.rs:57-58 */
+#define W_RANS_LOW 65536u  /* wire/mod.rs:60 */
+#define W_HEADER 16
+#define W_LANES 64         /* rans.rs:12 */
+#define W_MAX_ELEMENTS (1u << 24)
+
+/* The writer of wire/mod.rs:86-138: fields least significant bit first, the last byte padded. */
+typedef struct {
+    uint8_t *out;
+    size_t len;
+    uint64_t acc;
+    uint32_t n;
+} bitw;
+
+static void bw_put(bitw *w, uint64_t value, uint32_t bits) {
+    w->acc |= (value & ((1ull << bits) - 1)) << w->n;
+    w->n += bits;
+    while (w->n >= 8) {
+        w->out[w->len++] = (uint8_t)w->acc;
+        w->acc >>= 8;
+        w->n -= 8;
+    }
+}
+
+static void bw_put_wide(bitw *w, uint64_t value, uint32_t bits) {
+    if (bits <= 32) {
+        bw_put(w, value, bits);
+    } else {
+        bw_put(w, value, 32);
+        bw_put(w, value >> 32, bits - 32);
+    }
+}
+
+/* Elias gamma of v >= 1: k zeros, the leading one, the k bits below it (wire/mod.rs:122-129) */
+static void bw_gamma(bitw *w, uint64_t v) {
+    uint32_t k = 63 - (uint32_t)__builtin_clzll(v);
+    bw_put(w, 0, k);
+    bw_put(w, 1, 1);
+    bw_put_wide(w, v & ((1ull << k) - 1), k);
+}
+
+static size_t bw_finish(bitw *w) {
+    if (w->n > 0) w->out[w->len++] = (uint8_t)w->acc;
+    return w->len;
+}
+
+/* The reader of wire/mod.rs:141-218 as a bit cursor: a read of `bits` fails (Truncated) exactly
+ * when fewer remain, which is when the reference's refilled accumulator comes up short; bytes
+ * consumed = ceil(bits read / 8) (its pos - n / 8). */
+typedef struct {
+    const uint8_t *b;
+    size_t len;
+    uint64_t bit;
+} bitr;
+
+static int br_get(bitr *r, uint32_t bits, uint64_t *v) {
+    if (r->bit + bits > 8 * (uint64_t)r->len) return W_TRUNCATED;
+    uint64_t x = 0;
+    for (uint32_t i = 0; i < bits; i++, r->bit++) x |= (uint64_t)((r->b[r->bit >> 3] >> (r->bit & 7)) & 1) << i;
+    *v = x;
+    return W_OK;
+}
+
+static int br_gamma(bitr *r, uint64_t *v) {
+    uint32_t k = 0;
+    uint64_t b;
+    for (;;) {
+        int e = br_get(r, 1, &b);
+        if (e) return e;
+        if (b) break;
+        if (++k > 32) return W_MALFORMED;
+    }
+    uint64_t low;
+    int e = br_get(r, k, &low);
+    if (e) return e;
+    *v = (1ull << k) | low;
+    return W_OK;
+}
+
+static size_t br_consumed(const bitr *r) { return (size_t)((r->bit + 7) / 8); }
+
+/* wire/mod.rs:220-225 (commitment_bytes) */
+size_t orc_commitment_bytes(const uint32_t *primes, int limbs, int cols) {
+    uint64_t bits = 0;
+    for (int k = 0; k < limbs; k++) bits += orc_residue_bits(primes[k]);
+    return (size_t)((4ull * cols * N162 * bits + 7) / 8);
+}
+
+/* wire/mod.rs:226-243: comp [c][t][k][s] centered; the entries in VerticallyAlignedMatrix::iter
+ * order (column by column, rows 0..3), each limb's 162 slots as residues in [0, q) */
+size_t orc_pack_commitment(const uint32_t *primes, int limbs, int cols, const int16_t *comp, uint8_t *out) {
+    bitw w = {out, 0, 0, 0};
+    for (size_t e = 0; e < 4 * (size_t)cols; e++)
+        for (int k = 0; k < limbs; k++) {
+            const int32_t q = (int32_t)primes[k];
+            const uint32_t bits = orc_residue_bits(primes[k]);
+            for (int s = 0; s < N162; s++) {
+                int32_t v = comp[(e * limbs + k) * N162 + s] % q;
+                if (v < 0) v += q;
+                bw_put(&w, (uint64_t)v, bits);
+            }
+        }
+    return bw_finish(&w);
+}
+
+/* wire/mod.rs:246-301 (the Clear arm) */
+int orc_unpack_commitment(const uint32_t *primes, int limbs, int cols, const uint8_t *bytes, size_t len,
+                          int16_t *comp) {
+    if (len != orc_commitment_bytes(primes, limbs, cols)) return W_MALFORMED;
+    bitr r = {bytes, len, 0};
+    for (size_t e = 0; e < 4 * (size_t)cols; e++)
+        for (int k = 0; k < limbs; k++) {
+            const uint32_t q = primes[k], bits = orc_residue_bits(q), half = (q - 1) / 2;
+            for (int s = 0; s < N162; s++) {
+                uint64_t v;
+                int err = br_get(&r, bits, &v);
+                if (err) return err;
+                if (v >= q) return W_MALFORMED;
+                comp[(e * limbs + k) * N162 + s] = (int16_t)(v > half ? (int32_t)v - (int32_t)q : (int32_t)v);
+            }
+        }
+    return W_OK;
+}
+
+/* wire/mod.rs:399-431 */
+int orc_unpack_dropped(const uint32_t *primes, int limbs, int cols, uint32_t d, const uint8_t *bytes, size_t len,
+                       uint16_t *top, uint16_t *digits) {
+    const size_t count = (size_t)cols * N;
+    uint64_t per = orc_top_bits(primes[0], d);
+    for (int k = 1; k < limbs; k++) per += orc_residue_bits(primes[k]);
+    if (d == 0 || len != (count * per + 7) / 8) return W_MALFORMED;
+    const uint32_t tb = orc_top_bits(primes[0], d), bound = ((primes[0] - 1) + (1u << (d - 1))) >> d;
+    bitr r = {bytes, len, 0};
+    for (size_t i = 0; i < count; i++) {
+        uint64_t v;
+        int err = br_get(&r, tb, &v);
+        if (err) return err;
+        if (v > bound) return W_MALFORMED;
+        top[i] = (uint16_t)v;
+        for (int k = 1; k < limbs; k++) {
+            if ((err = br_get(&r, orc_residue_bits(primes[k]), &v))) return err;
+            if (v >= primes[k]) return W_MALFORMED;
+            digits[(size_t)(k - 1) * count + i] = (uint16_t)v;
+        }
+    }
+    return W_OK;
+}
+
+/* wire/mod.rs:445-453: 64 + 64 + 34 bits an element */
+size_t orc_pack_f162(const uint64_t *x, size_t n, uint8_t *out) {
+    bitw w = {out, 0, 0, 0};
+    for (size_t i = 0; i < n; i++) {
+        bw_put_wide(&w, x[3 * i], 64);
+        bw_put_wide(&w, x[3 * i + 1], 64);
+        bw_put_wide(&w, x[3 * i + 2], 34);
+    }
+    return bw_finish(&w);
+}
+
+/* wire/mod.rs:456-470 */
+int orc_unpack_f162(const uint8_t *bytes, size_t len, size_t n, uint64_t *x) {
+    if (len != (n * 162 + 7) / 8) return W_MALFORMED;
+    bitr r = {bytes, len, 0};
+    for (size_t i = 0; i < n; i++) {
+        int err;
+        if ((err = br_get(&r, 64, &x[3 * i])) || (err = br_get(&r, 64, &x[3 * i + 1])) ||
+            (err = br_get(&r, 34, &x[3 * i + 2])))
+            return err;
+    }
+    return W_OK;
+}
+
+/* rans.rs:220-223 */
+static uint32_t raw_bits(uint32_t length) { return length <= 1 ? 0 : 32 - (uint32_t)__builtin_clz(length - 1); }
+
+typedef struct {
+    uint64_t key;
+    size_t i;
+} ranked;
+
+/* larger key first, then the smaller index */
+static int ranked_cmp(const void *a, const void *b) {
+    const ranked *x = (const ranked *)a, *y = (const ranked *)b;
+    if (x->key != y->key) return x->key > y->key ? -1 : 1;
+    return x->i < y->i ? -1 : (x->i > y->i);
+}
+
+/* rans.rs:236-271: counts[0 .. n - 1) the symbols, counts[n - 1] the escape -> freq summing to
+ * 4096 (or all zero for an empty message) */
+void orc_rans_quantize(const uint64_t *counts, size_t n, uint32_t *freq) {
+    const size_t esc = n - 1;
+    uint64_t *w = (uint64_t *)malloc(n * sizeof(uint64_t));
+    ranked *rk = (ranked *)malloc(n * sizeof(ranked));
+    memcpy(w, counts, n * sizeof(uint64_t));
+    size_t occ = 0;
+    for (size_t i = 0; i < esc; i++)
+        if (w[i]) rk[occ++] = (ranked){w[i], i};
+    if (occ > W_SCALE - 1) {
+        qsort(rk, occ, sizeof(ranked), ranked_cmp);
+        for (size_t j = W_SCALE - 1; j < occ; j++) {
+            w[esc] += w[rk[j].i];
+            w[rk[j].i] = 0;
+        }
+    }
+    uint64_t total = 0;
+    occ = 0;
+    for (size_t i = 0; i < n; i++) {
+        total += w[i];
+        freq[i] = 0;
+        if (w[i]) occ++;
+    }
+    if (occ) {
+        const uint64_t spare = W_SCALE - occ;
+        uint64_t used = 0;
+        size_t m = 0;
+        for (size_t i = 0; i < n; i++) {
+            if (!w[i]) continue;
+            const uint64_t share = w[i] * spare / total;
+            freq[i] = 1 + (uint32_t)share;
+            used += share;
+            rk[m++] = (ranked){w[i] * spare - share * total, i};
+        }
+        qsort(rk, m, sizeof(ranked), ranked_cmp);
+        for (uint64_t j = 0; j < spare - used; j++) freq[rk[j].i]++;
+    }
+    free(rk);
+    free(w);
+}
+
+/* A bound on orc_rans_encode's output for `count` ring elements. */
+size_t orc_rans_bound(size_t count) { return W_HEADER + (65537 * 25 + 7) / 8 + 4 * W_LANES + 4 * count * N + 2; }
+
+/* The rANS step of rans.rs:306-333 written with the exact quotient: x -> (x / f) 2^12 + x mod f +
+ * start, after emitting the low word when x >= f 2^20. (The reference's reciprocal and bias
+ * compute the same quotient.) */
+static void put_symbol(uint32_t *x, uint16_t *words, size_t *top, uint32_t f, uint32_t start) {
+    if ((*x >> 20) >= f) {
+        words[(*top)++] = (uint16_t)*x;
+        *x >>= 16;
+    }
+    *x = ((*x / f) << 12) + *x % f + start;
+}
+
+static void put_bits(uint32_t *x, uint16_t *words, size_t *top, uint32_t value, uint32_t bits) {
+    if (!bits) return;
+    if ((*x >> (32 - bits)) != 0) {
+        words[(*top)++] = (uint16_t)*x;
+        *x >>= 16;
+    }
+    *x = (*x << bits) | value;
+}
+
+static void le32(uint8_t *p, uint32_t v) {
+    for (int i = 0; i < 4; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+static uint32_t rd32(const uint8_t *p) { return p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24; }
+
+/* rans.rs:18-99: v is `count` ring elements of 648 i16, flattened */
+size_t orc_rans_encode(const int16_t *v, size_t count, int ntt, uint16_t base_q, uint8_t *out) {
+    const size_t coefs = count * N;
+    out[0] = 1;
+    out[1] = ntt ? 1 : 0;
+    out[2] = (uint8_t)base_q;
+    out[3] = (uint8_t)(base_q >> 8);
+    le32(out + 4, (uint32_t)count);
+    if (coefs == 0) {
+        le32(out + 8, 0);
+        le32(out + 12, 0);
+        return W_HEADER;
+    }
+    int32_t low = 32767, high = -32768;
+    for (size_t i = 0; i < coefs; i++) {
+        if (v[i] < low) low = v[i];
+        if (v[i] > high) high = v[i];
+    }
+    const uint32_t length = (uint32_t)(high - low + 1);
+    le32(out + 8, (uint32_t)low);
+    le32(out + 12, length);
+    uint64_t *counts = (uint64_t *)calloc(length + 1, sizeof(uint64_t));
+    uint32_t *freq = (uint32_t *)malloc((length + 1) * sizeof(uint32_t));
+    uint32_t *start = (uint32_t *)malloc((length + 1) * sizeof(uint32_t));
+    for (size_t i = 0; i < coefs; i++) counts[v[i] - low]++;
+    orc_rans_quantize(counts, length + 1, freq);
+    bitw w = {out + W_HEADER, 0, 0, 0};
+    for (uint32_t s = 0; s <= length; s++) bw_gamma(&w, (uint64_t)freq[s] + 1);
+    size_t len = W_HEADER + bw_finish(&w);
+    uint32_t acc = 0;
+    uint64_t escaped = 0;
+    for (uint32_t s = 0; s <= length; s++) {
+        start[s] = acc;
+        acc += freq[s];
+        if (s < length && !freq[s]) escaped += counts[s];
+    }
+    const uint32_t raw = raw_bits(length);
+    uint32_t x[W_LANES];
+    for (int k = 0; k < W_LANES; k++) x[k] = W_RANS_LOW;
+    uint16_t *words = (uint16_t *)malloc((coefs + escaped + 1) * sizeof(uint16_t));
+    size_t top = 0;
+    for (size_t g = (coefs + W_LANES - 1) / W_LANES; g-- > 0;) {
+        const size_t lanes = coefs - g * W_LANES < W_LANES ? coefs - g * W_LANES : W_LANES;
+        for (size_t k = lanes; k-- > 0;) {
+            const uint32_t s = (uint32_t)(v[g * W_LANES + k] - low);
+            if (!freq[s]) {
+                put_bits(&x[k], words, &top, s, raw);
+                put_symbol(&x[k], words, &top, freq[length], start[length]);
+            } else {
+                put_symbol(&x[k], words, &top, freq[s], start[s]);
+            }
+        }
+    }
+    for (int k = 0; k < W_LANES; k++, len += 4) le32(out + len, x[k]);
+    for (size_t i = 0; i < top; i++, len += 2) {
+        out[len] = (uint8_t)words[top - 1 - i];
+        out[len + 1] = (uint8_t)(words[top - 1 - i] >> 8);
+    }
+    free(words);
+    free(start);
+    free(freq);
+    free(counts);
+    return len;
+}
+
+/* The element count a coded message announces (0 for a header too short to hold one). */
+uint32_t orc_rans_count(const uint8_t *bytes, size_t len) { return len >= W_HEADER ? rd32(bytes + 4) : 0; }
+
+/* rans.rs:101-196 with the scalar Decoder of 336-388; v holds cap ring elements */
+int orc_rans_decode(const uint8_t *bytes, size_t len, int16_t *v, size_t cap, uint32_t *count_out, int *ntt) {
+    if (len < W_HEADER) return W_TRUNCATED;
+    if (bytes[0] != 1 || bytes[1] > 1) return W_MALFORMED;
+    const uint32_t count = rd32(bytes + 4), length = rd32(bytes + 12);
+    const int32_t low = (int32_t)rd32(bytes + 8);
+    *ntt = bytes[1];
+    *count_out = count;
+    if (count == 0) return len == W_HEADER && length == 0 ? W_OK : W_MALFORMED;
+    if (length == 0 || length > (1u << 16) || count > W_MAX_ELEMENTS || low < -32768 ||
+        (int64_t)low + length - 1 > 32767)
+        return W_MALFORMED;
+    if (count > cap) return W_CAPACITY;
+    int err = W_OK;
+    uint32_t *freq = (uint32_t *)malloc((length + 1) * sizeof(uint32_t));
+    uint32_t *table = (uint32_t *)malloc(W_SCALE * 3 * sizeof(uint32_t)); /* f, slot - start, symbol */
+    bitr r = {bytes + W_HEADER, len - W_HEADER, 0};
+    uint64_t total = 0;
+    for (uint32_t s = 0; s <= length && !err; s++) {
+        uint64_t g;
+        if ((err = br_gamma(&r, &g))) break;
+        total += g - 1;
+        if (total > W_SCALE) err = W_MALFORMED;
+        freq[s] = (uint32_t)(g - 1);
+    }
+    if (!err && total != W_SCALE) err = W_MALFORMED;
+    if (!err) {
+        uint32_t acc = 0;
+        for (uint32_t s = 0; s <= length; s++) {
+            for (uint32_t slot = acc; slot < acc + freq[s]; slot++) {
+                table[3 * slot] = freq[s];
+                table[3 * slot + 1] = slot - acc;
+                table[3 * slot + 2] = s;
+            }
+            acc += freq[s];
+        }
+        const size_t used = W_HEADER + br_consumed(&r);
+        const uint8_t *rest = bytes + used;
+        const size_t rest_len = len - used;
+        uint32_t x[W_LANES];
+        if (rest_len < 4 * W_LANES) err = W_TRUNCATED;
+        for (int k = 0; k < W_LANES && !err; k++) {
+            x[k] = rd32(rest + 4 * k);
+            if (x[k] < W_RANS_LOW) err = W_MALFORMED;
+        }
+        const uint8_t *stream = rest + 4 * W_LANES;
+        const size_t stream_len = err ? 0 : rest_len - 4 * W_LANES;
+        if (!err && stream_len % 2) err = W_MALFORMED;
+        const size_t nwords = stream_len / 2, coefs = (size_t)count * N;
+        const uint32_t raw = raw_bits(length);
+        size_t pos = 0;
+        for (size_t g = 0; g * W_LANES < coefs && !err; g++) {
+            const size_t lanes = coefs - g * W_LANES < W_LANES ? coefs - g * W_LANES : W_LANES;
+            for (size_t k = 0; k < lanes && !err; k++) {
+                const uint32_t *e = table + 3 * (x[k] & (W_SCALE - 1));
+                uint32_t y = e[0] * (x[k] >> 12) + e[1], s = e[2];
+                uint32_t wd = pos < nwords ? (uint32_t)stream[2 * pos] | (uint32_t)stream[2 * pos + 1] << 8 : 0;
+                if (y < W_RANS_LOW) {
+                    x[k] = (y << 16) | wd;
+                    pos++;
+                } else {
+                    x[k] = y;
+                }
+                if (s == length) {
+                    s = 0;
+                    if (raw) {
+                        s = x[k] & ((1u << raw) - 1);
+                        y = x[k] >> raw;
+                        wd = pos < nwords ? (uint32_t)stream[2 * pos] | (uint32_t)stream[2 * pos + 1] << 8 : 0;
+                        if (y < W_RANS_LOW) {
+                            x[k] = (y << 16) | wd;
+                            pos++;
+                        } else {
+                            x[k] = y;
+                        }
+                    }
+                    if (s >= length) err = W_MALFORMED;
+                }
+                v[g * W_LANES + k] = (int16_t)(low + (int32_t)s);
+            }
+            if (!err && pos > nwords) err = W_TRUNCATED;
+        }
+        if (!err && 2 * pos != stream_len) err = W_MALFORMED;
+        for (int k = 0; k < W_LANES && !err; k++)
+            if (x[k] != W_RANS_LOW) err = W_MALFORMED;
+    }
+    free(table);
+    free(freq);
+    return err;
+}
+
+/* rans.rs:500-524 */
+double orc_entropy_bytes(const int16_t *v, size_t n) {
+    if (!n) return 0.0;
+    uint64_t *c = (uint64_t *)calloc(65536, sizeof(uint64_t));
+    for (size_t i = 0; i < n; i++) c[(uint16_t)(v[i] + 32768)]++;
+    double bits = 0.0;
+    for (int i = 0; i < 65536; i++)
+        if (c[i]) {
+            const double p = (double)c[i] / (double)n;
+            bits += -(double)c[i] * log2(p);
+        }
+    free(c);
+    return bits / 8.0;
 }
