@@ -175,6 +175,34 @@ def wire_bench(lbn, prover, params, folded, reps=5):
             "sigma": round(float(np.sqrt(np.mean(folded.elements.astype(np.float64) ** 2))), 2)}
 
 
+def verify_bench(lbn, prover, verifier, commitment, challenges, point, folded, row, reps=3):
+    """The verify side of the step (verifier.rs:178-231, fold.rs:799-873) for the step's own
+    outputs: the folded commitment and the Clear opening check, on the host and on the GPU (the
+    folded witness and the commitment copied in, the bit planes of v through the commit GEMM).
+    Wall clock; both must accept."""
+    import numpy as np
+
+    def timed(f, n):
+        ts, out = [], None
+        for _ in range(n):
+            t0 = time.perf_counter()
+            out = f()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return out, round(float(np.median(ts)), 3)
+
+    fc_host, fold_host_ms = timed(lambda: verifier.fold_commitment(commitment, challenges), 1)
+    fc_dev, fold_dev_ms = timed(lambda: verifier.fold_commitment(commitment, challenges, device=prover), reps)
+    assert np.array_equal(fc_host.rows, fc_dev.rows)
+    fr = verifier.fold_row_evaluation(row, challenges)
+    args = (commitment, challenges, point, fc_host, folded, fr)
+    _, verify_host_ms = timed(lambda: verifier.verify_opening(*args), 1)
+    _, verify_dev_ms = timed(lambda: verifier.verify_opening(*args, device=prover), reps)
+    return {"fold_commitment_ms": {"host": fold_host_ms, "device": fold_dev_ms},
+            "verify_opening_ms": {"host": verify_host_ms, "device": verify_dev_ms},
+            "note": "host = the library's C++ verifier on one thread; device = the same checks on the GPU, "
+                    "inputs copied in over PCIe; both accepted"}
+
+
 def batched(lbn, reps):
     """The batched commitment of 64 uneven witnesses (2^16 .. 2^22 F162, a quarter of them lifted B128
     traces, one key) against the same 64 commitments one at a time on the GPU. Both leave their
@@ -455,6 +483,8 @@ def main():
     }
     log("wire...")
     line["wire"] = wire_bench(lbn, prover, params, folded)
+    log("verify...")
+    line["verify"] = verify_bench(lbn, prover, verifier, c2, challenges, point, folded, row)
     if not args.no_batched:
         log("batched commit...")
         line["batched"] = batched(lbn, 3)

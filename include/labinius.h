@@ -244,6 +244,18 @@ int lbn_verify_opening_bit_dropped(const lbn_pp *pp, const uint16_t *top, const 
                                    const lbn_challenge *c, size_t r, const int16_t *v, size_t len_ring,
                                    const lbn_f162 *p0, size_t p0_len, const lbn_f162 *folded_row_value);
 
+/* The verifier's heavy steps on a prover's device (the prover is only the device context: its key,
+ * stream and scratch; nothing of its own commitment or fold is read or changed). Same results as
+ * the host entry points above:
+ * verifier.rs:186-224 and eval.rs:202-249 (as lbn_verify_opening_clear): the folded witness's norm,
+ * A v decomposed against the folded commitment, and its row check; LBN_OK or LBN_REJECTED */
+int lbn_verify_opening_clear_device(lbn_prover *prover, const int16_t *folded_commitment, const int16_t *v,
+                                    size_t len_ring, const lbn_f162 *p0, size_t p0_len,
+                                    const lbn_f162 *folded_row_value);
+/* Verifier::fold_commitment, fold.rs:799-873 (as lbn_fold_commitment) */
+int lbn_fold_commitment_device(lbn_prover *prover, const int16_t *components, const lbn_challenge *c, size_t r,
+                               int16_t *out);
+
 /* Wire forms (wire/mod.rs, wire/rans.rs). Packers and the encoder write `out` (capacity `cap`)
  * and set *len; with out == NULL only *len is set. Unpackers and the decoder return LBN_OK,
  * LBN_WIRE_TRUNCATED or LBN_WIRE_MALFORMED exactly where the reference returns Ok,

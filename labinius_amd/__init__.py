@@ -709,13 +709,18 @@ class Verifier:
                                                    _p(u), ctypes.c_size_t(len(u)), arr))
         return FoldingChallenges(list(arr))
 
-    def fold_commitment(self, commitment, challenges):
+    def fold_commitment(self, commitment, challenges, device=None):
+        """fold.rs:799-873; with device (a Prover of the same parameters) on its GPU."""
         p = self.params
         out = np.zeros((4, len(p.primes()), N162), np.int16)
         comp = np.ascontiguousarray(commitment.components, np.int16)
         arr = challenges.array()
-        _check(lib().lbn_fold_commitment(ctypes.c_void_p(self.pp._h), _p(comp), arr, ctypes.c_size_t(len(arr)),
-                                         _p(out)))
+        if device is not None:
+            _check(lib().lbn_fold_commitment_device(ctypes.c_void_p(device._h), _p(comp), arr,
+                                                    ctypes.c_size_t(len(arr)), _p(out)))
+        else:
+            _check(lib().lbn_fold_commitment(ctypes.c_void_p(self.pp._h), _p(comp), arr, ctypes.c_size_t(len(arr)),
+                                             _p(out)))
         return FoldedCommitment(p.primes(), out)
 
     def fold_row_evaluation(self, row_evaluation, challenges):
@@ -746,9 +751,11 @@ class Verifier:
                                      ctypes.byref(hi)))
         return lo.value | (hi.value << 64)
 
-    def verify_opening(self, commitment, challenges, point, folded_commitment, folded_witness, folded_row_value):
+    def verify_opening(self, commitment, challenges, point, folded_commitment, folded_witness, folded_row_value,
+                       device=None):
         """Verifier::verify_opening (verifier.rs:281-337): the Clear message carries the folded
-        commitment; the bit-dropped one does not (pass None)."""
+        commitment; the bit-dropped one does not (pass None). With device (a Prover of the same
+        parameters) the Clear checks run on its GPU."""
         if self.params.dropped_bits() > 0:
             return self._verify_opening_bd(commitment, challenges, point, folded_witness, folded_row_value)
         fc = np.ascontiguousarray(folded_commitment.rows, np.int16)
@@ -757,8 +764,13 @@ class Verifier:
         fr = np.ascontiguousarray(folded_row_value, np.uint64)
         if fc.shape != (4, len(self.params.primes()), N162):
             raise VerificationError("Rejected")
-        rc = lib().lbn_verify_opening_clear(ctypes.c_void_p(self.pp._h), _p(fc), _p(v), ctypes.c_size_t(len(v)),
-                                            _p(p0), ctypes.c_size_t(len(p0)), _p(fr))
+        if device is not None:
+            rc = lib().lbn_verify_opening_clear_device(ctypes.c_void_p(device._h), _p(fc), _p(v),
+                                                       ctypes.c_size_t(len(v)), _p(p0), ctypes.c_size_t(len(p0)),
+                                                       _p(fr))
+        else:
+            rc = lib().lbn_verify_opening_clear(ctypes.c_void_p(self.pp._h), _p(fc), _p(v), ctypes.c_size_t(len(v)),
+                                                _p(p0), ctypes.c_size_t(len(p0)), _p(fr))
         if rc == LBN_REJECTED:
             raise VerificationError("Rejected")
         _check(rc)

@@ -152,6 +152,14 @@ struct F162v {
     uint64_t w[3];
 };
 
+// The verifier's fold of a commitment (fold.rs:799-873): per limb, the challenge c_j evaluated at
+// slot s as sum_i (-1)^sign_i (-1)^p_i pw[(mult v_s p_i) mod cond] with v_s = POW3_SLOT_EXP[s] and
+// pw the limb's root powers (at pw_off in one array).
+struct FoldCommConsts {
+    uint32_t limbs;
+    uint32_t q[GARNER_MAX], cond[GARNER_MAX], mult[GARNER_MAX], pw_off[GARNER_MAX];
+};
+
 // A bit-packed wire object (wire/mod.rs:226-243, 386-397): a sequence of records, each the same
 // run of segments, each `count` fields of `bits` bits, least significant bit first with nothing
 // between them. Field i of segment l of record r is the u16 at src[off + r rec_stride + i stride]

@@ -187,6 +187,25 @@ struct lbn_wire_ws {
     float ms[3] = {};  // the last encode: total, statistics, the rANS kernel
 };
 
+// The device verifier's scratch (verify.cpp), allocated at its first use.
+struct lbn_verify_ws {
+    bool ready = false;
+    int16_t *d_v = nullptr;        // the folded witness
+    uint64_t *d_w = nullptr;       // its bit planes as F162 columns
+    uint8_t *d_bytes = nullptr;    // their commit layout
+    uint32_t *d_yacc = nullptr, *d_raw = nullptr;
+    int16_t *d_comp = nullptr;     // A times each plane, decomposed
+    uint64_t *d_u = nullptr;       // the planes' row values
+    uint64_t *d_stat = nullptr;    // [0] squared norm, [1] largest |v|
+    int16_t *d_fcomp = nullptr;    // a commitment to fold
+    int16_t *d_fout = nullptr;     // the folded commitment
+    int64_t *d_facc = nullptr;
+    uint32_t *d_pw = nullptr;      // every limb's root powers
+    uint16_t *d_p3 = nullptr;      // POW3_SLOT_EXP
+    lbn_challenge *d_ch = nullptr;
+    size_t cap_v = 0, cap_fcomp = 0, cap_ch = 0;
+};
+
 struct lbn_opening {
     lbn_prover *owner = nullptr;
     uint8_t *d_bytes_f = nullptr;  // the witness in the fold's byte layout
@@ -197,6 +216,13 @@ void batch_free(lbn_batch_ws *b);
 // bd/digits.rs:121-182 on the device into pr->d_bd_out (dropped.cpp)
 int bd_digits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw);
 void wire_free(lbn_wire_ws &w);
+// Prepass, GEMM and finish of ncol columns (column_len F162 each, at d_w) under the prover's key,
+// into d_comp ([c][4][limb][162]) and d_raw ([limb][c][648]) on the prover's stream (prover.cpp).
+// d_bytes holds bytes_size(ncol, kb_c) bytes whose padding reads as zero; d_yacc ncol x
+// YACC_PLANES x limbs x 648 words.
+int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint8_t *d_bytes, uint32_t *d_yacc,
+                          uint32_t *d_raw, int16_t *d_comp);
+void verify_free(lbn_verify_ws &w);
 }
 
 struct lbn_prover {
@@ -226,4 +252,5 @@ struct lbn_prover {
     lbn_batch_ws *batch = nullptr;
     lbn_switch_ws sw;
     lbn_wire_ws wire;
+    lbn_verify_ws vfy;
 };

@@ -1109,6 +1109,97 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
 }
 
 // =============================================================================================
+// the verifier on the device (verifier.rs:186-224, fold.rs:799-873)
+// =============================================================================================
+
+// The folded witness v ([len_ring][648] int16) as F162 columns for the commit path: column b < nb
+// holds bit b of u = v + half, column nb all ones (J), column nb + 1 bit 0 of v; so that
+// A v = sum_b 2^b A u_b - half A J. Element e = 4 r + kk of a column holds coefficient 4 m + kk of
+// ring element r at bit m (f162.rs:32-41). One thread an element; the squared norm and the largest
+// |v| (verifier.rs:186-203) are gathered on the way.
+__global__ __launch_bounds__(256) void verify_planes_kernel(const int16_t *__restrict__ v, uint32_t len_ring,
+                                                            int half, uint32_t nb, uint64_t *__restrict__ w,
+                                                            unsigned long long *__restrict__ stat) {
+    const uint64_t column_len = 4ull * len_ring, e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint64_t sq = 0;
+    int mx = 0;
+    if (e < column_len) {
+        const int16_t *vr = v + (e >> 2) * 648 + (e & 3);
+#pragma unroll
+        for (int word = 0; word < 3; word++) {
+            uint64_t pl[15] = {}, par = 0;
+            const int m1 = word < 2 ? 64 : 34;
+#pragma unroll 2
+            for (int i = 0; i < m1; i++) {
+                const int x = vr[4 * (64 * word + i)];
+                sq += (uint64_t)((int64_t)x * x);
+                mx = max(mx, x < 0 ? -x : x);
+                const uint32_t u = (uint32_t)(x + half);
+#pragma unroll
+                for (int b = 0; b < 15; b++) pl[b] |= (uint64_t)((u >> b) & 1) << i;
+                par |= (uint64_t)(x & 1) << i;
+            }
+#pragma unroll
+            for (int b = 0; b < 15; b++)
+                if ((uint32_t)b < nb) w[((uint64_t)b * column_len + e) * 3 + word] = pl[b];
+            w[((uint64_t)nb * column_len + e) * 3 + word] = word < 2 ? ~0ull : (1ull << 34) - 1;
+            w[((uint64_t)(nb + 1) * column_len + e) * 3 + word] = par;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        sq += __shfl_xor(sq, o);
+        mx = max(mx, __shfl_xor(mx, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stat[0], (unsigned long long)sq);
+        atomicMax(&stat[1], (unsigned long long)mx);
+    }
+}
+
+// acc[t][k][s] += sum over this block's columns j of c_j(slot s of limb k) x comp[j][t][k][s]
+// (fold.rs:799-873): block (column chunk, limb), thread s.
+constexpr int FC_CHUNK = 32;
+__global__ __launch_bounds__(192) void fold_comm_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r,
+                                                        const int16_t *__restrict__ comp, FoldCommConsts fc,
+                                                        const uint32_t *__restrict__ pw,
+                                                        const uint16_t *__restrict__ p3,
+                                                        unsigned long long *__restrict__ acc) {
+    const uint32_t s = threadIdx.x, k = blockIdx.y, j0 = blockIdx.x * FC_CHUNK;
+    if (s >= 162) return;
+    const uint32_t q = fc.q[k], cond = fc.cond[k], mv = fc.mult[k] * p3[s], limbs = fc.limbs;
+    const uint32_t *pwk = pw + fc.pw_off[k];
+    int64_t a[4] = {0, 0, 0, 0};
+    for (uint32_t j = j0; j < min(j0 + FC_CHUNK, r); j++) {
+        const lbn_challenge_dev c = ch[j];
+        int64_t cv = 0;
+        for (uint32_t i = 0; i < c.weight; i++) {
+            const uint32_t p = c.positions[i];
+            const int64_t t = pwk[(mv * p) % cond];
+            cv += (((c.signs >> i) & 1) ^ (p & 1)) ? -t : t;
+        }
+        int64_t cs = cv % (int64_t)q;  // centered
+        if (cs < 0) cs += q;
+        if (cs > (int64_t)(q - 1) / 2) cs -= q;
+#pragma unroll
+        for (int t = 0; t < 4; t++) a[t] += cs * comp[(((uint64_t)j * 4 + t) * limbs + k) * 162 + s];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) atomicAdd(&acc[(t * limbs + k) * 162 + s], (unsigned long long)a[t]);
+}
+
+__global__ void fold_comm_finish_kernel(const long long *__restrict__ acc, FoldCommConsts fc,
+                                        int16_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, n = 4 * fc.limbs * 162;
+    if (i >= n) return;
+    const uint32_t k = (i / 162) % fc.limbs;
+    const int64_t q = fc.q[k];
+    int64_t x = acc[i] % q;
+    if (x < 0) x += q;
+    out[i] = (int16_t)(x > (q - 1) / 2 ? x - q : x);
+}
+
+// =============================================================================================
 // wire formats (wire/mod.rs, wire/rans.rs)
 // =============================================================================================
 
@@ -1576,6 +1667,25 @@ hipError_t launch_rans_scatter(hipStream_t s, const uint32_t *staged, const uint
     LBN_LAUNCH(rans_count_kernel, grid, dim3(256), 0, s, staged, groups, cnt);
     LBN_LAUNCH(rans_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, cnt, groups, offs, stat);
     LBN_LAUNCH(rans_scatter_kernel, grid, dim3(256), 0, s, staged, escw, offs, stat, groups, stream);
+    return hipSuccess;
+}
+
+hipError_t launch_verify_planes(hipStream_t s, const int16_t *v, uint32_t len_ring, int half, uint32_t nb,
+                                uint64_t *w, unsigned long long *stat) {
+    const uint64_t column_len = 4ull * len_ring;
+    LBN_LAUNCH(verify_planes_kernel, dim3((uint32_t)((column_len + 255) / 256)), dim3(256), 0, s, v, len_ring, half, nb,
+               w, stat);
+    return hipSuccess;
+}
+
+hipError_t launch_fold_comm(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, const int16_t *comp,
+                            const FoldCommConsts &fc, const uint32_t *pw, const uint16_t *p3, long long *acc,
+                            int16_t *out) {
+    const uint32_t n = 4 * fc.limbs * 162;
+    LBN_LAUNCH(fill_u32_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, s, (uint32_t *)acc, (size_t)(2 * n), 0u);
+    if (r) LBN_LAUNCH(fold_comm_kernel, dim3((r + FC_CHUNK - 1) / FC_CHUNK, fc.limbs), dim3(192), 0, s, ch, r, comp, fc,
+                      pw, p3, (unsigned long long *)acc);
+    LBN_LAUNCH(fold_comm_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, s, (const long long *)acc, fc, out);
     return hipSuccess;
 }
 

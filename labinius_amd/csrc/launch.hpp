@@ -48,6 +48,13 @@ hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pi
                                   const uint64_t r[3], int top, uint64_t *out);
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u);
+// the device verifier: the folded witness's bit planes (stat: squared norm, largest |v|, added
+// to); the fold of a commitment (acc: 4 x limbs x 162 scratch)
+hipError_t launch_verify_planes(hipStream_t s, const int16_t *v, uint32_t len_ring, int half, uint32_t nb,
+                                uint64_t *w, unsigned long long *stat);
+hipError_t launch_fold_comm(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, const int16_t *comp,
+                            const FoldCommConsts &fc, const uint32_t *pw, const uint16_t *p3, long long *acc,
+                            int16_t *out);
 // wire formats: a bit-packed object into ceil(total_bits / 32) words; the encoder's statistics
 // (stat: min, max; hist: nhist u64 counts, cleared first); the one-wave rANS encoder, which stages
 // each group's words at fixed places; their scatter into the stream

@@ -350,6 +350,7 @@ static void prover_free(lbn_prover *pr) {
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
     batch_free(pr->batch);
     wire_free(pr->wire);
+    verify_free(pr->vfy);
     for (hipEvent_t &e : pr->ev)
         if (e) (void)hipEventDestroy(e);
     if (pr->stream) (void)hipStreamDestroy(pr->stream);
@@ -422,6 +423,35 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
         op->d_bytes_f = bytes_f;
         *opening = op;
     }
+    return LBN_OK;
+}
+
+int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint8_t *d_bytes, uint32_t *d_yacc,
+                          uint32_t *d_raw, int16_t *d_comp) {
+    lbn_pp *pp = pr->pp;
+    const Shape &s = pp->shape;
+    const size_t limbs = s.primes.size();
+    hipStream_t st = pr->stream;
+    HIP_TRY(launch_prepass(st, d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)ncol, d_bytes,
+                           pp->kb_c, nullptr, 0));
+    GemmArgs g{};
+    g.tiles = pp->d_tiles;
+    g.afrag = pp->d_afrag;
+    g.lut = pp->d_lut;
+    g.ee = pp->d_ee;
+    g.out = pp->d_out;
+    g.bytes = d_bytes;
+    g.kbs = pp->kb_c;
+    g.a_kb_stride = A_STEP;
+    g.y = d_yacc;
+    g.y_batch = YACC_PLANES * limbs * N;
+    g.batches = (uint32_t)ncol;
+    g.groups = (uint32_t)((ncol + GROUP_BATCH - 1) / GROUP_BATCH);
+    g.ntiles = pp->ntiles;
+    g.sched_t = 8;
+    HIP_TRY(launch_tile_gemm(st, g, 0));
+    HIP_TRY(launch_commit_finish(st, d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, d_raw, d_comp,
+                                 (uint32_t)ncol));
     return LBN_OK;
 }
 

@@ -618,3 +618,14 @@ def entropy_bytes(v):
     """rans.rs:500-524"""
     v = np.ascontiguousarray(v, np.int16).ravel()
     return lib().orc_entropy_bytes(_p(v), ctypes.c_size_t(v.size))
+
+
+def a_times_v(primes, len_ring, a, v):
+    """verifier.rs:205-224: A v decomposed, [4][limb][162] int16 (a as key_expand returns it)."""
+    pr = np.array(primes, np.uint32)
+    a = np.ascontiguousarray(a, np.int16)
+    v = np.ascontiguousarray(v, np.int16)
+    assert v.size == len_ring * N
+    out = np.zeros((4, len(primes), N162), np.int16)
+    lib().orc_a_times_v(_p(pr), ctypes.c_int(len(primes)), ctypes.c_int(len_ring), _p(a), _p(v), _p(out))
+    return out

@@ -388,7 +388,6 @@ void orc_eval_quad_at(uint32_t q, const uint32_t *a, uint32_t u, uint32_t *r01) 
 
 /* scalar.rs:254-265: leaf-wise product of two quadratic-slot transforms */
 static void mul_quad_slots(const prime_tab *t, const uint64_t *a, const uint64_t *b, uint64_t *c) {
-    uint64_t q = t->q;
     for (int j = 0; j < 324; j++) {
         uint64_t cj = t->leafc[j];
         uint64_t a0 = a[2 * j], a1 = a[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
@@ -1339,27 +1338,13 @@ void orc_fold_binary(const uint64_t *u, const orc_challenge *ch, size_t cols, ui
     memcpy(out, acc, 24);
 }
 
-/* verifier.rs:178-231 for the Clear mode: returns 1 on acceptance. fold_cap per
- * scheme/params.rs:263-265. */
-int orc_verify_folded_opening(const orc_params *p, const int16_t *a, const int16_t *folded_comm,
-                              const int16_t *v, const uint64_t *p0, const uint64_t *u_folded) {
+/* verifier.rs:205-224: A v per limb (v of len_ring ring elements, any int16), decomposed:
+ * out[row][limb][slot], centered */
+void orc_a_times_v(const uint32_t *primes, int limbs, int len_ring, const int16_t *a, const int16_t *v,
+                   int16_t *out) {
     init_tables();
-    int limbs = (int)p->n_primes;
-    int rows = (int)(p->witness_log_len - p->column_log_len);
-    int len_ring = (1 << rows) / 4;
-    int32_t half = (int32_t)((p->primes[0] - 1) / 2);
-    uint64_t normsq = 0;
-    int32_t worst = 0;
-    for (size_t i = 0; i < (size_t)len_ring * N; i++) {
-        int32_t x = v[i];
-        int32_t ax = x < 0 ? -x : x;
-        if (ax > worst) worst = ax;
-        normsq += (uint64_t)((int64_t)x * x);
-    }
-    double capd = ceil(108.0 * (double)(((size_t)1 << p->witness_log_len) / 4 * N));
-    if (worst > half || normsq > (uint64_t)capd) return 0;
     for (int k = 0; k < limbs; k++) {
-        const prime_tab *t = prime(p->primes[k]);
+        const prime_tab *t = prime(primes[k]);
         uint64_t q = t->q;
         uint64_t y[N];
         memset(y, 0, sizeof y);
@@ -1381,11 +1366,36 @@ int orc_verify_folded_opening(const orc_params *p, const int16_t *a, const int16
         uint32_t y32[N];
         for (int u = 0; u < N; u++) y32[u] = (uint32_t)y[u];
         int16_t d[4 * N162];
-        orc_decompose(p->primes[k], y32, d);
+        orc_decompose(primes[k], y32, d);
         for (int row = 0; row < 4; row++)
-            for (int s = 0; s < N162; s++)
-                if (d[row * N162 + s] != folded_comm[(row * limbs + k) * N162 + s]) return 0;
+            for (int s = 0; s < N162; s++) out[(row * limbs + k) * N162 + s] = d[row * N162 + s];
     }
+}
+
+/* verifier.rs:178-231 for the Clear mode: returns 1 on acceptance. fold_cap per
+ * scheme/params.rs:263-265. */
+int orc_verify_folded_opening(const orc_params *p, const int16_t *a, const int16_t *folded_comm,
+                              const int16_t *v, const uint64_t *p0, const uint64_t *u_folded) {
+    init_tables();
+    int limbs = (int)p->n_primes;
+    int rows = (int)(p->witness_log_len - p->column_log_len);
+    int len_ring = (1 << rows) / 4;
+    int32_t half = (int32_t)((p->primes[0] - 1) / 2);
+    uint64_t normsq = 0;
+    int32_t worst = 0;
+    for (size_t i = 0; i < (size_t)len_ring * N; i++) {
+        int32_t x = v[i];
+        int32_t ax = x < 0 ? -x : x;
+        if (ax > worst) worst = ax;
+        normsq += (uint64_t)((int64_t)x * x);
+    }
+    double capd = ceil(108.0 * (double)(((size_t)1 << p->witness_log_len) / 4 * N));
+    if (worst > half || normsq > (uint64_t)capd) return 0;
+    int16_t *d = (int16_t *)malloc(sizeof(int16_t) * 4 * N162 * (size_t)limbs);
+    orc_a_times_v(p->primes, limbs, len_ring, a, v, d);
+    int same = memcmp(d, folded_comm, sizeof(int16_t) * 4 * N162 * (size_t)limbs) == 0;
+    free(d);
+    if (!same) return 0;
     /* eval.rs:202-249: eq(p0) . (v mod 2) == u' with element 4m + k = component k of v_m */
     size_t wdim = (size_t)1 << rows;
     uint64_t *eq = (uint64_t *)malloc(sizeof(uint64_t) * 3 * wdim);
