@@ -129,12 +129,22 @@ __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups,
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
 }
 
-__device__ __forceinline__ uint32_t mod_q_i64(long long z, uint32_t q, double qinv) {
-    const long long t = (long long)floor((double)z * qinv);
-    long long m = z - t * (long long)q;
-    if (m < 0) m += q;
-    if (m >= (long long)q) m -= q;
-    return (uint32_t)m;
+// ll + 2^7 mid + 2^14 hh mod q, in [0, q), from the three int32 accumulators of a product. The sum
+// is exact in f64 (|.| < 2^46); t = rint(z / q) is within 2^-17 of z / q's nearest integer, so the
+// exact remainder z - t q (one fma) lies in (-q, q).
+__device__ __forceinline__ uint32_t mod_q_classes(int ll, int mid, int hh, double qd, double qinv) {
+    const double z = __builtin_fma(16384.0, (double)hh, __builtin_fma(128.0, (double)mid, (double)ll));
+    const double t = __builtin_rint(z * qinv);
+    const int r = (int)__builtin_fma(-t, qd, z);
+    return (uint32_t)(r < 0 ? r + (int)qd : r);
+}
+
+// x mod q for x < 2^52, the same way
+__device__ __forceinline__ uint32_t mod_q_u64(unsigned long long x, double qd, double qinv) {
+    const double z = (double)x;
+    const double t = __builtin_rint(z * qinv);
+    const int r = (int)__builtin_fma(-t, qd, z);
+    return (uint32_t)(r < 0 ? r + (int)qd : r);
 }
 
 // The block order: item -> (tile, group). The 32 items an XCD runs at once (item & 7 is the XCD a
@@ -319,7 +329,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zbuf[j][m]; the D fragment of a 32 x 32
     // block holds, in register r, row 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
     const uint32_t q = td.q;
-    const double qinv = 1.0 / (double)q;
+    const double qd = (double)q, qinv = 1.0 / qd;
 #pragma unroll
     for (int mb = 0; mb < 3; mb++)
 #pragma unroll
@@ -328,9 +338,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int rr = 4 * r4 + r;
-                const long long z = (long long)acc[mb][0][rr] + ((long long)acc[mb][1][rr] << 7) +
-                                    ((long long)acc[mb][2][rr] << 14);
-                z4[r] = mod_q_i64(z, q, qinv);
+                z4[r] = mod_q_classes(acc[mb][0][rr], acc[mb][1][rr], acc[mb][2][rr], qd, qinv);
             }
             const int m0 = 32 * mb + 8 * r4 + 4 * (lane >> 5);
             *(uint4 *)&zbuf[jl * ZROW + m0] = make_uint4(z4[0], z4[1], z4[2], z4[3]);
@@ -345,9 +353,14 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         const uint32_t *zb = zbuf + (TILE_USED * b) * ZROW + m;
 #pragma unroll 9
         for (int n = 0; n < TILE_USED; n++) sum += (unsigned long long)ee[n * TILE + m] * zb[n * ZROW];
-        const uint32_t v = (uint32_t)(sum % q);
+        const uint32_t v = mod_q_u64(sum, qd, qinv);  // sum < 81 q^2 < 2^37
         uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
-        *yp = g.add ? (*yp + v) % q : v;
+        if (g.add) {
+            const uint32_t a = *yp + v;  // both in [0, q)
+            *yp = a >= q ? a - q : a;
+        } else {
+            *yp = v;
+        }
     }
 }
 
