@@ -359,6 +359,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-commit comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving comparison")
     ap.add_argument("--no-switch", action="store_true", help="skip the cross-field switch timing")
+    ap.add_argument("--no-wire", action="store_true", help="skip the wire encoding timing")
+    ap.add_argument("--no-verify", action="store_true", help="skip the verify-side timing")
     ap.add_argument("--serving-rate", type=float, default=100.0, help="requests per second")
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -481,10 +483,12 @@ def main():
                                             "unit": "TOP/s",
                                             "executed_frac_of_random_rate": round(xops / (gemm_ms * 1e-3) / 1e12 / rate_r, 4)}},
     }
-    log("wire...")
-    line["wire"] = wire_bench(lbn, prover, params, folded)
-    log("verify...")
-    line["verify"] = verify_bench(lbn, prover, verifier, c2, challenges, point, folded, row)
+    if not args.no_wire:
+        log("wire...")
+        line["wire"] = wire_bench(lbn, prover, params, folded)
+    if not args.no_verify:
+        log("verify...")
+        line["verify"] = verify_bench(lbn, prover, verifier, c2, challenges, point, folded, row)
     if not args.no_batched:
         log("batched commit...")
         line["batched"] = batched(lbn, 3)

@@ -173,8 +173,8 @@ struct lbn_wire_ws {
     size_t cap_in = 0;        // coefficients
     uint32_t *d_staged = nullptr;  // the encoder's words, a u32 per coefficient ([group][lane])
     uint16_t *d_escw = nullptr;    // escapes' raw-bits words, [group][lane]
-    uint32_t *d_cnt = nullptr;     // words each group pushed
-    uint64_t *d_offs = nullptr;    // words pushed before each group
+    uint32_t *d_cnt = nullptr;     // words pushed inside each group's block of 64, then block totals
+    uint64_t *d_offs = nullptr;    // words pushed before each block
     size_t cap_groups = 0;
     uint16_t *d_stream = nullptr;
     size_t cap_words = 0;

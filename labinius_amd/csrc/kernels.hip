@@ -1424,30 +1424,50 @@ __global__ __launch_bounds__(64) void rans_encode_kernel(const int16_t *__restri
     stat[4 + k] = x;
 }
 
-// The words each group pushed: one wave a group.
+// Per block of 64 consecutive groups: the words each group pushed, their sums over the block's
+// later groups (pushed first: the groups run backwards) and the block's total.
 __global__ __launch_bounds__(256) void rans_count_kernel(const uint32_t *__restrict__ staged, uint64_t G,
-                                                         uint32_t *__restrict__ cnt) {
-    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= G) return;
-    const uint32_t w = staged[g * 64 + (threadIdx.x & 63)];
-    const uint32_t c = __popcll(__builtin_amdgcn_ballot_w64((w >> 16) & 1)) +
-                       __popcll(__builtin_amdgcn_ballot_w64((w >> 17) & 1));
-    if ((threadIdx.x & 63) == 0) cnt[g] = c;
+                                                         uint32_t *__restrict__ loc, uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t c[64];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t g0 = (uint64_t)blockIdx.x * 64;
+    for (uint32_t i = 0; i < 16; i++) {
+        const uint32_t gl = wave * 16 + i;
+        uint32_t n = 0;
+        if (g0 + gl < G) {
+            const uint32_t w = staged[(g0 + gl) * 64 + lane];
+            n = __popcll(__builtin_amdgcn_ballot_w64((w >> 16) & 1)) +
+                __popcll(__builtin_amdgcn_ballot_w64((w >> 17) & 1));
+        }
+        if (lane == 0) c[gl] = n;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t own = c[lane];
+        uint32_t x = own;  // inclusive suffix sum
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_down(x, o, 64);
+            if (lane + o < 64) x += y;
+        }
+        if (g0 + lane < G) loc[g0 + lane] = x - own;
+        if (lane == 0) bsum[blockIdx.x] = x;
+    }
 }
 
-// offs[g] = the words pushed before group g, i.e. by the groups after it (they run first), and
-// the total (stat[2..3]); one block: each thread sums a run of groups in push order, the block
-// scans the sums, and each thread writes its run's offsets.
+// boff[b] = the words pushed before block b's groups, i.e. by the blocks after it, and the total
+// (stat[2..3]); one block: each thread sums a run of blocks in push order, the block scans the
+// sums, and each thread writes its run's offsets.
 constexpr int SCAN_THREADS = 1024;
-__global__ __launch_bounds__(SCAN_THREADS) void rans_scan_kernel(const uint32_t *__restrict__ cnt, uint64_t G,
-                                                                 uint64_t *__restrict__ offs,
+__global__ __launch_bounds__(SCAN_THREADS) void rans_scan_kernel(const uint32_t *__restrict__ bsum, uint64_t nblk,
+                                                                 uint64_t *__restrict__ boff,
                                                                  uint32_t *__restrict__ stat) {
     __shared__ uint64_t part[SCAN_THREADS];
     const uint32_t t = threadIdx.x;
-    const uint64_t span = (G + SCAN_THREADS - 1) / SCAN_THREADS, r0 = min((uint64_t)t * span, G),
-                   r1 = min(r0 + span, G);
+    const uint64_t span = (nblk + SCAN_THREADS - 1) / SCAN_THREADS, r0 = min((uint64_t)t * span, nblk),
+                   r1 = min(r0 + span, nblk);
     uint64_t sum = 0;
-    for (uint64_t r = r0; r < r1; r++) sum += cnt[G - 1 - r];  // r counts groups in push order
+    for (uint64_t r = r0; r < r1; r++) sum += bsum[nblk - 1 - r];  // r counts blocks in push order
     part[t] = sum;
     __syncthreads();
     for (uint32_t d = 1; d < SCAN_THREADS; d <<= 1) {  // inclusive scan (Hillis-Steele)
@@ -1458,9 +1478,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void rans_scan_kernel(const uint32_t 
     }
     uint64_t run = part[t] - sum;
     for (uint64_t r = r0; r < r1; r++) {
-        const uint64_t g = G - 1 - r;
-        offs[g] = run;
-        run += cnt[g];
+        const uint64_t b = nblk - 1 - r;
+        boff[b] = run;
+        run += bsum[b];
     }
     if (t == SCAN_THREADS - 1) {
         stat[2] = (uint32_t)part[t];
@@ -1472,7 +1492,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void rans_scan_kernel(const uint32_t 
 // of lanes above k were pushed before lane k's, an escape's raw-bits word before its symbol's.
 __global__ __launch_bounds__(256) void rans_scatter_kernel(const uint32_t *__restrict__ staged,
                                                            const uint16_t *__restrict__ escw,
-                                                           const uint64_t *__restrict__ offs,
+                                                           const uint32_t *__restrict__ loc,
+                                                           const uint64_t *__restrict__ boff,
                                                            const uint32_t *__restrict__ stat, uint64_t G,
                                                            uint16_t *__restrict__ stream) {
     const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1483,7 +1504,7 @@ __global__ __launch_bounds__(256) void rans_scatter_kernel(const uint32_t *__res
     const uint64_t m1 = __builtin_amdgcn_ballot_w64(n1), m2 = __builtin_amdgcn_ballot_w64(n2);
     const uint64_t above = k == 63 ? 0ull : ~0ull << (k + 1);
     const uint64_t top = (uint64_t)stat[2] | (uint64_t)stat[3] << 32;
-    const uint64_t p = offs[g] + __popcll(m1 & above) + __popcll(m2 & above);
+    const uint64_t p = boff[g >> 6] + loc[g] + __popcll(m1 & above) + __popcll(m2 & above);
     if (n1) stream[top - 1 - p] = escw[g * 64 + k];
     if (n2) stream[top - 1 - (p + (n1 ? 1 : 0))] = (uint16_t)w;
 }
@@ -1676,10 +1697,13 @@ hipError_t launch_rans_encode(hipStream_t s, const int16_t *v, uint64_t coefs, c
 hipError_t launch_rans_scatter(hipStream_t s, const uint32_t *staged, const uint16_t *escw, uint32_t *cnt,
                                uint64_t *offs, uint32_t *stat, uint64_t groups, uint16_t *stream) {
     if (groups == 0) return hipSuccess;
-    const dim3 grid((uint32_t)((groups + 3) / 4));
-    LBN_LAUNCH(rans_count_kernel, grid, dim3(256), 0, s, staged, groups, cnt);
-    LBN_LAUNCH(rans_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, cnt, groups, offs, stat);
-    LBN_LAUNCH(rans_scatter_kernel, grid, dim3(256), 0, s, staged, escw, offs, stat, groups, stream);
+    // cnt: the per-group sums inside their block, then the block totals; offs: the block offsets
+    const uint64_t nblk = (groups + 63) / 64;
+    uint32_t *bsum = cnt + groups;
+    LBN_LAUNCH(rans_count_kernel, dim3((uint32_t)nblk), dim3(256), 0, s, staged, groups, cnt, bsum);
+    LBN_LAUNCH(rans_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, (const uint32_t *)bsum, nblk, offs, stat);
+    LBN_LAUNCH(rans_scatter_kernel, dim3((uint32_t)((groups + 3) / 4)), dim3(256), 0, s, staged, escw,
+               (const uint32_t *)cnt, (const uint64_t *)offs, (const uint32_t *)stat, groups, stream);
     return hipSuccess;
 }
 

@@ -63,7 +63,8 @@ hipError_t launch_wire_stats(hipStream_t s, const int16_t *v, uint64_t coefs, in
                              uint32_t nhist);
 hipError_t launch_rans_encode(hipStream_t s, const int16_t *v, uint64_t coefs, const RansSym *sym, uint32_t nsym,
                               int low, uint32_t raw, uint32_t *staged, uint16_t *escw, uint32_t *stat);
-// the staged words' counts (cnt, a u32 a group), their scan (offs) and the stream
+// the staged words' counts (cnt: groups + ceil(groups / 64) u32), their scan (offs: ceil(groups / 64)
+// u64) and the stream
 hipError_t launch_rans_scatter(hipStream_t s, const uint32_t *staged, const uint16_t *escw, uint32_t *cnt,
                                uint64_t *offs, uint32_t *stat, uint64_t groups, uint16_t *stream);
 

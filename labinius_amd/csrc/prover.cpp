@@ -374,6 +374,10 @@ static int take_bytes_f(lbn_prover *pr, uint8_t **out) {
 }
 
 
+// The commit GEMM's schedule: 8 tiles x 4 groups an XCD round, or, with under 32 groups (a
+// few columns), 32 tiles x 1 group, so that the groups spread over the 8 XCDs instead of two.
+static uint32_t commit_sched(uint32_t groups) { return groups < 32 ? 32 : 8; }
+
 static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
@@ -402,7 +406,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     g.batches = (uint32_t)s.r;
     g.groups = (uint32_t)((s.r + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
-    g.sched_t = 8;
+    g.sched_t = commit_sched(g.groups);
     HIP_TRY(launch_tile_gemm(st, g, 0));
     HIP_TRY(hipEventRecord(pr->ev[2], st));
     HIP_TRY(launch_commit_finish(st, pr->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, pr->d_raw,
@@ -448,7 +452,7 @@ int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint
     g.batches = (uint32_t)ncol;
     g.groups = (uint32_t)((ncol + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
-    g.sched_t = 8;
+    g.sched_t = commit_sched(g.groups);
     HIP_TRY(launch_tile_gemm(st, g, 0));
     HIP_TRY(launch_commit_finish(st, d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, d_raw, d_comp,
                                  (uint32_t)ncol));

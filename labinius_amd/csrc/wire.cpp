@@ -296,8 +296,8 @@ int encode_device(lbn_prover *pr, const int16_t *d_v, size_t count, int ntt, uin
         w.cap_groups = 0;
         if ((rc = grow(&w.d_staged, c1, groups * RANS_LANES))) return rc;
         if ((rc = grow(&w.d_escw, c2, groups * RANS_LANES))) return rc;
-        if ((rc = grow(&w.d_cnt, c3, groups))) return rc;
-        if ((rc = grow(&w.d_offs, c4, groups))) return rc;
+        if ((rc = grow(&w.d_cnt, c3, groups + (groups + 63) / 64))) return rc;
+        if ((rc = grow(&w.d_offs, c4, (groups + 63) / 64))) return rc;
         w.cap_groups = groups;
     }
     if ((rc = grow(&w.d_stream, w.cap_words, nwords))) return rc;
