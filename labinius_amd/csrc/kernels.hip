@@ -629,6 +629,38 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
+// t = sum_j eq1_j u_j over F162 (eval.rs:185-193): a product per thread, XOR-reduced through the
+// wave and the block, then into t (cleared first) with atomic XORs.
+__global__ __launch_bounds__(256) void claim_kernel(const uint64_t *__restrict__ eq1, const uint64_t *__restrict__ u,
+                                                    uint32_t n, unsigned long long *__restrict__ t) {
+    uint64_t a0 = 0, a1 = 0, a2 = 0;
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+        uint64_t pr[3];
+        f162_mul_dev(eq1 + 3 * (size_t)j, u + 3 * (size_t)j, pr);
+        a0 ^= pr[0];
+        a1 ^= pr[1];
+        a2 ^= pr[2];
+    }
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        a0 ^= shfl_xor_u64(a0, o);
+        a1 ^= shfl_xor_u64(a1, o);
+        a2 ^= shfl_xor_u64(a2, o);
+    }
+    __shared__ uint64_t part[4][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[wave][0] = a0;
+        part[wave][1] = a1;
+        part[wave][2] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint64_t x = part[0][threadIdx.x] ^ part[1][threadIdx.x] ^ part[2][threadIdx.x] ^ part[3][threadIdx.x];
+        atomicXor(&t[threadIdx.x], (unsigned long long)x);
+    }
+}
+
 // Visibility of a wave's own LDS writes to its other lanes, without a block barrier.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1723,6 +1755,12 @@ hipError_t launch_fold_comm(hipStream_t s, const lbn_challenge_dev *ch, uint32_t
     if (r) LBN_LAUNCH(fold_comm_kernel, dim3((r + FC_CHUNK - 1) / FC_CHUNK, fc.limbs), dim3(192), 0, s, ch, r, comp, fc,
                       pw, p3, (unsigned long long *)acc);
     LBN_LAUNCH(fold_comm_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, s, (const long long *)acc, fc, out);
+    return hipSuccess;
+}
+
+hipError_t launch_claim(hipStream_t s, const uint64_t *eq1, const uint64_t *u, uint32_t n, uint64_t *t) {
+    const uint32_t blocks = std::max(1u, std::min(64u, (n + 255) / 256));
+    LBN_LAUNCH(claim_kernel, dim3(blocks), dim3(256), 0, s, eq1, u, n, (unsigned long long *)t);
     return hipSuccess;
 }
 

@@ -48,6 +48,8 @@ hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pi
                                   const uint64_t r[3], int top, uint64_t *out);
 hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
                            uint32_t cols, uint64_t *u);
+// t (3 u64, cleared first) ^= sum_j eq1_j u_j over F162, j < n
+hipError_t launch_claim(hipStream_t s, const uint64_t *eq1, const uint64_t *u, uint32_t n, uint64_t *t);
 // the device verifier: the folded witness's bit planes (stat: squared norm, largest |v|, added
 // to); the fold of a commitment (acc: 4 x limbs x 162 scratch)
 hipError_t launch_verify_planes(hipStream_t s, const int16_t *v, uint32_t len_ring, int half, uint32_t nb,

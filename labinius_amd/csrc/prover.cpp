@@ -343,6 +343,7 @@ static int build_device(lbn_pp *pp) {
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
                     pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_u, pr->d_wscratch,
+                    pr->d_p1, pr->d_eq1, pr->d_eqpart1, pr->d_t,
                     pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out, pr->sw.d_eq, pr->sw.d_a, pr->sw.d_p,
                     pr->sw.d_small};
     for (void *b : bufs)
@@ -497,7 +498,8 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     return LBN_OK;
 }
 
-static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, lbn_f162 *u) {
+// eq(p0) and the row values u into pr->d_u, on the prover's stream (events 8..10)
+static int row_eval_launch(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0) {
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
     hipStream_t st = pr->stream;
@@ -507,12 +509,42 @@ static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
     HIP_TRY(hipEventRecord(pr->ev[9], st));
     HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_u, s.r * 6, 0));
     HIP_TRY(launch_row_eval(st, d_w, pr->d_eq, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
-    HIP_TRY(hipEventRecord(pr->ev[10], st));
-    HIP_TRY(hipMemcpyAsync(u, pr->d_u, s.r * sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    return LBN_OK;
+}
+
+static void row_eval_timings(lbn_prover *pr) {
     pr->timings[8] = elapsed(pr->ev[8], pr->ev[10]);
     pr->timings[9] = elapsed(pr->ev[8], pr->ev[9]);
     pr->timings[10] = elapsed(pr->ev[9], pr->ev[10]);
+}
+
+static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, lbn_f162 *u) {
+    int rc = row_eval_launch(pr, d_w, p0);
+    if (rc) return rc;
+    hipStream_t st = pr->stream;
+    HIP_TRY(hipEventRecord(pr->ev[10], st));
+    HIP_TRY(hipMemcpyAsync(u, pr->d_u, pr->pp->shape.r * sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    row_eval_timings(pr);
+    return LBN_OK;
+}
+
+// The row values, then the claim t = sum_j eq(p1)_j u_j (eval.rs:185-193) on the device: only t
+// comes back.
+static int mle_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, const lbn_f162 *p1,
+                           lbn_f162 *t) {
+    const Shape &s = pr->pp->shape;
+    int rc = row_eval_launch(pr, d_w, p0);
+    if (rc) return rc;
+    hipStream_t st = pr->stream;
+    HIP_TRY(hipMemcpyAsync(pr->d_p1, p1, s.cl * sizeof(lbn_f162), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_eq(st, pr->d_p1, s.cl, pr->d_eqpart1, pr->d_eq1));
+    HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_t, 6, 0));
+    HIP_TRY(launch_claim(st, pr->d_eq1, pr->d_u, (uint32_t)s.r, pr->d_t));
+    HIP_TRY(hipEventRecord(pr->ev[10], st));
+    HIP_TRY(hipMemcpyAsync(t, pr->d_t, sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    row_eval_timings(pr);
     return LBN_OK;
 }
 
@@ -614,6 +646,10 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     TRY_RC(dalloc(&pr->d_p0, 3 * (size_t)std::max<uint32_t>(s.rows, 1)));
     TRY_RC(dalloc(&pr->d_eq, 3 * s.column_len));
     TRY_RC(dalloc(&pr->d_eqpart, 3 * 256 * (size_t)((s.rows + 7) / 8)));
+    TRY_RC(dalloc(&pr->d_p1, 3 * (size_t)std::max<uint32_t>(s.cl, 1)));
+    TRY_RC(dalloc(&pr->d_eq1, 3 * ((size_t)1 << s.cl)));
+    TRY_RC(dalloc(&pr->d_eqpart1, 3 * 256 * (size_t)((s.cl + 7) / 8)));
+    TRY_RC(dalloc(&pr->d_t, 3));
     TRY_RC(dalloc(&pr->d_u, 3 * s.r));
 
     // warm-up (scheme/prover.rs:39-49): a zero witness through every kernel once
@@ -775,21 +811,7 @@ int lbn_mle_evaluate(lbn_prover *pr, const lbn_dwitness *w, const lbn_f162 *p0, 
     const Shape &s = pr->pp->shape;
     if (w->n != s.n || p0_len != s.rows || p1_len != s.cl)
         return fail(LBN_ERR_SHAPE, "the evaluation point does not match the witness");
-    std::vector<lbn_f162> u(s.r);
-    int rc = row_eval_device(pr, w->d_w, p0, u.data());
-    if (rc) return rc;
-    // eval.rs:185-193: u . eq(p1)
-    std::vector<F162> ps(p1_len);
-    for (size_t k = 0; k < p1_len; k++) std::memcpy(ps[k].w, p1[k].w, 24);
-    std::vector<F162> eq = eq_table(ps.data(), (int)p1_len);
-    F162 acc{{0, 0, 0}};
-    for (size_t j = 0; j < s.r; j++) {
-        F162 uj;
-        std::memcpy(uj.w, u[j].w, 24);
-        acc = f162_add(acc, f162_mul(eq[j], uj));
-    }
-    std::memcpy(t->w, acc.w, 24);
-    return LBN_OK;
+    return mle_eval_device(pr, w->d_w, p0, p1, t);
 }
 
 // Test-only (include/labinius_debug.h): one MFMA with the tile GEMM's lane layouts.
