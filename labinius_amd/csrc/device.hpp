@@ -37,7 +37,9 @@ constexpr int A_STEP = 2 * 2 * NBLK * FRAG;  // A fragments per K step: [kh][dig
 constexpr int GROUP_BATCH = 3;               // batch elements per group
 constexpr int GROUP_COLS = 256;              // N of a group (243 used)
 constexpr int GROUP_FRAGS = GROUP_COLS / BLK;
-constexpr int EE_TILE = TILE_USED * TILE;    // E of a tile, [n 81][m 96] u16
+constexpr int TILE_ROWS = 82;               // rows a tile may use (a quadratic commit tile's first half: 82)
+constexpr int EP = TILE_USED + 1;            // E and Z rows padded to an even length (pairs of n)
+constexpr int EE_TILE = TILE_ROWS * EP + 4;  // E of a tile, [m 82][n 82] u16 (zero at n = 81), 16-byte padded
 
 // The input bytes in MFMA B-fragment order. Byte n of batch element b at K index k lives at
 //

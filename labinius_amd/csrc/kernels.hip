@@ -19,6 +19,7 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 namespace lbn {
 
@@ -108,20 +109,19 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // Blocks are ordered so that the 32 blocks an XCD runs at once are 8 tiles x 4 groups: each
 // A and each group's bytes then come from L2 for 4 and 8 blocks respectively.
 constexpr int GEMM_THREADS = 512;
-// Z rows in LDS are padded from 96 to 100 words: the 32 inputs a half-wave stores then start on 16
-// distinct groups of 4 banks instead of 2.
-constexpr int ZROW = TILE + 4;
+// The epilogue's Z in LDS: [batch 3][m 82][n 82] u16 (Z < q < 2^15), rows padded to pairs of n.
+constexpr int ZP_BYTES = (GROUP_BATCH * TILE_ROWS * EP * 2 + 15) / 16 * 16;
 // A is staged per K step in an LDS buffer of AP 16-byte words per thread (A_STEP bytes used).
 constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
 constexpr int A_LDS = AP * GEMM_THREADS * 16;
 // LDS (the kernel's only LDS is the dynamic block, at address 0):
 //   K loop:   [0, 64 K) replicated LUT | plain LUT (1 KB) | two A buffers
-//   epilogue: [0, ...) Z [256][ZROW] words, overwriting the above
+//   epilogue: [0, ZP_BYTES) Z, overwriting the above
 //   E, loaded at item start and read in the epilogue, above both.
 // The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
 // address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
 constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
-constexpr int EE_OFF = GROUP_COLS * ZROW * 4;
+constexpr int EE_OFF = (ABUF_OFF + 2 * A_LDS > ZP_BYTES ? ABUF_OFF + 2 * A_LDS : ZP_BYTES);
 constexpr int GEMM_LDS = EE_OFF + EE_TILE * 2;
 static_assert(ABUF_OFF + 2 * A_LDS <= EE_OFF, "K-loop LDS overlaps E");
 
@@ -185,8 +185,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 
     uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
     uint8_t *abuf = smem + ABUF_OFF;                  // [2][A_LDS]
-    uint32_t *zbuf = (uint32_t *)smem;                // [256][ZROW], after the loop
-    uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [81][96]
+    uint16_t *zp = (uint16_t *)smem;                  // [3][82][82], after the loop
+    uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][82]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t lv = g.lut[td.lut_off + tid];     // entry (parity tid >> 8, byte tid & 255)
@@ -269,12 +269,12 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // E and the output slots are loaded now, behind the K loop; E's LDS area is not used before
     // the epilogue. 81 x 96 x 2 bytes are 972 16-byte words, two per thread.
     static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
-    static_assert(GROUP_BATCH * TILE <= GEMM_THREADS, "one epilogue output a thread");
+    static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
     constexpr int EE_V = EE_TILE * 2 / 16;
     const uint4 *eesrc = (const uint4 *)(g.ee + td.ee_off);
     uint4 ee0 = eesrc[tid], ee1 = make_uint4(0, 0, 0, 0);
     if (tid + GEMM_THREADS < EE_V) ee1 = eesrc[tid + GEMM_THREADS];
-    const int slot = tid < GROUP_BATCH * TILE ? g.out[td.out_off + tid % TILE] : -1;
+    const int slot = tid < 2 * GROUP_BATCH * TILE_ROWS ? g.out[td.out_off + (tid >> 1) % TILE_ROWS] : -1;
 
     const uint32_t kbs = g.kbn;  // steps of this launch, counted from kb0
     load_a(0);
@@ -326,34 +326,49 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         __syncthreads();
     }
 
-    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zbuf[j][m]; the D fragment of a 32 x 32
-    // block holds, in register r, row 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
+    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zp[b][m][n] (m < 82) for this lane's input
+    // j = 81 b + n; the D fragment of a 32 x 32 block holds, in register r, row
+    // 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
     const uint32_t q = td.q;
     const double qd = (double)q, qinv = 1.0 / qd;
+    if (jl < GROUP_BATCH * TILE_USED) {
+        uint16_t *zj = zp + (jl / TILE_USED) * TILE_ROWS * EP + jl % TILE_USED;
 #pragma unroll
-    for (int mb = 0; mb < 3; mb++)
+        for (int mb = 0; mb < 3; mb++)
 #pragma unroll
-        for (int r4 = 0; r4 < 4; r4++) {
-            uint32_t z4[4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rr = 4 * r4 + r;
-                z4[r] = mod_q_classes(acc[mb][0][rr], acc[mb][1][rr], acc[mb][2][rr], qd, qinv);
+            for (int rr = 0; rr < 16; rr++) {
+                const int m = 32 * mb + 8 * (rr >> 2) + 4 * (lane >> 5) + (rr & 3);
+                if (m < TILE_ROWS)
+                    zj[m * EP] = (uint16_t)mod_q_classes(acc[mb][0][rr], acc[mb][1][rr], acc[mb][2][rr], qd, qinv);
             }
-            const int m0 = 32 * mb + 8 * r4 + 4 * (lane >> 5);
-            *(uint4 *)&zbuf[jl * ZROW + m0] = make_uint4(z4[0], z4[1], z4[2], z4[3]);
-        }
+    }
     __syncthreads();
-    // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m]
+    // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (pairs of n
+    // 0..20 and 21..40), each pair one v_dot2_u32_u16: E, Z < q < 2^14.3, so five pairs stay
+    // within a u32 before it is added to the u64 sum
     {
-        const int b = tid / TILE, m = tid % TILE;
-        const uint32_t batch = grp * GROUP_BATCH + b;
-        if (slot < 0 || batch >= g.batches) return;  // slot < 0 for tid >= 288 too
+        const int o = tid >> 1, half = tid & 1;
+        const int b = o / TILE_ROWS, m = o % TILE_ROWS;
+        const uint32_t *ep = (const uint32_t *)(ee + m * EP) + 21 * half;
+        const uint32_t *zq = (const uint32_t *)(zp + (b * TILE_ROWS + m) * EP) + 21 * half;
+        const int np = 21 - half;  // pairs of this half
         unsigned long long sum = 0;
-        const uint32_t *zb = zbuf + (TILE_USED * b) * ZROW + m;
-#pragma unroll 9
-        for (int n = 0; n < TILE_USED; n++) sum += (unsigned long long)ee[n * TILE + m] * zb[n * ZROW];
-        const uint32_t v = mod_q_u64(sum, qd, qinv);  // sum < 81 q^2 < 2^37
+        if (tid < 2 * GROUP_BATCH * TILE_ROWS) {
+#pragma unroll
+            for (int c = 0; c < 21; c += 5) {
+                uint32_t s5 = 0;
+#pragma unroll
+                for (int i = c; i < c + 5 && i < 21; i++)
+                    if (i < np)
+                        s5 = __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, ep[i]), __builtin_bit_cast(u16x2, zq[i]),
+                                                    s5, false);
+                sum += s5;
+            }
+        }
+        sum += __shfl_xor(sum, 1, 64);
+        const uint32_t batch = grp * GROUP_BATCH + b;
+        if (half || slot < 0 || batch >= g.batches) return;  // slot < 0 for tid >= 492 too
+        const uint32_t v = mod_q_u64(sum, qd, qinv);  // sum < 81 q^2 < 2^35
         uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
         if (g.add) {
             const uint32_t a = *yp + v;  // both in [0, q)

@@ -117,10 +117,12 @@ static HostTile quad_fold_tile(const Prime &P, int k, int b) {
     return h;
 }
 
-// E transposed, [n < 81][m < 96] (mod q): the GEMM's epilogue reads it by input n.
+// E of the rows a tile may use, [m < 82][n < 82] (mod q, zero at n = 81): the GEMM's epilogue
+// reads a row two n at a time. Rows from 82 on are padding in every tile (the builders above use
+// m < 81, or m < 82 for a quadratic commit tile's first half).
 static void pack_ee(const HostTile &h, uint16_t *dst) {
-    for (int n = 0; n < TILE_USED; n++)
-        for (int m = 0; m < TILE; m++) dst[n * TILE + m] = (uint16_t)h.E[m * TILE + n];
+    for (int m = 0; m < TILE_ROWS; m++)
+        for (int n = 0; n < TILE_USED; n++) dst[m * EP + n] = (uint16_t)h.E[m * TILE + n];
 }
 
 // The A operand of a commit tile in MFMA lane order: [kb][kh][digit][mb][lane][16 bytes], lane

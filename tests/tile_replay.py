@@ -8,7 +8,9 @@ import numpy as np
 N = 648
 TILE, KSTEP, NBLK, FRAG = 96, 64, 3, 1024
 A_STEP = 2 * 2 * NBLK * FRAG
-EE_TILE = 81 * TILE
+EP = 82  # E rows padded to pairs of n
+TILE_ROWS = 82  # rows a tile may use
+EE_TILE = TILE_ROWS * EP + 4
 YACC_PLANES = 2  # commit output planes: one per input parity of a quadratic limb
 DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("ee_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
                  ("par", "<u4"), ("y_off", "<u4")])
@@ -55,8 +57,12 @@ def a_matrix(afrag, kbs):
 
 
 def ee_matrix(ee):
-    """[n 81][m 96] -> E[m][n] (mod q)"""
-    return ee.reshape(81, TILE).astype(np.int64).T.copy()
+    """[m 82][n 82] (zero at n = 81) -> E[m][n] (mod q) for m < 96, the padding rows zero"""
+    E = np.zeros((TILE, 81), np.int64)
+    t = ee[:TILE_ROWS * EP].reshape(TILE_ROWS, EP).astype(np.int64)
+    assert not t[:, 81].any()
+    E[:TILE_ROWS] = t[:, :81]
+    return E
 
 
 def x_digits(bytes_kn, lut, par):
