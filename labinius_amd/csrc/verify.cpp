@@ -44,9 +44,8 @@ FoldCommConsts fold_consts(const Shape &s, std::vector<uint32_t> *pw) {
     return fc;
 }
 
-int verify_ready(lbn_prover *pr) {
+int verify_alloc(lbn_prover *pr) {
     lbn_verify_ws &w = pr->vfy;
-    if (w.ready) return LBN_OK;
     const lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
     const size_t limbs = s.primes.size();
@@ -72,6 +71,14 @@ int verify_ready(lbn_prover *pr) {
     HIP_TRY(hipMemcpy(w.d_p3, tables().pow3_slot_exp, N162 * sizeof(uint16_t), hipMemcpyHostToDevice));
     w.ready = true;
     return LBN_OK;
+}
+
+// The verifier's scratch, allocated at its first use (all of it, or none).
+int verify_ready(lbn_prover *pr) {
+    if (pr->vfy.ready) return LBN_OK;
+    const int rc = verify_alloc(pr);
+    if (rc) verify_free(pr->vfy);
+    return rc;
 }
 
 }  // namespace

@@ -172,14 +172,21 @@ int grow(T **p, size_t &cap, size_t need) {
     return LBN_OK;
 }
 
-int wire_ready(lbn_wire_ws &w) {
-    if (w.d_stat) return LBN_OK;
+int wire_alloc(lbn_wire_ws &w) {
     int rc;
     if ((rc = dalloc(&w.d_stat, RANS_STAT))) return rc;
     if ((rc = dalloc(&w.d_hist, NHIST))) return rc;
     if ((rc = dalloc(&w.d_sym, NHIST))) return rc;
     for (hipEvent_t &e : w.ev) HIP_TRY(hipEventCreate(&e));
     return LBN_OK;
+}
+
+// The encoder's fixed scratch, allocated at its first use (all of it, or none).
+int wire_ready(lbn_wire_ws &w) {
+    if (w.d_stat && w.d_hist && w.d_sym && w.ev[4]) return LBN_OK;
+    const int rc = wire_alloc(w);
+    if (rc) wire_free(w);
+    return rc;
 }
 
 // A packed object on the device from src, back to the host.
