@@ -88,6 +88,8 @@ struct Transcript {
 };
 
 uint64_t sample_short_challenge(Transcript &t, uint32_t weight, double bound, lbn_challenge *out);
+// n challenges in a row (FoldingChallenges::derive), the same as n calls of the above
+void sample_short_challenges(Transcript &t, size_t n, uint32_t weight, double bound, lbn_challenge *out);
 
 }  // namespace lbn
 

@@ -3,6 +3,8 @@
 // CPU, as in the reference; only commit / fold / row evaluation run on the GPU.
 #include <cstring>
 
+#include <thread>
+
 #include "internal.hpp"
 
 namespace lbn {
@@ -186,14 +188,42 @@ void attempt(ByteStream &x, uint32_t weight, lbn_challenge &out) {
 
 }  // namespace
 
-uint64_t sample_short_challenge(Transcript &t, uint32_t weight, double bound, lbn_challenge *out) {
+namespace {
+
+uint64_t sample_from(Blake3::Output reader, uint32_t weight, double bound, lbn_challenge *out) {
     const double bound_sq = bound * bound + 1e-12;
-    ByteStream x(t.reader("short-challenge", 15));
+    ByteStream x(reader);
     for (uint64_t attempts = 1;; attempts++) {
         attempt(x, weight, *out);
         sign(*out);
         if (within(*out, bound_sq)) return attempts;
     }
+}
+
+}  // namespace
+
+uint64_t sample_short_challenge(Transcript &t, uint32_t weight, double bound, lbn_challenge *out) {
+    return sample_from(t.reader("short-challenge", 15), weight, bound, out);
+}
+
+// n challenges in a row: challenge j reads its own stream (the transcript's state, its counter
+// value, the label), so the streams are taken in order and the rejection sampling runs on threads.
+void sample_short_challenges(Transcript &t, size_t n, uint32_t weight, double bound, lbn_challenge *out) {
+    std::vector<Blake3::Output> readers;
+    readers.reserve(n);
+    for (size_t j = 0; j < n; j++) readers.push_back(t.reader("short-challenge", 15));
+    const size_t nt = std::min<size_t>(std::max<size_t>(1, n / 16), std::max(1u, std::thread::hardware_concurrency()));
+    auto work = [&](size_t j0, size_t j1) {
+        for (size_t j = j0; j < j1; j++) sample_from(readers[j], weight, bound, &out[j]);
+    };
+    if (nt <= 1) {
+        work(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < nt; k++) th.emplace_back(work, n * k / nt, n * (k + 1) / nt);
+    work(0, n / nt);
+    for (auto &x : th) x.join();
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -298,15 +328,17 @@ int lbn_derive_evaluation_point(const lbn_pp *pp, lbn_transcript *tr, const int1
     absorb_parameters(t, pp);
     t.absorb_bytes("labinius/commitment", 19);
     t.absorb_u64(s.r);
-    const size_t limbs = s.primes.size(), per = 2 * N162 * limbs;
-    std::vector<uint8_t> buf(per);
+    // absorb_elements per column: the count 4, then the column's 4 entries with no per-element
+    // prefix, as one contiguous stream, so that the hasher takes whole subtrees on threads
+    const size_t limbs = s.primes.size(), per = 2 * N162 * limbs, col = 8 + 4 * per;
+    std::vector<uint8_t> buf(s.r * col);
     for (size_t c = 0; c < s.r; c++) {
-        t.state.update_u64(4);  // absorb_elements: the column's 4 entries, no per-element prefix
-        for (int row = 0; row < 4; row++) {
-            std::memcpy(buf.data(), comp + (c * 4 + row) * limbs * N162, per);  // little-endian i16
-            t.state.update(buf.data(), per);
-        }
+        uint8_t *d = buf.data() + c * col;
+        const uint64_t four = 4;
+        std::memcpy(d, &four, 8);  // little-endian u64
+        std::memcpy(d + 8, comp + c * 4 * limbs * N162, 4 * per);  // little-endian i16, rows 0..3
     }
+    t.state.update(buf.data(), buf.size());
     derive_point(t, s, p0, p1);
     return LBN_OK;
 }
@@ -318,7 +350,7 @@ int lbn_derive_folding_challenges(const lbn_pp *pp, lbn_transcript *tr, const lb
     Transcript &t = tr->t;
     t.absorb_bytes("labinius/row-evaluation", 23);
     t.absorb_bytes(u, 24 * r);
-    for (size_t j = 0; j < pp->shape.r; j++) sample_short_challenge(t, 28, 12.0, &out[j]);
+    sample_short_challenges(t, r, 28, 12.0, out);
     return LBN_OK;
 }
 

@@ -120,6 +120,32 @@ def test_transcript_and_challenges_match_oracle(lbn, orc):
     assert ac.fill(b"x", 33) == bc.fill(b"x", 33)
 
 
+def test_many_columns_point_and_challenges_match_oracle(lbn, orc):
+    # 1024 columns: the commitment (2.7 MB) is absorbed in one update that takes whole subtrees on
+    # threads, and the 1024 challenges are sampled on threads; both must equal the oracle's
+    M = lbn.Modulus
+    params = lbn.Params(17, 10, [M(9721)])
+    seed = bytes([0x21]) * 32
+    pp = lbn.PublicParameters.host_only(params, seed)
+    verifier = lbn.Verifier(pp)
+    primes, cols = params.primes(), params.columns()
+    assert cols == 1024
+    rng = np.random.default_rng(12)
+    comp = rng.integers(-1500, 1501, (cols, 4, len(primes), 162)).astype(np.int16)
+    commitment = lbn.Commitment(primes, cols, comp, None)
+    t, ot = lbn.Transcript(b"labinius/test/cols"), orc.Transcript(b"labinius/test/cols")
+    point = verifier.derive_evaluation_point(t, commitment)
+    op0, op1 = orc.derive_evaluation_point(ot, orc.make_params(17, 10, primes, seed), comp)
+    assert np.array_equal(point.p0, op0) and np.array_equal(point.p1, op1)
+    u = rng.integers(0, 2**63, (cols, 3), dtype=np.uint64)
+    u[:, 2] &= np.uint64((1 << 34) - 1)
+    ch = verifier.derive_folding_challenges(t, lbn.RowEvaluation(u))
+    och = orc.derive_folding_challenges(ot, u)
+    assert len(ch.challenges) == cols
+    for a, b in zip(ch.challenges, och):
+        assert bytes(a.positions) == bytes(b.positions) and a.signs == b.signs
+
+
 def test_f162_matches_oracle(lbn, orc):
     rng = np.random.default_rng(8)
     for _ in range(50):
