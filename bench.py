@@ -43,12 +43,13 @@ def tile_ops(params):
 
 def executed_ops(params):
     """int8 MFMA ops the commit GEMM executes, padding included: per (tile, group of 3 columns) and
-    K step of 64 ring elements, 8 waves x 24 v_mfma_i32_32x32x32_i8 of 2 x 32^3 ops (a tile is 96
-    rows of which 81 are used, a group 256 inputs of which 243)."""
+    K step of 64 ring elements, 8 waves x 44 v_mfma_i32_16x16x64_i8 of 2 x 16 x 16 x 64 ops (per
+    wave and 16-input block: five 16-row blocks x 4 digit products for rows 0..79, and 2 for a
+    block holding rows 80 and 81's digits; a group is 256 inputs of which 243)."""
     ntiles = sum(16 if (q - 1) % 972 == 0 and (q - 1) % 1944 != 0 else 8 for q in params.primes())
     groups = -(-params.columns() // 3)
     kbs = -(-params.len_ring() // 64)
-    return ntiles * groups * kbs * 8 * 24 * 2 * 32 ** 3
+    return ntiles * groups * kbs * 8 * 44 * 2 * 16 * 16 * 64
 
 
 def mfma_rate(L, random_operands):

@@ -20,8 +20,8 @@ extern "C" {
 int lbn_debug_tables(const lbn_params *params, const uint8_t seed[32], int which, void *dst, size_t cap,
                      size_t *bytes);
 
-/* One v_mfma_i32_32x32x32_i8 on the device with the operand and result lane layouts the tile GEMM
- * uses: a is 32 x 32 (row-major [m][k]), b is 32 x 32 ([k][n]), d receives the 32 x 32 int32 product. */
+/* One v_mfma_i32_16x16x64_i8 on the device with the operand and result lane layouts the tile GEMM
+ * uses: a is 16 x 64 (row-major [m][k]), b is 64 x 16 ([k][n]), d receives the 16 x 16 int32 product. */
 int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d);
 
 /* The tile GEMM's K chunk in steps of 64 (0: the default, the largest the int32 accumulators hold
@@ -30,7 +30,7 @@ int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d);
 int lbn_debug_gemm_chunk(uint32_t kb);
 
 /* Measurement, no counterpart in the reference: the device's int8 MFMA rate in TOP/s
- * (v_mfma_i32_32x32x32_i8, 8 independent chains a wave, all CUs), on constant operands
+ * (v_mfma_i32_16x16x64_i8, 16 independent chains a wave, all CUs), on constant operands
  * (random_operands = 0) or on random ones, which draw more power and clock lower. */
 int lbn_debug_mfma_rate(int random_operands, double *tops);
 

@@ -29,10 +29,12 @@ namespace lbn {
 
 constexpr int TILE = 96;         // padded M of a tile (81 rows used)
 constexpr int TILE_USED = 81;
-constexpr int KSTEP = 64;        // K per step of the GEMM loop: two v_mfma_i32_32x32x32_i8
-constexpr int BLK = 32;          // MFMA block (32 x 32)
+constexpr int KSTEP = 64;        // K per step of the GEMM loop: one v_mfma_i32_16x16x64_i8
+// The operands are staged in 32 x 32 fragment order (lane l: row or input l & 31, K 16 (l >> 5)
+// of a 32-K half); the GEMM reads them as 16 x 64 pieces.
+constexpr int BLK = 32;          // staged fragment block (32 rows or inputs x 32 K)
 constexpr int NBLK = TILE / BLK; // 3 blocks per tile side
-constexpr int FRAG = 1024;       // bytes of one 32 x 32 int8 operand fragment (64 lanes x 16 B)
+constexpr int FRAG = 1024;       // bytes of one staged 32 x 32 int8 fragment (64 lanes x 16 B)
 constexpr int A_STEP = 2 * 2 * NBLK * FRAG;  // A fragments per K step: [kh][digit][mb]
 constexpr int GROUP_BATCH = 3;               // batch elements per group
 constexpr int GROUP_COLS = 256;              // N of a group (243 used)

@@ -100,11 +100,17 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // 256 inputs j = 81 b + n (243 used) over all K, then y[b][out[m]] = sum_n E[m][n] Z[m][81 b + n]
 // mod q.
 //
-// Eight waves, wave w owning inputs 32 w .. 32 w + 31. A (both digits, three 32-row blocks) is
+// Eight waves, wave w owning inputs 32 w .. 32 w + 31 as two 16-input blocks. A (both digits) is
 // staged per K step in LDS, double-buffered and shared by the eight waves. Each wave builds its
-// own B fragments in registers: one 16-byte load per lane gives the 16 bytes of its input and K
-// range, and the tile's LUT (in LDS) turns every byte into its two digits. Z is kept as three
+// own B fragments in registers: two 16-byte loads per lane give the 16 bytes of its two inputs'
+// K range, and the tile's LUT (in LDS) turns every byte into its two digits. Z is kept as three
 // int32 accumulators per element: ll = A_l x_l, mid = A_l x_h + A_h x_l, hh = A_h x_h.
+//
+// The products are v_mfma_i32_16x16x64_i8. Rows 0..79 are five 16-row blocks with four digit
+// products each; rows 80 and 81 (81 = 5 x 16 + 1, and a quadratic commit tile's first half uses
+// 82) share a sixth block whose rows are [A80_l, A80_h, A81_l, A81_h], so two products (with x_l
+// and with x_h) give all eight of their digit products. A step is 44 MFMAs of 32 K ops a wave,
+// where 96 rows of 32 x 32 x 32 blocks took 24 of 64 K: 17% less MFMA work.
 //
 // Blocks are ordered so that the 32 blocks an XCD runs at once are 8 tiles x 4 groups: each
 // A and each group's bytes then come from L2 for 4 and 8 blocks respectively.
@@ -121,9 +127,9 @@ constexpr int A_LDS = AP * GEMM_THREADS * 16;
 // The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
 // address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
 constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
-constexpr int EE_OFF = (ABUF_OFF + 2 * A_LDS > ZP_BYTES ? ABUF_OFF + 2 * A_LDS : ZP_BYTES);
+constexpr int EE_OFF = (ABUF_OFF + 3 * A_LDS > ZP_BYTES ? ABUF_OFF + 3 * A_LDS : ZP_BYTES);
 constexpr int GEMM_LDS = EE_OFF + EE_TILE * 2;
-static_assert(ABUF_OFF + 2 * A_LDS <= EE_OFF, "K-loop LDS overlaps E");
+static_assert(ABUF_OFF + 3 * A_LDS <= EE_OFF, "K-loop LDS overlaps E");
 
 __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
@@ -184,7 +190,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
  const TileDesc td = g.tiles[tile];
 
     uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
-    uint8_t *abuf = smem + ABUF_OFF;                  // [2][A_LDS]
+    uint8_t *abuf = smem + ABUF_OFF;                  // [3][A_LDS]
     uint16_t *zp = (uint16_t *)smem;                  // [3][82][82], after the loop
     uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][82]
 
@@ -200,40 +206,56 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         lut[tid] = (uint16_t)lv;
     }
 
-    const int jl = 32 * wave + (lane & 31);  // this lane's input within the group
-    const int par = ((int)td.par + jl % TILE_USED) & 1;
-    const uint16_t *L = lut + (par << 8);
-    // byte 0 of a lookup address: this lane's parity and copy; byte 1 comes from the input word
-    const uint32_t lsel = (uint32_t)(par << 7) | (uint32_t)((lane & 31) << 2);
+    // v_mfma_i32_16x16x64_i8 operands: lane holds row (A) or input (B) c16 = lane & 15 of a 16-block
+    // at K group kg = lane >> 4, i.e. K 16 kg .. 16 kg + 15 of the step's 64. A wave owns 32 inputs
+    // as two 16-blocks nb, so a lane holds inputs jl and jl + 16. The staged layouts are in
+    // 32 x 32 x 32 fragment order (lane l: row or input l & 31, K 32 kh + 16 (l >> 5) + byte), so
+    // the 16-byte piece of (row or input x, K group kg) sits at lane (x & 31) + 32 (kg & 1) of
+    // half kh = kg >> 1 there.
+    const int c16 = lane & 15, kg = lane >> 4;
+    const int wgrp = __builtin_amdgcn_readfirstlane(wave >> 2);  // waves 0-3: 0, waves 4-7: 1
+    const int jl = 32 * wave + c16;  // this lane's input of block nb = 0 within the group
+    const int off32 = c16 + 32 * (kg & 1), kh = kg >> 1;
+    // byte 0 of a lookup address: the input's parity and this lane's LUT copy; byte 1 comes from
+    // the input word
+    uint32_t lsel[2];
+    const uint16_t *L[2];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        const int par = ((int)td.par + (jl + 16 * nb) % TILE_USED) & 1;
+        lsel[nb] = (uint32_t)(par << 7) | (uint32_t)((lane & 31) << 2);
+        L[nb] = lut + (par << 8);
+    }
 
-    const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + lane) * 16;
+    // the (group, wave) byte fragments: [kb][kh 2][lane 64][16]; this lane's two pieces a step
+    const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + kh * 64 + off32) * 16;
     const int8_t *abase = g.afrag + td.a_off;
 
     uint4 raw[2];
     auto load_raw = [&](uint32_t kb) {
 #pragma unroll
-        for (int kh = 0; kh < 2; kh++) raw[kh] = *(const uint4 *)(bp + ((uint64_t)(g.kb0 + kb) * 2 + kh) * 1024);
+        for (int nb = 0; nb < 2; nb++) raw[nb] = *(const uint4 *)(bp + (uint64_t)(g.kb0 + kb) * 2048 + 256 * nb);
     };
     v4i bl[2], bh[2];
     auto build = [&]() {
 #pragma unroll
-        for (int kh = 0; kh < 2; kh++) {
-            const uint32_t w4[4] = {raw[kh].x, raw[kh].y, raw[kh].z, raw[kh].w};
+        for (int nb = 0; nb < 2; nb++) {
+            const uint32_t w4[4] = {raw[nb].x, raw[nb].y, raw[nb].z, raw[nb].w};
             int ol[4], oh[4];
 #pragma unroll
             for (int d = 0; d < 4; d++) {
                 const uint32_t r = w4[d];
                 uint32_t v0, v1, v2, v3;
                 if (REP) {
-                    v0 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0400u));
-                    v1 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0500u));
-                    v2 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0600u));
-                    v3 = *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0700u));
+                    v0 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0400u));
+                    v1 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0500u));
+                    v2 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0600u));
+                    v3 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0700u));
                 } else {
-                    v0 = L[r & 255];
-                    v1 = L[(r >> 8) & 255];
-                    v2 = L[(r >> 16) & 255];
-                    v3 = L[r >> 24];
+                    v0 = L[nb][r & 255];
+                    v1 = L[nb][(r >> 8) & 255];
+                    v2 = L[nb][(r >> 16) & 255];
+                    v3 = L[nb][r >> 24];
                 }
                 // p01 = [v0.b0, v1.b0, v0.b1, v1.b1]; then the four low and the four high digits
                 const uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
@@ -241,8 +263,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
                 ol[d] = (int)__builtin_amdgcn_perm(p23, p01, 0x05040100u);
                 oh[d] = (int)__builtin_amdgcn_perm(p23, p01, 0x07060302u);
             }
-            bl[kh] = (v4i){ol[0], ol[1], ol[2], ol[3]};
-            bh[kh] = (v4i){oh[0], oh[1], oh[2], oh[3]};
+            bl[nb] = (v4i){ol[0], ol[1], ol[2], ol[3]};
+            bh[nb] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
     // Every thread loads and stores AP 16-byte words of A a step, with no branch: the compiler can
@@ -259,15 +281,28 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
         for (int i = 0; i < AP; i++) *(v4i *)(abuf + buf * A_LDS + (tid + GEMM_THREADS * i) * 16) = apre[i];
     };
+    // this lane's A pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
+    // (mb16 < 5, rows 0..79) and, for the fifth block, rows 80 and 81: lane c16 < 4 holds digit
+    // c16 & 1 of row 80 + (c16 >> 1), the other lanes zero
+    const int aoff = (kh * 2 * NBLK * 64 + off32) * 16;
+    const int xdig = c16 & 1, xrow = (c16 >> 1) & 1;
+    const int xoff = ((kh * 2 + xdig) * NBLK * 64 + 2 * 64 + 16 + xrow + 32 * (kg & 1)) * 16;
+    const int xm = c16 < 4 ? -1 : 0;  // a mask rather than a select, which would branch in the loop
+    const v4i xmask = (v4i){xm, xm, xm, xm};
 
-    v16i acc[3][3];
+    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the fifth block's [nb][B digit]
+    v4i acc[5][2][3], xacc[2][2];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 5; a++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) acc[a][c] = (v16i){};
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[a][n][c] = (v4i){};
+#pragma unroll
+    for (int n = 0; n < 2; n++) xacc[n][0] = xacc[n][1] = (v4i){};
 
     // E and the output slots are loaded now, behind the K loop; E's LDS area is not used before
-    // the epilogue. 81 x 96 x 2 bytes are 972 16-byte words, two per thread.
+    // the epilogue. 82 x 82 x 2 bytes (+ padding) are 841 16-byte words, two per thread.
     static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
     static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
     constexpr int EE_V = EE_TILE * 2 / 16;
@@ -277,70 +312,97 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     const int slot = tid < 2 * GROUP_BATCH * TILE_ROWS ? g.out[td.out_off + (tid >> 1) % TILE_ROWS] : -1;
 
     const uint32_t kbs = g.kbn;  // steps of this launch, counted from kb0
+    // The two waves of a SIMD (w and w + 4) run each step's halves, the products of step kb and
+    // the lookups of step kb + 1, in opposite phase: waves 0-3 meet the step barrier between the
+    // two, waves 4-7 before the products, so one wave's lookups run while the other's products
+    // are in the MFMA pipe. A is then read for two steps at once (waves 0-3 one step ahead) and
+    // is staged in three buffers; the step that a thread stages is kb + 1 (waves 0-3) or kb + 2
+    // (waves 4-7), both into the buffer that no wave reads in that interval.
     load_a(0);
     load_raw(0);
     ((uint4 *)ee)[tid] = ee0;
     if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
     store_a(0);
-    __syncthreads();  // the LUT, E and A(0)
-    build();
-    load_a(kbs > 1 ? 1 : 0);
-    load_raw(kbs > 1 ? 1 : 0);
-    // One basic block per step, so that the next step's lookups and loads are scheduled among
-    // this step's MFMAs: past the last step the prefetches repeat the last one and the
-    // B fragments they build are not used.
-    for (uint32_t kb = 0; kb < kbs; kb++) {
-        const uint8_t *ab = abuf + (kb & 1) * A_LDS;
-        v4i cl[2], ch[2];
-#pragma unroll
-        for (int kh = 0; kh < 2; kh++) {
-            cl[kh] = bl[kh];
-            ch[kh] = bh[kh];
-        }
-        build();                          // step kb + 1 (its bytes are loaded)
-        load_raw(min(kb + 2, kbs - 1));   // a whole step ahead, and older than the A loads below
-#pragma unroll
-        for (int kh = 0; kh < 2; kh++) {
-            v4i a0[3], a1[3];
-#pragma unroll
-            for (int mb = 0; mb < 3; mb++) {
-                a0[mb] = *(const v4i *)(ab + (((kh * 2 + 0) * 3 + mb) * 64 + lane) * 16);
-                a1[mb] = *(const v4i *)(ab + (((kh * 2 + 1) * 3 + mb) * 64 + lane) * 16);
-            }
-            // the two products that share an accumulator are three MFMAs apart
-#pragma unroll
-            for (int mb = 0; mb < 3; mb++)
-                acc[mb][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mb], cl[kh], acc[mb][0], 0, 0, 0);
-#pragma unroll
-            for (int mb = 0; mb < 3; mb++)
-                acc[mb][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0[mb], ch[kh], acc[mb][1], 0, 0, 0);
-#pragma unroll
-            for (int mb = 0; mb < 3; mb++)
-                acc[mb][2] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mb], ch[kh], acc[mb][2], 0, 0, 0);
-#pragma unroll
-            for (int mb = 0; mb < 3; mb++)
-                acc[mb][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1[mb], cl[kh], acc[mb][1], 0, 0, 0);
-        }
-        store_a((kb + 1) & 1);  // A(kb + 1); buffer (kb + 1) & 1 was last read at step kb - 1
-        load_a(min(kb + 2, kbs - 1));
-        __syncthreads();
+    if (wgrp) {
+        load_a(kbs > 1 ? 1 : 0);
+        store_a(1);
     }
+    __syncthreads();  // the LUT, E, A(0) and waves 4-7's part of A(1)
+    build();
+    load_raw(kbs > 1 ? 1 : 0);
+    load_a(min(1u + wgrp, kbs - 1));
+    for (uint32_t kb = 0; kb < kbs; kb++) {
+        if (wgrp) __syncthreads();
+        const uint8_t *ab = abuf + (kb % 3) * A_LDS;
+        // The digit classes in turn over all ten (row block, input block) pairs, so that the two
+        // products sharing a mid accumulator are 24 MFMAs apart: a dependent MFMA waits for its
+        // accumulator in the pipe, and 16 x 16 x 64 products need about 16 in between to run at
+        // full rate.
+        v4i a0[5], a1[5];
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                acc[mb][nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bl[nb], acc[mb][nb][0], 0, 0, 0);
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], acc[mb][nb][1], 0, 0, 0);
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++)
+            a1[mb] = *(const v4i *)(ab + aoff + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+        {
+            const v4i ax = *(const v4i *)(ab + xoff) & xmask;
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], xacc[nb][0], 0, 0, 0);
+                xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], xacc[nb][1], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                acc[mb][nb][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bh[nb], acc[mb][nb][2], 0, 0, 0);
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+                acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
+        store_a((kb + 1 + wgrp) % 3);
+        load_a(min(kb + 2 + wgrp, kbs - 1));
+        if (!wgrp) __syncthreads();
+        build();                          // step kb + 1 (its bytes are loaded)
+        load_raw(min(kb + 2, kbs - 1));   // a whole step ahead
+    }
+    __syncthreads();  // waves 4-7's last products read the LDS that the epilogue overwrites
 
-    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zp[b][m][n] (m < 82) for this lane's input
-    // j = 81 b + n; the D fragment of a 32 x 32 block holds, in register r, row
-    // 8 (r >> 2) + 4 (lane >> 5) + (r & 3) and column lane & 31
+    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zp[b][m][n] for this lane's inputs
+    // j = 81 b + n; the D block of a 16 x 16 product holds, in register r, row 4 kg + r and
+    // column c16. In the fifth block, rows 0..3 (lanes of K group 0) are row 80's and row 81's
+    // digit products: with B's low digit [l80, h80, l81, h81] x l, with its high digit x h.
     const uint32_t q = td.q;
     const double qd = (double)q, qinv = 1.0 / qd;
-    if (jl < GROUP_BATCH * TILE_USED) {
-        uint16_t *zj = zp + (jl / TILE_USED) * TILE_ROWS * EP + jl % TILE_USED;
 #pragma unroll
-        for (int mb = 0; mb < 3; mb++)
+    for (int nb = 0; nb < 2; nb++) {
+        const int j = jl + 16 * nb;
+        if (j >= GROUP_BATCH * TILE_USED) continue;
+        uint16_t *zj = zp + (j / TILE_USED) * TILE_ROWS * EP + j % TILE_USED;
 #pragma unroll
-            for (int rr = 0; rr < 16; rr++) {
-                const int m = 32 * mb + 8 * (rr >> 2) + 4 * (lane >> 5) + (rr & 3);
-                if (m < TILE_ROWS)
-                    zj[m * EP] = (uint16_t)mod_q_classes(acc[mb][0][rr], acc[mb][1][rr], acc[mb][2][rr], qd, qinv);
+        for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int m = 16 * mb + 4 * kg + rr;
+                zj[m * EP] = (uint16_t)mod_q_classes(acc[mb][nb][0][rr], acc[mb][nb][1][rr], acc[mb][nb][2][rr], qd, qinv);
             }
+        if (kg == 0) {
+            const v4i xl = xacc[nb][0], xh = xacc[nb][1];
+            zj[80 * EP] = (uint16_t)mod_q_classes(xl[0], xh[0] + xl[1], xh[1], qd, qinv);
+            zj[81 * EP] = (uint16_t)mod_q_classes(xl[2], xh[2] + xl[3], xh[3], qd, qinv);
+        }
     }
     __syncthreads();
     // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (pairs of n
@@ -1107,28 +1169,29 @@ __global__ void garner_kernel(const int16_t *__restrict__ cen, GarnerConsts gc, 
         if (k < (int)gc.limbs) digits[(size_t)(k - 1) * count + i] = (uint16_t)m[k];
 }
 
-// One v_mfma_i32_32x32x32_i8 with the operand and result lane layouts the tile GEMM assumes:
-// A[m][k] and B[k][n] row-major 32 x 32; lane l holds A row l & 31 and B column l & 31 at
-// K 16 (l >> 5) .. + 15; D register r is row 8 (r >> 2) + 4 (l >> 5) + (r & 3), column l & 31.
+// One v_mfma_i32_16x16x64_i8 with the operand and result lane layouts the tile GEMM assumes:
+// A[m][k] row-major 16 x 64 and B[k][n] row-major 64 x 16; lane l holds A row l & 15 and B
+// column l & 15 at K 16 (l >> 4) .. + 15; D register r is row 4 (l >> 4) + r, column l & 15.
 __global__ void mfma_probe_kernel(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                                   int32_t *__restrict__ D) {
     const int lane = threadIdx.x;
     int8_t ab[16], bb[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-        ab[j] = A[(lane & 31) * 32 + 16 * (lane >> 5) + j];
-        bb[j] = B[(16 * (lane >> 5) + j) * 32 + (lane & 31)];
+        ab[j] = A[(lane & 15) * 64 + 16 * (lane >> 4) + j];
+        bb[j] = B[(16 * (lane >> 4) + j) * 16 + (lane & 15)];
     }
     v4i a, b;
     __builtin_memcpy(&a, ab, 16);
     __builtin_memcpy(&b, bb, 16);
-    v16i acc = {};
-    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
+    v4i acc = {};
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 16; r++) D[(8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 32 + (lane & 31)] = acc[r];
+    for (int r = 0; r < 4; r++) D[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[r];
 }
 
-// The int8 MFMA rate: each wave issues iters x 8 independent v_mfma_i32_32x32x32_i8, on operands
+// The int8 MFMA rate: each wave issues iters x 16 independent v_mfma_i32_16x16x64_i8 (fewer chains
+// leave the pipe waiting on accumulators), on operands
 // that are constant, or random per lane and alternating between two sets. Random operands toggle
 // more bits, draw more power and run at a lower clock.
 __device__ __forceinline__ int rate_hash(uint32_t x) {
@@ -1147,19 +1210,19 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(int random, uint32_t ite
         a1 = (v4i){rate_hash(16 * t + 8), rate_hash(16 * t + 9), rate_hash(16 * t + 10), rate_hash(16 * t + 11)};
         b1 = (v4i){rate_hash(16 * t + 12), rate_hash(16 * t + 13), rate_hash(16 * t + 14), rate_hash(16 * t + 15)};
     }
-    v16i acc[8];
+    v4i acc[16];
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc[i] = (v16i){};
+    for (int i = 0; i < 16; i++) acc[i] = (v4i){};
     for (uint32_t it = 0; it < iters; it++) {
 #pragma unroll
-        for (int i = 0; i < 8; i++)
-            acc[i] = __builtin_amdgcn_mfma_i32_32x32x32_i8((i & 1) ? a1 : a0, (i & 1) ? b1 : b0, acc[i], 0, 0, 0);
+        for (int i = 0; i < 16; i++)
+            acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8((i & 1) ? a1 : a0, (i & 1) ? b1 : b0, acc[i], 0, 0, 0);
     }
     int s = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < 16; i++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) s ^= acc[i][r];
+        for (int r = 0; r < 4; r++) s ^= acc[i][r];
     sink[t] = s;
 }
 

@@ -823,12 +823,12 @@ int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d) {
     int32_t *dd = nullptr;
     int rc = LBN_OK;
     if (hipMalloc((void **)&da, 1024) != hipSuccess || hipMalloc((void **)&db, 1024) != hipSuccess ||
-        hipMalloc((void **)&dd, 4096) != hipSuccess) {
+        hipMalloc((void **)&dd, 1024) != hipSuccess) {
         rc = fail(LBN_ERR_OOM, "probe buffers");
     } else if (hipMemcpy(da, a, 1024, hipMemcpyHostToDevice) != hipSuccess ||
                hipMemcpy(db, b, 1024, hipMemcpyHostToDevice) != hipSuccess ||
                launch_mfma_probe(nullptr, da, db, dd) != hipSuccess ||
-               hipMemcpy(d, dd, 4096, hipMemcpyDeviceToHost) != hipSuccess) {
+               hipMemcpy(d, dd, 1024, hipMemcpyDeviceToHost) != hipSuccess) {
         rc = fail(LBN_ERR_HIP, "probe launch");
     }
     if (da) (void)hipFree(da);
@@ -861,7 +861,7 @@ int lbn_debug_mfma_rate(int random_operands, double *tops) {
         hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
         rc = fail(LBN_ERR_HIP, "rate launch");
     } else {
-        *tops = (double)blocks * 4 * iters * 8 * (32.0 * 32 * 32 * 2) / (ms * 1e-3) / 1e12;
+        *tops = (double)blocks * 4 * iters * 16 * (16.0 * 16 * 64 * 2) / (ms * 1e-3) / 1e12;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

@@ -29,9 +29,9 @@ def _device():
 def test_mfma_lane_layout(lbn):
     say("case: mfma probe")
     rng = np.random.default_rng(1)
-    a = rng.integers(-128, 128, (32, 32)).astype(np.int8)
-    b = rng.integers(-128, 128, (32, 32)).astype(np.int8)
-    d = np.zeros((32, 32), np.int32)
+    a = rng.integers(-128, 128, (16, 64)).astype(np.int8)
+    b = rng.integers(-128, 128, (64, 16)).astype(np.int8)
+    d = np.zeros((16, 16), np.int32)
     rc = lbn.lib().lbn_debug_mfma_probe(a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
                                         d.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
