@@ -238,28 +238,27 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     };
     v4i bl[2], bh[2];
     auto build = [&]() {
+        uint32_t v[2][16];
+        // all 32 lookups first, so that their latencies overlap
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) {
             const uint32_t w4[4] = {raw[nb].x, raw[nb].y, raw[nb].z, raw[nb].w};
+#pragma unroll
+            for (int d = 0; d < 4; d++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint32_t r = w4[d];
+                    v[nb][4 * d + e] = REP ? *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0400u + 0x100u * e))
+                                           : L[nb][(r >> (8 * e)) & 255];
+                }
+        }
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
             int ol[4], oh[4];
 #pragma unroll
             for (int d = 0; d < 4; d++) {
-                const uint32_t r = w4[d];
-                uint32_t v0, v1, v2, v3;
-                if (REP) {
-                    v0 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0400u));
-                    v1 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0500u));
-                    v2 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0600u));
-                    v3 = *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0700u));
-                } else {
-                    v0 = L[nb][r & 255];
-                    v1 = L[nb][(r >> 8) & 255];
-                    v2 = L[nb][(r >> 16) & 255];
-                    v3 = L[nb][r >> 24];
-                }
-                // p01 = [v0.b0, v1.b0, v0.b1, v1.b1]; then the four low and the four high digits
-                const uint32_t p01 = __builtin_amdgcn_perm(v1, v0, 0x05010400u);
-                const uint32_t p23 = __builtin_amdgcn_perm(v3, v2, 0x05010400u);
+                const uint32_t p01 = __builtin_amdgcn_perm(v[nb][4 * d + 1], v[nb][4 * d], 0x05010400u);
+                const uint32_t p23 = __builtin_amdgcn_perm(v[nb][4 * d + 3], v[nb][4 * d + 2], 0x05010400u);
                 ol[d] = (int)__builtin_amdgcn_perm(p23, p01, 0x05040100u);
                 oh[d] = (int)__builtin_amdgcn_perm(p23, p01, 0x07060302u);
             }
@@ -328,6 +327,13 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         store_a(1);
     }
     __syncthreads();  // the LUT, E, A(0) and waves 4-7's part of A(1)
+    // the low-digit A pieces of the next step are read ahead, at the end of the step before
+    v4i a0[5];
+    auto read_a0 = [&](const uint8_t *ab) {
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+    };
+    read_a0(abuf);
     build();
     load_raw(kbs > 1 ? 1 : 0);
     load_a(min(1u + wgrp, kbs - 1));
@@ -338,9 +344,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         // products sharing a mid accumulator are 24 MFMAs apart: a dependent MFMA waits for its
         // accumulator in the pipe, and 16 x 16 x 64 products need about 16 in between to run at
         // full rate.
-        v4i a0[5], a1[5];
-#pragma unroll
-        for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+        v4i a1[5];
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
 #pragma unroll
@@ -377,6 +381,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         if (!wgrp) __syncthreads();
         build();                          // step kb + 1 (its bytes are loaded)
         load_raw(min(kb + 2, kbs - 1));   // a whole step ahead
+        read_a0(abuf + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
     }
     __syncthreads();  // waves 4-7's last products read the LDS that the epilogue overwrites
 
