@@ -511,3 +511,12 @@ def test_f64_reciprocal_quotient_is_exact():
     ok = x < fi * (1 << 20)
     q = np.floor(x.astype(np.float64) * rcp).astype(np.int64)
     assert np.array_equal(q[ok], (x // fi)[ok])
+    # The encoder also divides a state it will shed (x > f 2^20 - 1, any x < 2^32) and uses the
+    # quotient's top: x / f >> 16 = (x >> 16) / f. Both checked up to 2^32 - 1.
+    top = (1 << 32) - 1
+    n = np.concatenate([rng.integers(1 << 20, 1 << 32, 300), [top]])[None, :] // fi
+    for dx in (-1, 0, fi - 1):
+        x = np.minimum(n * fi + dx, top)
+        q = np.floor(x.astype(np.float64) * rcp).astype(np.int64)
+        assert np.array_equal(q, x // fi)
+        assert np.array_equal(q >> 16, (x >> 16) // fi)
