@@ -154,6 +154,12 @@ def matrix_seed(seed):
     return lib().orc_matrix_seed((ctypes.c_uint8 * 32).from_buffer_copy(bytes(seed)))
 
 
+def xorshift_step(x):
+    """rng.rs:10-14: the xorshift64* state transition."""
+    lib().orc_xorshift_step.restype = ctypes.c_uint64
+    return lib().orc_xorshift_step(ctypes.c_uint64(int(x)))
+
+
 def key_expand(primes, len_ring, seed64):
     pr = np.array(primes, np.uint32)
     a = np.zeros((len(primes), len_ring, N), np.int16)

@@ -27,8 +27,14 @@
  *   - BLAKE3: published test vectors of the spec (hash of "", "abc", keyed and derive-key of the
  *     empty input) plus agreement with the product's independently structured implementation at
  *     every input length around the chunk and tree boundaries;
- *   - A expansion, Witness::random, the transcript framing and the challenge sampler: parity
- *     unpinned (no reference execution is possible in this container); restated byte by byte;
+ *   - A expansion: the state transition is checked to be linear with order 2^64 - 1 over GF(2)
+ *     (a full-period xorshift, as the published generator), the draws uniform over the centred
+ *     range; which full-period triple, the seed mixing and the draw order stay unpinned;
+ *   - challenge sampler: the statistics the reference's tests/challenge.rs asserts (uniform
+ *     positions, balanced signs, separation by absorbed data, acceptance at bound 15), plus
+ *     shape, bound and determinism; its exact byte stream stays unpinned;
+ *   - Witness::random and the transcript framing: parity unpinned (no reference execution is
+ *     possible in this container); restated byte by byte;
  *   - wire formats: the reference's own wire tests (tests/wire.rs) restated case for case (exact
  *     sizes, the one-word-per-lane stream of 16 + 5 + 4 * 64 + 2 bytes, round trips on honest and
  *     adversarial messages, which error each broken stream gets), known answers worked from the
@@ -672,13 +678,17 @@ void orc_blake3(int mode, const uint8_t *key, const uint8_t *in, size_t len, uin
 
 typedef struct { uint64_t s; } rng_t;
 static void rng_new(rng_t *r, uint64_t seed) { r->s = seed * 0x9E3779B97F4A7C15ull | 1; }
-static uint64_t rng_next(rng_t *r) {
-    uint64_t x = r->s;
+/* xorshift64*'s state transition (rng.rs:10-14), exported so that the tests can check it is the
+ * full-period linear map of the published generator */
+uint64_t orc_xorshift_step(uint64_t x) {
     x ^= x >> 12;
     x ^= x << 25;
     x ^= x >> 27;
-    r->s = x;
-    return x * 0x2545F4914F6CDD1Dull;
+    return x;
+}
+static uint64_t rng_next(rng_t *r) {
+    r->s = orc_xorshift_step(r->s);
+    return r->s * 0x2545F4914F6CDD1Dull;
 }
 static uint32_t rng_below(rng_t *r, uint32_t n) {
     return (uint32_t)(((rng_next(r) >> 32) * (uint64_t)n) >> 32);

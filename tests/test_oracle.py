@@ -201,6 +201,82 @@ def test_key_layout_and_range(orc):
     assert np.array_equal(a[0, :32], b[0])
 
 
+def _gf2_mul(a, b):
+    return (a.astype(np.int64) @ b.astype(np.int64)) & 1
+
+
+def _gf2_pow(m, e):
+    r = np.eye(64, dtype=np.int64)
+    while e:
+        if e & 1:
+            r = _gf2_mul(r, m)
+        m = _gf2_mul(m, m)
+        e >>= 1
+    return r
+
+
+def test_xorshift_is_the_full_period_generator(orc):
+    # rng.rs:10-14 is Marsaglia's xorshift over GF(2)^64: the oracle's step must be linear, and
+    # its matrix must have order 2^64 - 1 (the full period of the published (12, 25, 27) triple),
+    # which a wrong shift or direction breaks
+    cols = [orc.xorshift_step(1 << i) for i in range(64)]
+    m = np.array([[(c >> r) & 1 for c in cols] for r in range(64)], np.int64)  # column i = step(e_i)
+    rng = np.random.default_rng(31)
+    for _ in range(20):
+        a, b = (int(x) for x in rng.integers(0, 2**63, 2, dtype=np.uint64))
+        assert orc.xorshift_step(a ^ b) == orc.xorshift_step(a) ^ orc.xorshift_step(b)
+        v = np.array([(a >> i) & 1 for i in range(64)], np.int64)
+        out = (m @ v) & 1
+        assert sum(int(out[i]) << i for i in range(64)) == orc.xorshift_step(a)
+    period = (1 << 64) - 1
+    eye = np.eye(64, dtype=np.int64)
+    assert np.array_equal(_gf2_pow(m, period), eye)
+    for p in (3, 5, 17, 257, 641, 65537, 6700417):  # the prime factors of 2^64 - 1
+        assert not np.array_equal(_gf2_pow(m, period // p), eye), p
+
+
+def test_key_draws_are_uniform_and_centred(orc):
+    # key.rs:63-81: A_k[i][j] = below(q) - (q - 1) / 2 with below(q) = (hi32 * q) >> 32: every
+    # value of the centred range is hit about equally often, and each limb has its own stream
+    q = 97
+    a = orc.key_expand([q, 3889], 2048, 12345)
+    counts = np.bincount(a[0].ravel().astype(np.int64) + (q - 1) // 2, minlength=q)
+    assert counts.size == q and counts.min() > 0
+    expect = a[0].size / q
+    chi2 = float(((counts - expect) ** 2 / expect).sum())
+    assert chi2 < 160.0  # 96 degrees of freedom; the 0.999 quantile is about 145
+    assert not np.array_equal(a[0][:8] % q, a[1][:8] % q)
+
+
+def test_sampler_statistics(orc):
+    # tests/challenge.rs:106-125 (positions uniform over 20 000 unrejected attempts, within 15% of
+    # weight / 162 each), :169-191 (the signs are balanced), :37-60 (absorbed data separates the
+    # streams) and :235-271 (at bound 15 under two attempts per challenge on average)
+    t = orc.Transcript(b"oracle/uniform")
+    hits = np.zeros(162, np.int64)
+    plus = total = 0
+    n = 20000
+    for _ in range(n):
+        c, att = t.sample_short_challenge(28, 1e9)  # no rejection: one attempt each
+        assert att == 1
+        pos = list(c.positions[:c.weight])
+        hits[pos] += 1
+        s = sum((c.signs >> i) & 1 for i in range(c.weight))
+        plus += c.weight - s
+        total += c.weight
+    expect = n * 28 / 162
+    assert hits.min() > 0.85 * expect and hits.max() < 1.15 * expect
+    assert abs(plus / total - 0.5) < 0.05
+    a, b = orc.Transcript(b"oracle/sep"), orc.Transcript(b"oracle/sep")
+    a.absorb_bytes(b"x")
+    b.absorb_bytes(b"y")
+    ca, cb = a.sample_short_challenge(30, 1e9)[0], b.sample_short_challenge(30, 1e9)[0]
+    assert bytes(ca.positions) != bytes(cb.positions) or ca.signs != cb.signs
+    t = orc.Transcript(b"oracle/accept")
+    att = [t.sample_short_challenge(28, 15.0)[1] for _ in range(2000)]
+    assert np.mean(att) < 2.0
+
+
 def _challenges(orc, n, label=b"oracle/test"):
     t = orc.Transcript(label)
     return [t.sample_short_challenge()[0] for _ in range(n)]
