@@ -281,7 +281,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         for (int i = 0; i < AP; i++) *(v4i *)(abuf + buf * A_LDS + (tid + GEMM_THREADS * i) * 16) = apre[i];
     };
     // this lane's A pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
-    // (mb16 < 5, rows 0..79) and, for the fifth block, rows 80 and 81: lane c16 < 4 holds digit
+    // (mb16 < 5, rows 0..79) and, for the sixth block, rows 80 and 81: lane c16 < 4 holds digit
     // c16 & 1 of row 80 + (c16 >> 1), the other lanes zero
     const int aoff = (kh * 2 * NBLK * 64 + off32) * 16;
     const int xdig = c16 & 1, xrow = (c16 >> 1) & 1;
@@ -289,7 +289,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     const int xm = c16 < 4 ? -1 : 0;  // a mask rather than a select, which would branch in the loop
     const v4i xmask = (v4i){xm, xm, xm, xm};
 
-    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the fifth block's [nb][B digit]
+    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the sixth block's [nb][B digit]
     v4i acc[5][2][3], xacc[2][2];
 #pragma unroll
     for (int a = 0; a < 5; a++)
@@ -387,7 +387,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 
     // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zp[b][m][n] for this lane's inputs
     // j = 81 b + n; the D block of a 16 x 16 product holds, in register r, row 4 kg + r and
-    // column c16. In the fifth block, rows 0..3 (lanes of K group 0) are row 80's and row 81's
+    // column c16. In the sixth block, rows 0..3 (lanes of K group 0) are row 80's and row 81's
     // digit products: with B's low digit [l80, h80, l81, h81] x l, with its high digit x h.
     const uint32_t q = td.q;
     const double qd = (double)q, qinv = 1.0 / qd;
