@@ -52,6 +52,40 @@ def executed_ops(params):
     return ntiles * groups * kbs * 8 * 44 * 2 * 16 * 16 * 64
 
 
+def stage_rooflines(params, stage, ms):
+    """Per kernel stage: its algorithmic work, the roofline that bounds it (HBM at 8 TB/s or int8
+    MFMA at 5 POP/s), the achieved rate and the fraction; then the whole path's fraction, the sum
+    of the stages' floors over the measured step. Bytes are what each kernel must move at least:
+    W and the two byte layouts for the layout pass, yacc in and raw + components out for the
+    finish, the challenges' fragments, v-hat in and v out, eq, and W again for the row values.
+    The smallest kernels (challenges, inverse NTT, eq) are bound by their latency, far from either
+    roofline; they are listed for completeness."""
+    n, r, L, k = params.witness_len(), params.columns(), params.len_ring(), len(params.primes())
+    layout = lambda batches, kdim: -(-batches // 3) * 8 * -(-kdim // 64) * 2 * 1024
+    work = {
+        "commit_layout": ("hbm", 24 * n + layout(r, L) + layout(L, r)),
+        "commit_gemm": ("mfma", tile_ops(params)),
+        "commit_finish": ("hbm", r * 2 * k * 648 * 4 + k * r * 648 * 4 + r * 4 * k * 162 * 2),
+        "fold_challenges": ("hbm", 8 * -(-r // 64) * 12288),
+        "fold_gemm": ("mfma", 8 * 81 * 81 * 4 * 2 * L * r),
+        "fold_intt": ("hbm", L * 648 * 4 + L * 648 * 2),
+        "eval_eq": ("hbm", (n // r) * 24),
+        "eval_reduce": ("hbm", 24 * n),
+    }
+    out, floor = {}, 0.0
+    for name, (bound, w) in work.items():
+        t = stage.get(name, 0.0)
+        peak = PEAK_HBM_GBS * 1e9 if bound == "hbm" else PEAK_INT8_TOPS * 1e12
+        f = w / peak * 1e3
+        floor += f
+        ach = w / (t * 1e-3) if t > 0 else 0.0
+        out[name] = {"ms": round(t, 4), "bound": bound, "work": w, "work_unit": "bytes" if bound == "hbm" else "ops",
+                     "achieved": round(ach / 1e9, 1) if bound == "hbm" else round(ach / 1e12, 1),
+                     "unit": "GB/s" if bound == "hbm" else "TOP/s", "floor_ms": round(f, 4),
+                     "frac": round(f / t, 4) if t > 0 else None}
+    return out, round(floor, 4), round(floor / ms, 4)
+
+
 def mfma_rate(L, random_operands):
     """The device's int8 MFMA rate (TOP/s), measured by lbn_debug_mfma_rate."""
     t = ctypes.c_double()
@@ -453,6 +487,7 @@ def main():
     xops = executed_ops(params)
     rate_c, rate_r = mfma_rate(L, 0), mfma_rate(L, 1)
     ab = algorithmic_bytes(params)
+    sr, sr_floor, sr_frac = stage_rooflines(params, stage, ms)
     line = {
         "metric": "commit+fold+eval ms @2^24 bit-exact",
         "value": round(ms, 3),
@@ -474,6 +509,9 @@ def main():
         "d2h_inclusive_ms": round(d2h_ms, 3),
         "hbm_floor_ms": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3, 4),
         "hbm_roofline_frac": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3 / ms, 4),
+        "stage_rooflines": sr,
+        "whole_path_floor_ms": sr_floor,
+        "whole_path_roofline_frac": sr_frac,
         "roofline": {"kernel": "tile_gemm_kernel (commit)", "bound": "mfma", "achieved": round(achieved, 1),
                      "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                      "unit_note": "int8 MFMA operations (multiply and add counted separately), 10^12 per second",
