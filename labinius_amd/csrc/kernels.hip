@@ -753,7 +753,8 @@ __global__ __launch_bounds__(64 * RE_WAVES) void row_eval_kernel(const uint64_t 
                                                                 const uint64_t *__restrict__ eq, uint32_t wdim,
                                                                 uint32_t cols, uint32_t span,
                                                                 uint64_t *__restrict__ u) {
-    __shared__ uint64_t T[8][256][3];
+    __shared__ ulonglong2 T01[8][256];  // words 0 and 1 of each table entry
+    __shared__ uint64_t T2[8][256];     // word 2
     __shared__ uint64_t NT[8][2][16][3];  // nibble tables: low and high four elements
     __shared__ __attribute__((aligned(16))) uint32_t ws[RE_WAVES][2][3][2 * RE_CHUNK];  // [wave][col][word][elem, half]
     __shared__ uint64_t by[RE_WAVES][2][8][21];  // [wave][col][group][bit position / 8]
@@ -793,9 +794,8 @@ __global__ __launch_bounds__(64 * RE_WAVES) void row_eval_kernel(const uint64_t 
         for (int e = tid; e < 8 * 256; e += 64 * RE_WAVES) {
             const int g = e >> 8, m = e & 255;
             const uint64_t *lo = NT[g][0][m & 15], *hi = NT[g][1][m >> 4];
-            T[g][m][0] = lo[0] ^ hi[0];
-            T[g][m][1] = lo[1] ^ hi[1];
-            T[g][m][2] = lo[2] ^ hi[2];
+            T01[g][m] = make_ulonglong2(lo[0] ^ hi[0], lo[1] ^ hi[1]);
+            T2[g][m] = lo[2] ^ hi[2];
         }
         __syncthreads();  // the tables
 #pragma unroll
@@ -846,10 +846,11 @@ __global__ __launch_bounds__(64 * RE_WAVES) void row_eval_kernel(const uint64_t 
                         if (b < 162) {
 #pragma unroll
                             for (int g = 0; g < 8; g++) {
-                                const uint64_t *t = T[g][bb[g * 168 + b]];
-                                S[2 * pr + cw][sl][0] ^= t[0];
-                                S[2 * pr + cw][sl][1] ^= t[1];
-                                S[2 * pr + cw][sl][2] ^= t[2];
+                                const int m = bb[g * 168 + b];
+                                const ulonglong2 t01 = T01[g][m];
+                                S[2 * pr + cw][sl][0] ^= t01.x;
+                                S[2 * pr + cw][sl][1] ^= t01.y;
+                                S[2 * pr + cw][sl][2] ^= T2[g][m];
                             }
                         }
                     }
