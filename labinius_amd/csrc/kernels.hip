@@ -327,11 +327,14 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         store_a(1);
     }
     __syncthreads();  // the LUT, E, A(0) and waves 4-7's part of A(1)
-    // the low-digit A pieces of the next step are read ahead, at the end of the step before
-    v4i a0[5];
+    // the next step's A pieces (both digits) are read ahead, at the end of the step before
+    v4i a0[5], a1[5];
     auto read_a0 = [&](const uint8_t *ab) {
 #pragma unroll
         for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+#pragma unroll
+        for (int mb = 0; mb < 5; mb++)
+            a1[mb] = *(const v4i *)(ab + aoff + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
     };
     read_a0(abuf);
     build();
@@ -344,7 +347,6 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         // products sharing a mid accumulator are 24 MFMAs apart: a dependent MFMA waits for its
         // accumulator in the pipe, and 16 x 16 x 64 products need about 16 in between to run at
         // full rate.
-        v4i a1[5];
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
 #pragma unroll
@@ -355,9 +357,6 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], acc[mb][nb][1], 0, 0, 0);
-#pragma unroll
-        for (int mb = 0; mb < 5; mb++)
-            a1[mb] = *(const v4i *)(ab + aoff + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
         {
             const v4i ax = *(const v4i *)(ab + xoff) & xmask;
 #pragma unroll
