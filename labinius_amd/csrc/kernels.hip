@@ -343,6 +343,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     for (uint32_t kb = 0; kb < kbs; kb++) {
         if (wgrp) __syncthreads();
         const uint8_t *ab = abuf + (kb % 3) * A_LDS;
+        // the wave issuing products goes first at the SIMD's arbiter, the other wave's lookups
+        // fill in around them (the wave's own priority; the stream's stays the default)
+        __builtin_amdgcn_s_setprio(1);
         // The digit classes in turn over all ten (row block, input block) pairs, so that the two
         // products sharing a mid accumulator are 24 MFMAs apart: a dependent MFMA waits for its
         // accumulator in the pipe, and 16 x 16 x 64 products need about 16 in between to run at
@@ -375,6 +378,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
+        __builtin_amdgcn_s_setprio(0);
         store_a((kb + 1 + wgrp) % 3);
         load_a(min(kb + 2 + wgrp, kbs - 1));
         if (!wgrp) __syncthreads();
