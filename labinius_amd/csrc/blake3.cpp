@@ -9,7 +9,9 @@
 #include "blake3.hpp"
 
 #include <cstring>
+#include <algorithm>
 #include <array>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -114,25 +116,47 @@ void subtree_cv(const uint32_t key[8], uint32_t flags, const uint8_t *p, uint64_
     parent_cv(key, flags, l, r, out);
 }
 
-// The same over up to 16 threads: 16 (or fewer) equal subtrees, then their parents.
-void subtree_cv_threads(const uint32_t key[8], uint32_t flags, const uint8_t *p, uint64_t counter, uint64_t n,
-                        uint32_t out[8]) {
-    uint64_t parts = 16;
-    while (parts > n) parts /= 2;
-    const uint64_t per = n / parts;
-    std::vector<std::array<uint32_t, 8>> cvs(parts);
-    std::vector<std::thread> th;
-    for (uint64_t t = 1; t < parts; t++)
-        th.emplace_back([&, t] { subtree_cv(key, flags, p + t * per * 1024, counter + t * per, per, cvs[t].data()); });
-    subtree_cv(key, flags, p, counter, per, cvs[0].data());
-    for (auto &x : th) x.join();
-    for (uint64_t w = parts; w > 1; w /= 2)
-        for (uint64_t i = 0; i < w / 2; i++) parent_cv(key, flags, cvs[2 * i].data(), cvs[2 * i + 1].data(), cvs[i].data());
-    std::memcpy(out, cvs[0].data(), 32);
-}
+constexpr uint64_t SUBTREE_MIN = 16;         // chunks hashed as one subtree when the input allows
+constexpr size_t PARALLEL_MIN = 1 << 20;     // bytes of one update from which subtrees go to threads
+constexpr uint64_t LEAF = 32;                // chunks of one thread's task
 
-constexpr uint64_t SUBTREE_MIN = 16;    // chunks hashed as one subtree when the input allows
-constexpr uint64_t THREADS_MIN = 1024;  // chunks (1 MiB) from which the subtree uses threads
+struct Sub {
+    const uint8_t *p;
+    uint64_t counter, n;
+};
+
+// The chaining values of complete subtrees, all of them at once: each is split into leaves of at
+// most LEAF chunks (both are powers of two), the leaves are hashed on up to 16 threads, and each
+// subtree's leaves are merged pairwise as the tree does.
+void subtree_cvs(const uint32_t key[8], uint32_t flags, const std::vector<Sub> &subs,
+                 std::vector<std::array<uint32_t, 8>> &out) {
+    std::vector<Sub> leaves;
+    std::vector<size_t> first(subs.size());
+    for (size_t i = 0; i < subs.size(); i++) {
+        const uint64_t per = std::min(subs[i].n, LEAF);
+        first[i] = leaves.size();
+        for (uint64_t o = 0; o < subs[i].n; o += per) leaves.push_back({subs[i].p + o * 1024, subs[i].counter + o, per});
+    }
+    std::vector<std::array<uint32_t, 8>> cv(leaves.size());
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t k; (k = next.fetch_add(1)) < leaves.size();)
+            subtree_cv(key, flags, leaves[k].p, leaves[k].counter, leaves[k].n, cv[k].data());
+    };
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), leaves.size());
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+    out.resize(subs.size());
+    for (size_t i = 0; i < subs.size(); i++) {
+        std::array<uint32_t, 8> *c = cv.data() + first[i];
+        for (uint64_t w = subs[i].n / std::min(subs[i].n, LEAF); w > 1; w /= 2)
+            for (uint64_t j = 0; j < w / 2; j++) parent_cv(key, flags, c[2 * j].data(), c[2 * j + 1].data(), c[j].data());
+        out[i] = c[0];
+    }
+}
 
 }  // namespace
 
@@ -203,8 +227,31 @@ void Blake3::close_chunk() {
     blocks_ = 0;
 }
 
+// A large update runs its loop twice. The first run, on a copy, only records the complete
+// subtrees the loop takes; their chaining values are then computed together on threads; the
+// second run is the real one and takes each subtree's value from that list. The byte-wise work
+// (chunks before the input reaches an aligned subtree, and the tail) stays sequential.
+struct Blake3::Plan {
+    bool record = true;
+    std::vector<Sub> subs;
+    std::vector<std::array<uint32_t, 8>> cvs;
+    size_t next = 0;
+};
+
 void Blake3::update(const void *data, size_t len) {
     const uint8_t *p = (const uint8_t *)data;
+    if (len < PARALLEL_MIN) return update_loop(p, len, nullptr);
+    Plan plan;
+    {
+        Blake3 dry = *this;
+        dry.update_loop(p, len, &plan);
+    }
+    subtree_cvs(key_, flags_, plan.subs, plan.cvs);
+    plan.record = false;
+    update_loop(p, len, &plan);
+}
+
+void Blake3::update_loop(const uint8_t *p, size_t len, Plan *plan) {
     while (len) {
         if (buf_len_ == 64) {
             // more input follows the buffered block, so it is not the chunk's final one
@@ -226,9 +273,11 @@ void Blake3::update(const void *data, size_t len) {
             while (n * 2 * 1024 < len) n *= 2;
             while (n > 1 && (chunk_ & (n - 1))) n /= 2;
             if (n >= SUBTREE_MIN) {
-                uint32_t cv[8];
-                if (n >= THREADS_MIN)
-                    subtree_cv_threads(key_, flags_, p, chunk_, n, cv);
+                uint32_t cv[8] = {};
+                if (plan && plan->record)
+                    plan->subs.push_back({p, chunk_, n});  // a placeholder value on the dry run
+                else if (plan)
+                    std::memcpy(cv, plan->cvs[plan->next++].data(), sizeof cv);
                 else
                     subtree_cv(key_, flags_, p, chunk_, n, cv);
                 int level = 0;

@@ -31,7 +31,9 @@ class Blake3 {
     Output finalize() const;
 
    private:
+    struct Plan;
     Blake3(const uint32_t key[8], uint32_t flags);
+    void update_loop(const uint8_t *p, size_t len, Plan *plan);
     void close_chunk();
     void push_subtree(const uint32_t cv[8], int level);
     uint32_t key_[8];
