@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -146,7 +147,13 @@ void subtree_cvs(const uint32_t key[8], uint32_t flags, const std::vector<Sub> &
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), leaves.size());
     std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; t++) th.emplace_back(work);
+    for (size_t t = 1; t < nt; t++) {
+        try {
+            th.emplace_back(work);
+        } catch (const std::system_error &) {
+            break;  // no more threads: the ones running, and this one, take the rest of the leaves
+        }
+    }
     work();
     for (auto &x : th) x.join();
     out.resize(subs.size());

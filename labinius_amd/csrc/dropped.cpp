@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -188,7 +189,13 @@ int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, 
         work(0, s.r);
     } else {
         std::vector<std::thread> th;
-        for (size_t t = 0; t < nt; t++) th.emplace_back(work, s.r * t / nt, s.r * (t + 1) / nt);
+        for (size_t t = 0; t < nt; t++) {
+            try {
+                th.emplace_back(work, s.r * t / nt, s.r * (t + 1) / nt);
+            } catch (const std::system_error &) {
+                work(s.r * t / nt, s.r * (t + 1) / nt);  // no thread to be had: this range here
+            }
+        }
         for (auto &x : th) x.join();
     }
     return LBN_OK;

@@ -3,6 +3,7 @@
 // CPU, as in the reference; only commit / fold / row evaluation run on the GPU.
 #include <cstring>
 
+#include <system_error>
 #include <thread>
 
 #include "internal.hpp"
@@ -221,7 +222,13 @@ void sample_short_challenges(Transcript &t, size_t n, uint32_t weight, double bo
         return;
     }
     std::vector<std::thread> th;
-    for (size_t k = 1; k < nt; k++) th.emplace_back(work, n * k / nt, n * (k + 1) / nt);
+    for (size_t k = 1; k < nt; k++) {
+        try {
+            th.emplace_back(work, n * k / nt, n * (k + 1) / nt);
+        } catch (const std::system_error &) {
+            work(n * k / nt, n * (k + 1) / nt);  // no thread to be had: this range here
+        }
+    }
     work(0, n / nt);
     for (auto &x : th) x.join();
 }
