@@ -601,6 +601,11 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
 // reduction x^162 = x^81 + 1 (the same two-step fold as the host's f162_mul).
 // =============================================================================================
 
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m, 64), hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
 __device__ void f162_mul_dev(const uint64_t *a, const uint64_t *b, uint64_t *out) {
     uint64_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
     const uint64_t a0 = a[0], a1 = a[1], a2 = a[2], b0 = b[0], b1 = b[1], b2 = b[2];
@@ -626,11 +631,54 @@ __device__ void f162_mul_dev(const uint64_t *a, const uint64_t *b, uint64_t *out
     out[2] = (p2 & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
 }
 
+// a b over F162 by P consecutive lanes (P a power of two dividing 64): lane sub = lane % P takes the
+// bits j of a with j / (64 / P) == sub in every word, the partial products are XOR-summed across
+// the P lanes, and every lane gets the reduced product. The comb's dependency chain is P times
+// shorter than f162_mul_dev's.
+template <int P>
+__device__ void f162_mul_lanes(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    constexpr int J = 64 / P;
+    const int j0 = (int)(threadIdx.x % P) * J;
+    uint64_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
+    const uint64_t a0 = a[0], a1 = a[1], a2 = a[2], b0 = b[0], b1 = b[1], b2 = b[2];
+    for (int jj = 0; jj < J; jj++) {
+        const int j = j0 + jj;
+        const uint64_t s0 = b0 << j;
+        const uint64_t s1 = (b1 << j) | (j ? b0 >> (64 - j) : 0);
+        const uint64_t s2 = (b2 << j) | (j ? b1 >> (64 - j) : 0);
+        const uint64_t s3 = j ? b2 >> (64 - j) : 0;
+        const uint64_t m0 = 0ull - ((a0 >> j) & 1), m1 = 0ull - ((a1 >> j) & 1), m2 = 0ull - ((a2 >> j) & 1);
+        p0 ^= s0 & m0;
+        p1 ^= (s1 & m0) ^ (s0 & m1);
+        p2 ^= (s2 & m0) ^ (s1 & m1) ^ (s0 & m2);
+        p3 ^= (s3 & m0) ^ (s2 & m1) ^ (s1 & m2);
+        p4 ^= (s3 & m1) ^ (s2 & m2);
+        p5 ^= s3 & m2;
+    }
+#pragma unroll
+    for (int m = 1; m < P; m <<= 1) {
+        p0 ^= shfl_xor_u64(p0, m);
+        p1 ^= shfl_xor_u64(p1, m);
+        p2 ^= shfl_xor_u64(p2, m);
+        p3 ^= shfl_xor_u64(p3, m);
+        p4 ^= shfl_xor_u64(p4, m);
+        p5 ^= shfl_xor_u64(p5, m);
+    }
+    const uint64_t H0 = (p2 >> 34) | (p3 << 30), H1 = (p3 >> 34) | (p4 << 30);
+    const uint64_t H2w = (p4 >> 34) | (p5 << 30);
+    const uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+    const uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
+    out[0] = p0 ^ H0 ^ G0;
+    out[1] = p1 ^ H1 ^ G1 ^ (S0 << 17);
+    out[2] = (p2 & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
+}
+
 // eq(p0)[b] = prod_k (bit k of b ? p_k : 1 + p_k) (eval.rs:51-63), as a product of sub-tables over
 // groups of 8 coordinates: part[c][x] for the coordinates 8c .. 8c + 7 (x < 256), built as a
-// product tree of depth 3 (pairs of coordinates, quads, then the 256 entries).
-__global__ __launch_bounds__(256) void eq_part_kernel(const uint64_t *__restrict__ p0, uint32_t rows,
-                                                      uint64_t *__restrict__ part) {
+// product tree of depth 3 (pairs of coordinates, quads, then the 256 entries), each product on a
+// group of lanes: 8 for the 16 pairs and the 32 quads, 4 for the 256 entries (1024 threads).
+__global__ __launch_bounds__(1024) void eq_part_kernel(const uint64_t *__restrict__ p0, uint32_t rows,
+                                                       uint64_t *__restrict__ part) {
     __shared__ uint64_t pr[4][4][3];   // pair j (coordinates 2j, 2j + 1), 2 bits
     __shared__ uint64_t qd[2][16][3];  // quad j (coordinates 4j .. 4j + 3), 4 bits
     const uint32_t c = blockIdx.x, x = threadIdx.x;
@@ -646,41 +694,49 @@ __global__ __launch_bounds__(256) void eq_part_kernel(const uint64_t *__restrict
         f[1] = p0[3 * (k0 + k) + 1];
         f[2] = p0[3 * (k0 + k) + 2];
     };
-    if (x < 16) {
-        const uint32_t j = x >> 2, v = x & 3;
-        uint64_t f[3], g2[3];
+    if (x < 16 * 8) {  // whole waves: 8 lanes a product
+        const uint32_t m = x / 8, j = m >> 2, v = m & 3;
+        uint64_t f[3], g2[3], r[3];
         factor(2 * j, v & 1, f);
         factor(2 * j + 1, v >> 1, g2);
-        f162_mul_dev(f, g2, pr[j][v]);
+        f162_mul_lanes<8>(f, g2, r);
+        if (x % 8 == 0) pr[j][v][0] = r[0], pr[j][v][1] = r[1], pr[j][v][2] = r[2];
     }
     __syncthreads();
-    if (x < 32) {
-        const uint32_t j = x >> 4, v = x & 15;
-        f162_mul_dev(pr[2 * j][v & 3], pr[2 * j + 1][v >> 2], qd[j][v]);
+    if (x < 32 * 8) {
+        const uint32_t m = x / 8, j = m >> 4, v = m & 15;
+        uint64_t r[3];
+        f162_mul_lanes<8>(pr[2 * j][v & 3], pr[2 * j + 1][v >> 2], r);
+        if (x % 8 == 0) qd[j][v][0] = r[0], qd[j][v][1] = r[1], qd[j][v][2] = r[2];
     }
     __syncthreads();
-    if (x >= (1u << nk)) return;
+    const uint32_t e = x / 4;  // entry < 256; its 4 lanes stay together, so each wave is whole
     uint64_t acc[3];
-    f162_mul_dev(qd[0][x & 15], qd[1][x >> 4], acc);
-    uint64_t *o = part + 3 * ((size_t)c * 256 + x);
+    f162_mul_lanes<4>(qd[0][e & 15], qd[1][e >> 4], acc);
+    if (x % 4 != 0 || e >= (1u << nk)) return;
+    uint64_t *o = part + 3 * ((size_t)c * 256 + e);
     o[0] = acc[0];
     o[1] = acc[1];
     o[2] = acc[2];
 }
 
+// entry b on 4 lanes (EQ_LANES): the product of its sub-table entries
+constexpr int EQ_LANES = 4;
 __global__ void eq_kernel(const uint64_t *__restrict__ part, uint32_t rows, uint64_t *__restrict__ eq) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= (1u << rows)) return;
-    const uint64_t *f = part + 3 * (b & 255);
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x, b = id / EQ_LANES;
+    const bool live = b < (1u << rows);  // the grid is whole waves; lanes past the end compute entry 0
+    const uint32_t bb = live ? b : 0;
+    const uint64_t *f = part + 3 * (bb & 255);
     uint64_t acc[3] = {f[0], f[1], f[2]};
     for (uint32_t c = 1; 8 * c < rows; c++) {
-        const uint64_t *g2 = part + 3 * ((size_t)c * 256 + ((b >> (8 * c)) & 255));
+        const uint64_t *g2 = part + 3 * ((size_t)c * 256 + ((bb >> (8 * c)) & 255));
         uint64_t t[3];
-        f162_mul_dev(acc, g2, t);
+        f162_mul_lanes<EQ_LANES>(acc, g2, t);
         acc[0] = t[0];
         acc[1] = t[1];
         acc[2] = t[2];
     }
+    if (!live || id % EQ_LANES) return;
     eq[3 * (size_t)b] = acc[0];
     eq[3 * (size_t)b + 1] = acc[1];
     eq[3 * (size_t)b + 2] = acc[2];
@@ -707,11 +763,6 @@ __device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
     t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
     x ^= t ^ (t << 28);
     return x;
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-    const int lo = __shfl_xor((int)(uint32_t)v, m, 64), hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
 // t = sum_j eq1_j u_j over F162 (eval.rs:185-193): a product per thread, XOR-reduced through the
@@ -1762,8 +1813,8 @@ hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pi
 
 hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *part, uint64_t *eq) {
     const uint32_t n = 1u << rows;
-    LBN_LAUNCH(eq_part_kernel, dim3((rows + 7) / 8), dim3(256), 0, s, p0, rows, part);
-    LBN_LAUNCH(eq_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, rows, eq);
+    LBN_LAUNCH(eq_part_kernel, dim3((rows + 7) / 8), dim3(1024), 0, s, p0, rows, part);
+    LBN_LAUNCH(eq_kernel, dim3(((uint64_t)n * EQ_LANES + 255) / 256), dim3(256), 0, s, part, rows, eq);
     return hipSuccess;
 }
 
