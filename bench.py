@@ -132,50 +132,73 @@ def cpu_model():
     return platform.machine()
 
 
-def cpu_baseline(params, folded=None):
-    """The oracle (a single-threaded plain-C restatement of the reference algorithm) on a sample of
-    the same workload, scaled to the whole step: commit and row evaluation of a few columns (each
-    column is independent), and the fold of a few output ring elements (each depends on every
-    column at its own index)."""
+def cpu_baseline(params, w, commitment, row, claimed, point, challenges, folded, reps=3):
+    """The oracle (a single-threaded, scalar plain-C restatement of the reference algorithm, not the
+    reference's AVX-512 code) on a bounded sample of the same workload, scaled to the whole step:
+    the commit and the row evaluation of the witness's first columns (each column is independent)
+    and the fold of a few output ring elements (each depends on every column at its own index),
+    each the median of `reps` runs on one pinned core. The sample's outputs are also compared with
+    the GPU's, so that the step's result is checked bit for bit where it is measured: raw rows and
+    components of the sampled columns, their row values, the MLE claim from all row values, and
+    the sampled folded ring elements. The reference's own single-core figures (other hardware,
+    from its source comments) are quoted beside it."""
     import numpy as np
     import oracle as orc
+    affinity = sorted(os.sched_getaffinity(0))
+    core = affinity[0]
     try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        os.sched_setaffinity(0, {core})
     except Exception:
-        pass
-    cols, L = params.columns(), params.len_ring()
-    wdim = params.column_len()
-    ncols = 32
-    w = orc.witness_random(WITNESS_SEED, ncols * wdim)
-    a = orc.key_expand(params.primes(), L, orc.matrix_seed(MATRIX_SEED))
-    t0 = time.perf_counter()
-    orc.commit(params.primes(), L, ncols, a, w)
-    t_commit = (time.perf_counter() - t0) / ncols * cols
-    rng = np.random.default_rng(0)
-    p0 = rng.integers(0, 2**63, (params.row_log_len(), 3), dtype=np.uint64)
-    t0 = time.perf_counter()
-    orc.row_evaluate(w, p0)
-    t_eval = (time.perf_counter() - t0) / ncols * cols
-    # fold: output ring element i needs ring element i of every column; the sample's columns hold
-    # just the first nrows ring elements (same per-row work, smaller buffer)
-    nrows = 256
-    wf = orc.witness_random(bytes([0x3C]) * 32, cols * 4 * nrows)
-    tr = orc.Transcript(b"labinius/baseline")
-    ch = [tr.sample_short_challenge()[0] for _ in range(cols)]
-    t0 = time.perf_counter()
-    orc.fold_rows(params.primes()[0], nrows, cols, wf, ch, np.arange(nrows))
-    t_fold = (time.perf_counter() - t0) / nrows * L
+        core = None
+
+    def med(f):
+        ts, out = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = f()
+            ts.append(time.perf_counter() - t0)
+        return out, float(np.median(ts))
+
+    primes, cols, L, wdim = params.primes(), params.columns(), params.len_ring(), params.column_len()
+    ncols, nrows = 32, 128
+    a = orc.key_expand(primes, L, orc.matrix_seed(MATRIX_SEED))
+    ws = w.elements[:ncols * wdim]
+    (raw, comp), t_c = med(lambda: orc.commit(primes, L, ncols, a, ws))
+    u, t_e = med(lambda: orc.row_evaluate(ws, point.p0))
+    rows = np.arange(0, L, L // nrows)[:nrows]
+    fv, t_f = med(lambda: orc.fold_rows(primes[0], L, cols, w.elements, challenges.challenges, rows))
+    checked = {
+        "columns": f"0..{ncols - 1} of {cols}: raw rows, components and row values",
+        "fold_rows": f"{nrows} of {L} folded ring elements (every {L // nrows}th)",
+        "claim": "from all row values",
+        "raw_and_components": bool(np.array_equal(commitment.raw[:, :ncols], raw)
+                                   and np.array_equal(commitment.components[:ncols], comp)),
+        "row_values": bool(np.array_equal(row.values[:ncols], u)),
+        "claim_equal": bool(np.array_equal(claimed, orc.claim(row.values, point.p1))),
+        "folded": bool(np.array_equal(folded.elements[rows], fv)),
+    }
+    if not all(checked[k] for k in ("raw_and_components", "row_values", "claim_equal", "folded")):
+        raise AssertionError(f"the GPU's step differs from the oracle: {checked}")
+    t_commit, t_eval, t_fold = t_c / ncols * cols, t_e / ncols * cols, t_f / nrows * L
     total_ms = 1e3 * (t_commit + t_eval + t_fold)
-    extra = {}
-    if folded is not None:  # the wire encode of the step's folded witness (rans.rs:18-99), whole
-        t0 = time.perf_counter()
-        orc.rans_encode(folded, params.primes()[0])
-        extra["wire_encode_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
-    return {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "port", **extra,
-            "sample": f"oracle (plain C, one pinned thread) on {ncols} of {cols} columns for commit and row "
-                      f"evaluation and {nrows} of {L} output ring elements for the fold, scaled linearly; "
-                      f"commit {1e3 * t_commit:.0f} ms, eval {1e3 * t_eval:.0f} ms, fold {1e3 * t_fold:.0f} ms; "
-                      f"host {cpu_model()}"}
+    t0 = time.perf_counter()
+    orc.rans_encode(folded.elements, primes[0])
+    encode_ms = 1e3 * (time.perf_counter() - t0)
+    return {"value": round(total_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "wire_encode_ms": round(encode_ms, 2),
+            "stages_ms": {"commit": round(1e3 * t_commit, 1), "eval": round(1e3 * t_eval, 1),
+                          "fold": round(1e3 * t_fold, 1)},
+            "sample": f"the oracle (scalar plain C, not the reference's AVX-512 path) on one pinned core "
+                      f"(core {core} of {len(affinity)} in this process's affinity, os.cpu_count() "
+                      f"{os.cpu_count()}, {cpu_model()}): commit and row evaluation of {ncols} of {cols} "
+                      f"columns and the fold of {nrows} of {L} output ring elements, each the median of "
+                      f"{reps} runs, scaled linearly to the whole step",
+            "parity_checked": checked,
+            "reference_published": {
+                "hardware": "Intel i7-11850H, one core (the reference's own source comments; other hardware)",
+                "suite_l_commit_ms": 295.9, "suite_l_fold_ms": 68.1,
+                "source": "crates/pcs/src/simd/commit.rs:100-101, crates/pcs/src/fold.rs:25-29",
+                "note": "suite l is 2^22 F162 in 512 columns; xl has 4x the witness"}}
 
 
 def wire_bench(lbn, prover, params, folded, reps=5):
@@ -477,9 +500,32 @@ def main():
     # the same step with the outputs copied back (PCIe included)
     t0 = time.perf_counter()
     c2, o2 = prover.commit_resident(rw)
-    prover.mle_evaluate(rw, point)
+    claimed = prover.mle_evaluate(rw, point)
     folded = prover.fold(o2, challenges)
     d2h_ms = 1e3 * (time.perf_counter() - t0)
+
+    # what a drop-in Prover::commit(&Witness) pays on top: W over PCIe from pageable host memory
+    # (scheme/prover.rs:55-91 takes a host witness); the upload alone, and lbn_commit of the host
+    # witness (upload + commit, outputs left in HBM), each the median of 3
+    e_host = np.ascontiguousarray(w.elements, np.uint64)
+    h2d, host_commit = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        tmp = prover.upload(w)
+        h2d.append(1e3 * (time.perf_counter() - t0))
+        del tmp
+        op = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        lbn._check(L.lbn_commit(hp, e_host.ctypes.data_as(vp), ctypes.c_size_t(len(e_host)), None, None,
+                                ctypes.byref(op)))
+        host_commit.append(1e3 * (time.perf_counter() - t0))
+        lbn._check(L.lbn_recycle(hp, op))
+    host_witness = {"h2d_ms": round(float(np.median(h2d)), 3), "bytes": int(e_host.nbytes),
+                    "h2d_gb_per_s": round(e_host.nbytes / (float(np.median(h2d)) * 1e-3) / 1e9, 2),
+                    "host_witness_commit_ms": round(float(np.median(host_commit)), 3),
+                    "resident_commit_ms": round(stage["commit"], 3),
+                    "note": "lbn_commit of a host witness (Prover::commit(&Witness)): the upload from pageable "
+                            "host memory plus the commit; never the bench's value, which has W resident"}
 
     gemm_ms = stage["commit_gemm"]
     ops = tile_ops(params)
@@ -507,6 +553,7 @@ def main():
                    "moduli": params.primes()},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
         "d2h_inclusive_ms": round(d2h_ms, 3),
+        "host_witness": host_witness,
         "hbm_floor_ms": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3, 4),
         "hbm_roofline_frac": round(ab["total"] / (PEAK_HBM_GBS * 1e9) * 1e3 / ms, 4),
         "stage_rooflines": sr,
@@ -539,7 +586,7 @@ def main():
         line["switch"] = switch_bench(lbn)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
-        line["cpu_baseline"] = cpu_baseline(params, folded.elements)
+        line["cpu_baseline"] = cpu_baseline(params, w, c2, row, claimed, point, challenges, folded)
     print(json.dumps(line), flush=True)
 
 

@@ -34,6 +34,16 @@ int lbn_debug_gemm_chunk(uint32_t kb);
  * (random_operands = 0) or on random ones, which draw more power and clock lower. */
 int lbn_debug_mfma_rate(int random_operands, double *tops);
 
+/* One v_mfma_f32_16x16x128_f8f6f4 with FP4 (e2m1) operands, given as each lane's operand registers:
+ * a and b are 64 lanes x 16 bytes, d receives 64 lanes x 4 floats (the result registers); scaled
+ * selects E8M0 scales 127 (2^0) instead of 0. No counterpart in the reference: it pins the lane
+ * layouts the row evaluation's FP4 products assume. */
+int lbn_debug_mfma_f4_probe(const uint8_t *a, const uint8_t *b, float *d, int scaled);
+
+/* Measurement, no counterpart in the reference: the device's FP4 MFMA rate in TFLOP/s
+ * (v_mfma_f32_16x16x128_f8f6f4, 16 independent chains a wave, all CUs, 0 / 0.5 operands). */
+int lbn_debug_mfma_f4_rate(int random_operands, double *tflops);
+
 #ifdef __cplusplus
 }
 #endif

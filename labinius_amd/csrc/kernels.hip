@@ -19,6 +19,8 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 namespace lbn {
@@ -1286,6 +1288,59 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(int random, uint32_t ite
     sink[t] = s;
 }
 
+// One v_mfma_f32_16x16x128_f8f6f4 with both operands in FP4 (e2m1, format code 4): lane l's A and
+// B operand registers are the 16 bytes at A + 16 l and B + 16 l (its 32 nibbles, low nibble of
+// byte 0 first), and its 4 result registers go to D + 4 l. scaled = 0 passes E8M0 scale 0 to both
+// sides (the unscaled encoding), scaled = 1 passes 127 (2^0).
+__global__ void mfma_f4_probe_kernel(const uint32_t *__restrict__ A, const uint32_t *__restrict__ B,
+                                     float *__restrict__ D, int scaled) {
+    const int lane = threadIdx.x;
+    v8i a = {}, b = {};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        a[j] = (int)A[4 * lane + j];
+        b[j] = (int)B[4 * lane + j];
+    }
+    v4f acc = {};
+    if (scaled)
+        acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 4, 4, 0, 127, 0, 127);
+    else
+        acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 4, 4, 0, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) D[4 * lane + r] = acc[r];
+}
+
+// The FP4 MFMA rate: 16 independent v_mfma_f32_16x16x128_f8f6f4 chains a wave on operands of
+// nibbles 0 / 0.5 (a 0-1 pattern: constant, or random per lane and alternating between two sets).
+__global__ __launch_bounds__(256) void mfma_f4_rate_kernel(int random, uint32_t iters, float *__restrict__ sink) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    v8i a0 = {0x11111111, 0x11111111, 0x11111111, 0x11111111, 0, 0, 0, 0}, b0 = a0, a1 = a0, b1 = a0;
+    if (random) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            a0[j] = rate_hash(16 * t + j) & 0x11111111;
+            b0[j] = rate_hash(16 * t + 4 + j) & 0x11111111;
+            a1[j] = rate_hash(16 * t + 8 + j) & 0x11111111;
+            b1[j] = rate_hash(16 * t + 12 + j) & 0x11111111;
+        }
+    }
+    v4f acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = (v4f){};
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            acc[i] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4((i & 1) ? a1 : a0, (i & 1) ? b1 : b0, acc[i], 4,
+                                                                      4, 0, 0, 0, 0);
+    }
+    float s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) s += acc[i][r];
+    sink[t] = s;
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id < n) p[id] = v;
@@ -1697,6 +1752,16 @@ hipError_t launch_mfma_rate(hipStream_t s, int random, uint32_t blocks, uint32_t
 
 hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D) {
     LBN_LAUNCH(mfma_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D);
+    return hipSuccess;
+}
+
+hipError_t launch_mfma_f4_probe(hipStream_t s, const uint32_t *A, const uint32_t *B, float *D, int scaled) {
+    LBN_LAUNCH(mfma_f4_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D, scaled);
+    return hipSuccess;
+}
+
+hipError_t launch_mfma_f4_rate(hipStream_t s, int random, uint32_t blocks, uint32_t iters, float *sink) {
+    LBN_LAUNCH(mfma_f4_rate_kernel, dim3(blocks), dim3(256), 0, s, random, iters, sink);
     return hipSuccess;
 }
 

@@ -837,6 +837,52 @@ int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d) {
     return rc;
 }
 
+// Test-only (include/labinius_debug.h): one FP4 MFMA on raw lane registers.
+int lbn_debug_mfma_f4_probe(const uint8_t *a, const uint8_t *b, float *d, int scaled) {
+    if (!a || !b || !d) return fail(LBN_ERR_ARG, "null argument");
+    uint32_t *da = nullptr, *db = nullptr;
+    float *dd = nullptr;
+    int rc = LBN_OK;
+    if (hipMalloc((void **)&da, 1024) != hipSuccess || hipMalloc((void **)&db, 1024) != hipSuccess ||
+        hipMalloc((void **)&dd, 1024) != hipSuccess) {
+        rc = fail(LBN_ERR_OOM, "probe buffers");
+    } else if (hipMemcpy(da, a, 1024, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(db, b, 1024, hipMemcpyHostToDevice) != hipSuccess ||
+               launch_mfma_f4_probe(nullptr, da, db, dd, scaled) != hipSuccess ||
+               hipMemcpy(d, dd, 1024, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = fail(LBN_ERR_HIP, "probe launch");
+    }
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dd) (void)hipFree(dd);
+    return rc;
+}
+
+// Measurement (include/labinius_debug.h): the FP4 MFMA rate in TFLOP/s.
+int lbn_debug_mfma_f4_rate(int random_operands, double *tflops) {
+    if (!tflops) return fail(LBN_ERR_ARG, "null argument");
+    const uint32_t blocks = 1024, iters = 4000;
+    float *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LBN_OK;
+    float ms = 0;
+    if (hipMalloc((void **)&sink, (size_t)blocks * 256 * sizeof(float)) != hipSuccess) return fail(LBN_ERR_OOM, "rate sink");
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        launch_mfma_f4_rate(nullptr, random_operands, blocks, 16, sink) != hipSuccess ||
+        hipEventRecord(e0, nullptr) != hipSuccess ||
+        launch_mfma_f4_rate(nullptr, random_operands, blocks, iters, sink) != hipSuccess ||
+        hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(LBN_ERR_HIP, "rate launch");
+    } else {
+        *tflops = (double)blocks * 4 * iters * 16 * (16.0 * 16 * 128 * 2) / (ms * 1e-3) / 1e12;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(sink);
+    return rc;
+}
+
 // Test-only (include/labinius_debug.h): the GEMM's K chunk in steps, 0 for the exact-range default.
 int lbn_debug_gemm_chunk(uint32_t kb) {
     if (kb > KB_EXACT) return fail(LBN_ERR_ARG, "a K chunk beyond the accumulators' exact range");
