@@ -147,6 +147,16 @@ int lbn_claim(const lbn_f162 *u, size_t r, const lbn_f162 *p1, size_t p1_len, lb
 int lbn_mle_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *p0, size_t p0_len,
                      const lbn_f162 *p1, size_t p1_len, lbn_f162 *t);
 
+/* No counterpart in the reference: lbn_row_evaluate for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It needs
+ * no prover and allocates nothing: d_w (n F162), d_p0 (p0_len F162) and d_u (n / 2^p0_len F162,
+ * overwritten) are device pointers, the work is queued on `stream` (a hipStream_t, NULL for the
+ * default stream) and the call returns without waiting. The eq tensor lives in a static device
+ * array of 0.8 MB per device, so p0_len is at most 15 (LBN_ERR_SHAPE above); calls that overlap
+ * on one device share that array and race. u equals lbn_row_evaluate's. */
+int lbn_row_evaluate_noalloc(const lbn_f162 *d_w, size_t n, const lbn_f162 *d_p0, size_t p0_len, lbn_f162 *d_u,
+                             void *stream);
+
 /* Device time of each stage of the prover's last commit / fold / row evaluation, in ms, measured
  * with HIP events on the prover's stream:
  *   [0] commit total  [1] commit layout pass  [2] commit GEMM  [3] commit finish

@@ -12,7 +12,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblabinius.so")
-HOST_SOURCES = ["ring.cpp", "blake3.cpp", "protocol.cpp", "prover.cpp", "dropped.cpp", "batch.cpp", "switch.cpp", "wire.cpp", "verify.cpp"]
+HOST_SOURCES = ["ring.cpp", "blake3.cpp", "protocol.cpp", "prover.cpp", "dropped.cpp", "batch.cpp", "switch.cpp", "wire.cpp", "verify.cpp",
+                "noalloc.cpp"]
 DEVICE_SOURCES = ["kernels.hip"]
 
 
