@@ -151,8 +151,8 @@ int lbn_mle_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It needs
  * no prover and allocates nothing: d_w (n F162), d_p0 (p0_len F162) and d_u (n / 2^p0_len F162,
  * overwritten) are device pointers, the work is queued on `stream` (a hipStream_t, NULL for the
- * default stream) and the call returns without waiting. The eq tensor lives in a static device
- * array of 0.8 MB per device, so p0_len is at most 15 (LBN_ERR_SHAPE above); calls that overlap
+ * default stream) and the call returns without waiting. The eq tensor and its operands live in a
+ * static device array of 3.7 MB per device, so p0_len is at most 15 (LBN_ERR_SHAPE above); calls that overlap
  * on one device share that array and race. u equals lbn_row_evaluate's. */
 int lbn_row_evaluate_noalloc(const lbn_f162 *d_w, size_t n, const lbn_f162 *d_p0, size_t p0_len, lbn_f162 *d_u,
                              void *stream);

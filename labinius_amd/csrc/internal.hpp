@@ -240,6 +240,7 @@ struct lbn_prover {
     uint32_t *d_vhat = nullptr;
     int16_t *d_v = nullptr;
     uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_eqpart = nullptr, *d_u = nullptr;
+    uint32_t *d_eqf = nullptr;  // the row evaluation's eq operands (row_eval_scratch_u32)
     uint64_t *d_p1 = nullptr, *d_eq1 = nullptr, *d_eqpart1 = nullptr, *d_t = nullptr;  // the MLE claim
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
     // the bit-dropped digits (grow-only): raw rows from the host, centered coefficients, top + digits

@@ -7,12 +7,13 @@
 //   intt_kernel           a limb's inverse transform -> centered coefficients (the folded witness)
 //   garner_kernel         the bit-dropped commitment's mixed-radix digits and top values
 //   eq_kernel             the eq tensor over F162
-//   row_eval_kernel       u_j = sum_i eq_i W[i + wdim j] over F162
+//   row_eval_f4_kernel    u_j = sum_i eq_i W[i + wdim j] over F162, a GF(2) product on FP4 MFMA
 //
 // The algebra of the tile GEMM is in device.hpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -744,29 +745,6 @@ __global__ void eq_kernel(const uint64_t *__restrict__ part, uint32_t rows, uint
     eq[3 * (size_t)b + 2] = acc[2];
 }
 
-// u_j = sum_i eq_i W[i + wdim j] (eval.rs:171-182) by bit positions: with S_b the XOR of eq_i over
-// the i whose element has bit b set, u_j = sum_b x^b S_b mod x^162 + x^81 + 1. S_b is gathered
-// eight elements at a time (the method of the four Russians): per group of 8 consecutive i a
-// 256-entry table T[m] = XOR of eq_{8g+k} over the bits k of m, shared by every column; then each
-// bit position b of a column costs one lookup per group, at the byte "bits b of the 8 elements".
-//
-// A block takes 32 columns (four per wave) and a span of chunks of 64 elements: per chunk it
-// builds the 8 tables in LDS once for all 32 columns; each wave then takes its columns two at a
-// time, transposes their chunk into per-bit-position bytes (8 x 8 bit transposes) in its own LDS
-// area and looks them up; a lane owns bit positions lane, lane + 64 and lane + 128.
-constexpr int RE_CHUNK = 64, RE_WAVES = 8, RE_CPW = 4, RE_COLS = RE_WAVES * RE_CPW;
-
-__device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
-    // byte t, bit p -> byte p, bit t
-    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
-    x ^= t ^ (t << 7);
-    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
-    x ^= t ^ (t << 14);
-    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
-    x ^= t ^ (t << 28);
-    return x;
-}
-
 // t = sum_j eq1_j u_j over F162 (eval.rs:185-193): a product per thread, XOR-reduced through the
 // wave and the block, then into t (cleared first) with atomic XORs.
 __global__ __launch_bounds__(256) void claim_kernel(const uint64_t *__restrict__ eq1, const uint64_t *__restrict__ u,
@@ -799,156 +777,237 @@ __global__ __launch_bounds__(256) void claim_kernel(const uint64_t *__restrict__
     }
 }
 
-// Visibility of a wave's own LDS writes to its other lanes, without a block barrier.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+// ---------------------------------------------------------------------------------------------
+// The row evaluation as a GF(2) product on FP4 MFMA.
+//
+// u_j = sum_i eq_i W_ij over GF(2^162) is, before the reduction by x^162 + x^81 + 1, the
+// polynomial P_j = sum_{b,k} x^{b+k} S_j[b][k] with S_j[b][k] = parity of the number of i whose
+// W_ij has bit b and eq_i has bit k. That count is an integer matrix product, bits(W_j)^T
+// bits(eq) over K = the row's elements, which v_mfma_f32_16x16x128_f8f6f4 computes exactly with
+// FP4 (e2m1) operands: a W bit is the nibble 0.5, an eq bit the nibble 2.0, so a product is 1 and
+// the f32 sums are exact integers below 2^24. Both sides are padded from 162 to 176 bits (11
+// tiles of 16). A chunk is 128 elements (one K step); per chunk the block
+//   - transposes the column's 128 x 192-bit chunk into bit planes P[b][q] (bit b of elements
+//     32 q .. 32 q + 31), by 16 x 16 bit transposes on 16-lane rows (DPP butterflies);
+//   - stages the chunk's eq operands (precomputed once per evaluation by eqfrag_kernel, the same
+//     for every column) in LDS;
+//   - multiplies: wave w owns the W tiles B = w, w + 4, w + 8 (rows 16 B .. 16 B + 15), expands
+//     each lane's plane word into its 32 operand nibbles, and runs them against the 11 eq tiles.
+// The f32 counts' parities are then summed along the anti-diagonals b + k (XOR) and reduced.
+// ---------------------------------------------------------------------------------------------
+constexpr int RF_CHUNK = 128, RF_TILES = 11, RF_BITS = 16 * RF_TILES;  // 176
+constexpr int RF_FRAG_U32 = RF_TILES * 64 * 4;                          // u32 of eq operands per chunk
+
+// eq operands: frag[c][N][lane][4 u32], nibble 8 r + j of a lane (register r, bits 4 j .. 4 j + 3)
+// is 2.0 (0x4) when bit 16 N + (lane & 15) of eq_{128 c + 32 (lane >> 4) + 8 r + j} is set.
+__global__ __launch_bounds__(256) void eqfrag_kernel(const uint64_t *__restrict__ eq, uint32_t nch,
+                                                     uint32_t *__restrict__ frag) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= nch * RF_TILES * 64) return;
+    const uint32_t lane = id & 63, nt = (id >> 6) % RF_TILES, c = id / (64 * RF_TILES);
+    const uint32_t bit = 16 * nt + (lane & 15);
+    uint32_t r4[4] = {0, 0, 0, 0};
+    if (bit < 162) {
+        const uint64_t *e = eq + 3 * ((size_t)RF_CHUNK * c + 32 * (lane >> 4)) + (bit >> 6);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) r4[r] |= (uint32_t)((e[3 * (8 * r + j)] >> (bit & 63)) & 1) << (4 * j + 2);
+    }
+    uint4 *o = (uint4 *)frag + (size_t)id;
+    *o = make_uint4(r4[0], r4[1], r4[2], r4[3]);
 }
 
-__global__ __launch_bounds__(64 * RE_WAVES) void row_eval_kernel(const uint64_t *__restrict__ w,
-                                                                const uint64_t *__restrict__ eq, uint32_t wdim,
-                                                                uint32_t cols, uint32_t span,
-                                                                uint64_t *__restrict__ u) {
-    __shared__ ulonglong2 T01[8][256];  // words 0 and 1 of each table entry
-    __shared__ uint64_t T2[8][256];     // word 2
-    __shared__ uint64_t NT[8][2][16][3];  // nibble tables: low and high four elements
-    __shared__ __attribute__((aligned(16))) uint32_t ws[RE_WAVES][2][3][2 * RE_CHUNK];  // [wave][col][word][elem, half]
-    __shared__ uint64_t by[RE_WAVES][2][8][21];  // [wave][col][group][bit position / 8]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t j0 = blockIdx.x * RE_COLS, nch = wdim / RE_CHUNK;
+// lanes l and l ^ D of each 16-lane row exchange x, by DPP: quad_perm for D = 1, 2; for D = 4 the
+// quad mirror (l ^ 3) then the half-row mirror (l ^ 7); row_ror:8 for D = 8
+template <int D>
+__device__ __forceinline__ uint32_t dpp_xor(uint32_t x) {
+    if constexpr (D == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    if constexpr (D == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    if constexpr (D == 4)
+        return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xF, 0xF, false), 0x141,
+                                                  0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);
+}
+// one stage of two 16 x 16 bit transposes (the two 16-bit halves) over a 16-lane row: lanes without
+// bit D take the partner's positions without it into theirs with it (rotated left by D), lanes
+// with bit D the partner's positions with it into theirs without it (rotated right by D); rot and
+// keep are the lane's rotation (32 - D or D) and the positions it keeps.
+template <int D>
+__device__ __forceinline__ uint32_t bt16_stage(uint32_t v, uint32_t rot, uint32_t keep) {
+    const uint32_t y = dpp_xor<D>(v);
+    return (v & keep) | (__builtin_amdgcn_alignbit(y, y, rot) & ~keep);
+}
+// a lane's (rot, keep) for stage D with positions-without-D mask m
+template <int D>
+__device__ __forceinline__ void bt16_lane(uint32_t x16, uint32_t m, uint32_t &rot, uint32_t &keep) {
+    rot = (x16 & D) ? D : 32 - D;
+    keep = (x16 & D) ? ~m : m;
+}
+
+// 8 bits -> 8 nibbles of 0.5 (0x1): bit j to bit 4 j
+__device__ __forceinline__ uint32_t spread_nibbles(uint32_t x) {
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    return (x | (x << 3)) & 0x11111111u;
+}
+
+// A block takes one column and a span of chunks, RF_G chunks a phase: the phase's W words and eq
+// operands are loaded one phase ahead into registers; each 16-lane row transposes 16 elements x
+// one 32-bit word (two 16 x 16 transposes) by DPP butterflies into the bit planes, kept in LDS as
+// P[q][b] = bits b of elements 32 q .. 32 q + 31 (two 16-bit halves). In the products, the eq
+// operands are read from LDS RF_LOOK tiles ahead of their MFMAs.
+constexpr int RF_G = 2, RF_LOOK = 4;
+__global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__restrict__ w,
+                                                             const uint4 *__restrict__ frag, uint32_t wdim,
+                                                             uint32_t cols, uint32_t span,
+                                                             unsigned long long *__restrict__ u) {
+    __shared__ uint4 ebuf[RF_G][RF_TILES * 64];
+    __shared__ uint32_t planes[RF_G][4][RF_BITS];
+    __shared__ uint64_t red[4][6];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, x16 = tid & 15;
+    const uint32_t j = blockIdx.x, nch = wdim / RF_CHUNK;
     const uint32_t ch0 = blockIdx.y * span, ch1 = min(ch0 + span, nch);
-    // this wave's column c is j0 + wave + RE_WAVES c
-    bool live[RE_CPW];
+    const uint32_t *wc = w + (size_t)j * wdim * 6;  // the column as u32 words, 6 per element
+    uint32_t r8, k8, r4, k4, r2, k2, r1, k1;
+    bt16_lane<8>(x16, 0x00FF00FFu, r8, k8);
+    bt16_lane<4>(x16, 0x0F0F0F0Fu, r4, k4);
+    bt16_lane<2>(x16, 0x33333333u, r2, k2);
+    bt16_lane<1>(x16, 0x55555555u, r1, k1);
+    v4f acc[3][RF_TILES];
 #pragma unroll
-    for (int c = 0; c < RE_CPW; c++) live[c] = j0 + wave + RE_WAVES * c < cols;
-    uint64_t S[RE_CPW][3][3];
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int a = 0; a < RE_CPW; a++)
+        for (int n = 0; n < RF_TILES; n++) acc[a][n] = (v4f){0, 0, 0, 0};
+    // per chunk, this thread's three transposition tasks (g, s) = (t / 6, t % 6), t = 16 p + tid / 16:
+    // word s of element 16 g + x16; and three of the chunk's 704 eq operands
+    uint32_t word[RF_G][3];
+    uint4 ef[RF_G][3];
+    auto fetch = [&](uint32_t ph) {
 #pragma unroll
-        for (int b = 0; b < 3; b++) S[a][b][0] = S[a][b][1] = S[a][b][2] = 0;
-
-    for (uint32_t ch = ch0; ch < ch1; ch++) {
-        const uint32_t i0 = ch * RE_CHUNK;
-        __syncthreads();  // the previous chunk's tables are no longer read
-        if (tid < 256) {
-            // nibble table (g, half h, x): XOR of eq_{8g + 4h + k} over the bits k of x
-            const int g = tid >> 5, h = (tid >> 4) & 1, x = tid & 15;
-            const uint64_t *e = eq + 3 * ((size_t)i0 + 8 * g + 4 * h);
-            uint64_t a0 = 0, a1 = 0, a2 = 0;
+        for (int c = 0; c < RF_G; c++) {
+            const uint32_t ch = ph + c;
+            const bool live = ch < ch1;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint64_t msk = 0ull - (uint64_t)((x >> k) & 1);
-                a0 ^= e[3 * k] & msk;
-                a1 ^= e[3 * k + 1] & msk;
-                a2 ^= e[3 * k + 2] & msk;
+            for (int p = 0; p < 3; p++) {
+                const uint32_t t = 16 * p + (tid >> 4), g = t / 6, sw = t % 6;
+                word[c][p] = live ? wc[((size_t)RF_CHUNK * ch + 16 * g + x16) * 6 + sw] : 0u;
+                const uint32_t f = tid + 256 * p;
+                ef[c][p] = live && f < RF_TILES * 64 ? frag[(size_t)ch * RF_TILES * 64 + f] : make_uint4(0, 0, 0, 0);
             }
-            NT[g][h][x][0] = a0;
-            NT[g][h][x][1] = a1;
-            NT[g][h][x][2] = a2;
         }
+    };
+    // the MFMAs of one phase for a wave with NB W tiles
+    auto mma = [&](auto nbc, uint32_t ph) {
+        constexpr int NB = decltype(nbc)::value;
+#pragma unroll
+        for (int c = 0; c < RF_G; c++) {
+            if (ph + c >= ch1) break;  // block-uniform
+            uint4 bq[RF_LOOK];
+#pragma unroll
+            for (int n = 0; n < RF_LOOK; n++) bq[n] = ebuf[c][n * 64 + lane];
+            v8i aop[NB];
+#pragma unroll
+            for (int a = 0; a < NB; a++) {
+                const uint32_t pw = planes[c][lane >> 4][16 * (wave + 4 * a) + (lane & 15)];
+                aop[a] = (v8i){(int)spread_nibbles(pw & 0xFFu), (int)spread_nibbles((pw >> 8) & 0xFFu),
+                               (int)spread_nibbles((pw >> 16) & 0xFFu), (int)spread_nibbles(pw >> 24), 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int n = 0; n < RF_TILES; n++) {
+                const uint4 b4 = bq[n % RF_LOOK];
+                if (n + RF_LOOK < RF_TILES) bq[n % RF_LOOK] = ebuf[c][(n + RF_LOOK) * 64 + lane];
+                const v8i bop = (v8i){(int)b4.x, (int)b4.y, (int)b4.z, (int)b4.w, 0, 0, 0, 0};
+#pragma unroll
+                for (int a = 0; a < NB; a++)
+                    acc[a][n] =
+                        __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aop[a], bop, acc[a][n], 4, 4, 0, 0, 0, 0);
+            }
+        }
+    };
+    if (ch0 < ch1) fetch(ch0);
+    for (uint32_t ph = ch0; ph < ch1; ph += RF_G) {
+        __syncthreads();  // the previous phase's planes and eq operands are no longer read
+#pragma unroll
+        for (int c = 0; c < RF_G; c++)
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                const uint32_t f = tid + 256 * p;
+                if (f < RF_TILES * 64) ebuf[c][f] = ef[c][p];
+                const uint32_t t = 16 * p + (tid >> 4), g = t / 6, sw = t % 6;
+                uint32_t v = word[c][p];
+                v = bt16_stage<8>(v, r8, k8);
+                v = bt16_stage<4>(v, r4, k4);
+                v = bt16_stage<2>(v, r2, k2);
+                v = bt16_stage<1>(v, r1, k1);
+                // low half: bit 32 s + x16 of elements 16 g .. 16 g + 15; high half: bit 32 s + 16 + x16
+                uint16_t *pl = (uint16_t *)&planes[c][g >> 1][0] + (g & 1);
+                const uint32_t b0 = 32 * sw + x16, b1 = b0 + 16;
+                pl[2 * b0] = b0 < 162 ? (uint16_t)v : (uint16_t)0;
+                if (b1 < RF_BITS) pl[2 * b1] = b1 < 162 ? (uint16_t)(v >> 16) : (uint16_t)0;
+            }
         __syncthreads();
-        for (int e = tid; e < 8 * 256; e += 64 * RE_WAVES) {
-            const int g = e >> 8, m = e & 255;
-            const uint64_t *lo = NT[g][0][m & 15], *hi = NT[g][1][m >> 4];
-            T01[g][m] = make_ulonglong2(lo[0] ^ hi[0], lo[1] ^ hi[1]);
-            T2[g][m] = lo[2] ^ hi[2];
-        }
-        __syncthreads();  // the tables
+        if (ph + RF_G < ch1) fetch(ph + RF_G);
+        if (wave == 3)
+            mma(std::integral_constant<int, 2>{}, ph);
+        else
+            mma(std::integral_constant<int, 3>{}, ph);
+    }
+    // the counts' parities along the anti-diagonals: lane l holds S[16 B + 4 (l >> 4) + r][16 N + (l & 15)]
+    const int nb = wave == 3 ? 2 : 3;
+    uint64_t pl[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int pr = 0; pr < RE_CPW / 2; pr++) {
-            wave_sync();  // the previous pair's words and bytes are no longer read
-            // both columns' chunks, word-major as dwords: ws[c][k][2 i + half] = half of word k of element i
+    for (int a = 0; a < 3; a++) {
+        if (a >= nb) continue;
+        const int B = (int)wave + 4 * a;
 #pragma unroll
-            for (int cw = 0; cw < 2; cw++)
-                if (live[2 * pr + cw]) {
-                    const uint64_t *src = w + 3 * ((size_t)(j0 + wave + RE_WAVES * (2 * pr + cw)) * wdim + i0);
-                    for (int x = lane; x < RE_CHUNK * 3; x += 64) {
-                        const uint64_t v = src[x];
-                        ws[wave][cw][x % 3][2 * (x / 3)] = (uint32_t)v;
-                        ws[wave][cw][x % 3][2 * (x / 3) + 1] = (uint32_t)(v >> 32);
-                    }
-                }
-            wave_sync();
-            // item (g, word k, byte c): bits 64 k + 8 c .. + 7 of the group's 8 elements, gathered
-            // with byte permutes: byte c of the 8 words, then an 8 x 8 bit transpose
+        for (int n = 0; n < RF_TILES; n++) {
+            uint64_t nib = 0;
 #pragma unroll
-            for (int cw = 0; cw < 2; cw++)
-                if (live[2 * pr + cw])
-                    for (int it = lane; it < 8 * 24; it += 64) {
-                        const int g = it / 24, k = (it >> 3) % 3, c = it & 7;
-                        if (8 * k + c > 20) continue;  // bit positions past 167
-                        const uint4 *q = (const uint4 *)&ws[wave][cw][k][16 * g];
-                        const uint4 d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3];
-                        // element t's dword (c < 4: low half, else high half)
-                        const bool hiw = c >= 4;
-                        const uint32_t e0 = hiw ? d0.y : d0.x, e1 = hiw ? d0.w : d0.z, e2 = hiw ? d1.y : d1.x,
-                                       e3 = hiw ? d1.w : d1.z, e4 = hiw ? d2.y : d2.x, e5 = hiw ? d2.w : d2.z,
-                                       e6 = hiw ? d3.y : d3.x, e7 = hiw ? d3.w : d3.z;
-                        const uint32_t cb = c & 3, sel2 = cb | ((cb + 4) << 8);
-                        const uint32_t x01 = __builtin_amdgcn_perm(e1, e0, sel2), x23 = __builtin_amdgcn_perm(e3, e2, sel2);
-                        const uint32_t x45 = __builtin_amdgcn_perm(e5, e4, sel2), x67 = __builtin_amdgcn_perm(e7, e6, sel2);
-                        const uint32_t lo = __builtin_amdgcn_perm(x23, x01, 0x05040100u);
-                        const uint32_t hi = __builtin_amdgcn_perm(x67, x45, 0x05040100u);
-                        by[wave][cw][g][8 * k + c] = transpose8x8(((uint64_t)hi << 32) | lo);
-                    }
-            wave_sync();  // the pair's bytes
-#pragma unroll
-            for (int cw = 0; cw < 2; cw++)
-                if (live[2 * pr + cw]) {
-                    const uint8_t *bb = (const uint8_t *)by[wave][cw];
-#pragma unroll
-                    for (int sl = 0; sl < 3; sl++) {
-                        const int b = lane + 64 * sl;
-                        if (b < 162) {
-#pragma unroll
-                            for (int g = 0; g < 8; g++) {
-                                const int m = bb[g * 168 + b];
-                                const ulonglong2 t01 = T01[g][m];
-                                S[2 * pr + cw][sl][0] ^= t01.x;
-                                S[2 * pr + cw][sl][1] ^= t01.y;
-                                S[2 * pr + cw][sl][2] ^= T2[g][m];
-                            }
-                        }
-                    }
-                }
+            for (int r = 0; r < 4; r++) nib |= (uint64_t)((uint32_t)acc[a][n][r] & 1u) << r;
+            const int pos = 16 * (B + n);  // B is wave-uniform, so the word is too
+            pl[pos >> 6] ^= nib << (pos & 63);
         }
     }
-    // u_j ^= sum_b x^b S_b mod x^162 + x^81 + 1
+    const uint32_t o = 4 * (lane >> 4) + (lane & 15);  // the lane's offset b + k - 16 (B + N) - r
+    uint64_t p[6];
 #pragma unroll
-    for (int cw = 0; cw < RE_CPW; cw++) {
-        const uint32_t j = j0 + wave + RE_WAVES * cw;
-        if (j >= cols) continue;  // wave-uniform
-        uint64_t p[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 5; q >= 0; q--) p[q] = (pl[q] << o) | (o && q ? pl[q - 1] >> (64 - o) : 0);
 #pragma unroll
-        for (int sl = 0; sl < 3; sl++) {
-            const int b = lane + 64 * sl;
-            if (b >= 162) continue;
-            // x^b S as 6 words: S lands at word sl with the shift lane (b = 64 sl + lane)
-            const uint64_t *sv = S[cw][sl];
+    for (int q = 0; q < 6; q++)
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                p[sl + q] ^= sv[q] << lane;
-                if (lane) p[sl + q + 1] ^= sv[q] >> (64 - lane);
-            }
-        }
+        for (int m = 32; m >= 1; m >>= 1) p[q] ^= shfl_xor_u64(p[q], m);
+    if (lane == 0)
 #pragma unroll
-        for (int q = 0; q < 6; q++)
+        for (int q = 0; q < 6; q++) red[wave][q] = p[q];
+    __syncthreads();
+    if (tid == 0 && ch0 < ch1) {
+        uint64_t t[6];
 #pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) p[q] ^= shfl_xor_u64(p[q], m);
-        if (lane == 0) {
-            const uint64_t H0 = (p[2] >> 34) | (p[3] << 30), H1 = (p[3] >> 34) | (p[4] << 30);
-            const uint64_t H2w = (p[4] >> 34) | (p[5] << 30);
-            const uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
-            const uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
-            const uint64_t o0 = p[0] ^ H0 ^ G0;
-            const uint64_t o1 = p[1] ^ H1 ^ G1 ^ (S0 << 17);
-            const uint64_t o2 = (p[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17);
-            atomicXor((unsigned long long *)&u[3 * j], (unsigned long long)o0);
-            atomicXor((unsigned long long *)&u[3 * j + 1], (unsigned long long)o1);
-            atomicXor((unsigned long long *)&u[3 * j + 2], (unsigned long long)o2);
-        }
+        for (int q = 0; q < 6; q++) t[q] = red[0][q] ^ red[1][q] ^ red[2][q] ^ red[3][q];
+        const uint64_t H0 = (t[2] >> 34) | (t[3] << 30), H1 = (t[3] >> 34) | (t[4] << 30);
+        const uint64_t H2w = (t[4] >> 34) | (t[5] << 30);
+        const uint64_t G0 = (H1 >> 17) | (H2w << 47), G1 = H2w >> 17;
+        const uint64_t S0 = H0 ^ G0, S1 = (H1 & ((1ull << 17) - 1)) ^ G1;
+        atomicXor(&u[3 * (size_t)j], (unsigned long long)(t[0] ^ H0 ^ G0));
+        atomicXor(&u[3 * (size_t)j + 1], (unsigned long long)(t[1] ^ H1 ^ G1 ^ (S0 << 17)));
+        atomicXor(&u[3 * (size_t)j + 2],
+                  (unsigned long long)((t[2] & ((1ull << 34) - 1)) ^ H2w ^ (S0 >> 47) ^ (S1 << 17)));
     }
 }
+
+// An 8 x 8 bit matrix in a u64 (byte t = row t), transposed.
+__device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
+    // byte t, bit p -> byte p, bit t
+    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+    x ^= t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+    x ^= t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+    x ^= t ^ (t << 28);
+    return x;
+}
+
 
 // =============================================================================================
 // the cross-field switch (switch.rs:137-164, fields/crossfield.rs:149-476, fields/sumcheck.rs:100-145)
@@ -1883,14 +1942,21 @@ hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t 
     return hipSuccess;
 }
 
-hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
-                           uint32_t cols, uint64_t *u) {
-    // the kernel's chunks are 64 elements: wdim (2^row_log_len >= 128) is a multiple. Chunks are
-    // spread over the grid's y so that there are about two blocks per CU.
-    const uint32_t nch = wdim / RE_CHUNK, gx = (cols + RE_COLS - 1) / RE_COLS;
-    const uint32_t gy_want = std::max(1u, std::min(nch, (512 + gx - 1) / gx));
-    const uint32_t span = (nch + gy_want - 1) / gy_want, gy = (nch + span - 1) / span;
-    LBN_LAUNCH(row_eval_kernel, dim3(gx, gy), dim3(64 * RE_WAVES), 0, s, w, eq, wdim, cols, span, u);
+size_t row_eval_scratch_u32(uint32_t wdim) { return (size_t)(wdim / RF_CHUNK) * RF_FRAG_U32; }
+
+hipError_t launch_row_eval_f4(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t *frag, uint32_t wdim,
+                              uint32_t cols, uint64_t *u) {
+    // wdim = 2^row_log_len >= 128: whole chunks. A block per (column, span of chunks); the chunks
+    // are split over the grid's y until there are about 1024 blocks, each span at most 2^17
+    // chunks (the f32 counts stay below 2^24).
+    const uint32_t nch = wdim / RF_CHUNK;
+    LBN_LAUNCH(eqfrag_kernel, dim3((nch * RF_TILES * 64 + 255) / 256), dim3(256), 0, s, eq, nch, frag);
+    const uint32_t gy_want = std::max(1u, std::min(nch, (1024 + cols - 1) / cols));
+    uint32_t span = std::min((nch + gy_want - 1) / gy_want, 1u << 17);
+    span = (span + RF_G - 1) / RF_G * RF_G;  // whole phases
+    const uint32_t gy = (nch + span - 1) / span;
+    LBN_LAUNCH(row_eval_f4_kernel, dim3(cols, gy), dim3(256), 0, s, (const uint32_t *)w, (const uint4 *)frag, wdim,
+               cols, span, (unsigned long long *)u);
     return hipSuccess;
 }
 

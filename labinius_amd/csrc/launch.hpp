@@ -48,8 +48,11 @@ hipError_t launch_switch_msg(hipStream_t s, const uint64_t *a, const uint64_t *p
 // round i's fold by r (arrays of 2 half -> half) and round i + 1's message (half >= 2)
 hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pin, uint64_t *pout, uint32_t half,
                                   const uint64_t r[3], int top, uint64_t *out);
-hipError_t launch_row_eval(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t wdim,
-                           uint32_t cols, uint64_t *u);
+// u_j = sum_i eq_i w[i + wdim j] over F162 (u cleared first) by the FP4-MFMA GF(2) product; frag
+// holds row_eval_scratch_u32(wdim) u32 of scratch (the eq operands)
+hipError_t launch_row_eval_f4(hipStream_t s, const uint64_t *w, const uint64_t *eq, uint32_t *frag, uint32_t wdim,
+                              uint32_t cols, uint64_t *u);
+size_t row_eval_scratch_u32(uint32_t wdim);
 // t (3 u64, cleared first) ^= sum_j eq1_j u_j over F162, j < n
 hipError_t launch_claim(hipStream_t s, const uint64_t *eq1, const uint64_t *u, uint32_t n, uint64_t *t);
 // the device verifier: the folded witness's bit planes (stat: squared norm, largest |v|, added

@@ -1,9 +1,9 @@
 // The row evaluation without a prover and without allocating (lbn_row_evaluate_noalloc).
 //
 // The kernels of lbn_row_evaluate need scratch beyond their inputs and output: the eq tensor's
-// 8-coordinate sub-tables and the eq tensor itself. Here that scratch is one file-scope device
-// array, `scratch`, sized for row_log_len <= ROW_NOALLOC_MAX_ROWS; each device the library runs on
-// holds its own copy until the process ends. Its address on a device is looked up once, with that
+// 8-coordinate sub-tables, the eq tensor itself and its FP4 operands. Here that scratch is one
+// file-scope device array, `scratch`, sized for row_log_len <= ROW_NOALLOC_MAX_ROWS; each device
+// the library runs on holds its own copy until the process ends. Its address on a device is looked up once, with that
 // device current, by hipGetSymbolAddress, and kept per device; host code uses only that address.
 // Every call on one device shares the array, so calls that overlap on one device race (the
 // caller orders them, for instance on one stream).
@@ -17,7 +17,9 @@ namespace lbn {
 constexpr uint32_t ROW_NOALLOC_MAX_ROWS = 15;
 constexpr size_t ROW_NOALLOC_PART_WORDS = 3 * 256 * ((ROW_NOALLOC_MAX_ROWS + 7) / 8);
 constexpr size_t ROW_NOALLOC_EQ_WORDS = 3 * ((size_t)1 << ROW_NOALLOC_MAX_ROWS);
-constexpr size_t ROW_NOALLOC_WORDS = ROW_NOALLOC_PART_WORDS + ROW_NOALLOC_EQ_WORDS;  // 798 720 bytes
+// the FP4 product's eq operands, u32 (row_eval_scratch_u32 of 2^15 elements), as u64 words
+constexpr size_t ROW_NOALLOC_FRAG_WORDS = ((size_t)1 << ROW_NOALLOC_MAX_ROWS) / 128 * 11 * 64 * 4 / 2;
+constexpr size_t ROW_NOALLOC_WORDS = ROW_NOALLOC_PART_WORDS + ROW_NOALLOC_EQ_WORDS + ROW_NOALLOC_FRAG_WORDS;  // 3.68 MB
 
 static __device__ __align__(256) uint64_t scratch[ROW_NOALLOC_WORDS];
 
@@ -62,10 +64,13 @@ int lbn_row_evaluate_noalloc(const lbn_f162 *d_w, size_t n, const lbn_f162 *d_p0
     int rc = scratch_address(&base);
     if (rc) return rc;
     uint64_t *part = base, *eq = base + ROW_NOALLOC_PART_WORDS;
+    uint32_t *frag = (uint32_t *)(eq + ROW_NOALLOC_EQ_WORDS);
+    if (row_eval_scratch_u32((uint32_t)wdim) > 2 * ROW_NOALLOC_FRAG_WORDS)
+        return fail(LBN_ERR_SHAPE, "the eq operands do not fit the static scratch");
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(launch_eq(st, (const uint64_t *)d_p0, (uint32_t)p0_len, part, eq));
     HIP_TRY(launch_fill_u32(st, (uint32_t *)d_u, cols * 6, 0));
-    HIP_TRY(launch_row_eval(st, (const uint64_t *)d_w, eq, (uint32_t)wdim, (uint32_t)cols, (uint64_t *)d_u));
+    HIP_TRY(launch_row_eval_f4(st, (const uint64_t *)d_w, eq, frag, (uint32_t)wdim, (uint32_t)cols, (uint64_t *)d_u));
     return LBN_OK;
 }
 

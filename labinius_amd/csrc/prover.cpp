@@ -344,7 +344,7 @@ static int build_device(lbn_pp *pp) {
 
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
-                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_u, pr->d_wscratch,
+                    pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_eqf, pr->d_u, pr->d_wscratch,
                     pr->d_p1, pr->d_eq1, pr->d_eqpart1, pr->d_t,
                     pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out, pr->sw.d_eq, pr->sw.d_a, pr->sw.d_p,
                     pr->sw.d_small};
@@ -510,7 +510,7 @@ static int row_eval_launch(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
     HIP_TRY(launch_eq(st, pr->d_p0, s.rows, pr->d_eqpart, pr->d_eq));
     HIP_TRY(hipEventRecord(pr->ev[9], st));
     HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_u, s.r * 6, 0));
-    HIP_TRY(launch_row_eval(st, d_w, pr->d_eq, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
+    HIP_TRY(launch_row_eval_f4(st, d_w, pr->d_eq, pr->d_eqf, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
     return LBN_OK;
 }
 
@@ -648,6 +648,7 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     TRY_RC(dalloc(&pr->d_p0, 3 * (size_t)std::max<uint32_t>(s.rows, 1)));
     TRY_RC(dalloc(&pr->d_eq, 3 * s.column_len));
     TRY_RC(dalloc(&pr->d_eqpart, 3 * 256 * (size_t)((s.rows + 7) / 8)));
+    TRY_RC(dalloc(&pr->d_eqf, row_eval_scratch_u32((uint32_t)s.column_len)));
     TRY_RC(dalloc(&pr->d_p1, 3 * (size_t)std::max<uint32_t>(s.cl, 1)));
     TRY_RC(dalloc(&pr->d_eq1, 3 * ((size_t)1 << s.cl)));
     TRY_RC(dalloc(&pr->d_eqpart1, 3 * 256 * (size_t)((s.cl + 7) / 8)));

@@ -108,7 +108,7 @@ int lbn_verify_opening_clear_device(lbn_prover *pr, const int16_t *fc, const int
     if ((rc = commit_columns_device(pr, w.d_w, ncol, w.d_bytes, w.d_yacc, w.d_raw, w.d_comp))) return rc;
     HIP_TRY(launch_eq(st, pr->d_p0, s.rows, pr->d_eqpart, pr->d_eq));
     HIP_TRY(launch_fill_u32(st, (uint32_t *)w.d_u, ncol * 6, 0));
-    HIP_TRY(launch_row_eval(st, w.d_w, pr->d_eq, (uint32_t)s.column_len, (uint32_t)ncol, w.d_u));
+    HIP_TRY(launch_row_eval_f4(st, w.d_w, pr->d_eq, pr->d_eqf, (uint32_t)s.column_len, (uint32_t)ncol, w.d_u));
     std::vector<int16_t> comp(ncol * 4 * limbs * N162);
     uint64_t stat[2], u[3 * MAX_COLS];
     HIP_TRY(hipMemcpyAsync(comp.data(), w.d_comp, comp.size() * sizeof(int16_t), hipMemcpyDeviceToHost, st));
