@@ -366,45 +366,65 @@ def switch_bench(lbn, reps=5):
     return out
 
 
-def serving_bench(lbn, rate, count):
-    """Poisson arrivals of commit -> evaluate -> fold requests (sizes 2^16 40 %, 2^18 30 %, 2^20 20 %,
-    2^22 8 %, 2^24 2 %; Clear or bit-dropped at random) through the continuous-batching server,
-    then the same stream one request at a time. Latency is arrival to folded witness."""
+def serving_bench(lbn, rates, count):
+    """The serving loop (the native lbn_server): a stream of `count` commit -> evaluate -> fold requests
+    with sizes in the mix's exact proportions (2^16 40 %, 2^18 30 %, 2^20 20 %, 2^22 8 %, 2^24 2 %),
+    Clear or bit-dropped at random, at each Poisson arrival rate (None: all at once, the saturating
+    case), through the continuous-batching server and through the same server run serially (one
+    request at a time, unbatched). Latency is arrival to folded witness on the host; commits/s is
+    completed requests over the span from the first arrival to the last completion."""
     import numpy as np
     from labinius_amd import serving
     from labinius_amd import workloads as wl
-    spec = wl.serving_requests(rate, count)
     rng = np.random.default_rng(1)
-    reqs = []
-    for t, e, d, i in spec:
+    spec = wl.serving_requests_stratified(None, count)
+    wits = []
+    for _, e, d, i in spec:
         x = rng.integers(0, 2**63, (1 << e, 3), dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, (1 << e, 3),
                                                                                                   dtype=np.uint64)
         x[:, 2] &= np.uint64((1 << 34) - 1)
-        reqs.append((i, e, d, lbn.Witness.from_elements(wl.witness_params(e), x), t))
-    server = serving.Server(MATRIX_SEED)
-    warm = [(10**6 + k, e, 8 * (k % 2), lbn.Witness.random(wl.witness_params(e), bytes([k]) * 32), 0.0)
-            for k, e in enumerate(server.workers)]
-    serving.run_stream(server, warm)
-    server.batches.clear()
+        wits.append(lbn.Witness.from_elements(wl.witness_params(e), x))
     t0 = time.time()
-    mixed = serving.run_stream(server, reqs)
-    server.close()
-    log(f"served {len(mixed)} in {time.time() - t0:.1f} s, {len(server.batches)} batched commitments")
-    unb = serving.serve_unbatched(MATRIX_SEED, reqs)
-    sizes = {}
-    for _, e, _, _, _ in reqs:
-        sizes[e] = sizes.get(e, 0) + 1
-    stage = lambda rs, a, b: round(1e3 * float(np.mean([getattr(r, b) - getattr(r, a) for r in rs])), 3)
-    return {"rate_per_s": rate, "sizes": {f"2^{e}": c for e, c in sorted(sizes.items())},
-            "bit_dropped": sum(1 for r in reqs if r[2]),
-            "served": dict(serving.summarize(mixed), batches=len(server.batches),
-                           mean_requests_per_batch=round(len(mixed) / max(1, len(server.batches)), 2),
-                           queue_ms=stage(mixed, "arrival", "admitted"),
-                           commit_ms=stage(mixed, "admitted", "committed"), round_ms=stage(mixed, "committed", "done"),
-                           peak_hbm_gb=round(server.peak_reserved / 1e9, 3)),
-            "unbatched": dict(serving.summarize(unb), queue_ms=stage(unb, "arrival", "admitted"),
-                              commit_ms=stage(unb, "admitted", "committed"), round_ms=stage(unb, "committed", "done")),
-            "hbm_budget_gb": 200}
+    served = serving.Server(MATRIX_SEED)
+    serial = serving.Server(MATRIX_SEED, serial=True, workers=1)
+    log(f"servers set up in {time.time() - t0:.1f} s")
+    warm = [(10**6 + k, e, wl.SERVING_DROPPED[e] * (k % 2), wits[[s_[1] for s_ in spec].index(e)], 0.0)
+            for k, e in enumerate([16, 18, 20, 22, 24] * 2)]
+    serving.run_stream(served, warm)
+    serving.run_stream(serial, warm)
+    out = {"requests_per_rate": count, "sizes": {f"2^{e}": sum(1 for s_ in spec if s_[1] == e) for e in (16, 18, 20, 22, 24)},
+           "bit_dropped": sum(1 for s_ in spec if s_[2]), "hbm_budget_gb": served.budget / 1e9,
+           "fixed_gb": {"served": round(served.stats()["fixed_bytes"] / 1e9, 3),
+                        "serial": round(serial.stats()["fixed_bytes"] / 1e9, 3)},
+           "rates": []}
+    for rate in rates:
+        arr = wl.serving_requests_stratified(rate, count)
+        reqs = [(i, e, d, wits[i], t) for (t, e, d, i) in arr]
+        before = served.stats()
+        mixed = serving.run_stream(served, reqs)
+        after = served.stats()
+        alone = serving.run_stream(serial, reqs)
+        ms = serving.summarize(mixed)
+        ms["batches"] = after["batches"] - before["batches"]
+        ms["pool_mallocs"] = after["pool_mallocs"] - before["pool_mallocs"]
+        su = serving.summarize(alone)
+        stage = lambda rs, a, b: round(1e3 * float(np.mean([getattr(r, b) - getattr(r, a) for r in rs])), 3)
+        for m, rs in ((ms, mixed), (su, alone)):
+            m["queue_ms"] = stage(rs, "arrival", "admitted")
+            m["commit_ms"] = stage(rs, "admitted", "committed")
+            m["round_ms"] = stage(rs, "committed", "done")
+        big = [r.latency_ms() for r in mixed if r.e == 24]
+        ms["latency_2^24_ms"] = [round(x, 3) for x in big]
+        out["rates"].append({"rate_per_s": rate if rate else "all at once", "served": ms, "unbatched": su,
+                             "speedup_commits_per_s": round(ms["commits_per_s"] / max(su["commits_per_s"], 1e-9), 3)})
+        log(f"serving at {rate}/s: served {ms['commits_per_s']}/s p99 {ms['p99_ms']} ms; "
+            f"unbatched {su['commits_per_s']}/s p99 {su['p99_ms']} ms")
+    st = served.stats()
+    out["pool"] = {"obtained_gb": round(st["pool_obtained"] / 1e9, 3), "mallocs": st["pool_mallocs"],
+                   "reuses": st["pool_reuses"], "peak_reserved_gb": round(st["reserved_peak"] / 1e9, 3)}
+    served.close()
+    serial.close()
+    return out
 
 
 def main():
@@ -419,8 +439,9 @@ def main():
     ap.add_argument("--no-switch", action="store_true", help="skip the cross-field switch timing")
     ap.add_argument("--no-wire", action="store_true", help="skip the wire encoding timing")
     ap.add_argument("--no-verify", action="store_true", help="skip the verify-side timing")
-    ap.add_argument("--serving-rate", type=float, default=100.0, help="requests per second")
-    ap.add_argument("--serving-requests", type=int, default=120)
+    ap.add_argument("--serving-rates", default="100,300,1000,0",
+                    help="Poisson arrival rates, requests per second (0: all at once)")
+    ap.add_argument("--serving-requests", type=int, default=100)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (read 1 GiB, write 1 GiB) five times, to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -580,7 +601,8 @@ def main():
         line["batched"] = batched(lbn, 3)
     if not args.no_serving:
         log("serving...")
-        line["serving"] = serving_bench(lbn, args.serving_rate, args.serving_requests)
+        line["serving"] = serving_bench(lbn, [float(x) or None for x in args.serving_rates.split(",")],
+                                        args.serving_requests)
     if not args.no_switch:
         log("switch...")
         line["switch"] = switch_bench(lbn)

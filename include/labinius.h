@@ -56,7 +56,9 @@ enum {
     LBN_ERR_NO_DEVICE = 14,
     /* WireError, wire/mod.rs:67-74 */
     LBN_WIRE_TRUNCATED = 15,         /* the byte string ran out before the object did */
-    LBN_WIRE_MALFORMED = 16          /* not a coding of any object of the expected shape */
+    LBN_WIRE_MALFORMED = 16,         /* not a coding of any object of the expected shape */
+    /* the serving loop: a request that alone needs more device memory than the server's budget */
+    LBN_ERR_ADMISSION = 17
 };
 
 enum { LBN_OPENING_CLEAR = 0, LBN_OPENING_BIT_DROPPED = 1, LBN_OPENING_RECURSIVE = 2 };
@@ -157,6 +159,18 @@ int lbn_mle_evaluate(lbn_prover *prover, const lbn_dwitness *w, const lbn_f162 *
 int lbn_row_evaluate_noalloc(const lbn_f162 *d_w, size_t n, const lbn_f162 *d_p0, size_t p0_len, lbn_f162 *d_u,
                              void *stream);
 
+/* A device memory pool (no counterpart in the reference; its Prover recycles one workspace,
+ * key.rs:185-191, scheme/prover.rs:116-131). A prover given a pool takes its resident witnesses
+ * (lbn_witness_upload, lbn_witness_upload_params) and its fold layouts from it and gives them back
+ * on release, never calling hipFree while the pool lives; cap_bytes bounds what the pool obtains
+ * from the device (LBN_ERR_OOM past it). Thread-safe; provers on several threads may share it. */
+typedef struct lbn_pool lbn_pool;
+int lbn_pool_create(uint64_t cap_bytes, lbn_pool **out);
+void lbn_pool_release(lbn_pool *pool);  /* after every prover and witness that uses it */
+int lbn_pool_stats(lbn_pool *pool, uint64_t *obtained, uint64_t *in_use, uint64_t *peak_in_use, uint64_t *mallocs,
+                   uint64_t *reuses);
+int lbn_prover_set_pool(lbn_prover *prover, lbn_pool *pool);
+
 /* Device time of each stage of the prover's last commit / fold / row evaluation, in ms, measured
  * with HIP events on the prover's stream:
  *   [0] commit total  [1] commit layout pass  [2] commit GEMM  [3] commit finish
@@ -168,14 +182,80 @@ int lbn_prover_timings(const lbn_prover *prover, float *ms, size_t n);
  * result equals Prover::commit, scheme/prover.rs:55, of that witness alone). A witness of other
  * Params that share this prover's key (the same row_log_len and moduli, so the same A, key.rs:48)
  * is uploaded with lbn_witness_upload_params; lbn_commit_batch commits `count` of them in one
- * captured HIP graph: components[i] ([cols_i][4][limb][162]) and raw[i] ([limb][cols_i][648])
- * may be NULL. The GEMM's block schedule is autotuned per total column count. */
+ * launch sequence (layout, GEMM, finish; captured once as a HIP graph and replayed when the same
+ * total width comes twice in a row): components[i] ([cols_i][4][limb][162]) and raw[i]
+ * ([limb][cols_i][648]) may be NULL. The GEMM's block schedule is autotuned per power-of-two class
+ * of the total column count. */
 int lbn_witness_upload_params(lbn_prover *prover, const lbn_params *params, const lbn_f162 *w, size_t n,
                               lbn_dwitness **out);
 int lbn_commit_batch(lbn_prover *prover, const lbn_dwitness *const *ws, size_t count, int16_t *const *components,
                      uint32_t *const *raw);
 /* The last batch's schedule (tiles per XCD round, persistent blocks or 0) and its device ms. */
 int lbn_batch_info(const lbn_prover *prover, uint32_t *sched_t, uint32_t *persistent_blocks, float *ms);
+
+/* The serving loop (no counterpart in the reference, whose callers run one round at a time): a
+ * server commits, evaluates and folds requests of many sizes that share one key (the same
+ * row_log_len and moduli as `key`, so the same A), with continuous batching. A scheduler thread
+ * admits queued requests in arrival order while their device memory fits the budget (the
+ * witness and the fold's layout, lbn_server_request_bytes, from the server's device pool), uploads
+ * them, and commits every request it admitted in one batched commitment; requests that arrive
+ * while others are on the GPU join the next batch. `workers` threads, each with its own provers
+ * and HIP streams, then run each request's round: the bit-dropped digits when the mode drops bits
+ * (scheme/prover.rs:69-72), the evaluation point (verifier.rs:61-100), the row values and the
+ * claim (witness.rs:94-105), the folding challenges (commitment.rs:217-228) and the fold of the
+ * still-resident witness (prover.rs:120-131), overlapping on the device with each other and with
+ * the next batch. Each result is the request's own round, the same as alone (its transcript
+ * starts from its own domain). serial = 1 runs the same stream one request at a time, unbatched,
+ * with nothing in flight beside it: the comparison. */
+typedef struct lbn_server lbn_server;
+typedef struct {
+    uint64_t hbm_budget;        /* device bytes the server may hold: its own workspaces plus requests */
+    uint32_t workers;           /* round threads (0: 4) */
+    uint32_t serial;            /* 1: one request at a time, no batching, no overlap */
+    uint32_t max_batch_columns; /* columns of one batched commitment (0: 16384) */
+    uint32_t n_sizes;           /* witness_log_len of the sizes to set up in advance (others on first use) */
+    uint32_t sizes[8];
+} lbn_server_config;
+/* One request's outputs, in the caller's buffers, sized by its Params (include/labinius.h layouts):
+ * components (Clear) and raw always, top and digits when it drops bits; the point (p0: row_log_len,
+ * p1: column_log_len F162), the row values u (columns F162), the claim (1 F162), the challenges
+ * (columns) and the folded witness v (len_ring x 648). */
+typedef struct {
+    int16_t *components;
+    uint32_t *raw;
+    uint16_t *top, *digits;
+    lbn_f162 *p0, *p1, *u, *claim;
+    lbn_challenge *challenges;
+    int16_t *v;
+} lbn_server_outputs;
+typedef struct {
+    int status;                /* LBN_OK or the error the request met (it alone fails) */
+    double arrival_s, admitted_s, committed_s, done_s;  /* seconds since the server started */
+    uint64_t bytes;            /* the device memory it held */
+    uint32_t batch;            /* which batched commitment took it */
+} lbn_server_info;
+typedef struct {
+    uint64_t requests, failed, batches, batched_requests, batched_columns;
+    uint64_t reserved_peak, fixed_bytes, pool_obtained, pool_mallocs, pool_reuses;
+} lbn_server_stats;
+enum { LBN_SERVER_FAIL_FOLD = 1 };  /* test hook: the request's fold is given an invalid challenge */
+int lbn_server_create(const lbn_params *key, const uint8_t matrix_seed[32], const lbn_server_config *config,
+                      lbn_server **out);
+/* Queues a request; w (n F162) and the output buffers stay the caller's and must stay valid until
+ * lbn_server_wait returns for the ticket. LBN_ERR_ADMISSION when the request alone exceeds the
+ * budget, LBN_ERR_SHAPE when its Params do not share the server's key. */
+int lbn_server_submit(lbn_server *server, const lbn_params *params, const lbn_f162 *w, size_t n,
+                      const uint8_t *domain, size_t domain_len, const lbn_server_outputs *outputs, uint32_t flags,
+                      uint64_t *ticket);
+/* Blocks until the request is done; returns LBN_OK (the request's own status is in info). */
+int lbn_server_wait(lbn_server *server, uint64_t ticket, lbn_server_info *info);
+int lbn_server_stats_get(lbn_server *server, lbn_server_stats *stats);
+/* A new budget for admissions from now on (requests already admitted keep their memory). */
+int lbn_server_set_budget(lbn_server *server, uint64_t hbm_budget);
+/* Finishes every queued request, then stops the threads and frees the device memory. */
+void lbn_server_destroy(lbn_server *server);
+/* Device bytes a request of these Params holds from admission to the end of its fold. */
+uint64_t lbn_server_request_bytes(const lbn_params *params);
 
 /* Transcript, challenge.rs:66-160 */
 lbn_transcript *lbn_transcript_new(const uint8_t *domain, size_t len);

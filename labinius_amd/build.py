@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblabinius.so")
 HOST_SOURCES = ["ring.cpp", "blake3.cpp", "protocol.cpp", "prover.cpp", "dropped.cpp", "batch.cpp", "switch.cpp", "wire.cpp", "verify.cpp",
-                "noalloc.cpp"]
+                "noalloc.cpp", "pool.cpp", "server.cpp"]
 DEVICE_SOURCES = ["kernels.hip"]
 
 

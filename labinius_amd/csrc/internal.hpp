@@ -94,7 +94,14 @@ void sample_short_challenges(Transcript &t, size_t n, uint32_t weight, double bo
 }  // namespace lbn
 
 struct lbn_pp;
+struct lbn_pool;
 namespace lbn {
+// the device pool (pool.cpp): a buffer of at least `bytes` (same_tag: it last held a layout with
+// this tag), and its return
+int pool_alloc(lbn_pool *pool, uint64_t bytes, uint64_t tag, void **out, bool *same_tag);
+void pool_free(lbn_pool *pool, void *p);
+uint64_t pool_rounded(uint64_t bytes);
+void pool_set_cap(lbn_pool *pool, uint64_t cap);
 // verifier.rs:41-57
 void absorb_parameters(Transcript &t, const lbn_pp *pp);
 // verifier.rs:85-99
@@ -139,6 +146,7 @@ struct lbn_pp {
 
 struct lbn_dwitness {
     lbn_prover *owner = nullptr;
+    lbn_pool *pool = nullptr;  // d_w came from this pool (else from hipMalloc)
     uint64_t *d_w = nullptr;
     size_t n = 0;
     size_t cols = 0;  // its own Params::columns(); the key (row length, moduli) is the prover's
@@ -154,9 +162,9 @@ struct lbn_batch_ws {
     const uint64_t **h_cols = nullptr;  // pinned
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
-    size_t graph_cols = 0;
+    size_t graph_cols = 0, last_cols = 0;
     uint32_t graph_t = 0, graph_persist = 0;
-    std::vector<std::array<uint32_t, 3>> tuned;  // (columns, sched_t, persistent blocks)
+    std::vector<std::array<uint32_t, 3>> tuned;  // (width class, sched_t, persistent blocks)
     hipEvent_t ev[2] = {};
     float ms = 0;
     uint32_t sched_t = 8, persist = 0;
@@ -224,12 +232,17 @@ void wire_free(lbn_wire_ws &w);
 // YACC_PLANES x limbs x 648 words.
 int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint8_t *d_bytes, uint32_t *d_yacc,
                           uint32_t *d_raw, int16_t *d_comp);
+// the batched commitment's workspace sized for cols columns in advance (batch.cpp)
+int commit_batch_reserve(lbn_prover *pr, size_t cols);
+// a resident witness of n F162 for the prover (from its pool when it has one), uploaded from w
+int witness_alloc_upload(lbn_prover *pr, const lbn_f162 *w, size_t n, size_t cols, lbn_dwitness **out);
 void verify_free(lbn_verify_ws &w);
 }
 
 struct lbn_prover {
     lbn_pp *pp = nullptr;
     hipStream_t stream = nullptr;
+    lbn_pool *pool = nullptr;  // where uploads and fold layouts come from (NULL: hipMalloc / hipFree)
     // workspace
     uint8_t *d_bytes_c = nullptr;
     std::vector<uint8_t *> free_bytes_f;  // recycled fold layouts

@@ -49,6 +49,12 @@ int make_shape(const lbn_params *p, Shape *s) {
     for (uint32_t i = 0; i < p->n_extra; i++) s->primes.push_back(p->extra[i]);
     s->opening = p->opening;
     s->dropped_bits = p->opening == LBN_OPENING_BIT_DROPPED ? p->dropped_bits : 0;
+    // bd/digits.rs:141-182 keeps digits and the rounded top in u16 and needs a second modulus for the
+    // digits: every entry point rejects other bit-dropped shapes (a reference assert!)
+    if (p->opening == LBN_OPENING_BIT_DROPPED && (s->dropped_bits < 1 || s->dropped_bits > 15))
+        return fail(LBN_ERR_SHAPE, "dropped bits out of range (1 to 15)");
+    if (p->opening == LBN_OPENING_BIT_DROPPED && s->primes.size() < 2)
+        return fail(LBN_ERR_SHAPE, "a bit-dropped commitment needs two or more moduli");
     return LBN_OK;
 }
 

@@ -350,7 +350,7 @@ static void prover_free(lbn_prover *pr) {
                     pr->sw.d_small};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);
+    for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);  // a pooled prover keeps none
     batch_free(pr->batch);
     wire_free(pr->wire);
     verify_free(pr->vfy);
@@ -363,19 +363,56 @@ static void prover_free(lbn_prover *pr) {
 
 static size_t bytes_f_size(const lbn_pp *pp) { return bytes_size((uint32_t)pp->shape.len_ring, pp->kb_f); }
 
+// The fold's layout buffer: recycled by the prover, or from its pool. Padding columns are never
+// written by the layout pass and must read as zero: a fresh buffer, or a pooled one that last held
+// another layout, is cleared.
 static int take_bytes_f(lbn_prover *pr, uint8_t **out) {
+    const size_t sz = bytes_f_size(pr->pp);
+    if (pr->pool) {
+        const uint64_t tag = ((uint64_t)pr->pp->kb_f << 32) | pr->pp->shape.len_ring;
+        bool same = false;
+        void *p = nullptr;
+        int rc = pool_alloc(pr->pool, sz, tag, &p, &same);
+        if (rc) return rc;
+        *out = (uint8_t *)p;
+        if (!same) {
+            const hipError_t e = hipMemsetAsync(*out, 0, sz, pr->stream);
+            if (e != hipSuccess) {
+                pool_free(pr->pool, p);
+                return fail(LBN_ERR_HIP, std::string("layout clear: ") + hipGetErrorString(e));
+            }
+        }
+        return LBN_OK;
+    }
     if (!pr->free_bytes_f.empty()) {
         *out = pr->free_bytes_f.back();
         pr->free_bytes_f.pop_back();
         return LBN_OK;
     }
-    const size_t sz = bytes_f_size(pr->pp);
     HIP_TRY(hipMalloc((void **)out, sz));
-    // padding columns are never written by the layout pass and must read as zero
-    HIP_TRY(hipMemset(*out, 0, sz));
+    const hipError_t e = hipMemset(*out, 0, sz);
+    if (e != hipSuccess) {
+        (void)hipFree(*out);
+        return fail(LBN_ERR_HIP, std::string("layout clear: ") + hipGetErrorString(e));
+    }
     return LBN_OK;
 }
 
+// A layout buffer back to where take_bytes_f found it.
+static void give_bytes_f(lbn_prover *pr, uint8_t *b) {
+    if (!b) return;
+    if (pr->pool)
+        pool_free(pr->pool, b);
+    else
+        pr->free_bytes_f.push_back(b);
+}
+
+// Returns a taken layout buffer on every early return, unless the opening took it.
+struct BytesGuard {
+    lbn_prover *pr;
+    uint8_t *b;
+    ~BytesGuard() { give_bytes_f(pr, b); }
+};
 
 // The commit GEMM's schedule: 8 tiles x 4 groups an XCD round, or, with under 32 groups (a
 // few columns), 32 tiles x 1 group, so that the groups spread over the 8 XCDs instead of two.
@@ -390,6 +427,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
         int rc = take_bytes_f(pr, &bytes_f);
         if (rc) return rc;
     }
+    BytesGuard guard{pr, bytes_f};
     hipStream_t st = pr->stream;
     HIP_TRY(hipEventRecord(pr->ev[0], st));
     HIP_TRY(launch_prepass(st, d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r,
@@ -429,6 +467,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
         op->owner = pr;
         op->d_bytes_f = bytes_f;
         *opening = op;
+        guard.b = nullptr;  // the opening holds it now
     }
     return LBN_OK;
 }
@@ -547,6 +586,34 @@ static int mle_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
     HIP_TRY(hipMemcpyAsync(t, pr->d_t, sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     row_eval_timings(pr);
+    return LBN_OK;
+}
+
+int witness_alloc_upload(lbn_prover *pr, const lbn_f162 *w, size_t n, size_t cols, lbn_dwitness **out) {
+    lbn_dwitness *d = new lbn_dwitness();
+    d->owner = pr;
+    d->n = n;
+    d->cols = cols;
+    const size_t bytes = n * sizeof(lbn_f162);
+    if (pr->pool) {
+        bool same = false;
+        void *p = nullptr;
+        const int rc = pool_alloc(pr->pool, bytes, 0, &p, &same);
+        if (rc) {
+            delete d;
+            return rc;
+        }
+        d->pool = pr->pool;
+        d->d_w = (uint64_t *)p;
+    } else if (hipMalloc((void **)&d->d_w, bytes) != hipSuccess) {
+        delete d;
+        return fail(LBN_ERR_OOM, "witness allocation");
+    }
+    if (hipMemcpy(d->d_w, w, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        lbn_dwitness_release(d);
+        return fail(LBN_ERR_HIP, "witness upload");
+    }
+    *out = d;
     return LBN_OK;
 }
 
@@ -702,28 +769,17 @@ void lbn_prover_release(lbn_prover *pr) {
 int lbn_witness_upload(lbn_prover *pr, const lbn_f162 *w, size_t n, lbn_dwitness **out) {
     if (!pr || !w || !out) return fail(LBN_ERR_ARG, "null argument");
     if (n != pr->pp->shape.n) return fail(LBN_ERR_WITNESS_LENGTH, "the witness is not witness_len() elements");
-    lbn_dwitness *d = new lbn_dwitness();
-    d->owner = pr;
-    d->n = n;
-    d->cols = pr->pp->shape.r;
-    if (hipMalloc((void **)&d->d_w, n * sizeof(lbn_f162)) != hipSuccess) {
-        delete d;
-        return fail(LBN_ERR_OOM, "witness allocation");
-    }
-    if (hipMemcpy(d->d_w, w, n * sizeof(lbn_f162), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(d->d_w);
-        delete d;
-        return fail(LBN_ERR_HIP, "witness upload");
-    }
-    *out = d;
-    return LBN_OK;
+    return witness_alloc_upload(pr, w, n, pr->pp->shape.r, out);
 }
 
 void *lbn_dwitness_device_ptr(lbn_dwitness *w) { return w ? (void *)w->d_w : nullptr; }
 
 void lbn_dwitness_release(lbn_dwitness *w) {
     if (!w) return;
-    if (w->d_w) (void)hipFree(w->d_w);
+    if (w->pool)
+        pool_free(w->pool, w->d_w);
+    else if (w->d_w)
+        (void)hipFree(w->d_w);
     delete w;
 }
 
@@ -754,7 +810,7 @@ int lbn_fold(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch, size_t r,
     for (size_t j = 0; j < r; j++)
         if (ch[j].weight > 32) return fail(LBN_ERR_ARG, "challenge weight above 32");
     int rc = fold_device(pr, op, ch, r, v);
-    pr->free_bytes_f.push_back(op->d_bytes_f);
+    give_bytes_f(pr, op->d_bytes_f);
     delete op;
     return rc;
 }
@@ -773,14 +829,14 @@ int lbn_fold_resident(lbn_prover *pr, const lbn_dwitness *w, const lbn_challenge
     const hipError_t e = launch_prepass(pr->stream, w->d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring,
                                         (uint32_t)s.r, nullptr, 0, op.d_bytes_f, pr->pp->kb_f);
     rc = e == hipSuccess ? fold_device(pr, &op, ch, r, v) : fail(LBN_ERR_HIP, hipGetErrorString(e));
-    pr->free_bytes_f.push_back(op.d_bytes_f);
+    give_bytes_f(pr, op.d_bytes_f);
     return rc;
 }
 
 int lbn_recycle(lbn_prover *pr, lbn_opening *op) {
     if (!pr || !op) return fail(LBN_ERR_ARG, "null argument");
     if (op->owner != pr) return fail(LBN_ERR_SHAPE, "the opening belongs to another prover");
-    pr->free_bytes_f.push_back(op->d_bytes_f);
+    give_bytes_f(pr, op->d_bytes_f);
     delete op;
     return LBN_OK;
 }

@@ -64,6 +64,34 @@ def batch_witness(i, e, lifted):
     return w
 
 
+def serving_sizes_stratified(count, seed=0x57A71F1):
+    """Exponents in the serving mix's exact proportions (at least one of each size), in a seeded
+    random order: 40 % 2^16, 30 % 2^18, 20 % 2^20, 8 % 2^22, 2 % 2^24."""
+    rng = Rng(seed)
+    counts = [max(1, int(round(p * count))) for _, p in SERVING_SIZES]
+    counts[0] += count - sum(counts)
+    out = [e for (e, _), c in zip(SERVING_SIZES, counts) for _ in range(c)]
+    for i in range(len(out) - 1, 0, -1):  # Fisher-Yates
+        j = rng.below(i + 1)
+        out[i], out[j] = out[j], out[i]
+    return out
+
+
+def serving_requests_stratified(rate_per_s, count, seed=0x5E4F1CE):
+    """Poisson arrivals at rate_per_s (None: all at once) with stratified sizes and Clear or
+    bit-dropped at random: (arrival seconds, exponent, dropped bits or 0, index)."""
+    import math
+    rng = Rng(seed)
+    sizes = serving_sizes_stratified(count, seed ^ 0x5A5A)
+    t, out = 0.0, []
+    for i, e in enumerate(sizes):
+        if rate_per_s:
+            t += -math.log(1.0 - rng.uniform()) / rate_per_s
+        d = SERVING_DROPPED[e] if rng.below(2) else 0
+        out.append((t, e, d, i))
+    return out
+
+
 def serving_requests(rate_per_s, count, seed=0x5E4F1CE):
     """Poisson arrivals at rate_per_s: (arrival seconds, exponent, dropped bits or 0, index)."""
     import math
