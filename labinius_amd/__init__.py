@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblabinius.so")
+# LABINIUS_LIB names another build of the same library (a sanitizer build of the host code, say)
+LIB_PATH = os.environ.get("LABINIUS_LIB") or os.path.join(HERE, "liblabinius.so")
 
 N = 648
 N162 = 162
@@ -108,10 +109,11 @@ def lib():
         # When torch (ROCm build) is present, its HIP runtime must be the one in the process before
         # the library resolves libamdhip64.so.7: loading the system runtime first and torch's
         # second leaves the library without a device.
-        try:
-            import torch  # noqa: F401
-        except Exception:
-            pass
+        if not os.environ.get("LABINIUS_NO_TORCH"):
+            try:
+                import torch  # noqa: F401
+            except Exception:
+                pass
         if not os.path.exists(LIB_PATH):
             raise LabiniusError(-1, f"{LIB_PATH} is not built (run __graft_entry__.build())")
         L = ctypes.CDLL(LIB_PATH)

@@ -24,10 +24,16 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def build(verbose=False, sanitize=False, out_path=None):
+    """sanitize: the host code with AddressSanitizer and UndefinedBehaviorSanitizer (device code as
+    always), into `out_path`; such a library runs with the clang ASan runtime preloaded."""
     cc = hipcc()
     common = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
               "-I", os.path.join(HERE, "..", "include")]
+    if sanitize:
+        common = ["-O1", "-g"] + common[1:] + ["-Xarch_host", "-fsanitize=address", "-Xarch_host",
+                                               "-fsanitize=undefined", "-Xarch_host", "-fno-sanitize-recover=undefined",
+                                               "-Xarch_host", "-fno-omit-frame-pointer"]
     tmp = tempfile.mkdtemp(prefix="lbn_build_")
     objs = []
     jobs = []
@@ -58,12 +64,14 @@ def build(verbose=False):
                 sys.stderr.write("\n".join(lines) + "\n")
         if failed:
             raise RuntimeError("compilation failed: " + ", ".join(failed))
-        tmp_lib = LIB + ".tmp"
-        subprocess.run([cc, "-shared", "--offload-arch=gfx950", "-o", tmp_lib, *objs], check=True)
-        os.replace(tmp_lib, LIB)
+        lib_path = out_path or LIB
+        tmp_lib = lib_path + ".tmp"
+        link = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-shared-libsan"] if sanitize else []
+        subprocess.run([cc, "-shared", "--offload-arch=gfx950", *link, "-o", tmp_lib, *objs], check=True)
+        os.replace(tmp_lib, lib_path)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    return LIB
+    return lib_path
 
 
 if __name__ == "__main__":

@@ -7,12 +7,12 @@
 // upwards, which is the spec's rule that a left subtree holds the largest power-of-two number of
 // chunks.
 #include "blake3.hpp"
+#include "threads.hpp"
 
 #include <cstring>
 #include <algorithm>
 #include <array>
 #include <atomic>
-#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -146,16 +146,7 @@ void subtree_cvs(const uint32_t key[8], uint32_t flags, const std::vector<Sub> &
     };
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), leaves.size());
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; t++) {
-        try {
-            th.emplace_back(work);
-        } catch (const std::system_error &) {
-            break;  // no more threads: the ones running, and this one, take the rest of the leaves
-        }
-    }
-    work();
-    for (auto &x : th) x.join();
+    lbn::run_ranges(nt, nt, [&](size_t, size_t) { work(); });  // hashing allocates nothing
     out.resize(subs.size());
     for (size_t i = 0; i < subs.size(); i++) {
         std::array<uint32_t, 8> *c = cv.data() + first[i];
@@ -247,15 +238,22 @@ struct Blake3::Plan {
 
 void Blake3::update(const void *data, size_t len) {
     const uint8_t *p = (const uint8_t *)data;
-    if (len < PARALLEL_MIN) return update_loop(p, len, nullptr);
-    Plan plan;
-    {
-        Blake3 dry = *this;
-        dry.update_loop(p, len, &plan);
+    if (len >= PARALLEL_MIN) {
+        try {
+            Plan plan;
+            {
+                Blake3 dry = *this;
+                dry.update_loop(p, len, &plan);
+            }
+            subtree_cvs(key_, flags_, plan.subs, plan.cvs);
+            plan.record = false;
+            update_loop(p, len, &plan);  // takes the planned values only: allocates nothing
+            return;
+        } catch (const std::exception &) {
+            // no memory for the plan: the state is untouched (the plan ran on a copy), hash in order
+        }
     }
-    subtree_cvs(key_, flags_, plan.subs, plan.cvs);
-    plan.record = false;
-    update_loop(p, len, &plan);
+    update_loop(p, len, nullptr);
 }
 
 void Blake3::update_loop(const uint8_t *p, size_t len, Plan *plan) {

@@ -3,7 +3,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -151,7 +150,17 @@ using namespace lbn;
 
 extern "C" {
 
+static int drop_bits_host(const lbn_params *params, const uint32_t *raw, uint16_t *top, uint16_t *digits);
+
 int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, uint16_t *digits) {
+    try {
+        return drop_bits_host(params, raw, top, digits);
+    } catch (const std::exception &) {
+        return fail(LBN_ERR_OOM, "the digits' working memory");
+    }
+}
+
+static int drop_bits_host(const lbn_params *params, const uint32_t *raw, uint16_t *top, uint16_t *digits) {
     if (!params || !raw || !top || !digits) return fail(LBN_ERR_ARG, "null argument");
     Shape s;
     int rc = make_shape(params, &s);
@@ -185,19 +194,7 @@ int lbn_drop_bits(const lbn_params *params, const uint32_t *raw, uint16_t *top, 
         }
     };
     const size_t nt = std::min<size_t>(std::max<size_t>(1, s.r / 64), std::max(1u, std::thread::hardware_concurrency()));
-    if (nt <= 1) {
-        work(0, s.r);
-    } else {
-        std::vector<std::thread> th;
-        for (size_t t = 0; t < nt; t++) {
-            try {
-                th.emplace_back(work, s.r * t / nt, s.r * (t + 1) / nt);
-            } catch (const std::system_error &) {
-                work(s.r * t / nt, s.r * (t + 1) / nt);  // no thread to be had: this range here
-            }
-        }
-        for (auto &x : th) x.join();
-    }
+    if (!run_ranges(s.r, nt, work)) return fail(LBN_ERR_OOM, "the digits' working memory");
     return LBN_OK;
 }
 

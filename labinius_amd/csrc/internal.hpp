@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/labinius.h"
@@ -15,6 +16,7 @@
 #include "blake3.hpp"
 #include "device.hpp"
 #include "ring.hpp"
+#include "threads.hpp"
 
 namespace lbn {
 
@@ -30,6 +32,8 @@ int fail(int code, const std::string &msg);
     } while (0)
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+
 
 template <typename T>
 int dalloc(T **p, size_t count) {

@@ -3,7 +3,6 @@
 // CPU, as in the reference; only commit / fold / row evaluation run on the GPU.
 #include <cstring>
 
-#include <system_error>
 #include <thread>
 
 #include "internal.hpp"
@@ -223,20 +222,7 @@ void sample_short_challenges(Transcript &t, size_t n, uint32_t weight, double bo
     auto work = [&](size_t j0, size_t j1) {
         for (size_t j = j0; j < j1; j++) sample_from(readers[j], weight, bound, &out[j]);
     };
-    if (nt <= 1) {
-        work(0, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (size_t k = 1; k < nt; k++) {
-        try {
-            th.emplace_back(work, n * k / nt, n * (k + 1) / nt);
-        } catch (const std::system_error &) {
-            work(n * k / nt, n * (k + 1) / nt);  // no thread to be had: this range here
-        }
-    }
-    work(0, n / nt);
-    for (auto &x : th) x.join();
+    (void)run_ranges(n, nt, work);  // sampling allocates nothing: no share can throw
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -344,7 +330,12 @@ int lbn_derive_evaluation_point(const lbn_pp *pp, lbn_transcript *tr, const int1
     // absorb_elements per column: the count 4, then the column's 4 entries with no per-element
     // prefix, as one contiguous stream, so that the hasher takes whole subtrees on threads
     const size_t limbs = s.primes.size(), per = 2 * N162 * limbs, col = 8 + 4 * per;
-    std::vector<uint8_t> buf(s.r * col);
+    std::vector<uint8_t> buf;
+    try {
+        buf.resize(s.r * col);
+    } catch (const std::exception &) {
+        return fail(LBN_ERR_OOM, "the commitment's transcript bytes");
+    }
     for (size_t c = 0; c < s.r; c++) {
         uint8_t *d = buf.data() + c * col;
         const uint64_t four = 4;
@@ -363,7 +354,11 @@ int lbn_derive_folding_challenges(const lbn_pp *pp, lbn_transcript *tr, const lb
     Transcript &t = tr->t;
     t.absorb_bytes("labinius/row-evaluation", 23);
     t.absorb_bytes(u, 24 * r);
-    sample_short_challenges(t, r, 28, 12.0, out);
+    try {
+        sample_short_challenges(t, r, 28, 12.0, out);
+    } catch (const std::exception &) {  // the readers' list; the transcript has absorbed u
+        return fail(LBN_ERR_OOM, "the challenges' streams");
+    }
     return LBN_OK;
 }
 

@@ -110,7 +110,7 @@ int copy_out(const std::vector<uint8_t> &b, uint8_t *out, size_t cap, size_t *le
     if (len) *len = b.size();
     if (!out) return LBN_OK;
     if (cap < b.size()) return fail(LBN_ERR_SHAPE, "output buffer too small");
-    std::memcpy(out, b.data(), b.size());
+    if (!b.empty()) std::memcpy(out, b.data(), b.size());  // an empty vector's data() may be null
     return LBN_OK;
 }
 
