@@ -106,7 +106,8 @@ static uint32_t width_class(size_t R) {
 }
 
 // Candidate schedules, timed on this device for this width class: tiles per XCD round and a full
-// or a persistent grid (two blocks' worth of rounds per CU).
+// or a persistent grid (one or two blocks' worth of rounds per CU). A persistent grid with every
+// tile in one XCD round keeps each block on one tile, so it replicates the tile's LUT once.
 static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p) {
     lbn_batch_ws *b = pr->batch;
     for (const auto &e : b->tuned)
@@ -116,7 +117,7 @@ static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p
             return LBN_OK;
         }
     static const uint32_t ts[] = {4, 8, 16};
-    static const uint32_t ps[] = {0, 512};
+    static const uint32_t ps[] = {0, 256, 512};
     float best = 1e30f;
     *best_t = 8;
     *best_p = 0;

@@ -124,15 +124,18 @@ constexpr int ZP_BYTES = (GROUP_BATCH * TILE_ROWS * EP * 2 + 15) / 16 * 16;
 constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
 constexpr int A_LDS = AP * GEMM_THREADS * 16;
 // LDS (the kernel's only LDS is the dynamic block, at address 0):
-//   K loop:   [0, 64 K) replicated LUT | plain LUT (1 KB) | two A buffers
-//   epilogue: [0, ZP_BYTES) Z, overwriting the above
+//   K loop:   [0, 64 K) replicated LUT | plain LUT (1 KB) | three A buffers
+//   epilogue: Z in the A buffers' place, so that the LUT outlives the item
 //   E, loaded at item start and read in the epilogue, above both.
+// A block whose next item has the same tile (a persistent grid over one tile's groups) keeps the
+// LUT and E from the item before.
 // The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
 // address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
 constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
-constexpr int EE_OFF = (ABUF_OFF + 3 * A_LDS > ZP_BYTES ? ABUF_OFF + 3 * A_LDS : ZP_BYTES);
+constexpr int EE_OFF = ABUF_OFF + 3 * A_LDS;
 constexpr int GEMM_LDS = EE_OFF + EE_TILE * 2;
-static_assert(ABUF_OFF + 3 * A_LDS <= EE_OFF, "K-loop LDS overlaps E");
+static_assert(ZP_BYTES <= 3 * A_LDS, "the epilogue's Z fits the A buffers");
+static_assert(GEMM_LDS <= 160 * 1024, "the GEMM's LDS fits a CU");
 
 __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
@@ -167,7 +170,7 @@ __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, u
 }
 
 template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem);
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep);
 
 // REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
 // half-wave read distinct banks; its set-up (eight 16-byte stores a thread) pays from 8 K steps.
@@ -175,10 +178,12 @@ template <bool REP>
 __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides
+    uint32_t prev = 0xFFFFFFFFu;
     for (uint32_t item = blockIdx.x; item < g.items; item += gridDim.x) {
         uint32_t tile, grp;
         if (!gemm_item_of(g, item, tile, grp)) continue;
-        gemm_item<REP>(g, tile, grp, smem);
+        gemm_item<REP>(g, tile, grp, smem, tile == prev);
+        prev = tile;
         __syncthreads();  // the next item overwrites the LDS
     }
 }
@@ -189,24 +194,26 @@ typedef __attribute__((address_space(3))) const uint32_t lds_u32;
 __device__ __forceinline__ lds_u32 *lds_word(uint32_t a) { return (lds_u32 *)(uintptr_t)(a + LUTR_OFF); }
 
 template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem) {
- const TileDesc td = g.tiles[tile];
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep) {
+    const TileDesc td = g.tiles[tile];
 
     uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
     uint8_t *abuf = smem + ABUF_OFF;                  // [3][A_LDS]
-    uint16_t *zp = (uint16_t *)smem;                  // [3][82][82], after the loop
+    uint16_t *zp = (uint16_t *)abuf;                  // [3][82][82], after the loop
     uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][82]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t lv = g.lut[td.lut_off + tid];     // entry (parity tid >> 8, byte tid & 255)
-    if (REP) {
-        // its 32 copies: eight 16-byte stores of one value
-        const uint4 v4 = make_uint4(lv, lv, lv, lv);
-        uint8_t *row = smem + LUTR_OFF + (tid & 255) * 256 + (tid >> 8) * 128;
+    if (!keep) {
+        const uint32_t lv = g.lut[td.lut_off + tid];  // entry (parity tid >> 8, byte tid & 255)
+        if (REP) {
+            // its 32 copies: eight 16-byte stores of one value
+            const uint4 v4 = make_uint4(lv, lv, lv, lv);
+            uint8_t *row = smem + LUTR_OFF + (tid & 255) * 256 + (tid >> 8) * 128;
 #pragma unroll
-        for (int c = 0; c < 8; c++) *(uint4 *)(row + 16 * c) = v4;
-    } else {
-        lut[tid] = (uint16_t)lv;
+            for (int c = 0; c < 8; c++) *(uint4 *)(row + 16 * c) = v4;
+        } else {
+            lut[tid] = (uint16_t)lv;
+        }
     }
 
     // v_mfma_i32_16x16x64_i8 operands: lane holds row (A) or input (B) c16 = lane & 15 of a 16-block
@@ -309,8 +316,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
     constexpr int EE_V = EE_TILE * 2 / 16;
     const uint4 *eesrc = (const uint4 *)(g.ee + td.ee_off);
-    uint4 ee0 = eesrc[tid], ee1 = make_uint4(0, 0, 0, 0);
-    if (tid + GEMM_THREADS < EE_V) ee1 = eesrc[tid + GEMM_THREADS];
+    uint4 ee0 = make_uint4(0, 0, 0, 0), ee1 = make_uint4(0, 0, 0, 0);
+    if (!keep) {
+        ee0 = eesrc[tid];
+        if (tid + GEMM_THREADS < EE_V) ee1 = eesrc[tid + GEMM_THREADS];
+    }
     const int slot = tid < 2 * GROUP_BATCH * TILE_ROWS ? g.out[td.out_off + (tid >> 1) % TILE_ROWS] : -1;
 
     const uint32_t kbs = g.kbn;  // steps of this launch, counted from kb0
@@ -322,8 +332,10 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // (waves 4-7), both into the buffer that no wave reads in that interval.
     load_a(0);
     load_raw(0);
-    ((uint4 *)ee)[tid] = ee0;
-    if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
+    if (!keep) {
+        ((uint4 *)ee)[tid] = ee0;
+        if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
+    }
     store_a(0);
     if (wgrp) {
         load_a(kbs > 1 ? 1 : 0);

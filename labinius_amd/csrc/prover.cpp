@@ -501,6 +501,12 @@ int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint
     return LBN_OK;
 }
 
+// The fold GEMM runs on a persistent grid of 256 blocks (one a CU): its 8 tiles over 256 blocks
+// give every block one tile for all its items (gemm_item_of with 8 tiles an XCD round), so the
+// block replicates that tile's LUT and loads its E once, where a full grid paid both per item of
+// only ceil(columns / 64) K steps.
+constexpr uint32_t FOLD_BLOCKS = 256;
+
 static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch, size_t r, int16_t *v) {
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
@@ -525,7 +531,7 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     g.groups = (uint32_t)((s.len_ring + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles_f;
     g.sched_t = 8;
-    HIP_TRY(launch_tile_gemm(st, g, 0));
+    HIP_TRY(launch_tile_gemm(st, g, FOLD_BLOCKS));
     HIP_TRY(hipEventRecord(pr->ev[6], st));
     HIP_TRY(launch_intt(st, pr->d_vhat, (uint32_t)s.len_ring, pp->intt, pp->d_twi, pr->d_v));
     HIP_TRY(hipEventRecord(pr->ev[7], st));
