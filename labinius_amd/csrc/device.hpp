@@ -87,6 +87,7 @@ struct GemmArgs {
     uint32_t groups;          // ceil(batches / 3)
     uint32_t ntiles;
     uint32_t sched_t;         // tiles per XCD round (1 .. 32, a power of two); the rest are groups
+    uint32_t tile_major;      // 1: consecutive rounds share their tiles (and walk the groups)
     uint32_t items;           // work items (set by the launcher)
 };
 

@@ -160,12 +160,18 @@ __device__ __forceinline__ uint32_t mod_q_u64(unsigned long long x, double qd, d
 }
 
 // The block order: item -> (tile, group). The 32 items an XCD runs at once (item & 7 is the XCD a
-// block lands on when the grid is a multiple of 8) are t tiles x 32/t groups.
+// block lands on when the grid is a multiple of 8) are t tiles x 32/t groups. Rounds walk the tile
+// sets first (consecutive rounds share their groups' bytes in L2) or, tile-major, the groups
+// first: on a persistent grid of 256 blocks, block b's items b + 256 k are then rounds k with one
+// tile for ceil(groups / (256 / t)) rounds in a row, which keep that tile's LUT and E.
 __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, uint32_t &tile, uint32_t &grp) {
     const uint32_t t = g.sched_t, xcd = item & 7, s = item >> 3;
     const uint32_t round = s >> 5, within = s & 31, tgroups = (g.ntiles + t - 1) / t;
-    tile = t * (round % tgroups) + within % t;
-    grp = (32 / t) * (8 * (round / tgroups) + xcd) + within / t;
+    const uint32_t gblocks = (g.groups + 256 / t - 1) / (256 / t);
+    const uint32_t ts = g.tile_major ? round / gblocks : round % tgroups;
+    const uint32_t gb = g.tile_major ? round % gblocks : round / tgroups;
+    tile = t * ts + within % t;
+    grp = (32 / t) * (8 * gb + xcd) + within / t;
     return tile < g.ntiles && grp < g.groups;
 }
 
@@ -803,28 +809,36 @@ __global__ __launch_bounds__(256) void claim_kernel(const uint64_t *__restrict__
 //     32 q .. 32 q + 31), by 16 x 16 bit transposes on 16-lane rows (DPP butterflies);
 //   - stages the chunk's eq operands (precomputed once per evaluation by eqfrag_kernel, the same
 //     for every column) in LDS;
-//   - multiplies: wave w owns the W tiles B = w, w + 4, w + 8 (rows 16 B .. 16 B + 15), expands
+//   - multiplies: wave w owns the W tiles B = w, w + 4, w + 8 (rows 16 B .. 16 B + 15), masks
 //     each lane's plane word into its 32 operand nibbles, and runs them against the 11 eq tiles.
 // The f32 counts' parities are then summed along the anti-diagonals b + k (XOR) and reduced.
 // ---------------------------------------------------------------------------------------------
 constexpr int RF_CHUNK = 128, RF_TILES = 11, RF_BITS = 16 * RF_TILES;  // 176
 constexpr int RF_FRAG_U32 = RF_TILES * 64 * 4;                          // u32 of eq operands per chunk
 
-// eq operands: frag[c][N][lane][4 u32], nibble 8 r + j of a lane (register r, bits 4 j .. 4 j + 3)
-// is 2.0 (0x4) when bit 16 N + (lane & 15) of eq_{128 c + 32 (lane >> 4) + 8 r + j} is set.
+// The W operand of a lane takes its plane word's bits by masks, with no spreading: register r,
+// nibble j holds bit 4 j + r of the word (element 4 j + r of the lane's 32) at the nibble's bit r
+// for r < 3 (the values 0.5, 1.0, 2.0) and, shifted down, at bit 0 for r = 3 (0.5). The eq
+// operand's register r, nibble j then is eq's bit for the same element with the value that makes
+// each product 1: 2.0, 1.0, 0.5, 2.0 (0x4, 0x2, 0x1, 0x4) for r = 0 .. 3.
+
+// eq operands: frag[c][N][lane][4 u32]; register r, nibble j (bits 4 j .. 4 j + 3) of a lane is
+// that value when bit 16 N + (lane & 15) of eq_{128 c + 32 (lane >> 4) + 4 j + r} is set.
 __global__ __launch_bounds__(256) void eqfrag_kernel(const uint64_t *__restrict__ eq, uint32_t nch,
                                                      uint32_t *__restrict__ frag) {
     const uint32_t id = blockIdx.x * 256 + threadIdx.x;
     if (id >= nch * RF_TILES * 64) return;
     const uint32_t lane = id & 63, nt = (id >> 6) % RF_TILES, c = id / (64 * RF_TILES);
     const uint32_t bit = 16 * nt + (lane & 15);
+    constexpr uint32_t nibv[4] = {0x4u, 0x2u, 0x1u, 0x4u};
     uint32_t r4[4] = {0, 0, 0, 0};
     if (bit < 162) {
         const uint64_t *e = eq + 3 * ((size_t)RF_CHUNK * c + 32 * (lane >> 4)) + (bit >> 6);
 #pragma unroll
         for (int r = 0; r < 4; r++)
 #pragma unroll
-            for (int j = 0; j < 8; j++) r4[r] |= (uint32_t)((e[3 * (8 * r + j)] >> (bit & 63)) & 1) << (4 * j + 2);
+            for (int j = 0; j < 8; j++)
+                r4[r] |= (uint32_t)((e[3 * (4 * j + r)] >> (bit & 63)) & 1) * (nibv[r] << (4 * j));
     }
     uint4 *o = (uint4 *)frag + (size_t)id;
     *o = make_uint4(r4[0], r4[1], r4[2], r4[3]);
@@ -857,12 +871,6 @@ __device__ __forceinline__ void bt16_lane(uint32_t x16, uint32_t m, uint32_t &ro
     keep = (x16 & D) ? ~m : m;
 }
 
-// 8 bits -> 8 nibbles of 0.5 (0x1): bit j to bit 4 j
-__device__ __forceinline__ uint32_t spread_nibbles(uint32_t x) {
-    x = (x | (x << 12)) & 0x000F000Fu;
-    x = (x | (x << 6)) & 0x03030303u;
-    return (x | (x << 3)) & 0x11111111u;
-}
 
 // A block takes one column and a span of chunks, RF_G chunks a phase: the phase's W words and eq
 // operands are loaded one phase ahead into registers; each 16-lane row transposes 16 elements x
@@ -922,8 +930,8 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
 #pragma unroll
             for (int a = 0; a < NB; a++) {
                 const uint32_t pw = planes[c][lane >> 4][16 * (wave + 4 * a) + (lane & 15)];
-                aop[a] = (v8i){(int)spread_nibbles(pw & 0xFFu), (int)spread_nibbles((pw >> 8) & 0xFFu),
-                               (int)spread_nibbles((pw >> 16) & 0xFFu), (int)spread_nibbles(pw >> 24), 0, 0, 0, 0};
+                aop[a] = (v8i){(int)(pw & 0x11111111u), (int)(pw & 0x22222222u), (int)(pw & 0x44444444u),
+                               (int)((pw >> 3) & 0x11111111u), 0, 0, 0, 0};
             }
 #pragma unroll
             for (int n = 0; n < RF_TILES; n++) {

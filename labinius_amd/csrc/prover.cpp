@@ -418,6 +418,16 @@ struct BytesGuard {
 // few columns), 32 tiles x 1 group, so that the groups spread over the 8 XCDs instead of two.
 static uint32_t commit_sched(uint32_t groups) { return groups < 32 ? 32 : 8; }
 
+// The commit GEMM of a wide layout runs tile-major on a persistent grid of one block a CU: each
+// block keeps one tile (its replicated LUT and its E in LDS) for ceil(groups / 32) items in a row,
+// while the XCD's 8 tiles x 4 groups a round still share A and the groups' bytes in L2.
+constexpr uint32_t COMMIT_BLOCKS = 256;
+static void commit_schedule(GemmArgs &g) {
+    g.sched_t = commit_sched(g.groups);
+    g.tile_major = g.groups >= 64;
+}
+static uint32_t commit_blocks(const GemmArgs &g) { return g.tile_major ? COMMIT_BLOCKS : 0; }
+
 static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
@@ -447,8 +457,8 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     g.batches = (uint32_t)s.r;
     g.groups = (uint32_t)((s.r + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
-    g.sched_t = commit_sched(g.groups);
-    HIP_TRY(launch_tile_gemm(st, g, 0));
+    commit_schedule(g);
+    HIP_TRY(launch_tile_gemm(st, g, commit_blocks(g)));
     HIP_TRY(hipEventRecord(pr->ev[2], st));
     HIP_TRY(launch_commit_finish(st, pr->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, pr->d_raw,
                                  pr->d_comp, (uint32_t)s.r));
@@ -494,8 +504,8 @@ int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint
     g.batches = (uint32_t)ncol;
     g.groups = (uint32_t)((ncol + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
-    g.sched_t = commit_sched(g.groups);
-    HIP_TRY(launch_tile_gemm(st, g, 0));
+    commit_schedule(g);
+    HIP_TRY(launch_tile_gemm(st, g, commit_blocks(g)));
     HIP_TRY(launch_commit_finish(st, d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, d_raw, d_comp,
                                  (uint32_t)ncol));
     return LBN_OK;
