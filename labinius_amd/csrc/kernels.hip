@@ -862,7 +862,8 @@ __device__ __forceinline__ uint32_t dpp_xor(uint32_t x) {
 template <int D>
 __device__ __forceinline__ uint32_t bt16_stage(uint32_t v, uint32_t rot, uint32_t keep) {
     const uint32_t y = dpp_xor<D>(v);
-    return (v & keep) | (__builtin_amdgcn_alignbit(y, y, rot) & ~keep);
+    // keep ? v : rotated y, bit by bit: one v_bitop3 (truth table 0xE4 over v, y', keep)
+    return __builtin_amdgcn_bitop3_b32(v, __builtin_amdgcn_alignbit(y, y, rot), keep, 0xE4);
 }
 // a lane's (rot, keep) for stage D with positions-without-D mask m
 template <int D>
@@ -871,6 +872,10 @@ __device__ __forceinline__ void bt16_lane(uint32_t x16, uint32_t m, uint32_t &ro
     keep = (x16 & D) ? ~m : m;
 }
 
+
+// An FP4 operand is the instruction's first four registers (32 nibbles); it reads no more, so the
+// other four are left undefined rather than zeroed.
+__device__ __forceinline__ v8i fp4_operand(v4i x) { return __builtin_shufflevector(x, x, 0, 1, 2, 3, -1, -1, -1, -1); }
 
 // A block takes one column and a span of chunks, RF_G chunks a phase: the phase's W words and eq
 // operands are loaded one phase ahead into registers; each 16-lane row transposes 16 elements x
@@ -882,7 +887,7 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
                                                              const uint4 *__restrict__ frag, uint32_t wdim,
                                                              uint32_t cols, uint32_t span,
                                                              unsigned long long *__restrict__ u) {
-    __shared__ uint4 ebuf[RF_G][RF_TILES * 64];
+    __shared__ v4i ebuf[RF_G][RF_TILES * 64];  // vector types: struct copies would go through scratch
     __shared__ uint32_t planes[RF_G][4][RF_BITS];
     __shared__ uint64_t red[4][6];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, x16 = tid & 15;
@@ -902,18 +907,19 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
     // per chunk, this thread's three transposition tasks (g, s) = (t / 6, t % 6), t = 16 p + tid / 16:
     // word s of element 16 g + x16; and three of the chunk's 704 eq operands
     uint32_t word[RF_G][3];
-    uint4 ef[RF_G][3];
+    v4i ef[RF_G][3];
+    // A chunk past the span (its MFMAs are skipped) and operand slots past the 704 (never stored)
+    // load from clamped addresses inside the column, with no zero fill.
     auto fetch = [&](uint32_t ph) {
 #pragma unroll
         for (int c = 0; c < RF_G; c++) {
-            const uint32_t ch = ph + c;
-            const bool live = ch < ch1;
+            const uint32_t ch = min(ph + c, ch1 - 1);
 #pragma unroll
             for (int p = 0; p < 3; p++) {
                 const uint32_t t = 16 * p + (tid >> 4), g = t / 6, sw = t % 6;
-                word[c][p] = live ? wc[((size_t)RF_CHUNK * ch + 16 * g + x16) * 6 + sw] : 0u;
-                const uint32_t f = tid + 256 * p;
-                ef[c][p] = live && f < RF_TILES * 64 ? frag[(size_t)ch * RF_TILES * 64 + f] : make_uint4(0, 0, 0, 0);
+                word[c][p] = wc[((size_t)RF_CHUNK * ch + 16 * g + x16) * 6 + sw];
+                const uint32_t f = min(tid + 256 * p, (uint32_t)(RF_TILES * 64 - 1));
+                ef[c][p] = ((const v4i *)frag)[(size_t)ch * RF_TILES * 64 + f];
             }
         }
     };
@@ -923,21 +929,22 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
 #pragma unroll
         for (int c = 0; c < RF_G; c++) {
             if (ph + c >= ch1) break;  // block-uniform
-            uint4 bq[RF_LOOK];
+            v4i bq[RF_LOOK];
 #pragma unroll
             for (int n = 0; n < RF_LOOK; n++) bq[n] = ebuf[c][n * 64 + lane];
             v8i aop[NB];
 #pragma unroll
             for (int a = 0; a < NB; a++) {
                 const uint32_t pw = planes[c][lane >> 4][16 * (wave + 4 * a) + (lane & 15)];
-                aop[a] = (v8i){(int)(pw & 0x11111111u), (int)(pw & 0x22222222u), (int)(pw & 0x44444444u),
-                               (int)((pw >> 3) & 0x11111111u), 0, 0, 0, 0};
+                const v4i m4 = {(int)(pw & 0x11111111u), (int)(pw & 0x22222222u), (int)(pw & 0x44444444u),
+                                (int)((pw >> 3) & 0x11111111u)};
+                aop[a] = fp4_operand(m4);
             }
 #pragma unroll
             for (int n = 0; n < RF_TILES; n++) {
-                const uint4 b4 = bq[n % RF_LOOK];
+                const v4i b4 = bq[n % RF_LOOK];
                 if (n + RF_LOOK < RF_TILES) bq[n % RF_LOOK] = ebuf[c][(n + RF_LOOK) * 64 + lane];
-                const v8i bop = (v8i){(int)b4.x, (int)b4.y, (int)b4.z, (int)b4.w, 0, 0, 0, 0};
+                const v8i bop = fp4_operand(b4);
 #pragma unroll
                 for (int a = 0; a < NB; a++)
                     acc[a][n] =
@@ -961,13 +968,15 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
                 v = bt16_stage<2>(v, r2, k2);
                 v = bt16_stage<1>(v, r1, k1);
                 // low half: bit 32 s + x16 of elements 16 g .. 16 g + 15; high half: bit 32 s + 16 + x16
+                // (planes of bits 162 .. 175 are whatever the words hold there: their counts are
+                // dropped below)
                 uint16_t *pl = (uint16_t *)&planes[c][g >> 1][0] + (g & 1);
                 const uint32_t b0 = 32 * sw + x16, b1 = b0 + 16;
-                pl[2 * b0] = b0 < 162 ? (uint16_t)v : (uint16_t)0;
-                if (b1 < RF_BITS) pl[2 * b1] = b1 < 162 ? (uint16_t)(v >> 16) : (uint16_t)0;
+                pl[2 * b0] = (uint16_t)v;
+                if (b1 < RF_BITS) pl[2 * b1] = (uint16_t)(v >> 16);
             }
         __syncthreads();
-        if (ph + RF_G < ch1) fetch(ph + RF_G);
+        fetch(ph + RF_G);  // the next phase's, or (past the span) clamped loads nobody uses
         if (wave == 3)
             mma(std::integral_constant<int, 2>{}, ph);
         else
@@ -984,7 +993,8 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
         for (int n = 0; n < RF_TILES; n++) {
             uint64_t nib = 0;
 #pragma unroll
-            for (int r = 0; r < 4; r++) nib |= (uint64_t)((uint32_t)acc[a][n][r] & 1u) << r;
+            for (int r = 0; r < 4; r++)  // rows of W bits 162 .. 175 (tile 10) are padding
+                if (16 * B + 4 * (int)(lane >> 4) + r < 162) nib |= (uint64_t)((uint32_t)acc[a][n][r] & 1u) << r;
             const int pos = 16 * (B + n);  // B is wave-uniform, so the word is too
             pl[pos >> 6] ^= nib << (pos & 63);
         }
