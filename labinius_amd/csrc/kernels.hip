@@ -910,16 +910,24 @@ __global__ __launch_bounds__(256, 2) void row_eval_f4_kernel(const uint32_t *__r
     v4i ef[RF_G][3];
     // A chunk past the span (its MFMAs are skipped) and operand slots past the 704 (never stored)
     // load from clamped addresses inside the column, with no zero fill.
+    // The loads are a chunk's uniform base plus this thread's fixed offsets.
+    uint32_t woff[3], foff[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const uint32_t t = 16 * p + (tid >> 4), g = t / 6, sw = t % 6;
+        woff[p] = (16 * g + x16) * 6 + sw;
+        foff[p] = min(tid + 256 * p, (uint32_t)(RF_TILES * 64 - 1));
+    }
     auto fetch = [&](uint32_t ph) {
 #pragma unroll
         for (int c = 0; c < RF_G; c++) {
             const uint32_t ch = min(ph + c, ch1 - 1);
+            const uint32_t *wch = wc + (size_t)ch * RF_CHUNK * 6;
+            const v4i *fch = (const v4i *)frag + (size_t)ch * RF_TILES * 64;
 #pragma unroll
             for (int p = 0; p < 3; p++) {
-                const uint32_t t = 16 * p + (tid >> 4), g = t / 6, sw = t % 6;
-                word[c][p] = wc[((size_t)RF_CHUNK * ch + 16 * g + x16) * 6 + sw];
-                const uint32_t f = min(tid + 256 * p, (uint32_t)(RF_TILES * 64 - 1));
-                ef[c][p] = ((const v4i *)frag)[(size_t)ch * RF_TILES * 64 + f];
+                word[c][p] = wch[woff[p]];
+                ef[c][p] = fch[foff[p]];
             }
         }
     };
