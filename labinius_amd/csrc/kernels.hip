@@ -178,6 +178,19 @@ __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, u
 template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep);
 
+// The fourth word of a raw buffer descriptor (no stride, no format conversion) on this family.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+
+// A raw buffer descriptor over [p, p + bytes) for a block-uniform p (read from the first lane, so
+// the compiler keeps the descriptor in scalar registers).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void *p, uint32_t bytes) {
+    const uint64_t v = (uint64_t)p;
+    const uint64_t u = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)u, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes),
+                                             BUF_RSRC_WORD3);
+}
+
 // REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
 // half-wave read distinct banks; its set-up (eight 16-byte stores a thread) pays from 8 K steps.
 template <bool REP>
@@ -243,14 +256,20 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         L[nb] = lut + (par << 8);
     }
 
-    // the (group, wave) byte fragments: [kb][kh 2][lane 64][16]; this lane's two pieces a step
-    const uint8_t *bp = g.bytes + (((uint64_t)grp * GROUP_FRAGS + wave) * g.kbs * 2 * 64 + kh * 64 + off32) * 16;
-    const int8_t *abase = g.afrag + td.a_off;
+    // the (group, wave) byte fragments: [kb][kh 2][lane 64][16]; this lane's two pieces a step. Both
+    // streams are raw buffer loads: the region's base in the (uniform) descriptor, the step in the
+    // scalar offset, the lane's fixed offset in the vector one, so a step's address is no VALU work.
+    const uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);
+    const uint8_t *bbase = g.bytes + ((uint64_t)grp * GROUP_FRAGS + wave_u) * g.kbs * 2 * 64 * 16;
+    const __amdgpu_buffer_rsrc_t brs = uniform_rsrc(bbase, g.kbs * 2048u);
+    const uint32_t boff = (kh * 64 + off32) * 16;
+    const __amdgpu_buffer_rsrc_t ars = uniform_rsrc(g.afrag + td.a_off, g.kbs * g.a_kb_stride);
 
-    uint4 raw[2];
+    v4i raw[2];
     auto load_raw = [&](uint32_t kb) {
 #pragma unroll
-        for (int nb = 0; nb < 2; nb++) raw[nb] = *(const uint4 *)(bp + (uint64_t)(g.kb0 + kb) * 2048 + 256 * nb);
+        for (int nb = 0; nb < 2; nb++)
+            raw[nb] = __builtin_amdgcn_raw_buffer_load_b128(brs, boff + 256 * nb, (g.kb0 + kb) * 2048u, 0);
     };
     v4i bl[2], bh[2];
     auto build = [&]() {
@@ -258,7 +277,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         // all 32 lookups first, so that their latencies overlap
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) {
-            const uint32_t w4[4] = {raw[nb].x, raw[nb].y, raw[nb].z, raw[nb].w};
+            const uint32_t w4[4] = {(uint32_t)raw[nb][0], (uint32_t)raw[nb][1], (uint32_t)raw[nb][2],
+                                    (uint32_t)raw[nb][3]};
 #pragma unroll
             for (int d = 0; d < 4; d++)
 #pragma unroll
@@ -286,11 +306,13 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // then count the loads in flight and wait for the input bytes alone. Words past the step's
     // 768 repeat the last one and land in the buffer's padding.
     v4i apre[AP];
+    uint32_t aoffs[AP];
+#pragma unroll
+    for (int i = 0; i < AP; i++) aoffs[i] = min(tid + GEMM_THREADS * i, A_STEP / 16 - 1) * 16;
     auto load_a = [&](uint32_t kb) {
-        const int8_t *ak = abase + (size_t)(g.kb0 + kb) * g.a_kb_stride;
 #pragma unroll
         for (int i = 0; i < AP; i++)
-            apre[i] = *(const v4i *)(ak + min(tid + GEMM_THREADS * i, A_STEP / 16 - 1) * 16);
+            apre[i] = __builtin_amdgcn_raw_buffer_load_b128(ars, aoffs[i], (g.kb0 + kb) * g.a_kb_stride, 0);
     };
     auto store_a = [&](int buf) {
 #pragma unroll
@@ -298,12 +320,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     };
     // this lane's A pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
     // (mb16 < 5, rows 0..79) and, for the sixth block, rows 80 and 81: lane c16 < 4 holds digit
-    // c16 & 1 of row 80 + (c16 >> 1), the other lanes zero
+    // c16 & 1 of row 80 + (c16 >> 1); the other lanes read row 82, which every tile's A keeps zero
+    // (a tile uses at most 82 rows)
     const int aoff = (kh * 2 * NBLK * 64 + off32) * 16;
-    const int xdig = c16 & 1, xrow = (c16 >> 1) & 1;
+    const int xdig = c16 & 1, xrow = c16 < 4 ? (c16 >> 1) & 1 : 2;
     const int xoff = ((kh * 2 + xdig) * NBLK * 64 + 2 * 64 + 16 + xrow + 32 * (kg & 1)) * 16;
-    const int xm = c16 < 4 ? -1 : 0;  // a mask rather than a select, which would branch in the loop
-    const v4i xmask = (v4i){xm, xm, xm, xm};
 
     // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the sixth block's [nb][B digit]
     v4i acc[5][2][3], xacc[2][2];
@@ -382,7 +403,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], acc[mb][nb][1], 0, 0, 0);
         {
-            const v4i ax = *(const v4i *)(ab + xoff) & xmask;
+            const v4i ax = *(const v4i *)(ab + xoff);
 #pragma unroll
             for (int nb = 0; nb < 2; nb++) {
                 xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], xacc[nb][0], 0, 0, 0);
