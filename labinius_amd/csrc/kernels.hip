@@ -302,21 +302,18 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             bh[nb] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
-    // Every thread loads and stores AP 16-byte words of A a step, with no branch: the compiler can
-    // then count the loads in flight and wait for the input bytes alone. Words past the step's
+    // A goes from global memory straight into an LDS buffer (buffer loads to LDS, no registers):
+    // every thread moves AP 16-byte words a step, a wave 64 consecutive ones; words past the step's
     // 768 repeat the last one and land in the buffer's padding.
-    v4i apre[AP];
     uint32_t aoffs[AP];
 #pragma unroll
     for (int i = 0; i < AP; i++) aoffs[i] = min(tid + GEMM_THREADS * i, A_STEP / 16 - 1) * 16;
-    auto load_a = [&](uint32_t kb) {
+    auto dma_a = [&](uint32_t kb, int buf) {
 #pragma unroll
         for (int i = 0; i < AP; i++)
-            apre[i] = __builtin_amdgcn_raw_buffer_load_b128(ars, aoffs[i], (g.kb0 + kb) * g.a_kb_stride, 0);
-    };
-    auto store_a = [&](int buf) {
-#pragma unroll
-        for (int i = 0; i < AP; i++) *(v4i *)(abuf + buf * A_LDS + (tid + GEMM_THREADS * i) * 16) = apre[i];
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                ars, (__attribute__((address_space(3))) void *)(abuf + buf * A_LDS + (64 * wave_u + GEMM_THREADS * i) * 16),
+                16, aoffs[i], (g.kb0 + kb) * g.a_kb_stride, 0, 0);
     };
     // this lane's A pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
     // (mb16 < 5, rows 0..79) and, for the sixth block, rows 80 and 81: lane c16 < 4 holds digit
@@ -356,35 +353,35 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // two, waves 4-7 before the products, so one wave's lookups run while the other's products
     // are in the MFMA pipe. A is then read for two steps at once (waves 0-3 one step ahead) and
     // is staged in three buffers; the step that a thread stages is kb + 1 (waves 0-3) or kb + 2
-    // (waves 4-7), both into the buffer that no wave reads in that interval.
-    load_a(0);
+    // (waves 4-7), both into the buffer that no wave reads in that interval. A wave issues its A
+    // copy and the next step's input bytes just before its products, which read no LDS, so both
+    // land while the products run; the compiler waits for them before the next LDS access.
+    dma_a(0, 0);
     load_raw(0);
     if (!keep) {
         ((uint4 *)ee)[tid] = ee0;
         if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
     }
-    store_a(0);
-    if (wgrp) {
-        load_a(kbs > 1 ? 1 : 0);
-        store_a(1);
-    }
+    if (wgrp) dma_a(kbs > 1 ? 1 : 0, 1);
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
     __syncthreads();  // the LUT, E, A(0) and waves 4-7's part of A(1)
-    // the next step's A pieces (both digits) are read ahead, at the end of the step before
-    v4i a0[5], a1[5];
+    // the next step's A pieces (both digits, and the sixth block's) are read ahead, at the end of
+    // the step before
+    v4i a0[5], a1[5], ax;
     auto read_a0 = [&](const uint8_t *ab) {
 #pragma unroll
         for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
             a1[mb] = *(const v4i *)(ab + aoff + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+        ax = *(const v4i *)(ab + xoff);
     };
     read_a0(abuf);
     build();
-    load_raw(kbs > 1 ? 1 : 0);
-    load_a(min(1u + wgrp, kbs - 1));
     for (uint32_t kb = 0; kb < kbs; kb++) {
         if (wgrp) __syncthreads();
-        const uint8_t *ab = abuf + (kb % 3) * A_LDS;
+        dma_a(min(kb + 1 + wgrp, kbs - 1), (int)((kb + 1 + wgrp) % 3));
+        load_raw(min(kb + 1, kbs - 1));
         // the wave issuing products goes first at the SIMD's arbiter, the other wave's lookups
         // fill in around them (the wave's own priority; the stream's stays the default)
         __builtin_amdgcn_s_setprio(1);
@@ -402,13 +399,10 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], acc[mb][nb][1], 0, 0, 0);
-        {
-            const v4i ax = *(const v4i *)(ab + xoff);
 #pragma unroll
-            for (int nb = 0; nb < 2; nb++) {
-                xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], xacc[nb][0], 0, 0, 0);
-                xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], xacc[nb][1], 0, 0, 0);
-            }
+        for (int nb = 0; nb < 2; nb++) {
+            xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], xacc[nb][0], 0, 0, 0);
+            xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], xacc[nb][1], 0, 0, 0);
         }
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
@@ -421,13 +415,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
         __builtin_amdgcn_s_setprio(0);
-        store_a((kb + 1 + wgrp) % 3);
-        load_a(min(kb + 2 + wgrp, kbs - 1));
         if (!wgrp) __syncthreads();
-        build();                          // step kb + 1 (its bytes are loaded)
-        load_raw(min(kb + 2, kbs - 1));   // a whole step ahead
+        build();                                 // step kb + 1 (its bytes were loaded before the products)
         read_a0(abuf + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
     }
+    __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
     __syncthreads();  // waves 4-7's last products read the LDS that the epilogue overwrites
 
     // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zp[b][m][n] for this lane's inputs
