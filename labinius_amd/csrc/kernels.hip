@@ -178,6 +178,12 @@ __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, u
 template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep);
 
+// s_waitcnt immediates (this family's encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
+// 11:8): wait until at most n vector-memory operations of this wave are outstanding, nothing else.
+// Copies into LDS by buffer loads count there and are not waited for by a barrier; a wave waits
+// for its own before the barrier that publishes them.
+constexpr int waitcnt_vm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+
 // The fourth word of a raw buffer descriptor (no stride, no format conversion) on this family.
 constexpr int BUF_RSRC_WORD3 = 0x00020000;
 
@@ -379,7 +385,10 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     read_a0(abuf);
     build();
     for (uint32_t kb = 0; kb < kbs; kb++) {
-        if (wgrp) __syncthreads();
+        if (wgrp) {
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // this wave's copy of A(kb + 1) is in LDS
+            __syncthreads();
+        }
         dma_a(min(kb + 1 + wgrp, kbs - 1), (int)((kb + 1 + wgrp) % 3));
         load_raw(min(kb + 1, kbs - 1));
         // the wave issuing products goes first at the SIMD's arbiter, the other wave's lookups
@@ -415,7 +424,10 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
         __builtin_amdgcn_s_setprio(0);
-        if (!wgrp) __syncthreads();
+        if (!wgrp) {
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // this wave's copy of A(kb + 1) is in LDS
+            __syncthreads();
+        }
         build();                                 // step kb + 1 (its bytes were loaded before the products)
         read_a0(abuf + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
     }
