@@ -23,6 +23,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 namespace lbn {
 
@@ -141,18 +142,17 @@ __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups,
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
 }
 
-// ll + 2^7 mid + 2^14 hh mod q, in [0, q), from the three int32 accumulators of a product. The sum
-// is exact in f64 (|.| < 2^46); t = rint(z / q) is within 2^-17 of z / q's nearest integer, so the
-// exact remainder z - t q (one fma) lies in (-q, q).
-__device__ __forceinline__ uint32_t mod_q_classes(int ll, int mid, int hh, double qd, double qinv) {
+// ll + 2^7 mid + 2^14 hh mod q as a centred residue, from the three int32 accumulators of a
+// product. The sum is exact in f64 (|.| < 2^46); t = rint(z / q) is within 2^-17 of z / q's
+// nearest integer, so the exact remainder z - t q (one fma) lies in [-(q + 1) / 2, (q + 1) / 2].
+__device__ __forceinline__ int mod_q_classes(int ll, int mid, int hh, double qd, double qinv) {
     const double z = __builtin_fma(16384.0, (double)hh, __builtin_fma(128.0, (double)mid, (double)ll));
     const double t = __builtin_rint(z * qinv);
-    const int r = (int)__builtin_fma(-t, qd, z);
-    return (uint32_t)(r < 0 ? r + (int)qd : r);
+    return (int)__builtin_fma(-t, qd, z);
 }
 
-// x mod q for x < 2^52, the same way
-__device__ __forceinline__ uint32_t mod_q_u64(unsigned long long x, double qd, double qinv) {
+// x mod q in [0, q) for |x| < 2^52, the same way
+__device__ __forceinline__ uint32_t mod_q_i64(long long x, double qd, double qinv) {
     const double z = (double)x;
     const double t = __builtin_rint(z * qinv);
     const int r = (int)__builtin_fma(-t, qd, z);
@@ -434,7 +434,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
     __syncthreads();  // waves 4-7's last products read the LDS that the epilogue overwrites
 
-    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q into zp[b][m][n] for this lane's inputs
+    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q, centred (int16), into zp[b][m][n] for this lane's inputs
     // j = 81 b + n; the D block of a 16 x 16 product holds, in register r, row 4 kg + r and
     // column c16. In the sixth block, rows 0..3 (lanes of K group 0) are row 80's and row 81's
     // digit products: with B's low digit [l80, h80, l81, h81] x l, with its high digit x h.
@@ -460,23 +460,25 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     }
     __syncthreads();
     // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (pairs of n
-    // 0..20 and 21..40), each pair one v_dot2_u32_u16: E, Z < q < 2^14.3, so five pairs stay
-    // within a u32 before it is added to the u64 sum
+    // 0..20 and 21..40), each pair one v_dot2_i32_i16: 0 <= E < q and |Z| <= (q + 1) / 2 with
+    // q <= 19441, so five pairs (ten products) stay within an int32 before it is added to the
+    // int64 sum
     {
         const int o = tid >> 1, half = tid & 1;
         const int b = o / TILE_ROWS, m = o % TILE_ROWS;
         const uint32_t *ep = (const uint32_t *)(ee + m * EP) + 21 * half;
         const uint32_t *zq = (const uint32_t *)(zp + (b * TILE_ROWS + m) * EP) + 21 * half;
         const int np = 21 - half;  // pairs of this half
-        unsigned long long sum = 0;
+        static_assert(10ll * (19441 - 1) * ((19441 + 1) / 2) < (1ll << 31), "five pairs fit an int32 (q <= 19441)");
+        long long sum = 0;
         if (tid < 2 * GROUP_BATCH * TILE_ROWS) {
 #pragma unroll
             for (int c = 0; c < 21; c += 5) {
-                uint32_t s5 = 0;
+                int s5 = 0;
 #pragma unroll
                 for (int i = c; i < c + 5 && i < 21; i++)
                     if (i < np)
-                        s5 = __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, ep[i]), __builtin_bit_cast(u16x2, zq[i]),
+                        s5 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, ep[i]), __builtin_bit_cast(s16x2, zq[i]),
                                                     s5, false);
                 sum += s5;
             }
@@ -484,7 +486,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         sum += __shfl_xor(sum, 1, 64);
         const uint32_t batch = grp * GROUP_BATCH + b;
         if (half || slot < 0 || batch >= g.batches) return;  // slot < 0 for tid >= 492 too
-        const uint32_t v = mod_q_u64(sum, qd, qinv);  // sum < 81 q^2 < 2^35
+        const uint32_t v = mod_q_i64(sum, qd, qinv);  // |sum| < 81 q^2 / 2 < 2^34
         uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
         if (g.add) {
             const uint32_t a = *yp + v;  // both in [0, q)
