@@ -318,7 +318,7 @@ def batched(lbn, reps):
             "one_at_a_time_p99_ms": round(float(np.percentile(lat, 99)), 3),
             "speedup": round(one_ms / batch_ms, 3),
             "autotuned": {"tiles_per_xcd_round": info["tiles_per_xcd_round"],
-                          "persistent_blocks": info["persistent_blocks"]},
+                          "persistent_blocks": info["persistent_blocks"], "tile_major": info["tile_major"]},
             "graph": "one hipGraph launch per batch (layout, GEMM, finish)"}
 
 

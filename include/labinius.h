@@ -190,7 +190,8 @@ int lbn_witness_upload_params(lbn_prover *prover, const lbn_params *params, cons
                               lbn_dwitness **out);
 int lbn_commit_batch(lbn_prover *prover, const lbn_dwitness *const *ws, size_t count, int16_t *const *components,
                      uint32_t *const *raw);
-/* The last batch's schedule (tiles per XCD round, persistent blocks or 0) and its device ms. */
+/* The last batch's schedule (tiles per XCD round; persistent blocks or 0, with bit 31 set when its
+ * rounds run tile-major) and its device ms. */
 int lbn_batch_info(const lbn_prover *prover, uint32_t *sched_t, uint32_t *persistent_blocks, float *ms);
 
 /* The serving loop (no counterpart in the reference, whose callers run one round at a time): a

@@ -66,6 +66,10 @@ static int batch_reserve(lbn_prover *pr, size_t cols) {
     return LBN_OK;
 }
 
+// A schedule's persistent-grid word: the block count, and this bit when the rounds run tile-major
+// (gemm_item_of), so that a block keeps its tile's LUT and E over consecutive items.
+constexpr uint32_t TILE_MAJOR = 0x80000000u;
+
 // The kernel sequence of one batched commitment over R columns.
 static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t persist, bool with_prepass) {
     const lbn_pp *pp = pr->pp;
@@ -91,7 +95,8 @@ static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t per
     g.groups = (uint32_t)((R + GROUP_BATCH - 1) / GROUP_BATCH);
     g.ntiles = pp->ntiles;
     g.sched_t = sched_t;
-    HIP_TRY(launch_tile_gemm(st, g, persist));
+    g.tile_major = (persist & TILE_MAJOR) ? 1 : 0;
+    HIP_TRY(launch_tile_gemm(st, g, persist & ~TILE_MAJOR));
     HIP_TRY(launch_commit_finish(st, b->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, b->d_raw,
                                  b->d_comp, (uint32_t)R));
     return LBN_OK;
@@ -106,8 +111,9 @@ static uint32_t width_class(size_t R) {
 }
 
 // Candidate schedules, timed on this device for this width class: tiles per XCD round and a full
-// or a persistent grid (one or two blocks' worth of rounds per CU). A persistent grid with every
-// tile in one XCD round keeps each block on one tile, so it replicates the tile's LUT once.
+// or a persistent grid (one or two blocks' worth of rounds per CU), the rounds of a persistent
+// grid walking the tile sets first or (tile-major) the groups first. A block whose consecutive
+// items share their tile keeps its LUT and E in LDS.
 static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p) {
     lbn_batch_ws *b = pr->batch;
     for (const auto &e : b->tuned)
@@ -117,7 +123,7 @@ static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p
             return LBN_OK;
         }
     static const uint32_t ts[] = {4, 8, 16};
-    static const uint32_t ps[] = {0, 256, 512};
+    static const uint32_t ps[] = {0, 256, 512, 256 | TILE_MAJOR, 512 | TILE_MAJOR};
     float best = 1e30f;
     *best_t = 8;
     *best_p = 0;
