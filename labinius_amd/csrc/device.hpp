@@ -119,6 +119,7 @@ struct InttConsts {
     uint32_t quad;
     uint32_t w, w2, z6, inv2, inv3, det;
     uint32_t twi_off[7];    // offset of level l's inverse twiddles in the twi array
+    float qinv;             // 1 / q, for mulmod_f
 };
 
 // Mixed-radix (Garner) digits of the bit-dropped commitment (bd/digits.rs:134-182): x = m_0 +

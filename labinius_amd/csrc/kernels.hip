@@ -569,11 +569,19 @@ __global__ void chat_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r
     const uint32_t ex = expo[tau * TILE + m];
     if (c < r && ex != 0xFFFF) {
         const lbn_challenge_dev &cc = ch[c];
+        // the root exponent of position p is a p mod cond, a = mult ex mod cond: a p < cond 2^8 <
+        // 2^19 is exact in float, so its quotient from the float reciprocal is within one
+        const uint32_t a = (mult * ex) % cond;
+        const float cinv = 1.0f / (float)cond;
         int acc = 0;
         for (uint32_t i = 0; i < cc.weight; i++) {
             const uint32_t p = cc.positions[i];
             const int co = (1 - 2 * (int)((cc.signs >> i) & 1)) * ((p & 1) ? -1 : 1);
-            acc += co * (int)pw[(mult * ex * p) % cond];
+            const uint32_t ap = a * p;
+            int e = (int)(ap - (uint32_t)((float)ap * cinv) * cond);
+            e += e < 0 ? (int)cond : 0;
+            e -= e >= (int)cond ? (int)cond : 0;
+            acc += co * (int)pw[e];
         }
         int v = acc % (int)q;
         if (v < 0) v += q;
@@ -593,9 +601,19 @@ __global__ void chat_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r
 // the inverse transform of the base prime (scalar.rs:123-177, recursion/limbs.rs:76-127)
 // =============================================================================================
 
-__device__ __forceinline__ uint32_t mm(uint32_t a, uint32_t b, uint32_t q) {
-    return (uint32_t)(((uint64_t)a * b) % q);
+// a b mod q for a < 4q, b < q, q <= 19441: the product (< 2^31) by a 24-bit multiply, its quotient
+// by q from the float reciprocal (relative error below 2^-22, so within one of the true quotient),
+// and at most one correction either way.
+__device__ __forceinline__ uint32_t mulmod_f(uint32_t a, uint32_t b, uint32_t q, float qinv) {
+    const uint32_t p = __umul24(a, b);
+    const uint32_t t = (uint32_t)((float)p * qinv);
+    int r = (int)(p - __umul24(t, q));
+    r += r < 0 ? (int)q : 0;
+    r -= r >= (int)q ? (int)q : 0;
+    return (uint32_t)r;
 }
+// x mod q for x < 2q
+__device__ __forceinline__ uint32_t sub_q(uint32_t x, uint32_t q) { return x >= q ? x - q : x; }
 
 __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ vhat, InttConsts ic,
                                                    const uint32_t *__restrict__ twi,
@@ -603,7 +621,8 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
     __shared__ uint32_t u[648];
     const uint32_t i = blockIdx.x;
     const uint32_t q = ic.q;
-    for (int x = threadIdx.x; x < 648; x += 256) u[x] = vhat[(size_t)i * 648 + x] % q;
+    const float qf = ic.qinv;
+    for (int x = threadIdx.x; x < 648; x += 256) u[x] = vhat[(size_t)i * 648 + x];  // in [0, q)
     __syncthreads();
     const int radix_s[7] = {2, 2, 2, 3, 3, 3, 3};
     const int degree_s[8] = {648, 324, 162, 81, 27, 9, 3, 1};
@@ -620,24 +639,26 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
             const uint32_t zi = twi[ic.twi_off[level] + k];
             if (rad == 2) {
                 const uint32_t y0 = u[base + j], y1 = u[base + m + j];
-                u[base + j] = mm((y0 + y1) % q, ic.inv2, q);
-                u[base + m + j] = mm(mm((y0 + q - y1) % q, ic.inv2, q), zi, q);
+                u[base + j] = mulmod_f(y0 + y1, ic.inv2, q, qf);
+                u[base + m + j] = mulmod_f(mulmod_f(y0 + q - y1, ic.inv2, q, qf), zi, q, qf);
             } else {
                 const uint32_t y0 = u[base + j], y1 = u[base + m + j], y2 = u[base + 2 * m + j];
-                const uint32_t a0 = mm((y0 + y1 + y2) % q, ic.inv3, q);
-                const uint32_t t1 = mm((y0 + mm(ic.w2, y1, q) + mm(ic.w, y2, q)) % q, ic.inv3, q);
-                const uint32_t t2 = mm((y0 + mm(ic.w, y1, q) + mm(ic.w2, y2, q)) % q, ic.inv3, q);
+                const uint32_t a0 = mulmod_f(y0 + y1 + y2, ic.inv3, q, qf);
+                const uint32_t t1 =
+                    mulmod_f(y0 + mulmod_f(ic.w2, y1, q, qf) + mulmod_f(ic.w, y2, q, qf), ic.inv3, q, qf);
+                const uint32_t t2 =
+                    mulmod_f(y0 + mulmod_f(ic.w, y1, q, qf) + mulmod_f(ic.w2, y2, q, qf), ic.inv3, q, qf);
                 u[base + j] = a0;
-                u[base + m + j] = mm(t1, zi, q);
-                u[base + 2 * m + j] = mm(t2, mm(zi, zi, q), q);
+                u[base + m + j] = mulmod_f(t1, zi, q, qf);
+                u[base + 2 * m + j] = mulmod_f(t2, mulmod_f(zi, zi, q, qf), q, qf);
             }
         }
         __syncthreads();
     }
     for (int j = threadIdx.x; j < 324; j += 256) {
         const uint32_t y0 = u[j], y1 = u[j + 324];
-        const uint32_t a1 = mm((y0 + q - y1) % q, ic.det, q);
-        const uint32_t a0 = (y0 + q - mm(ic.z6, a1, q)) % q;
+        const uint32_t a1 = mulmod_f(y0 + q - y1, ic.det, q, qf);
+        const uint32_t a0 = sub_q(y0 + q - mulmod_f(ic.z6, a1, q, qf), q);
         v[(size_t)i * 648 + j] = centered(a0, q);
         v[(size_t)i * 648 + j + 324] = centered(a1, q);
     }

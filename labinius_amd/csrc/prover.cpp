@@ -187,6 +187,7 @@ static void intt_tables(const Prime &P, InttConsts &ic, std::vector<uint32_t> &t
     ic.inv2 = (uint32_t)P.inv2;
     ic.inv3 = (uint32_t)P.inv3;
     ic.det = (uint32_t)P.det;
+    ic.qinv = 1.0f / (float)P.q;
     twi.clear();
     for (int l = 0; l < 7; l++) {
         ic.twi_off[l] = (uint32_t)twi.size();
