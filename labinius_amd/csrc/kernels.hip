@@ -68,16 +68,55 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 #pragma unroll
             for (int h = 0; h < 6; h++) wd[k][h] = 0;
     }
+    // Byte n's bit t is coefficient 81 t + n, bit (81 t + n) >> 2 of element (81 t + n) & 3. For the
+    // word g of bytes 4 g .. 4 g + 3, bit 8 i + t is bit g of the run of element (t + i) & 3 that
+    // starts at bit 20 t + ((t + i) >> 2): so the 21 words are the transpose of the 32 runs (one
+    // funnel shift each), a 32 x 32 bit transpose in five exchange stages.
+    uint32_t x[32];
 #pragma unroll
-    for (int n = 0; n < TILE_USED; n++) {
-        uint32_t b = 0;
+    for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            const int m = 81 * t + n, k = m & 3, mp = m >> 2;
-            b |= ((wd[k][mp >> 5] >> (mp & 31)) & 1u) << t;
+            const int k = (t + i) & 3, p0 = 20 * t + ((t + i) >> 2), w = p0 >> 5, o = p0 & 31;
+            x[8 * i + t] = o ? __builtin_amdgcn_alignbit(wd[k][w + 1], wd[k][w], o) : wd[k][w];
         }
-        stc[n][cl * PRE_I + il] = (uint8_t)b;
-        stf[n][il * PRE_C + cl] = (uint8_t)b;
+    // rows k and k + j (bit j of k clear) exchange row k's columns with bit j set for row k + j's
+    // columns without it; the half- and byte-wide stages are one byte permute a word
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t a = x[k], b = x[k + 16];
+        x[k] = __builtin_amdgcn_perm(b, a, 0x05040100u);       // a's low half, b's low half above
+        x[k + 16] = __builtin_amdgcn_perm(b, a, 0x07060302u);  // a's high half, b's high half above
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < 32; k0 += 16)
+#pragma unroll
+        for (int k = k0; k < k0 + 8; k++) {
+            const uint32_t a = x[k], b = x[k + 8];
+            x[k] = __builtin_amdgcn_perm(b, a, 0x06020400u);      // bytes a0, b0, a2, b2
+            x[k + 8] = __builtin_amdgcn_perm(b, a, 0x07030501u);  // bytes a1, b1, a3, b3
+        }
+    auto stage = [&](auto jc, uint32_t mlo) {
+        constexpr int j = decltype(jc)::value;
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 2 * j)
+#pragma unroll
+            for (int k = k0; k < k0 + j; k++) {
+                const uint32_t a = x[k], b = x[k + j];
+                // a keeps its columns without bit j (mlo) and takes b's mlo columns shifted up;
+                // b keeps its columns with bit j and takes a's shifted down
+                x[k] = __builtin_amdgcn_bitop3_b32(a, b << j, mlo, 0xE4);
+                x[k + j] = __builtin_amdgcn_bitop3_b32(b, a >> j, ~mlo, 0xE4);
+            }
+    };
+    stage(std::integral_constant<int, 4>{}, 0x0F0F0F0Fu);
+    stage(std::integral_constant<int, 2>{}, 0x33333333u);
+    stage(std::integral_constant<int, 1>{}, 0x55555555u);
+#pragma unroll
+    for (int n = 0; n < TILE_USED; n++) {
+        const uint8_t b = (uint8_t)(x[n >> 2] >> (8 * (n & 3)));
+        stc[n][cl * PRE_I + il] = b;
+        stf[n][il * PRE_C + cl] = b;
     }
     __syncthreads();
     // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16)
