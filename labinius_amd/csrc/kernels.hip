@@ -368,16 +368,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     const int xdig = c16 & 1, xrow = c16 < 4 ? (c16 >> 1) & 1 : 2;
     const int xoff = ((kh * 2 + xdig) * NBLK * 64 + 2 * 64 + 16 + xrow + 32 * (kg & 1)) * 16;
 
-    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the sixth block's [nb][B digit]
+    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the sixth block's [nb][B digit];
+    // the first step's products start them from zero (an inline constant, no clearing moves)
     v4i acc[5][2][3], xacc[2][2];
-#pragma unroll
-    for (int a = 0; a < 5; a++)
-#pragma unroll
-        for (int n = 0; n < 2; n++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) acc[a][n][c] = (v4i){};
-#pragma unroll
-    for (int n = 0; n < 2; n++) xacc[n][0] = xacc[n][1] = (v4i){};
 
     // E and the output slots are loaded now, behind the K loop; E's LDS area is not used before
     // the epilogue. 82 x 82 x 2 bytes (+ padding) are 841 16-byte words, two per thread.
@@ -423,7 +416,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     };
     read_a0(abuf);
     build();
-    for (uint32_t kb = 0; kb < kbs; kb++) {
+    // step kb; FIRST: the step that starts the accumulators
+    auto step = [&](uint32_t kb, auto first) {
+        constexpr bool FIRST = decltype(first)::value;
+        const v4i z = {0, 0, 0, 0};
+        auto c = [&](const v4i &x) -> v4i { return FIRST ? z : x; };
         if (wgrp) {
             __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // this wave's copy of A(kb + 1) is in LDS
             __syncthreads();
@@ -441,22 +438,22 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         for (int mb = 0; mb < 5; mb++)
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bl[nb], acc[mb][nb][0], 0, 0, 0);
+                acc[mb][nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bl[nb], c(acc[mb][nb][0]), 0, 0, 0);
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], acc[mb][nb][1], 0, 0, 0);
+                acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], c(acc[mb][nb][1]), 0, 0, 0);
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) {
-            xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], xacc[nb][0], 0, 0, 0);
-            xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], xacc[nb][1], 0, 0, 0);
+            xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], c(xacc[nb][0]), 0, 0, 0);
+            xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], c(xacc[nb][1]), 0, 0, 0);
         }
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
 #pragma unroll
             for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bh[nb], acc[mb][nb][2], 0, 0, 0);
+                acc[mb][nb][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bh[nb], c(acc[mb][nb][2]), 0, 0, 0);
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
 #pragma unroll
@@ -469,7 +466,10 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         }
         build();                                 // step kb + 1 (its bytes were loaded before the products)
         read_a0(abuf + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
-    }
+    };
+    step(0, std::true_type{});
+    __builtin_amdgcn_sched_barrier(0);
+    for (uint32_t kb = 1; kb < kbs; kb++) step(kb, std::false_type{});
     __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
     __syncthreads();  // waves 4-7's last products read the LDS that the epilogue overwrites
 
