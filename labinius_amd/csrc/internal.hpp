@@ -258,7 +258,9 @@ struct lbn_prover {
     int16_t *d_v = nullptr;
     uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_eqpart = nullptr, *d_u = nullptr;
     uint32_t *d_eqf = nullptr;  // the row evaluation's eq operands (row_eval_scratch_u32)
-    uint64_t *d_p1 = nullptr, *d_eq1 = nullptr, *d_eqpart1 = nullptr, *d_t = nullptr;  // the MLE claim
+    uint64_t *d_p1 = nullptr, *d_eqpart1 = nullptr, *d_t = nullptr;  // the MLE claim (d_p1 inside d_p0's block)
+    uint64_t *h_pts = nullptr;       // pinned staging of p0 and p1 (one upload)
+    lbn_challenge *h_ch = nullptr;   // pinned staging of the folding challenges
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
     // the bit-dropped digits (grow-only): raw rows from the host, centered coefficients, top + digits
     uint32_t *d_bd_raw = nullptr;

@@ -785,11 +785,19 @@ __device__ void f162_mul_lanes(const uint64_t *a, const uint64_t *b, uint64_t *o
 // groups of 8 coordinates: part[c][x] for the coordinates 8c .. 8c + 7 (x < 256), built as a
 // product tree of depth 3 (pairs of coordinates, quads, then the 256 entries), each product on a
 // group of lanes: 8 for the 16 pairs and the 32 quads, 4 for the 256 entries (1024 threads).
-__global__ __launch_bounds__(1024) void eq_part_kernel(const uint64_t *__restrict__ p0, uint32_t rows,
-                                                       uint64_t *__restrict__ part) {
+// Blocks 0 .. nb0 - 1 build point 0's sub-tables (one a block), the others point 1's (p1 may be
+// null with nb1 = 0).
+__global__ __launch_bounds__(1024) void eq_part_kernel(const uint64_t *__restrict__ pa, uint32_t rows_a,
+                                                       uint64_t *__restrict__ part_a, uint32_t nb0,
+                                                       const uint64_t *__restrict__ pb, uint32_t rows_b,
+                                                       uint64_t *__restrict__ part_b) {
     __shared__ uint64_t pr[4][4][3];   // pair j (coordinates 2j, 2j + 1), 2 bits
     __shared__ uint64_t qd[2][16][3];  // quad j (coordinates 4j .. 4j + 3), 4 bits
-    const uint32_t c = blockIdx.x, x = threadIdx.x;
+    const bool second = blockIdx.x >= nb0;  // block-uniform
+    const uint32_t c = second ? blockIdx.x - nb0 : blockIdx.x, x = threadIdx.x;
+    const uint64_t *p0 = second ? pb : pa;
+    const uint32_t rows = second ? rows_b : rows_a;
+    uint64_t *part = second ? part_b : part_a;
     const uint32_t k0 = 8 * c, nk = min(8u, rows - k0);
     // a coordinate past the last one contributes the factor 1 (its bit is then always 0)
     auto factor = [&](uint32_t k, uint32_t bit, uint64_t *f) {
@@ -882,6 +890,7 @@ __global__ __launch_bounds__(256) void claim_kernel(const uint64_t *__restrict__
     }
 }
 
+
 // ---------------------------------------------------------------------------------------------
 // The row evaluation as a GF(2) product on FP4 MFMA.
 //
@@ -930,6 +939,97 @@ __global__ __launch_bounds__(256) void eqfrag_kernel(const uint64_t *__restrict_
     uint4 *o = (uint4 *)frag + (size_t)id;
     *o = make_uint4(r4[0], r4[1], r4[2], r4[3]);
 }
+
+// eq(p) entry b from its sub-tables: part[k][(b >> 8 k) & 255] multiplied over the tables, L lanes
+// a product (whole waves of lanes with live entries).
+template <int L>
+__device__ __forceinline__ void eq_entry(const uint64_t *__restrict__ part, uint32_t rows, uint32_t b, uint64_t acc[3]) {
+    const uint64_t *f = part + 3 * (b & 255);
+    acc[0] = f[0];
+    acc[1] = f[1];
+    acc[2] = f[2];
+    for (uint32_t k = 1; 8 * k < rows; k++) {
+        const uint64_t *g2 = part + 3 * ((size_t)k * 256 + ((b >> (8 * k)) & 255));
+        uint64_t t[3];
+        f162_mul_lanes<L>(acc, g2, t);
+        acc[0] = t[0];
+        acc[1] = t[1];
+        acc[2] = t[2];
+    }
+}
+
+// One block per 128-element chunk of eq(p0): the chunk's entries (EQ_LANES lanes each, into LDS and
+// eq), then the chunk's FP4 operands for the row evaluation (eqfrag_kernel's layout); the blocks
+// also clear the two output regions the evaluation accumulates into by XOR.
+constexpr int EF_THREADS = RF_CHUNK * EQ_LANES;  // 512
+__global__ __launch_bounds__(EF_THREADS) void eq_frag_kernel(const uint64_t *__restrict__ part, uint32_t rows,
+                                                             uint64_t *__restrict__ eq, uint32_t *__restrict__ frag,
+                                                             uint32_t *__restrict__ z0, uint32_t n0,
+                                                             uint32_t *__restrict__ z1, uint32_t n1) {
+    __shared__ uint64_t es[RF_CHUNK][3];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x, e = tid / EQ_LANES, b = RF_CHUNK * c + e;
+    uint64_t acc[3];
+    eq_entry<EQ_LANES>(part, rows, b, acc);
+    if (tid % EQ_LANES == 0) {
+        es[e][0] = acc[0];
+        es[e][1] = acc[1];
+        es[e][2] = acc[2];
+        eq[3 * (size_t)b] = acc[0];
+        eq[3 * (size_t)b + 1] = acc[1];
+        eq[3 * (size_t)b + 2] = acc[2];
+    }
+    for (uint32_t w = c * EF_THREADS + tid; w < n0; w += gridDim.x * EF_THREADS) z0[w] = 0;
+    for (uint32_t w = c * EF_THREADS + tid; w < n1; w += gridDim.x * EF_THREADS) z1[w] = 0;
+    __syncthreads();
+    constexpr uint32_t nibv[4] = {0x4u, 0x2u, 0x1u, 0x4u};  // as eqfrag_kernel
+    for (uint32_t f = tid; f < RF_TILES * 64; f += EF_THREADS) {
+        const uint32_t lane = f & 63, nt = f >> 6, bit = 16 * nt + (lane & 15);
+        uint32_t r4[4] = {0, 0, 0, 0};
+        if (bit < 162) {
+            const uint32_t e0 = 32 * (lane >> 4), wd = bit >> 6, sh = bit & 63;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    r4[r] |= (uint32_t)((es[e0 + 4 * j + r][wd] >> sh) & 1) * (nibv[r] << (4 * j));
+        }
+        ((uint4 *)frag)[(size_t)c * RF_TILES * 64 + f] = make_uint4(r4[0], r4[1], r4[2], r4[3]);
+    }
+}
+
+// The claim with eq(p1) built from its sub-tables in place: t ^= sum_j eq(p1)_j u_j.
+__global__ __launch_bounds__(256) void claim_parts_kernel(const uint64_t *__restrict__ part1, uint32_t rows1,
+                                                          const uint64_t *__restrict__ u, uint32_t n,
+                                                          unsigned long long *__restrict__ t) {
+    uint64_t a0 = 0, a1 = 0, a2 = 0;
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+        uint64_t e[3], pr[3];
+        eq_entry<1>(part1, rows1, j, e);
+        f162_mul_dev(e, u + 3 * (size_t)j, pr);
+        a0 ^= pr[0];
+        a1 ^= pr[1];
+        a2 ^= pr[2];
+    }
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        a0 ^= shfl_xor_u64(a0, o);
+        a1 ^= shfl_xor_u64(a1, o);
+        a2 ^= shfl_xor_u64(a2, o);
+    }
+    __shared__ uint64_t part[4][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[wave][0] = a0;
+        part[wave][1] = a1;
+        part[wave][2] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint64_t x = part[0][threadIdx.x] ^ part[1][threadIdx.x] ^ part[2][threadIdx.x] ^ part[3][threadIdx.x];
+        atomicXor(&t[threadIdx.x], (unsigned long long)x);
+    }
+}
+
 
 // lanes l and l ^ D of each 16-lane row exchange x, by DPP: quad_perm for D = 1, 2; for D = 4 the
 // quad mirror (l ^ 3) then the half-row mirror (l ^ 7); row_ror:8 for D = 8
@@ -2062,7 +2162,8 @@ hipError_t launch_switch_fold_msg(hipStream_t s, uint64_t *a, const uint64_t *pi
 
 hipError_t launch_eq(hipStream_t s, const uint64_t *p0, uint32_t rows, uint64_t *part, uint64_t *eq) {
     const uint32_t n = 1u << rows;
-    LBN_LAUNCH(eq_part_kernel, dim3((rows + 7) / 8), dim3(1024), 0, s, p0, rows, part);
+    LBN_LAUNCH(eq_part_kernel, dim3((rows + 7) / 8), dim3(1024), 0, s, p0, rows, part, (rows + 7) / 8,
+               (const uint64_t *)nullptr, 0u, (uint64_t *)nullptr);
     LBN_LAUNCH(eq_kernel, dim3(((uint64_t)n * EQ_LANES + 255) / 256), dim3(256), 0, s, part, rows, eq);
     return hipSuccess;
 }
@@ -2076,12 +2177,41 @@ hipError_t launch_row_eval_f4(hipStream_t s, const uint64_t *w, const uint64_t *
     // chunks (the f32 counts stay below 2^24).
     const uint32_t nch = wdim / RF_CHUNK;
     LBN_LAUNCH(eqfrag_kernel, dim3((nch * RF_TILES * 64 + 255) / 256), dim3(256), 0, s, eq, nch, frag);
+    return launch_row_eval_frag(s, w, frag, wdim, cols, u);
+}
+
+hipError_t launch_row_eval_frag(hipStream_t s, const uint64_t *w, const uint32_t *frag, uint32_t wdim, uint32_t cols,
+                                uint64_t *u) {
+    const uint32_t nch = wdim / RF_CHUNK;
     const uint32_t gy_want = std::max(1u, std::min(nch, (1024 + cols - 1) / cols));
     uint32_t span = std::min((nch + gy_want - 1) / gy_want, 1u << 17);
     span = (span + RF_G - 1) / RF_G * RF_G;  // whole phases
     const uint32_t gy = (nch + span - 1) / span;
     LBN_LAUNCH(row_eval_f4_kernel, dim3(cols, gy), dim3(256), 0, s, (const uint32_t *)w, (const uint4 *)frag, wdim,
                cols, span, (unsigned long long *)u);
+    return hipSuccess;
+}
+
+hipError_t launch_eq_parts(hipStream_t s, const uint64_t *p0, uint32_t rows0, uint64_t *part0, const uint64_t *p1,
+                           uint32_t rows1, uint64_t *part1) {
+    const uint32_t nb0 = (rows0 + 7) / 8, nb1 = p1 ? (rows1 + 7) / 8 : 0;
+    if (nb0 + nb1 == 0) return hipSuccess;
+    LBN_LAUNCH(eq_part_kernel, dim3(nb0 + nb1), dim3(1024), 0, s, p0, rows0, part0, nb0, p1, rows1, part1);
+    return hipSuccess;
+}
+
+hipError_t launch_eq_frag(hipStream_t s, const uint64_t *part, uint32_t rows, uint64_t *eq, uint32_t *frag,
+                          uint32_t *z0, uint32_t n0, uint32_t *z1, uint32_t n1) {
+    // 2^rows >= 128: whole chunks, a block each
+    LBN_LAUNCH(eq_frag_kernel, dim3((1u << rows) / RF_CHUNK), dim3(EF_THREADS), 0, s, part, rows, eq, frag, z0, n0,
+               z1, n1);
+    return hipSuccess;
+}
+
+hipError_t launch_claim_parts(hipStream_t s, const uint64_t *part1, uint32_t rows1, const uint64_t *u, uint32_t n,
+                              uint64_t *t) {
+    const uint32_t blocks = std::max(1u, std::min(64u, (n + 255) / 256));
+    LBN_LAUNCH(claim_parts_kernel, dim3(blocks), dim3(256), 0, s, part1, rows1, u, n, (unsigned long long *)t);
     return hipSuccess;
 }
 

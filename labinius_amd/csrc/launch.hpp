@@ -55,6 +55,17 @@ hipError_t launch_row_eval_f4(hipStream_t s, const uint64_t *w, const uint64_t *
 size_t row_eval_scratch_u32(uint32_t wdim);
 // t (3 u64, cleared first) ^= sum_j eq1_j u_j over F162, j < n
 hipError_t launch_claim(hipStream_t s, const uint64_t *eq1, const uint64_t *u, uint32_t n, uint64_t *t);
+// The evaluation's fused launches: both points' eq sub-tables in one launch (p1 may be null); eq(p0)
+// with the row evaluation's FP4 operands, clearing z0 (n0 words) and z1 (n1 words); the row values
+// from those operands; the claim with eq(p1) built from its sub-tables (rows1 <= 24).
+hipError_t launch_eq_parts(hipStream_t s, const uint64_t *p0, uint32_t rows0, uint64_t *part0, const uint64_t *p1,
+                           uint32_t rows1, uint64_t *part1);
+hipError_t launch_eq_frag(hipStream_t s, const uint64_t *part, uint32_t rows, uint64_t *eq, uint32_t *frag,
+                          uint32_t *z0, uint32_t n0, uint32_t *z1, uint32_t n1);
+hipError_t launch_row_eval_frag(hipStream_t s, const uint64_t *w, const uint32_t *frag, uint32_t wdim, uint32_t cols,
+                                uint64_t *u);
+hipError_t launch_claim_parts(hipStream_t s, const uint64_t *part1, uint32_t rows1, const uint64_t *u, uint32_t n,
+                              uint64_t *t);
 // the device verifier: the folded witness's bit planes (stat: squared norm, largest |v|, added
 // to); the fold of a commitment (acc: 4 x limbs x 162 scratch)
 hipError_t launch_verify_planes(hipStream_t s, const int16_t *v, uint32_t len_ring, int half, uint32_t nb,

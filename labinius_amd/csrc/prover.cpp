@@ -346,11 +346,13 @@ static int build_device(lbn_pp *pp) {
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
                     pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_eqf, pr->d_u, pr->d_wscratch,
-                    pr->d_p1, pr->d_eq1, pr->d_eqpart1, pr->d_t,
+                    pr->d_eqpart1, pr->d_t,
                     pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out, pr->sw.d_eq, pr->sw.d_a, pr->sw.d_p,
                     pr->sw.d_small};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    if (pr->h_pts) (void)hipHostFree(pr->h_pts);
+    if (pr->h_ch) (void)hipHostFree(pr->h_ch);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);  // a pooled prover keeps none
     batch_free(pr->batch);
     wire_free(pr->wire);
@@ -522,7 +524,8 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
     hipStream_t st = pr->stream;
-    HIP_TRY(hipMemcpyAsync(pr->d_ch, ch, r * sizeof(lbn_challenge), hipMemcpyHostToDevice, st));
+    std::memcpy(pr->h_ch, ch, r * sizeof(lbn_challenge));  // pinned staging: the copy is asynchronous
+    HIP_TRY(hipMemcpyAsync(pr->d_ch, pr->h_ch, r * sizeof(lbn_challenge), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(pr->ev[4], st));
     HIP_TRY(launch_chat(st, (const lbn_challenge_dev *)pr->d_ch, (uint32_t)r, pp->rpad, pp->d_expo_f, pp->d_pw_f,
                         pp->cond_f, pp->mult_f, s.primes[0], pr->d_afrag_f, pp->ntiles_f));
@@ -557,16 +560,23 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
 }
 
 // eq(p0) and the row values u into pr->d_u, on the prover's stream (events 8..10)
-static int row_eval_launch(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0) {
+// The row values (and, with p1, the claim's inputs): p0 and p1 in one upload from pinned staging,
+// both points' eq sub-tables in one launch, then eq(p0) with the FP4 operands (clearing u and t),
+// then the row values.
+static int row_eval_launch(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, const lbn_f162 *p1) {
     lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
     hipStream_t st = pr->stream;
-    HIP_TRY(hipMemcpyAsync(pr->d_p0, p0, s.rows * sizeof(lbn_f162), hipMemcpyHostToDevice, st));
+    std::memcpy(pr->h_pts, p0, s.rows * sizeof(lbn_f162));
+    if (p1) std::memcpy(pr->h_pts + 3 * (size_t)s.rows, p1, s.cl * sizeof(lbn_f162));
+    HIP_TRY(hipMemcpyAsync(pr->d_p0, pr->h_pts, (s.rows + (p1 ? s.cl : 0)) * sizeof(lbn_f162), hipMemcpyHostToDevice,
+                           st));
     HIP_TRY(hipEventRecord(pr->ev[8], st));
-    HIP_TRY(launch_eq(st, pr->d_p0, s.rows, pr->d_eqpart, pr->d_eq));
+    HIP_TRY(launch_eq_parts(st, pr->d_p0, s.rows, pr->d_eqpart, p1 ? pr->d_p1 : nullptr, s.cl, pr->d_eqpart1));
     HIP_TRY(hipEventRecord(pr->ev[9], st));
-    HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_u, s.r * 6, 0));
-    HIP_TRY(launch_row_eval_f4(st, d_w, pr->d_eq, pr->d_eqf, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
+    HIP_TRY(launch_eq_frag(st, pr->d_eqpart, s.rows, pr->d_eq, pr->d_eqf, (uint32_t *)pr->d_u, (uint32_t)(s.r * 6),
+                           (uint32_t *)pr->d_t, p1 ? 6u : 0u));
+    HIP_TRY(launch_row_eval_frag(st, d_w, pr->d_eqf, (uint32_t)s.column_len, (uint32_t)s.r, pr->d_u));
     return LBN_OK;
 }
 
@@ -577,7 +587,7 @@ static void row_eval_timings(lbn_prover *pr) {
 }
 
 static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, lbn_f162 *u) {
-    int rc = row_eval_launch(pr, d_w, p0);
+    int rc = row_eval_launch(pr, d_w, p0, nullptr);
     if (rc) return rc;
     hipStream_t st = pr->stream;
     HIP_TRY(hipEventRecord(pr->ev[10], st));
@@ -592,13 +602,10 @@ static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
 static int mle_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *p0, const lbn_f162 *p1,
                            lbn_f162 *t) {
     const Shape &s = pr->pp->shape;
-    int rc = row_eval_launch(pr, d_w, p0);
+    int rc = row_eval_launch(pr, d_w, p0, p1);
     if (rc) return rc;
     hipStream_t st = pr->stream;
-    HIP_TRY(hipMemcpyAsync(pr->d_p1, p1, s.cl * sizeof(lbn_f162), hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_eq(st, pr->d_p1, s.cl, pr->d_eqpart1, pr->d_eq1));
-    HIP_TRY(launch_fill_u32(st, (uint32_t *)pr->d_t, 6, 0));
-    HIP_TRY(launch_claim(st, pr->d_eq1, pr->d_u, (uint32_t)s.r, pr->d_t));
+    HIP_TRY(launch_claim_parts(st, pr->d_eqpart1, s.cl, pr->d_u, (uint32_t)s.r, pr->d_t));
     HIP_TRY(hipEventRecord(pr->ev[10], st));
     HIP_TRY(hipMemcpyAsync(t, pr->d_t, sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -729,12 +736,18 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
     TRY_RC(dalloc(&pr->d_afrag_f, (size_t)pp->ntiles_f * pp->kb_f * A_STEP));
     TRY_RC(dalloc(&pr->d_vhat, s.len_ring * N));
     TRY_RC(dalloc(&pr->d_v, s.len_ring * N));
-    TRY_RC(dalloc(&pr->d_p0, 3 * (size_t)std::max<uint32_t>(s.rows, 1)));
+    TRY_RC(dalloc(&pr->d_p0, 3 * ((size_t)s.rows + s.cl)));
+    pr->d_p1 = pr->d_p0 + 3 * (size_t)s.rows;
+    TRY_RC(hipHostMalloc((void **)&pr->h_pts, 3 * ((size_t)s.rows + s.cl) * sizeof(uint64_t), hipHostMallocDefault) ==
+                   hipSuccess
+               ? LBN_OK
+               : fail(LBN_ERR_OOM, "pinned staging"));
+    TRY_RC(hipHostMalloc((void **)&pr->h_ch, s.r * sizeof(lbn_challenge), hipHostMallocDefault) == hipSuccess
+               ? LBN_OK
+               : fail(LBN_ERR_OOM, "pinned staging"));
     TRY_RC(dalloc(&pr->d_eq, 3 * s.column_len));
     TRY_RC(dalloc(&pr->d_eqpart, 3 * 256 * (size_t)((s.rows + 7) / 8)));
     TRY_RC(dalloc(&pr->d_eqf, row_eval_scratch_u32((uint32_t)s.column_len)));
-    TRY_RC(dalloc(&pr->d_p1, 3 * (size_t)std::max<uint32_t>(s.cl, 1)));
-    TRY_RC(dalloc(&pr->d_eq1, 3 * ((size_t)1 << s.cl)));
     TRY_RC(dalloc(&pr->d_eqpart1, 3 * 256 * (size_t)((s.cl + 7) / 8)));
     TRY_RC(dalloc(&pr->d_t, 3));
     TRY_RC(dalloc(&pr->d_u, 3 * s.r));
