@@ -41,6 +41,16 @@ int dalloc(T **p, size_t count) {
     return LBN_OK;
 }
 
+// Waits for a stream by polling it: a round's calls wait for their own short work, and a polling
+// wait resumes the caller as soon as the stream drains, where a blocking one may first yield the
+// thread.
+inline hipError_t stream_wait(hipStream_t s) {
+    hipError_t e;
+    while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+    }
+    return e;
+}
+
 inline float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;

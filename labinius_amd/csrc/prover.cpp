@@ -468,7 +468,7 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     HIP_TRY(hipEventRecord(pr->ev[3], st));
     if (comp) HIP_TRY(hipMemcpyAsync(comp, pr->d_comp, s.r * 4 * limbs * N162 * sizeof(int16_t), hipMemcpyDeviceToHost, st));
     if (raw) HIP_TRY(hipMemcpyAsync(raw, pr->d_raw, limbs * s.r * N * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(stream_wait(st));
     pr->committed = true;
     pr->bd_ready = false;
     pr->timings[0] = elapsed(pr->ev[0], pr->ev[3]);
@@ -550,7 +550,7 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     HIP_TRY(launch_intt(st, pr->d_vhat, (uint32_t)s.len_ring, pp->intt, pp->d_twi, pr->d_v));
     HIP_TRY(hipEventRecord(pr->ev[7], st));
     if (v) HIP_TRY(hipMemcpyAsync(v, pr->d_v, s.len_ring * N * sizeof(int16_t), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(stream_wait(st));
     pr->folded = true;
     pr->timings[4] = elapsed(pr->ev[4], pr->ev[7]);
     pr->timings[5] = elapsed(pr->ev[4], pr->ev[5]);
@@ -592,7 +592,7 @@ static int row_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
     hipStream_t st = pr->stream;
     HIP_TRY(hipEventRecord(pr->ev[10], st));
     HIP_TRY(hipMemcpyAsync(u, pr->d_u, pr->pp->shape.r * sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(stream_wait(st));
     row_eval_timings(pr);
     return LBN_OK;
 }
@@ -608,7 +608,7 @@ static int mle_eval_device(lbn_prover *pr, const uint64_t *d_w, const lbn_f162 *
     HIP_TRY(launch_claim_parts(st, pr->d_eqpart1, s.cl, pr->d_u, (uint32_t)s.r, pr->d_t));
     HIP_TRY(hipEventRecord(pr->ev[10], st));
     HIP_TRY(hipMemcpyAsync(t, pr->d_t, sizeof(lbn_f162), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(stream_wait(st));
     row_eval_timings(pr);
     return LBN_OK;
 }
