@@ -596,44 +596,67 @@ __global__ void commit_finish_kernel(const uint32_t *__restrict__ yacc, uint32_t
 // The A fragments of the fold GEMM: tile tau, K = column c, M = row m; the value at a row is
 // c(-X^4) at the row's root, sum_i (-1)^{p_i} (1 - 2 s_i) root^{mult expo[m] p_i} (fold.rs:287-300
 // embeds c as the coefficient (-1)^m c_m of X^{4m}). One thread per (tile, column, row).
-__global__ void chat_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r, uint32_t rpad,
-                            const uint16_t *__restrict__ expo, const uint16_t *__restrict__ pw,
-                            uint32_t cond, uint32_t mult, uint32_t q, int8_t *__restrict__ afrag,
-                            uint32_t ntiles) {
-    const uint32_t total = ntiles * rpad * TILE;
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= total) return;
-    const uint32_t m = id % TILE, c = (id / TILE) % rpad, tau = id / (TILE * rpad);
-    int x = 0;
-    const uint32_t ex = expo[tau * TILE + m];
-    if (c < r && ex != 0xFFFF) {
-        const lbn_challenge_dev &cc = ch[c];
-        // the root exponent of position p is a p mod cond, a = mult ex mod cond: a p < cond 2^8 <
-        // 2^19 is exact in float, so its quotient from the float reciprocal is within one
-        const uint32_t a = (mult * ex) % cond;
-        const float cinv = 1.0f / (float)cond;
-        int acc = 0;
-        for (uint32_t i = 0; i < cc.weight; i++) {
-            const uint32_t p = cc.positions[i];
-            const int co = (1 - 2 * (int)((cc.signs >> i) & 1)) * ((p & 1) ? -1 : 1);
-            const uint32_t ap = a * p;
-            int e = (int)(ap - (uint32_t)((float)ap * cinv) * cond);
-            e += e < 0 ? (int)cond : 0;
-            e -= e >= (int)cond ? (int)cond : 0;
-            acc += co * (int)pw[e];
+// A block per (16 challenges, tile): the challenges and the root powers staged in LDS; thread
+// (row m, quarter g4) computes challenges 4 g4 .. 4 g4 + 3 of the 16 for row m. The 16 challenges
+// of a K group are 16 consecutive bytes of a row's fragment, so each thread writes its 4 low and 4
+// high digits as one word each.
+constexpr int CH_GROUP = 16, CH_THREADS = TILE * CH_GROUP / 4;  // 384
+constexpr uint32_t CH_PW_MAX = 1944;                           // the largest conductor
+__global__ __launch_bounds__(CH_THREADS) void chat_kernel(const lbn_challenge_dev *__restrict__ ch, uint32_t r,
+                                                          uint32_t rpad, const uint16_t *__restrict__ expo,
+                                                          const uint16_t *__restrict__ pw, uint32_t cond,
+                                                          uint32_t mult, uint32_t q, int8_t *__restrict__ afrag) {
+    __shared__ uint16_t pws[CH_PW_MAX];
+    __shared__ int16_t cps[CH_GROUP][32][2];  // a challenge's (position, signed coefficient) pairs
+    const uint32_t c0 = blockIdx.x * CH_GROUP, tau = blockIdx.y, tid = threadIdx.x;
+    for (uint32_t i = tid; i < cond; i += CH_THREADS) pws[i] = pw[i];
+    for (uint32_t x = tid; x < CH_GROUP * 32; x += CH_THREADS) {
+        const uint32_t k = x / 32, i = x % 32, c = c0 + k;
+        int16_t pos = 0, co = 0;  // past the weight (and for padding columns): coefficient 0
+        if (c < r && i < ch[c].weight) {
+            pos = ch[c].positions[i];
+            co = (int16_t)((1 - 2 * (int)((ch[c].signs >> i) & 1)) * ((pos & 1) ? -1 : 1));
         }
-        int v = acc % (int)q;
-        if (v < 0) v += q;
-        x = v > (int)(q - 1) / 2 ? v - (int)q : v;
+        cps[k][i][0] = pos;
+        cps[k][i][1] = co;
     }
-    const int xl = ((x + 64) & 127) - 64;
-    const int xh = (x - xl) / 128;
+    __syncthreads();
+    const uint32_t m = tid % TILE, g4 = tid / TILE;
+    const uint32_t ex = expo[tau * TILE + m];
+    uint32_t lo = 0, hi = 0;
+    if (ex != 0xFFFF) {
+        // the root exponent of position p is a p mod cond, a = mult ex mod cond: with a p < cond
+        // 2^8 < 2^19 and rc = floor(2^32 / cond) + 1 < 2^22, (a p rc) >> 32 (one 24-bit high
+        // product) is floor(a p / cond) or one more, so one correction gives the remainder
+        const uint32_t a = (mult * ex) % cond;
+        const uint32_t rc = (uint32_t)(0x100000000ull / cond) + 1;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int16_t(*cp)[2] = cps[4 * g4 + b];
+            int acc = 0;
+            // all 32 pairs (coefficient 0 past the weight): independent terms whose reads overlap
+#pragma unroll
+            for (uint32_t i = 0; i < 32; i++) {
+                const uint32_t ap = __umul24(a, (uint32_t)cp[i][0]);
+                int e = (int)(ap - __umul24(__umulhi(ap, rc), cond));
+                e += e < 0 ? (int)cond : 0;
+                acc += cp[i][1] * (int)pws[e];
+            }
+            int v = acc % (int)q;
+            if (v < 0) v += q;
+            const int x = v > (int)(q - 1) / 2 ? v - (int)q : v;
+            const int xl = ((x + 64) & 127) - 64;
+            const int xh = (x - xl) / 128;
+            lo |= (uint32_t)(uint8_t)(int8_t)xl << (8 * b);
+            hi |= (uint32_t)(uint8_t)(int8_t)xh << (8 * b);
+        }
+    }
     const uint32_t kbs = rpad / KSTEP;
-    const uint32_t kb = c / KSTEP, kk = c % KSTEP, kh = kk / 32, g2 = (kk % 32) / 16, b = kk % 16;
+    const uint32_t kb = c0 / KSTEP, kk = c0 % KSTEP, kh = kk / 32, g2 = (kk % 32) / 16;  // byte kk % 16 = 0
     const uint32_t mb = m / 32, lanev = 32 * g2 + (m % 32);
-    const size_t base = ((size_t)tau * kbs + kb) * (size_t)A_STEP;
-    afrag[base + (((kh * 2 + 0) * 3 + mb) * 64 + lanev) * 16 + b] = (int8_t)xl;
-    afrag[base + (((kh * 2 + 1) * 3 + mb) * 64 + lanev) * 16 + b] = (int8_t)xh;
+    int8_t *base = afrag + ((size_t)tau * kbs + kb) * (size_t)A_STEP;
+    ((uint32_t *)(base + (((kh * 2 + 0) * 3 + mb) * 64 + lanev) * 16))[g4] = lo;
+    ((uint32_t *)(base + (((kh * 2 + 1) * 3 + mb) * 64 + lanev) * 16))[g4] = hi;
 }
 
 // =============================================================================================
@@ -997,18 +1020,25 @@ __global__ __launch_bounds__(EF_THREADS) void eq_frag_kernel(const uint64_t *__r
     }
 }
 
-// The claim with eq(p1) built from its sub-tables in place: t ^= sum_j eq(p1)_j u_j.
+// The claim with eq(p1) built from its sub-tables in place: t ^= sum_j eq(p1)_j u_j, CL_LANES lanes
+// a product (the grid is whole groups; a group past n adds nothing).
+constexpr int CL_LANES = 8;
 __global__ __launch_bounds__(256) void claim_parts_kernel(const uint64_t *__restrict__ part1, uint32_t rows1,
                                                           const uint64_t *__restrict__ u, uint32_t n,
                                                           unsigned long long *__restrict__ t) {
     uint64_t a0 = 0, a1 = 0, a2 = 0;
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+    const uint32_t groups = gridDim.x * 256 / CL_LANES;
+    for (uint32_t g = (blockIdx.x * 256 + threadIdx.x) / CL_LANES; g < (n + groups - 1) / groups * groups;
+         g += groups) {
+        const uint32_t j = g < n ? g : 0;  // whole groups of lanes run the products
         uint64_t e[3], pr[3];
-        eq_entry<1>(part1, rows1, j, e);
-        f162_mul_dev(e, u + 3 * (size_t)j, pr);
-        a0 ^= pr[0];
-        a1 ^= pr[1];
-        a2 ^= pr[2];
+        eq_entry<CL_LANES>(part1, rows1, j, e);
+        f162_mul_lanes<CL_LANES>(e, u + 3 * (size_t)j, pr);
+        if (g < n && threadIdx.x % CL_LANES == 0) {
+            a0 ^= pr[0];
+            a1 ^= pr[1];
+            a2 ^= pr[2];
+        }
     }
 #pragma unroll
     for (int o = 32; o; o >>= 1) {
@@ -2103,9 +2133,10 @@ hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t li
 hipError_t launch_chat(hipStream_t s, const lbn_challenge_dev *ch, uint32_t r, uint32_t rpad,
                        const uint16_t *expo, const uint16_t *pw, uint32_t cond, uint32_t mult,
                        uint32_t q, int8_t *afrag, uint32_t ntiles) {
-    const uint32_t total = ntiles * rpad * TILE;
-    LBN_LAUNCH(chat_kernel, dim3((total + 255) / 256), dim3(256), 0, s, ch, r, rpad, expo, pw, cond, mult, q,
-               afrag, ntiles);
+    // rpad is a multiple of KSTEP, so of CH_GROUP; cond <= CH_PW_MAX (CONDUCTOR, CONDUCTOR_QUAD)
+    if (cond > CH_PW_MAX || rpad % CH_GROUP) return hipErrorInvalidValue;
+    LBN_LAUNCH(chat_kernel, dim3(rpad / CH_GROUP, ntiles), dim3(CH_THREADS), 0, s, ch, r, rpad, expo, pw, cond, mult,
+               q, afrag);
     return hipSuccess;
 }
 
@@ -2210,7 +2241,7 @@ hipError_t launch_eq_frag(hipStream_t s, const uint64_t *part, uint32_t rows, ui
 
 hipError_t launch_claim_parts(hipStream_t s, const uint64_t *part1, uint32_t rows1, const uint64_t *u, uint32_t n,
                               uint64_t *t) {
-    const uint32_t blocks = std::max(1u, std::min(64u, (n + 255) / 256));
+    const uint32_t blocks = std::max(1u, std::min(256u, (n * CL_LANES + 255) / 256));
     LBN_LAUNCH(claim_parts_kernel, dim3(blocks), dim3(256), 0, s, part1, rows1, u, n, (unsigned long long *)t);
     return hipSuccess;
 }
