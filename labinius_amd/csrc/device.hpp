@@ -119,8 +119,12 @@ struct InttConsts {
     uint32_t quad;
     uint32_t w, w2, z6, inv2, inv3, det;
     uint32_t twi_off[7];    // offset of level l's inverse twiddles in the twi array
+    uint32_t twi_count;     // the array's length (at most INTT_TWI_MAX)
     float qinv;             // 1 / q, for mulmod_f
 };
+// The inverse twiddles of one limb: one per sub-block of each level, 327 (splitting) or 487
+// (quadratic) in all.
+constexpr uint32_t INTT_TWI_MAX = 512;
 
 // Mixed-radix (Garner) digits of the bit-dropped commitment (bd/digits.rs:134-182): x = m_0 +
 // q_0 (m_1 + q_1 (m_2 + ...)) from the residues r_k, with inv[k][j] = q_j^-1 mod q_k.

@@ -681,10 +681,12 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
                                                    const uint32_t *__restrict__ twi,
                                                    int16_t *__restrict__ v) {
     __shared__ uint32_t u[648];
+    __shared__ uint32_t tw[INTT_TWI_MAX];  // every level's twiddles, read once with the input
     const uint32_t i = blockIdx.x;
     const uint32_t q = ic.q;
     const float qf = ic.qinv;
     for (int x = threadIdx.x; x < 648; x += 256) u[x] = vhat[(size_t)i * 648 + x];  // in [0, q)
+    for (uint32_t x = threadIdx.x; x < ic.twi_count; x += 256) tw[x] = twi[x];
     __syncthreads();
     const int radix_s[7] = {2, 2, 2, 3, 3, 3, 3};
     const int degree_s[8] = {648, 324, 162, 81, 27, 9, 3, 1};
@@ -698,7 +700,7 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
         const int nbf = 648 / rad;
         for (int b = threadIdx.x; b < nbf; b += 256) {
             const int k = b / m, j = b % m, base = k * n;
-            const uint32_t zi = twi[ic.twi_off[level] + k];
+            const uint32_t zi = tw[ic.twi_off[level] + k];
             if (rad == 2) {
                 const uint32_t y0 = u[base + j], y1 = u[base + m + j];
                 u[base + j] = mulmod_f(y0 + y1, ic.inv2, q, qf);

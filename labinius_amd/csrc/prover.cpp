@@ -193,6 +193,7 @@ static void intt_tables(const Prime &P, InttConsts &ic, std::vector<uint32_t> &t
         ic.twi_off[l] = (uint32_t)twi.size();
         twi.insert(twi.end(), P.twi[l].begin(), P.twi[l].end());
     }
+    ic.twi_count = (uint32_t)twi.size();  // <= INTT_TWI_MAX, checked where the tables are built
 }
 
 static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, HostTables &H) {
@@ -305,6 +306,8 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
 static int build_device(lbn_pp *pp) {
     HostTables H;
     build_host_tables(pp->shape, pp->key, H);
+    for (const InttConsts &ic : H.intt_limb)
+        if (ic.twi_count > INTT_TWI_MAX) return fail(LBN_ERR_HIP, "inverse twiddles exceed the kernel's table");
     pp->lpad = H.lpad;
     pp->rpad = H.rpad;
     pp->kb_c = H.kb_c;
