@@ -175,7 +175,8 @@ int lbn_prover_set_pool(lbn_prover *prover, lbn_pool *pool);
  * with HIP events on the prover's stream:
  *   [0] commit total  [1] commit layout pass  [2] commit GEMM  [3] commit finish
  *   [4] fold total    [5] challenge transform [6] fold GEMM    [7] inverse NTT
- *   [8] eval total    [9] eq table            [10] row reduce */
+ *   [8] eval total    [9] both points' eq sub-tables
+ *   [10] eq(p0) with its operands, the row values and the claim */
 int lbn_prover_timings(const lbn_prover *prover, float *ms, size_t n);
 
 /* Batched commitment (no counterpart in the reference, which commits one witness per call; each
