@@ -91,11 +91,15 @@ struct GemmArgs {
     uint32_t items;           // work items (set by the launcher)
 };
 
-// The int32 accumulators are exact while K x 9728 < 2^31: a digit is in [-64, 63] (low) or
-// [-76, 76] (high, q < 2^15), and the largest class, mid = A_l x_h + A_h x_l, adds at most
-// 2 x 64 x 76 = 9728 per K index. The launcher splits longer K into chunks of KB_EXACT steps and
-// the later chunks add their outputs mod q.
-constexpr uint32_t KB_EXACT = 2147483647u / (KSTEP * 9728u);  // 3449 steps, K = 220 736
+// Operands mod q are split into two int8 digits, x = x_l + 128 x_h: for q < UNSIGNED_DIGITS_Q
+// the residue in [0, q) gives 0 <= x_l, x_h < 128; for larger q (up to 19441) the centred residue
+// gives -64 <= x_l < 64 and |x_h| <= 76.
+constexpr uint32_t UNSIGNED_DIGITS_Q = 16384;
+// The int32 accumulators are exact while K x 32258 < 2^31: the largest class, mid = A_l x_h +
+// A_h x_l, adds at most 2 x 127 x 127 = 32258 per K index (non-negative digits; the centred ones
+// add at most 2 x 64 x 76). The launcher splits longer K into chunks of KB_EXACT steps and the
+// later chunks add their outputs mod q.
+constexpr uint32_t KB_EXACT = 2147483647u / (KSTEP * 32258u);  // 1040 steps, K = 66 560
 
 // The commit GEMM's output is y[c][limb][plane][648]. A splitting limb's 8 tiles write each of
 // the 648 slots once, into plane 0. A quadratic limb's slot 2j + t is written by the two tiles of

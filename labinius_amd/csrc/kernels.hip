@@ -644,9 +644,17 @@ __global__ __launch_bounds__(CH_THREADS) void chat_kernel(const lbn_challenge_de
             }
             int v = acc % (int)q;
             if (v < 0) v += q;
-            const int x = v > (int)(q - 1) / 2 ? v - (int)q : v;
-            const int xl = ((x + 64) & 127) - 64;
-            const int xh = (x - xl) / 128;
+            // the digits of the tile's operands (UNSIGNED_DIGITS_Q): of v in [0, q) below it, of
+            // the centred residue above
+            int xl, xh;
+            if (q < UNSIGNED_DIGITS_Q) {
+                xl = v & 127;
+                xh = v >> 7;
+            } else {
+                const int x = v > (int)(q - 1) / 2 ? v - (int)q : v;
+                xl = ((x + 64) & 127) - 64;
+                xh = (x - xl) / 128;
+            }
             lo |= (uint32_t)(uint8_t)(int8_t)xl << (8 * b);
             hi |= (uint32_t)(uint8_t)(int8_t)xh << (8 * b);
         }

@@ -12,6 +12,7 @@ EP = 82  # E rows padded to pairs of n
 TILE_ROWS = 82  # rows a tile may use
 EE_TILE = TILE_ROWS * EP + 4
 YACC_PLANES = 2  # commit output planes: one per input parity of a quadratic limb
+UNSIGNED_DIGITS_Q = 16384  # below it operands are split into non-negative digits, above it centred ones
 DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("ee_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
                  ("par", "<u4"), ("y_off", "<u4")])
 
@@ -137,7 +138,7 @@ def replay_fold(lbn, params, seed, w, challenges):
     rb = ring_bytes(w, cols, L)  # [c][i][n]
     vhat = np.zeros((L, N), np.int64)
     for d in desc:
-        # the chat kernel: A[c][m] = sum_i coef_i root^{mult expo[m] p_i}, centered, two digits
+        # the chat kernel: A[c][m] = sum_i coef_i root^{mult expo[m] p_i} mod q, two digits
         A = np.zeros((2, rpad, TILE), np.int64)
         ex = expo[d["out_off"]:d["out_off"] + TILE].astype(np.int64)
         for c, ch in enumerate(challenges):
@@ -150,8 +151,11 @@ def replay_fold(lbn, params, seed, w, challenges):
                     co = (1 - 2 * ((ch.signs >> i) & 1)) * (-1 if pp & 1 else 1)
                     acc += co * int(pw[(mult * int(ex[m]) * pp) % cond])
                 x = acc % q
-                x = x - q if x > (q - 1) // 2 else x
-                xl = ((x + 64) & 127) - 64
+                if q < UNSIGNED_DIGITS_Q:  # the residue's digits, both in [0, 128)
+                    xl = x & 127
+                else:  # the centred residue's
+                    x = x - q if x > (q - 1) // 2 else x
+                    xl = ((x + 64) & 127) - 64
                 A[0, c, m], A[1, c, m] = xl, (x - xl) // 128
         E = ee_matrix(ee[d["ee_off"]:d["ee_off"] + EE_TILE])
         lt = lut[d["lut_off"]:d["lut_off"] + 512].astype(np.int64)
