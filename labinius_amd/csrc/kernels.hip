@@ -160,22 +160,23 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 constexpr int GEMM_THREADS = 512;
 // The epilogue's Z in LDS: [batch 3][m 82][n 82] u16 (Z < q < 2^15), rows padded to pairs of n.
 constexpr int ZP_BYTES = (GROUP_BATCH * TILE_ROWS * EP * 2 + 15) / 16 * 16;
-// A is staged per K step in an LDS buffer of AP 16-byte words per thread (A_STEP bytes used).
-constexpr int AP = (A_STEP / 16 + GEMM_THREADS - 1) / GEMM_THREADS;
-constexpr int A_LDS = AP * GEMM_THREADS * 16;
+// A is staged per K step in an LDS buffer of A_STEP bytes: 768 16-byte words, the first 512 one
+// a thread, the other 256 one a thread of waves 0-3.
+constexpr int A_LDS = A_STEP;
+static_assert(A_STEP == 768 * 16 && GEMM_THREADS == 512, "two copies a thread of waves 0-3, one of waves 4-7");
 // LDS (the kernel's only LDS is the dynamic block, at address 0):
-//   K loop:   [0, 64 K) replicated LUT | plain LUT (1 KB) | three A buffers
-//   epilogue: Z in the A buffers' place, so that the LUT outlives the item
-//   E, loaded at item start and read in the epilogue, above both.
-// A block whose next item has the same tile (a persistent grid over one tile's groups) keeps the
-// LUT and E from the item before.
+//   [0, 64 K) replicated LUT | plain LUT (1 KB) | three A buffers | E | Z
+// E is loaded at item start and read in the epilogue. Z has its own region, so that the next
+// item's first A step can be copied into buffer 0 while the epilogue runs. A block whose next
+// item has the same tile (a persistent grid over one tile's groups) keeps the LUT and E from the
+// item before.
 // The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
 // address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
 constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
 constexpr int EE_OFF = ABUF_OFF + 3 * A_LDS;
-constexpr int GEMM_LDS = EE_OFF + EE_TILE * 2;
-static_assert(ZP_BYTES <= 3 * A_LDS, "the epilogue's Z fits the A buffers");
-static_assert(GEMM_LDS <= 160 * 1024, "the GEMM's LDS fits a CU");
+constexpr int ZP_OFF = EE_OFF + EE_TILE * 2;
+constexpr int GEMM_LDS = ZP_OFF + ZP_BYTES;
+static_assert(ZP_OFF % 16 == 0 && GEMM_LDS <= 160 * 1024, "the GEMM's LDS fits a CU");
 
 __host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
     return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
@@ -214,8 +215,16 @@ __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, u
     return tile < g.ntiles && grp < g.groups;
 }
 
+// What a block's item hands to the next: whether the next item exists, its tile and group, and
+// whether this item issued the next one's first A step (into buffer 0) and input bytes (raw).
+struct NextItem {
+    bool has;
+    uint32_t tile, grp;
+};
+
 template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep);
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep,
+                                          bool pre, const NextItem &nx, v4i (&raw)[2]);
 
 // s_waitcnt immediates (this family's encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
 // 11:8): wait until at most n vector-memory operations of this wave are outstanding, nothing else.
@@ -241,13 +250,26 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void *p, ui
 template <bool REP>
 __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
-    // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides
+    // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides.
+    // Each item issues the first A step and input bytes of the block's next item before its
+    // epilogue, so that they arrive while the epilogue runs.
+    uint32_t item = blockIdx.x, tile = 0, grp = 0;
+    while (item < g.items && !gemm_item_of(g, item, tile, grp)) item += gridDim.x;
     uint32_t prev = 0xFFFFFFFFu;
-    for (uint32_t item = blockIdx.x; item < g.items; item += gridDim.x) {
-        uint32_t tile, grp;
-        if (!gemm_item_of(g, item, tile, grp)) continue;
-        gemm_item<REP>(g, tile, grp, smem, tile == prev);
+    bool pre = false;
+    v4i raw[2];
+    while (item < g.items) {
+        NextItem nx;
+        uint32_t nitem = item + gridDim.x;
+        nx.tile = nx.grp = 0;
+        while (nitem < g.items && !gemm_item_of(g, nitem, nx.tile, nx.grp)) nitem += gridDim.x;
+        nx.has = nitem < g.items;
+        gemm_item<REP>(g, tile, grp, smem, tile == prev, pre, nx, raw);
         prev = tile;
+        tile = nx.tile;
+        grp = nx.grp;
+        item = nitem;
+        pre = nx.has;
         __syncthreads();  // the next item overwrites the LDS
     }
 }
@@ -258,12 +280,14 @@ typedef __attribute__((address_space(3))) const uint32_t lds_u32;
 __device__ __forceinline__ lds_u32 *lds_word(uint32_t a) { return (lds_u32 *)(uintptr_t)(a + LUTR_OFF); }
 
 template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep) {
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep,
+                                          bool pre, const NextItem &nx, v4i (&raw)[2]) {
     const TileDesc td = g.tiles[tile];
+    const uint64_t a_off_next = nx.has ? g.tiles[nx.tile].a_off : 0;  // read now, used after the K loop
 
     uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
     uint8_t *abuf = smem + ABUF_OFF;                  // [3][A_LDS]
-    uint16_t *zp = (uint16_t *)abuf;                  // [3][82][82], after the loop
+    uint16_t *zp = (uint16_t *)(smem + ZP_OFF);       // [3][82][82], after the loop
     uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][82]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -305,17 +329,20 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // streams are raw buffer loads: the region's base in the (uniform) descriptor, the step in the
     // scalar offset, the lane's fixed offset in the vector one, so a step's address is no VALU work.
     const uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);
-    const uint8_t *bbase = g.bytes + ((uint64_t)grp * GROUP_FRAGS + wave_u) * g.kbs * 2 * 64 * 16;
-    const __amdgpu_buffer_rsrc_t brs = uniform_rsrc(bbase, g.kbs * 2048u);
+    auto bytes_rsrc = [&](uint32_t gr) {
+        const uint8_t *bbase = g.bytes + ((uint64_t)gr * GROUP_FRAGS + wave_u) * g.kbs * 2 * 64 * 16;
+        return uniform_rsrc(bbase, g.kbs * 2048u);
+    };
+    const __amdgpu_buffer_rsrc_t brs = bytes_rsrc(grp);
     const uint32_t boff = (kh * 64 + off32) * 16;
     const __amdgpu_buffer_rsrc_t ars = uniform_rsrc(g.afrag + td.a_off, g.kbs * g.a_kb_stride);
 
-    v4i raw[2];
-    auto load_raw = [&](uint32_t kb) {
+    auto load_raw_from = [&](const __amdgpu_buffer_rsrc_t &rs, uint32_t kb) {
 #pragma unroll
         for (int nb = 0; nb < 2; nb++)
-            raw[nb] = __builtin_amdgcn_raw_buffer_load_b128(brs, boff + 256 * nb, (g.kb0 + kb) * 2048u, 0);
+            raw[nb] = __builtin_amdgcn_raw_buffer_load_b128(rs, boff + 256 * nb, (g.kb0 + kb) * 2048u, 0);
     };
+    auto load_raw = [&](uint32_t kb) { load_raw_from(brs, kb); };
     v4i bl[2], bh[2];
     auto build = [&]() {
         uint32_t v[2][16];
@@ -347,19 +374,18 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             bh[nb] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
-    // A goes from global memory straight into an LDS buffer (buffer loads to LDS, no registers):
-    // every thread moves AP 16-byte words a step, a wave 64 consecutive ones; words past the step's
-    // 768 repeat the last one and land in the buffer's padding.
-    uint32_t aoffs[AP];
+    // A goes from global memory straight into an LDS buffer (buffer loads to LDS, no registers): a
+    // step's 768 16-byte words, word w by thread w % 512, a wave 64 consecutive ones (the words
+    // from 512 on by waves 0-3).
+    auto dma_a_from = [&](const __amdgpu_buffer_rsrc_t &rs, uint32_t kb, int buf) {
 #pragma unroll
-    for (int i = 0; i < AP; i++) aoffs[i] = min(tid + GEMM_THREADS * i, A_STEP / 16 - 1) * 16;
-    auto dma_a = [&](uint32_t kb, int buf) {
-#pragma unroll
-        for (int i = 0; i < AP; i++)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                ars, (__attribute__((address_space(3))) void *)(abuf + buf * A_LDS + (64 * wave_u + GEMM_THREADS * i) * 16),
-                16, aoffs[i], (g.kb0 + kb) * g.a_kb_stride, 0, 0);
+        for (int i = 0; i < 2; i++)
+            if (i == 0 || wave_u < 4)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs, (__attribute__((address_space(3))) void *)(abuf + buf * A_LDS + (64 * wave_u + GEMM_THREADS * i) * 16),
+                    16, (tid + GEMM_THREADS * i) * 16, (g.kb0 + kb) * g.a_kb_stride, 0, 0);
     };
+    auto dma_a = [&](uint32_t kb, int buf) { dma_a_from(ars, kb, buf); };
     // this lane's A pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
     // (mb16 < 5, rows 0..79) and, for the sixth block, rows 80 and 81: lane c16 < 4 holds digit
     // c16 & 1 of row 80 + (c16 >> 1); the other lanes read row 82, which every tile's A keeps zero
@@ -394,8 +420,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // (waves 4-7), both into the buffer that no wave reads in that interval. A wave issues its A
     // copy and the next step's input bytes just before its products, which read no LDS, so both
     // land while the products run; the compiler waits for them before the next LDS access.
-    dma_a(0, 0);
-    load_raw(0);
+    // A(0) and the bytes of step 0 were issued by the block's item before, if any (pre).
+    if (!pre) {
+        dma_a(0, 0);
+        load_raw(0);
+    }
     if (!keep) {
         ((uint4 *)ee)[tid] = ee0;
         if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
@@ -471,7 +500,13 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     __builtin_amdgcn_sched_barrier(0);
     for (uint32_t kb = 1; kb < kbs; kb++) step(kb, std::false_type{});
     __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
-    __syncthreads();  // waves 4-7's last products read the LDS that the epilogue overwrites
+    __syncthreads();  // every wave is past its last read of the A buffers
+    // The block's next item: its A(0) into buffer 0 and its bytes of step 0, now, so that they
+    // arrive during the epilogue (which uses neither the A buffers nor raw).
+    if (nx.has) {
+        dma_a_from(uniform_rsrc(g.afrag + a_off_next, g.kbs * g.a_kb_stride), 0, 0);
+        load_raw_from(bytes_rsrc(nx.grp), 0);
+    }
 
     // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q, centred (int16), into zp[b][m][n] for this lane's inputs
     // j = 81 b + n; the D block of a 16 x 16 product holds, in register r, row 4 kg + r and
