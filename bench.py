@@ -86,10 +86,11 @@ def stage_rooflines(params, stage, ms):
     return out, round(floor, 4), round(floor / ms, 4)
 
 
-def mfma_rate(L, random_operands):
-    """The device's int8 MFMA rate (TOP/s), measured by lbn_debug_mfma_rate."""
+def mfma_rate(L, operands):
+    """The device's int8 MFMA rate (TOP/s) after a warm-up, measured by lbn_debug_mfma_rate:
+    operands 0 constant, 1 random bytes switching every product, 2 like the tile GEMM's digits."""
     t = ctypes.c_double()
-    rc = L.lbn_debug_mfma_rate(ctypes.c_int(random_operands), ctypes.byref(t))
+    rc = L.lbn_debug_mfma_rate(ctypes.c_int(operands), ctypes.byref(t))
     if rc != 0:
         raise RuntimeError(L.lbn_last_error().decode())
     return round(t.value, 1)
@@ -552,7 +553,7 @@ def main():
     ops = tile_ops(params)
     achieved = ops / (gemm_ms * 1e-3) / 1e12
     xops = executed_ops(params)
-    rate_c, rate_r = mfma_rate(L, 0), mfma_rate(L, 1)
+    rate_c, rate_r, rate_g = mfma_rate(L, 0), mfma_rate(L, 1), mfma_rate(L, 2)
     ab = algorithmic_bytes(params)
     sr, sr_floor, sr_frac = stage_rooflines(params, stage, ms)
     line = {
@@ -587,8 +588,12 @@ def main():
                      "ops_per_launch": ops, "launch_ms": round(gemm_ms, 4),
                      "executed_ops_per_launch": xops,
                      "measured_mfma_rate": {"constant_operands": rate_c, "random_operands": rate_r,
-                                            "unit": "TOP/s",
-                                            "executed_frac_of_random_rate": round(xops / (gemm_ms * 1e-3) / 1e12 / rate_r, 4)}},
+                                            "gemm_digit_operands": rate_g, "unit": "TOP/s",
+                                            "note": "after a warm-up of about 10 ms; random = random bytes "
+                                                    "switching every product (the worst case), gemm digit = "
+                                                    "operands like the tile GEMM's",
+                                            "executed_frac_of_gemm_digit_rate":
+                                                round(xops / (gemm_ms * 1e-3) / 1e12 / rate_g, 4)}},
     }
     if not args.no_wire:
         log("wire...")

@@ -30,9 +30,11 @@ int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d);
 int lbn_debug_gemm_chunk(uint32_t kb);
 
 /* Measurement, no counterpart in the reference: the device's int8 MFMA rate in TOP/s
- * (v_mfma_i32_16x16x64_i8, 16 independent chains a wave, all CUs), on constant operands
- * (random_operands = 0) or on random ones, which draw more power and clock lower. */
-int lbn_debug_mfma_rate(int random_operands, double *tops);
+ * (v_mfma_i32_16x16x64_i8, 16 independent chains a wave, all CUs, after a warm-up of about 10 ms),
+ * on constant operands (operands = 0), on random bytes switching every product (1), or on
+ * operands like the tile GEMM's: random digits in [0, 128) and [0, 64), B switching every eighth
+ * product (2). Operands that toggle more bits draw more power and clock lower. */
+int lbn_debug_mfma_rate(int operands, double *tops);
 
 /* One v_mfma_f32_16x16x128_f8f6f4 with FP4 (e2m1) operands, given as each lane's operand registers:
  * a and b are 64 lanes x 16 bytes, d receives 64 lanes x 4 floats (the result registers); scaled

@@ -40,8 +40,8 @@ constexpr int GROUP_BATCH = 3;               // batch elements per group
 constexpr int GROUP_COLS = 256;              // N of a group (243 used)
 constexpr int GROUP_FRAGS = GROUP_COLS / BLK;
 constexpr int TILE_ROWS = 82;               // rows a tile may use (a quadratic commit tile's first half: 82)
-constexpr int EP = TILE_USED + 1;            // E and Z rows padded to an even length (pairs of n)
-constexpr int EE_TILE = TILE_ROWS * EP + 4;  // E of a tile, [m 82][n 82] u16 (zero at n = 81), 16-byte padded
+constexpr int EP = 88;                      // E and Z rows padded to 176 bytes (16-byte aligned; zero from n = 81)
+constexpr int EE_TILE = TILE_ROWS * EP;      // E of a tile, [m 82][n 88] u16
 
 // The input bytes in MFMA B-fragment order. Byte n of batch element b at K index k lives at
 //
@@ -63,7 +63,7 @@ inline uint64_t bytes_size(uint32_t batches, uint32_t kbs) {
 struct TileDesc {
     uint64_t a_off;     // byte offset of the tile's A fragments: [kb][kh 2][digit 2][mb 3][lane 64][16]
     uint32_t lut_off;   // element offset (u16) of the tile's two 256-entry LUTs
-    uint32_t ee_off;    // element offset (u16) of the tile's E, [n 81][m 96]
+    uint32_t ee_off;    // element offset (u16) of the tile's E, [m 82][n 88]
     uint32_t out_off;   // element offset (i16) of the tile's 96 output slots (-1 = padding row)
     uint32_t q;
     uint32_t par;       // input n reads LUT (par + n) & 1

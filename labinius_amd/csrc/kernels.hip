@@ -35,9 +35,9 @@ namespace lbn {
 // coefficient 4 m + k of a ring element is bit m of its F162 element k (f162.rs:1-16). Written
 // twice in fragment order (byte_offset): the commit's (batch c, K = i) and the fold's (batch i,
 // K = c). A block covers 16 ring elements x 16 columns, one thread per ring element, and stages
-// the bytes in LDS twice, ring elements fastest and columns fastest, so that each store gathers
-// its 16 bytes with one aligned 16-byte read. Rows are padded to 272 bytes, which puts 16
-// consecutive n on distinct banks.
+// the bytes in LDS, ring elements fastest for the commit's stores and then, in the same 22 KB,
+// columns fastest for the fold's, so that each store gathers its 16 bytes with one aligned
+// 16-byte read. Rows are padded to 272 bytes, which puts 16 consecutive n on distinct banks.
 constexpr int PRE_I = 16, PRE_C = 16, PRE_ROW = PRE_I * PRE_C + 16;
 
 __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *__restrict__ w,
@@ -45,8 +45,7 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
                                                                 uint32_t column_len, uint32_t L, uint32_t r,
                                                                 uint8_t *__restrict__ bytes_c, uint32_t kbs_c,
                                                                 uint8_t *__restrict__ bytes_f, uint32_t kbs_f) {
-    __shared__ __attribute__((aligned(16))) uint8_t stc[TILE_USED][PRE_ROW];  // [n][c * 16 + i]
-    __shared__ __attribute__((aligned(16))) uint8_t stf[TILE_USED][PRE_ROW];  // [n][i * 16 + c]
+    __shared__ __attribute__((aligned(16))) uint8_t stc[TILE_USED][PRE_ROW];  // [n][c * 16 + i], then [n][i * 16 + c]
     const int il = threadIdx.x & (PRE_I - 1), cl = threadIdx.x / PRE_I;
     const uint32_t i0 = blockIdx.y * PRE_I, c0 = blockIdx.x * PRE_C;
     const uint32_t i = i0 + il, c = c0 + cl;
@@ -113,11 +112,7 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
     stage(std::integral_constant<int, 2>{}, 0x33333333u);
     stage(std::integral_constant<int, 1>{}, 0x55555555u);
 #pragma unroll
-    for (int n = 0; n < TILE_USED; n++) {
-        const uint8_t b = (uint8_t)(x[n >> 2] >> (8 * (n & 3)));
-        stc[n][cl * PRE_I + il] = b;
-        stf[n][il * PRE_C + cl] = b;
-    }
+    for (int n = 0; n < TILE_USED; n++) stc[n][cl * PRE_I + il] = (uint8_t)(x[n >> 2] >> (8 * (n & 3)));
     __syncthreads();
     // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16)
     if (bytes_c)
@@ -127,11 +122,15 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
             *(uint4 *)(bytes_c + byte_offset(c0 + cc, n, i0, kbs_c)) = *(const uint4 *)&stc[n][cc * PRE_I];
         }
     if (!bytes_f) return;
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < TILE_USED; n++) stc[n][il * PRE_C + cl] = (uint8_t)(x[n >> 2] >> (8 * (n & 3)));
+    __syncthreads();
     // fold: for each (i, n) the 16 bytes of columns c0 .. c0 + 15 (c0 is a multiple of 16)
     for (int u = threadIdx.x; u < PRE_I * TILE_USED; u += PRE_I * PRE_C) {
         const int ii = u / TILE_USED, n = u % TILE_USED;
         if (i0 + ii >= L) continue;
-        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0, kbs_f)) = *(const uint4 *)&stf[n][ii * PRE_C];
+        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0, kbs_f)) = *(const uint4 *)&stc[n][ii * PRE_C];
     }
 }
 
@@ -158,7 +157,7 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // Blocks are ordered so that the 32 blocks an XCD runs at once are 8 tiles x 4 groups: each
 // A and each group's bytes then come from L2 for 4 and 8 blocks respectively.
 constexpr int GEMM_THREADS = 512;
-// The epilogue's Z in LDS: [batch 3][m 82][n 82] u16 (Z < q < 2^15), rows padded to pairs of n.
+// The epilogue's Z in LDS: [batch 3][m 82][n 88] u16 (centred, |Z| <= (q + 1) / 2), rows of 176 bytes.
 constexpr int ZP_BYTES = (GROUP_BATCH * TILE_ROWS * EP * 2 + 15) / 16 * 16;
 // A is staged per K step in an LDS buffer of A_STEP bytes: 768 16-byte words, the first 512 one
 // a thread, the other 256 one a thread of waves 0-3.
@@ -287,8 +286,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 
     uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
     uint8_t *abuf = smem + ABUF_OFF;                  // [3][A_LDS]
-    uint16_t *zp = (uint16_t *)(smem + ZP_OFF);       // [3][82][82], after the loop
-    uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][82]
+    uint16_t *zp = (uint16_t *)(smem + ZP_OFF);       // [3][82][88], after the loop
+    uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][88]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (!keep) {
@@ -399,7 +398,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     v4i acc[5][2][3], xacc[2][2];
 
     // E and the output slots are loaded now, behind the K loop; E's LDS area is not used before
-    // the epilogue. 82 x 82 x 2 bytes (+ padding) are 841 16-byte words, two per thread.
+    // the epilogue. 82 x 88 x 2 bytes are 902 16-byte words, two per thread.
     static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
     static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
     constexpr int EE_V = EE_TILE * 2 / 16;
@@ -533,28 +532,28 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         }
     }
     __syncthreads();
-    // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (pairs of n
-    // 0..20 and 21..40), each pair one v_dot2_i32_i16: 0 <= E < q and |Z| <= (q + 1) / 2 with
-    // q <= 19441, so five pairs (ten products) stay within an int32 before it is added to the
-    // int64 sum
+    // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (n 0..39 and
+    // 40..87; E is zero from n = 81), reading four pairs of n at a time (16 bytes), each pair one
+    // v_dot2_i32_i16: 0 <= E < q and |Z| <= (q + 1) / 2 with q <= 19441, so four pairs (eight
+    // products) stay within an int32 before it is added to the int64 sum
     {
         const int o = tid >> 1, half = tid & 1;
         const int b = o / TILE_ROWS, m = o % TILE_ROWS;
-        const uint32_t *ep = (const uint32_t *)(ee + m * EP) + 21 * half;
-        const uint32_t *zq = (const uint32_t *)(zp + (b * TILE_ROWS + m) * EP) + 21 * half;
-        const int np = 21 - half;  // pairs of this half
-        static_assert(10ll * (19441 - 1) * ((19441 + 1) / 2) < (1ll << 31), "five pairs fit an int32 (q <= 19441)");
+        static_assert(EP * 2 == 11 * 16, "a row of E or Z is eleven 16-byte words");
+        const uint4 *ep = (const uint4 *)(ee + m * EP) + 5 * half;
+        const uint4 *zq = (const uint4 *)(zp + (b * TILE_ROWS + m) * EP) + 5 * half;
+        static_assert(8ll * (19441 - 1) * ((19441 + 1) / 2) < (1ll << 31), "four pairs fit an int32 (q <= 19441)");
         long long sum = 0;
         if (tid < 2 * GROUP_BATCH * TILE_ROWS) {
 #pragma unroll
-            for (int c = 0; c < 21; c += 5) {
-                int s5 = 0;
-#pragma unroll
-                for (int i = c; i < c + 5 && i < 21; i++)
-                    if (i < np)
-                        s5 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, ep[i]), __builtin_bit_cast(s16x2, zq[i]),
-                                                    s5, false);
-                sum += s5;
+            for (int c = 0; c < 6; c++) {
+                if (c == 5 && !half) break;  // half 0 has five words, half 1 six
+                const uint4 e = ep[c], z = zq[c];
+                int s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.x), __builtin_bit_cast(s16x2, z.x), 0, false);
+                s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.y), __builtin_bit_cast(s16x2, z.y), s4, false);
+                s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.z), __builtin_bit_cast(s16x2, z.z), s4, false);
+                s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.w), __builtin_bit_cast(s16x2, z.w), s4, false);
+                sum += s4;
             }
         }
         sum += __shfl_xor(sum, 1, 64);
@@ -1612,9 +1611,10 @@ __global__ void mfma_probe_kernel(const int8_t *__restrict__ A, const int8_t *__
 }
 
 // The int8 MFMA rate: each wave issues iters x 16 independent v_mfma_i32_16x16x64_i8 (fewer chains
-// leave the pipe waiting on accumulators), on operands
-// that are constant, or random per lane and alternating between two sets. Random operands toggle
-// more bits, draw more power and run at a lower clock.
+// leave the pipe waiting on accumulators). mode 0: constant operands; 1: random bytes per lane,
+// both operands switching between two sets every product; 2: operands like the tile GEMM's, low
+// digits in [0, 128) and high digits in [0, 64), A switching every product and B every eight.
+// Operands that toggle more bits draw more power and run at a lower clock.
 __device__ __forceinline__ int rate_hash(uint32_t x) {
     x *= 2654435761u;
     x ^= x >> 15;
@@ -1622,22 +1622,36 @@ __device__ __forceinline__ int rate_hash(uint32_t x) {
     x ^= x >> 13;
     return (int)x;
 }
-__global__ __launch_bounds__(256) void mfma_rate_kernel(int random, uint32_t iters, int32_t *__restrict__ sink) {
+__global__ __launch_bounds__(256) void mfma_rate_kernel(int mode, uint32_t iters, int32_t *__restrict__ sink) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     v4i a0 = {0x01010101, 0x01010101, 0x01010101, 0x01010101}, b0 = a0, a1 = a0, b1 = a0;
-    if (random) {
+    if (mode) {
         a0 = (v4i){rate_hash(16 * t), rate_hash(16 * t + 1), rate_hash(16 * t + 2), rate_hash(16 * t + 3)};
         b0 = (v4i){rate_hash(16 * t + 4), rate_hash(16 * t + 5), rate_hash(16 * t + 6), rate_hash(16 * t + 7)};
         a1 = (v4i){rate_hash(16 * t + 8), rate_hash(16 * t + 9), rate_hash(16 * t + 10), rate_hash(16 * t + 11)};
         b1 = (v4i){rate_hash(16 * t + 12), rate_hash(16 * t + 13), rate_hash(16 * t + 14), rate_hash(16 * t + 15)};
     }
+    if (mode == 2) {
+        a0 &= 0x7F7F7F7F;
+        b0 &= 0x7F7F7F7F;
+        a1 &= 0x3F3F3F3F;
+        b1 &= 0x3F3F3F3F;
+    }
     v4i acc[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = (v4i){};
-    for (uint32_t it = 0; it < iters; it++) {
+    if (mode == 2) {
+        for (uint32_t it = 0; it < iters; it++) {
 #pragma unroll
-        for (int i = 0; i < 16; i++)
-            acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8((i & 1) ? a1 : a0, (i & 1) ? b1 : b0, acc[i], 0, 0, 0);
+            for (int i = 0; i < 16; i++)
+                acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8((i & 1) ? a1 : a0, (i & 8) ? b1 : b0, acc[i], 0, 0, 0);
+        }
+    } else {
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8((i & 1) ? a1 : a0, (i & 1) ? b1 : b0, acc[i], 0, 0, 0);
+        }
     }
     int s = 0;
 #pragma unroll
@@ -2104,8 +2118,8 @@ __global__ __launch_bounds__(256) void rans_scatter_kernel(const uint32_t *__res
         if (e_ != hipSuccess) return e_;                                    \
     } while (0)
 
-hipError_t launch_mfma_rate(hipStream_t s, int random, uint32_t blocks, uint32_t iters, int32_t *sink) {
-    LBN_LAUNCH(mfma_rate_kernel, dim3(blocks), dim3(256), 0, s, random, iters, sink);
+hipError_t launch_mfma_rate(hipStream_t s, int mode, uint32_t blocks, uint32_t iters, int32_t *sink) {
+    LBN_LAUNCH(mfma_rate_kernel, dim3(blocks), dim3(256), 0, s, mode, iters, sink);
     return hipSuccess;
 }
 

@@ -17,7 +17,7 @@ struct lbn_challenge_dev {
 };
 
 hipError_t launch_mfma_probe(hipStream_t s, const int8_t *A, const int8_t *B, int32_t *D);
-hipError_t launch_mfma_rate(hipStream_t s, int random, uint32_t blocks, uint32_t iters, int32_t *sink);
+hipError_t launch_mfma_rate(hipStream_t s, int mode, uint32_t blocks, uint32_t iters, int32_t *sink);
 hipError_t launch_mfma_f4_probe(hipStream_t s, const uint32_t *A, const uint32_t *B, float *D, int scaled);
 hipError_t launch_mfma_f4_rate(hipStream_t s, int random, uint32_t blocks, uint32_t iters, float *sink);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);

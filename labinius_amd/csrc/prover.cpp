@@ -131,8 +131,8 @@ static HostTile quad_fold_tile(const Prime &P, int k, int b) {
     return h;
 }
 
-// E of the rows a tile may use, [m < 82][n < 82] (mod q, zero at n = 81): the GEMM's epilogue
-// reads a row two n at a time. Rows from 82 on are padding in every tile (the builders above use
+// E of the rows a tile may use, [m < 82][n < 88] (mod q, zero from n = 81): the GEMM's epilogue
+// reads a row eight n at a time. Rows from 82 on are padding in every tile (the builders above use
 // m < 81, or m < 82 for a quadratic commit tile's first half).
 static void pack_ee(const HostTile &h, uint16_t *dst) {
     for (int m = 0; m < TILE_ROWS; m++)
@@ -995,18 +995,21 @@ int lbn_debug_gemm_chunk(uint32_t kb) {
 }
 
 // Measurement (include/labinius_debug.h): the int8 MFMA rate in TOP/s on constant or random operands.
-int lbn_debug_mfma_rate(int random_operands, double *tops) {
+int lbn_debug_mfma_rate(int operands, double *tops) {
     if (!tops) return fail(LBN_ERR_ARG, "null argument");
-    const uint32_t blocks = 1024, iters = 4000;  // 4 waves a block, 4 blocks a CU
+    if (operands < 0 || operands > 2) return fail(LBN_ERR_ARG, "operands: 0, 1 or 2");
+    // 4 waves a block, 4 blocks a CU; a warm-up of about 10 ms first, so that the clock has
+    // settled where this load holds it
+    const uint32_t blocks = 1024, iters = 8000, warm = 20000;
     int32_t *sink = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = LBN_OK;
     float ms = 0;
     if (hipMalloc((void **)&sink, (size_t)blocks * 256 * sizeof(int32_t)) != hipSuccess) return fail(LBN_ERR_OOM, "rate sink");
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
-        launch_mfma_rate(nullptr, random_operands, blocks, 16, sink) != hipSuccess ||  // warm-up
+        launch_mfma_rate(nullptr, operands, blocks, warm, sink) != hipSuccess ||
         hipEventRecord(e0, nullptr) != hipSuccess ||
-        launch_mfma_rate(nullptr, random_operands, blocks, iters, sink) != hipSuccess ||
+        launch_mfma_rate(nullptr, operands, blocks, iters, sink) != hipSuccess ||
         hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
         rc = fail(LBN_ERR_HIP, "rate launch");

@@ -8,9 +8,9 @@ import numpy as np
 N = 648
 TILE, KSTEP, NBLK, FRAG = 96, 64, 3, 1024
 A_STEP = 2 * 2 * NBLK * FRAG
-EP = 82  # E rows padded to pairs of n
+EP = 88  # E rows padded to 176 bytes
 TILE_ROWS = 82  # rows a tile may use
-EE_TILE = TILE_ROWS * EP + 4
+EE_TILE = TILE_ROWS * EP
 YACC_PLANES = 2  # commit output planes: one per input parity of a quadratic limb
 UNSIGNED_DIGITS_Q = 16384  # below it operands are split into non-negative digits, above it centred ones
 DESC = np.dtype([("a_off", "<u8"), ("lut_off", "<u4"), ("ee_off", "<u4"), ("out_off", "<u4"), ("q", "<u4"),
@@ -58,10 +58,10 @@ def a_matrix(afrag, kbs):
 
 
 def ee_matrix(ee):
-    """[m 82][n 82] (zero at n = 81) -> E[m][n] (mod q) for m < 96, the padding rows zero"""
+    """[m 82][n 88] (zero from n = 81) -> E[m][n] (mod q) for m < 96, the padding rows zero"""
     E = np.zeros((TILE, 81), np.int64)
     t = ee[:TILE_ROWS * EP].reshape(TILE_ROWS, EP).astype(np.int64)
-    assert not t[:, 81].any()
+    assert not t[:, 81:].any()
     E[:TILE_ROWS] = t[:, :81]
     return E
 
