@@ -12,7 +12,7 @@
 //     y[u] = sum_n E[u][n] * Z[u][n],        Z[u][n] = sum_i A[i][u] * x_i[n],
 //
 // where Z is an integer GEMM (K = ring elements) run on int8 MFMA with both operands split into
-// two signed 7-bit digits. A "tile" is one such 96 x 81 (81 x 81 used) GEMM: 8 per splitting
+// two int8 digits (UNSIGNED_DIGITS_Q below). A "tile" is one such 96 x 81 (81 x 81 used) GEMM: 8 per splitting
 // limb, 16 per quadratic limb (4 sub-rings x 2 halves of the A rows x 2 parities of the inputs).
 // The fold is the same GEMM with K = columns, A = the challenges' transform, 8 tiles on the base.
 //
