@@ -214,16 +214,22 @@ __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, u
     return tile < g.ntiles && grp < g.groups;
 }
 
-// What a block's item hands to the next: whether the next item exists, its tile and group, and
-// whether this item issued the next one's first A step (into buffer 0) and input bytes (raw).
+// The block's next item: whether there is one, its tile and group.
 struct NextItem {
     bool has;
     uint32_t tile, grp;
 };
 
+// What an item hands to the next on its block: the registers of the first step (A(0)'s pieces
+// and the packed operands of step 0) and the input bytes, filled in by the item before (ready)
+// during its epilogue.
+struct ItemRegs {
+    v4i raw[2], a0[5], a1[5], ax, bl[2], bh[2];
+};
+
 template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep,
-                                          bool pre, const NextItem &nx, v4i (&raw)[2]);
+                                          bool ready, const NextItem &nx, ItemRegs &R);
 
 // s_waitcnt immediates (this family's encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
 // 11:8): wait until at most n vector-memory operations of this wave are outstanding, nothing else.
@@ -250,25 +256,25 @@ template <bool REP>
 __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
     extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
     // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides.
-    // Each item issues the first A step and input bytes of the block's next item before its
-    // epilogue, so that they arrive while the epilogue runs.
+    // Each item sets up the block's next item during its own epilogue (its LUT, A copies, input
+    // bytes and the first step's operands), so that the next item starts with its first products.
     uint32_t item = blockIdx.x, tile = 0, grp = 0;
     while (item < g.items && !gemm_item_of(g, item, tile, grp)) item += gridDim.x;
     uint32_t prev = 0xFFFFFFFFu;
-    bool pre = false;
-    v4i raw[2];
+    bool ready = false;
+    ItemRegs R;
     while (item < g.items) {
         NextItem nx;
         uint32_t nitem = item + gridDim.x;
         nx.tile = nx.grp = 0;
         while (nitem < g.items && !gemm_item_of(g, nitem, nx.tile, nx.grp)) nitem += gridDim.x;
         nx.has = nitem < g.items;
-        gemm_item<REP>(g, tile, grp, smem, tile == prev, pre, nx, raw);
+        gemm_item<REP>(g, tile, grp, smem, tile == prev, ready, nx, R);
         prev = tile;
         tile = nx.tile;
         grp = nx.grp;
         item = nitem;
-        pre = nx.has;
+        ready = nx.has;
         __syncthreads();  // the next item overwrites the LDS
     }
 }
@@ -280,9 +286,15 @@ __device__ __forceinline__ lds_u32 *lds_word(uint32_t a) { return (lds_u32 *)(ui
 
 template <bool REP>
 __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep,
-                                          bool pre, const NextItem &nx, v4i (&raw)[2]) {
+                                          bool ready, const NextItem &nx, ItemRegs &R) {
     const TileDesc td = g.tiles[tile];
-    const uint64_t a_off_next = nx.has ? g.tiles[nx.tile].a_off : 0;  // read now, used after the K loop
+    const TileDesc tdn = g.tiles[nx.has ? nx.tile : tile];  // the next item's, read now, used after the K loop
+    v4i(&raw)[2] = R.raw;
+    v4i(&a0)[5] = R.a0;
+    v4i(&a1)[5] = R.a1;
+    v4i &ax = R.ax;
+    v4i(&bl)[2] = R.bl;
+    v4i(&bh)[2] = R.bh;
 
     uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
     uint8_t *abuf = smem + ABUF_OFF;                  // [3][A_LDS]
@@ -290,8 +302,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][88]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (!keep) {
-        const uint32_t lv = g.lut[td.lut_off + tid];  // entry (parity tid >> 8, byte tid & 255)
+    // a tile's LUT into LDS: entry (parity tid >> 8, byte tid & 255), replicated or plain
+    auto set_lut = [&](const TileDesc &t) {
+        const uint32_t lv = g.lut[t.lut_off + tid];
         if (REP) {
             // its 32 copies: eight 16-byte stores of one value
             const uint4 v4 = make_uint4(lv, lv, lv, lv);
@@ -301,7 +314,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         } else {
             lut[tid] = (uint16_t)lv;
         }
-    }
+    };
+    if (!ready && !keep) set_lut(td);
 
     // v_mfma_i32_16x16x64_i8 operands: lane holds row (A) or input (B) c16 = lane & 15 of a 16-block
     // at K group kg = lane >> 4, i.e. K 16 kg .. 16 kg + 15 of the step's 64. A wave owns 32 inputs
@@ -315,14 +329,21 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     const int off32 = c16 + 32 * (kg & 1), kh = kg >> 1;
     // byte 0 of a lookup address: the input's parity and this lane's LUT copy; byte 1 comes from
     // the input word
-    uint32_t lsel[2];
-    const uint16_t *L[2];
+    struct Sel {
+        uint32_t lsel[2];
+        const uint16_t *L[2];
+    };
+    auto sel_of = [&](const TileDesc &t) {
+        Sel x;
 #pragma unroll
-    for (int nb = 0; nb < 2; nb++) {
-        const int par = ((int)td.par + (jl + 16 * nb) % TILE_USED) & 1;
-        lsel[nb] = (uint32_t)(par << 7) | (uint32_t)((lane & 31) << 2);
-        L[nb] = lut + (par << 8);
-    }
+        for (int nb = 0; nb < 2; nb++) {
+            const int par = ((int)t.par + (jl + 16 * nb) % TILE_USED) & 1;
+            x.lsel[nb] = (uint32_t)(par << 7) | (uint32_t)((lane & 31) << 2);
+            x.L[nb] = lut + (par << 8);
+        }
+        return x;
+    };
+    const Sel sel = sel_of(td);
 
     // the (group, wave) byte fragments: [kb][kh 2][lane 64][16]; this lane's two pieces a step. Both
     // streams are raw buffer loads: the region's base in the (uniform) descriptor, the step in the
@@ -342,8 +363,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             raw[nb] = __builtin_amdgcn_raw_buffer_load_b128(rs, boff + 256 * nb, (g.kb0 + kb) * 2048u, 0);
     };
     auto load_raw = [&](uint32_t kb) { load_raw_from(brs, kb); };
-    v4i bl[2], bh[2];
-    auto build = [&]() {
+    // the operands of one step from its input bytes (raw) through the LUT, with a tile's lookup
+    // selectors
+    auto build_with = [&](const Sel &x) {
         uint32_t v[2][16];
         // all 32 lookups first, so that their latencies overlap
 #pragma unroll
@@ -355,8 +377,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const uint32_t r = w4[d];
-                    v[nb][4 * d + e] = REP ? *lds_word(__builtin_amdgcn_perm(r, lsel[nb], 0x0c0c0400u + 0x100u * e))
-                                           : L[nb][(r >> (8 * e)) & 255];
+                    v[nb][4 * d + e] = REP ? *lds_word(__builtin_amdgcn_perm(r, x.lsel[nb], 0x0c0c0400u + 0x100u * e))
+                                           : x.L[nb][(r >> (8 * e)) & 255];
                 }
         }
 #pragma unroll
@@ -373,6 +395,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             bh[nb] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
+    auto build = [&]() { build_with(sel); };
     // A goes from global memory straight into an LDS buffer (buffer loads to LDS, no registers): a
     // step's 768 16-byte words, word w by thread w % 512, a wave 64 consecutive ones (the words
     // from 512 on by waves 0-3).
@@ -419,21 +442,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // (waves 4-7), both into the buffer that no wave reads in that interval. A wave issues its A
     // copy and the next step's input bytes just before its products, which read no LDS, so both
     // land while the products run; the compiler waits for them before the next LDS access.
-    // A(0) and the bytes of step 0 were issued by the block's item before, if any (pre).
-    if (!pre) {
-        dma_a(0, 0);
-        load_raw(0);
-    }
-    if (!keep) {
-        ((uint4 *)ee)[tid] = ee0;
-        if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
-    }
-    if (wgrp) dma_a(kbs > 1 ? 1 : 0, 1);
-    __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
-    __syncthreads();  // the LUT, E, A(0) and waves 4-7's part of A(1)
-    // the next step's A pieces (both digits, and the sixth block's) are read ahead, at the end of
-    // the step before
-    v4i a0[5], a1[5], ax;
+    // The next step's A pieces (both digits, and the sixth block's) are read ahead, at the end of
+    // the step before.
     auto read_a0 = [&](const uint8_t *ab) {
 #pragma unroll
         for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
@@ -442,8 +452,22 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             a1[mb] = *(const v4i *)(ab + aoff + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
         ax = *(const v4i *)(ab + xoff);
     };
-    read_a0(abuf);
-    build();
+    // Set-up: the LUT, A(0), waves 4-7's part of A(1), the bytes of step 0, then A(0)'s pieces and
+    // step 0's operands. The block's item before did all of it during its epilogue (ready); E,
+    // read only in the epilogue, is stored here when the tile changes.
+    if (!keep) {
+        ((uint4 *)ee)[tid] = ee0;
+        if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
+    }
+    if (!ready) {
+        dma_a(0, 0);
+        load_raw(0);
+        if (wgrp) dma_a(kbs > 1 ? 1 : 0, 1);
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
+        __syncthreads();                // the LUT, A(0) and waves 4-7's part of A(1)
+        read_a0(abuf);
+        build();
+    }
     // step kb; FIRST: the step that starts the accumulators
     auto step = [&](uint32_t kb, auto first) {
         constexpr bool FIRST = decltype(first)::value;
@@ -499,11 +523,16 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     __builtin_amdgcn_sched_barrier(0);
     for (uint32_t kb = 1; kb < kbs; kb++) step(kb, std::false_type{});
     __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
-    __syncthreads();  // every wave is past its last read of the A buffers
-    // The block's next item: its A(0) into buffer 0 and its bytes of step 0, now, so that they
-    // arrive during the epilogue (which uses neither the A buffers nor raw).
+    __syncthreads();  // every wave is past its last read of the A buffers and of the LUT
+    // The block's next item, set up now (the epilogue uses neither the LUT, the A buffers nor the
+    // operand registers): its LUT when the tile changes, A(0) into buffer 0, waves 4-7's part of
+    // A(1) into buffer 1, the bytes of step 0; after the epilogue's barrier, A(0)'s pieces and
+    // step 0's operands.
     if (nx.has) {
-        dma_a_from(uniform_rsrc(g.afrag + a_off_next, g.kbs * g.a_kb_stride), 0, 0);
+        if (nx.tile != tile) set_lut(tdn);
+        const __amdgpu_buffer_rsrc_t nars = uniform_rsrc(g.afrag + tdn.a_off, g.kbs * g.a_kb_stride);
+        dma_a_from(nars, 0, 0);
+        if (wgrp) dma_a_from(nars, kbs > 1 ? 1 : 0, 1);
         load_raw_from(bytes_rsrc(nx.grp), 0);
     }
 
@@ -531,7 +560,12 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             zj[81 * EP] = (uint16_t)mod_q_classes(xl[2], xh[2] + xl[3], xh[3], qd, qinv);
         }
     }
-    __syncthreads();
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's copies for the next item are complete
+    __syncthreads();  // Z, and the next item's LUT and A copies
+    if (nx.has) {
+        read_a0(abuf);
+        build_with(sel_of(tdn));
+    }
     // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (n 0..39 and
     // 40..87; E is zero from n = 81), reading four pairs of n at a time (16 bytes), each pair one
     // v_dot2_i32_i16: 0 <= E < q and |Z| <= (q + 1) / 2 with q <= 19441, so four pairs (eight
