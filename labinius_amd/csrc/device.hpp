@@ -89,6 +89,11 @@ struct GemmArgs {
     uint32_t sched_t;         // tiles per XCD round (1 .. 32, a power of two); the rest are groups
     uint32_t tile_major;      // 1: consecutive rounds share their tiles (and walk the groups)
     uint32_t items;           // work items (set by the launcher)
+    // set by the launcher: log2(sched_t); the tile sets and group blocks of the XCD rounds; and
+    // for each the multiplier ceil(2^32 / d) that divides every round index by d as a product's
+    // high word (0 where rounds x d reaches 2^32 and that is not exact: the kernel then divides)
+    uint32_t t_shift, tgroups, gblocks;
+    uint64_t tg_magic, gb_magic;
 };
 
 // Operands mod q are split into two int8 digits, x = x_l + 128 x_h: for q < UNSIGNED_DIGITS_Q

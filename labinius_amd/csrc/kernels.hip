@@ -203,14 +203,19 @@ __device__ __forceinline__ uint32_t mod_q_i64(long long x, double qd, double qin
 // sets first (consecutive rounds share their groups' bytes in L2) or, tile-major, the groups
 // first: on a persistent grid of 256 blocks, block b's items b + 256 k are then rounds k with one
 // tile for ceil(groups / (256 / t)) rounds in a row, which keep that tile's LUT and E.
+// All of it scalar work without a division (t is a power of two; round / d is a product's high
+// word), so that a block finds its next item in a few instructions.
 __device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, uint32_t &tile, uint32_t &grp) {
-    const uint32_t t = g.sched_t, xcd = item & 7, s = item >> 3;
-    const uint32_t round = s >> 5, within = s & 31, tgroups = (g.ntiles + t - 1) / t;
-    const uint32_t gblocks = (g.groups + 256 / t - 1) / (256 / t);
-    const uint32_t ts = g.tile_major ? round / gblocks : round % tgroups;
-    const uint32_t gb = g.tile_major ? round % gblocks : round / tgroups;
-    tile = t * ts + within % t;
-    grp = (32 / t) * (8 * gb + xcd) + within / t;
+    const uint32_t sh = g.t_shift, xcd = item & 7, s = item >> 3;
+    const uint32_t round = s >> 5, within = s & 31;
+    const uint32_t d = g.tile_major ? g.gblocks : g.tgroups;
+    const uint64_t magic = g.tile_major ? g.gb_magic : g.tg_magic;
+    const uint32_t major = magic ? (uint32_t)(((uint64_t)round * magic) >> 32) : round / d;
+    const uint32_t minor = round - major * d;
+    const uint32_t ts = g.tile_major ? major : minor;  // round / gblocks or round % tgroups
+    const uint32_t gb = g.tile_major ? minor : major;  // round % gblocks or round / tgroups
+    tile = (ts << sh) + (within & ((1u << sh) - 1));
+    grp = ((8 * gb + xcd) << (5 - sh)) + (within >> sh);
     return tile < g.ntiles && grp < g.groups;
 }
 
@@ -228,8 +233,23 @@ struct ItemRegs {
 };
 
 template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep,
-                                          bool ready, const NextItem &nx, ItemRegs &R);
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td, TileDesc &tdu, int slot, uint32_t tile,
+                                          uint32_t grp, uint8_t *smem, bool ready, const NextItem &nx, ItemRegs &R);
+
+// A tile descriptor read by vector loads, as scalar (wave-uniform) values: taken once its loads
+// are complete, so that the item that uses it starts without waiting on a load.
+__device__ __forceinline__ TileDesc uniform_desc(const TileDesc &t) {
+    TileDesc u;
+    u.a_off = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(t.a_off >> 32)) << 32) |
+              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)t.a_off);
+    u.lut_off = __builtin_amdgcn_readfirstlane(t.lut_off);
+    u.ee_off = __builtin_amdgcn_readfirstlane(t.ee_off);
+    u.out_off = __builtin_amdgcn_readfirstlane(t.out_off);
+    u.q = __builtin_amdgcn_readfirstlane(t.q);
+    u.par = __builtin_amdgcn_readfirstlane(t.par);
+    u.y_off = __builtin_amdgcn_readfirstlane(t.y_off);
+    return u;
+}
 
 // s_waitcnt immediates (this family's encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt
 // 11:8): wait until at most n vector-memory operations of this wave are outstanding, nothing else.
@@ -250,6 +270,27 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void *p, ui
                                              BUF_RSRC_WORD3);
 }
 
+// A tile's E into LDS (read only in the epilogue: 82 x 88 x 2 bytes, 902 16-byte words, two a
+// thread) and this thread's output slot (two epilogue threads an output; -1 for none). The
+// loads are issued before `wait` runs (a barrier where the LDS area is still being read), the
+// stores after it.
+static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
+static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
+template <typename Wait>
+__device__ __forceinline__ int load_tile_consts(const GemmArgs &g, const TileDesc &td, uint8_t *smem, Wait wait) {
+    constexpr int EE_V = EE_TILE * 2 / 16;
+    const int tid = threadIdx.x;
+    const uint4 *eesrc = (const uint4 *)(g.ee + td.ee_off);
+    const uint4 ee0 = eesrc[tid];
+    const uint4 ee1 = tid + GEMM_THREADS < EE_V ? eesrc[tid + GEMM_THREADS] : make_uint4(0, 0, 0, 0);
+    const int slot = tid < 2 * GROUP_BATCH * TILE_ROWS ? g.out[td.out_off + (tid >> 1) % TILE_ROWS] : -1;
+    wait();
+    uint4 *ee = (uint4 *)(smem + EE_OFF);
+    ee[tid] = ee0;
+    if (tid + GEMM_THREADS < EE_V) ee[tid + GEMM_THREADS] = ee1;
+    return slot;
+}
+
 // REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
 // half-wave read distinct banks; its set-up (eight 16-byte stores a thread) pays from 8 K steps.
 template <bool REP>
@@ -260,22 +301,32 @@ __global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) 
     // bytes and the first step's operands), so that the next item starts with its first products.
     uint32_t item = blockIdx.x, tile = 0, grp = 0;
     while (item < g.items && !gemm_item_of(g, item, tile, grp)) item += gridDim.x;
-    uint32_t prev = 0xFFFFFFFFu;
     bool ready = false;
     ItemRegs R;
+    // An item's tile descriptor is read during the item before, and E and the output
+    // slots are loaded when the tile changes, between two items: no item starts by waiting on a
+    // load. E is first read in the first item's epilogue, after its K steps' barriers.
+    if (item >= g.items) return;
+    TileDesc td = uniform_desc(g.tiles[tile]);
+    int slot = load_tile_consts(g, td, smem, [] {});
     while (item < g.items) {
         NextItem nx;
         uint32_t nitem = item + gridDim.x;
         nx.tile = nx.grp = 0;
         while (nitem < g.items && !gemm_item_of(g, nitem, nx.tile, nx.grp)) nitem += gridDim.x;
         nx.has = nitem < g.items;
-        gemm_item<REP>(g, tile, grp, smem, tile == prev, ready, nx, R);
-        prev = tile;
+        TileDesc tdu;
+        gemm_item<REP>(g, td, tdu, slot, tile, grp, smem, ready, nx, R);
+        // A new tile's E overwrites this one's once every wave is past its E.Z sums. Otherwise
+        // the next item writes the LDS only through its K steps (A copies into buffers that the
+        // epilogue does not read), and no wave passes its first step barrier before every wave
+        // has finished this epilogue, so this item's Z is safe too.
+        if (nx.has && nx.tile != tile) slot = load_tile_consts(g, tdu, smem, [] { __syncthreads(); });
         tile = nx.tile;
         grp = nx.grp;
         item = nitem;
         ready = nx.has;
-        __syncthreads();  // the next item overwrites the LDS
+        td = tdu;
     }
 }
 
@@ -285,10 +336,8 @@ typedef __attribute__((address_space(3))) const uint32_t lds_u32;
 __device__ __forceinline__ lds_u32 *lds_word(uint32_t a) { return (lds_u32 *)(uintptr_t)(a + LUTR_OFF); }
 
 template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint32_t grp, uint8_t *smem, bool keep,
-                                          bool ready, const NextItem &nx, ItemRegs &R) {
-    const TileDesc td = g.tiles[tile];
-    const TileDesc tdn = g.tiles[nx.has ? nx.tile : tile];  // the next item's, read now, used after the K loop
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td, TileDesc &tdu, int slot, uint32_t tile,
+                                          uint32_t grp, uint8_t *smem, bool ready, const NextItem &nx, ItemRegs &R) {
     v4i(&raw)[2] = R.raw;
     v4i(&a0)[5] = R.a0;
     v4i(&a1)[5] = R.a1;
@@ -315,7 +364,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
             lut[tid] = (uint16_t)lv;
         }
     };
-    if (!ready && !keep) set_lut(td);
+    if (!ready) set_lut(td);
 
     // v_mfma_i32_16x16x64_i8 operands: lane holds row (A) or input (B) c16 = lane & 15 of a 16-block
     // at K group kg = lane >> 4, i.e. K 16 kg .. 16 kg + 15 of the step's 64. A wave owns 32 inputs
@@ -420,19 +469,6 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     // the first step's products start them from zero (an inline constant, no clearing moves)
     v4i acc[5][2][3], xacc[2][2];
 
-    // E and the output slots are loaded now, behind the K loop; E's LDS area is not used before
-    // the epilogue. 82 x 88 x 2 bytes are 902 16-byte words, two per thread.
-    static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
-    static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
-    constexpr int EE_V = EE_TILE * 2 / 16;
-    const uint4 *eesrc = (const uint4 *)(g.ee + td.ee_off);
-    uint4 ee0 = make_uint4(0, 0, 0, 0), ee1 = make_uint4(0, 0, 0, 0);
-    if (!keep) {
-        ee0 = eesrc[tid];
-        if (tid + GEMM_THREADS < EE_V) ee1 = eesrc[tid + GEMM_THREADS];
-    }
-    const int slot = tid < 2 * GROUP_BATCH * TILE_ROWS ? g.out[td.out_off + (tid >> 1) % TILE_ROWS] : -1;
-
     const uint32_t kbs = g.kbn;  // steps of this launch, counted from kb0
     // The two waves of a SIMD (w and w + 4) run each step's halves, the products of step kb and
     // the lookups of step kb + 1, in opposite phase: waves 0-3 meet the step barrier between the
@@ -453,12 +489,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
         ax = *(const v4i *)(ab + xoff);
     };
     // Set-up: the LUT, A(0), waves 4-7's part of A(1), the bytes of step 0, then A(0)'s pieces and
-    // step 0's operands. The block's item before did all of it during its epilogue (ready); E,
-    // read only in the epilogue, is stored here when the tile changes.
-    if (!keep) {
-        ((uint4 *)ee)[tid] = ee0;
-        if (tid + GEMM_THREADS < EE_V) ((uint4 *)ee)[tid + GEMM_THREADS] = ee1;
-    }
+    // step 0's operands. The block's item before did all of it during its epilogue (ready).
     if (!ready) {
         dma_a(0, 0);
         load_raw(0);
@@ -521,16 +552,20 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     };
     step(0, std::true_type{});
     __builtin_amdgcn_sched_barrier(0);
+    // the next item's tile descriptor, read after step 0's barrier: a barrier waits for every load
+    // a wave has issued, and the next one is a step away
+    const TileDesc tdn = g.tiles[nx.has ? nx.tile : tile];
     for (uint32_t kb = 1; kb < kbs; kb++) step(kb, std::false_type{});
     __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
     __syncthreads();  // every wave is past its last read of the A buffers and of the LUT
+    tdu = uniform_desc(tdn);  // its loads are complete
     // The block's next item, set up now (the epilogue uses neither the LUT, the A buffers nor the
     // operand registers): its LUT when the tile changes, A(0) into buffer 0, waves 4-7's part of
     // A(1) into buffer 1, the bytes of step 0; after the epilogue's barrier, A(0)'s pieces and
     // step 0's operands.
     if (nx.has) {
-        if (nx.tile != tile) set_lut(tdn);
-        const __amdgpu_buffer_rsrc_t nars = uniform_rsrc(g.afrag + tdn.a_off, g.kbs * g.a_kb_stride);
+        if (nx.tile != tile) set_lut(tdu);
+        const __amdgpu_buffer_rsrc_t nars = uniform_rsrc(g.afrag + tdu.a_off, g.kbs * g.a_kb_stride);
         dma_a_from(nars, 0, 0);
         if (wgrp) dma_a_from(nars, kbs > 1 ? 1 : 0, 1);
         load_raw_from(bytes_rsrc(nx.grp), 0);
@@ -564,7 +599,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, uint32_t tile, uint
     __syncthreads();  // Z, and the next item's LUT and A copies
     if (nx.has) {
         read_a0(abuf);
-        build_with(sel_of(tdn));
+        build_with(sel_of(tdu));
     }
     // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (n 0..39 and
     // 40..87; E is zero from n = 81), reading four pairs of n at a time (16 bytes), each pair one
@@ -2200,6 +2235,14 @@ hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persisten
     GemmArgs a = g;
     if (a.sched_t == 0 || a.sched_t > 32 || (a.sched_t & (a.sched_t - 1))) a.sched_t = 8;
     a.items = gemm_items(a.ntiles, a.groups, a.sched_t);
+    a.t_shift = (uint32_t)__builtin_ctz(a.sched_t);
+    a.tgroups = (a.ntiles + a.sched_t - 1) / a.sched_t;
+    a.gblocks = (a.groups + 256 / a.sched_t - 1) / (256 / a.sched_t);
+    // n / d = (n * ceil(2^32 / d)) >> 32 exactly while n * d < 2^32 (n < the rounds, tgroups x gblocks)
+    const uint64_t rounds = (uint64_t)a.tgroups * a.gblocks;
+    auto magic = [&](uint32_t d) { return rounds * d < (1ull << 32) ? ((1ull << 32) + d - 1) / d : 0ull; };
+    a.tg_magic = magic(a.tgroups);
+    a.gb_magic = magic(a.gblocks);
     uint32_t grid = a.items;
     if (persistent_blocks && persistent_blocks < grid) grid = (persistent_blocks + 7) / 8 * 8;
     // K in chunks the int32 accumulators hold exactly; the later chunks add mod q
