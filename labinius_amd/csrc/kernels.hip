@@ -190,6 +190,19 @@ __device__ __forceinline__ int mod_q_classes(int ll, int mid, int hh, double qd,
     return (int)__builtin_fma(-t, qd, z);
 }
 
+// The same for non-negative classes of a short K: with 7-bit digits, ll + 2^7 mid is at most
+// K x 127^2 x 257 < 2^32 for K <= KB_U32 steps of 64, so it is summed exactly as a u32 first
+// (one conversion and one f64 fma fewer, which is what bounds the epilogue: every f64
+// instruction and conversion issues at half the rate of a 32-bit one).
+constexpr uint32_t KB_U32 = 4294967295u / (KSTEP * 127u * 127u * 257u);  // 16 steps, K = 1024
+static_assert(KB_U32 >= 16, "the fold's 1024 columns take the short form");
+__device__ __forceinline__ int mod_q_short(int ll, int mid, int hh, double qd, double qinv) {
+    const uint32_t w = (uint32_t)ll + ((uint32_t)mid << 7);
+    const double z = __builtin_fma(16384.0, (double)hh, (double)w);
+    const double t = __builtin_rint(z * qinv);
+    return (int)__builtin_fma(-t, qd, z);
+}
+
 // x mod q in [0, q) for |x| < 2^52, the same way
 __device__ __forceinline__ uint32_t mod_q_i64(long long x, double qd, double qinv) {
     const double z = (double)x;
@@ -577,24 +590,30 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
     // digit products: with B's low digit [l80, h80, l81, h81] x l, with its high digit x h.
     const uint32_t q = td.q;
     const double qd = (double)q, qinv = 1.0 / qd;
+    auto epilogue1 = [&](auto reduce) {
 #pragma unroll
-    for (int nb = 0; nb < 2; nb++) {
-        const int j = jl + 16 * nb;
-        if (j >= GROUP_BATCH * TILE_USED) continue;
-        uint16_t *zj = zp + (j / TILE_USED) * TILE_ROWS * EP + j % TILE_USED;
+        for (int nb = 0; nb < 2; nb++) {
+            const int j = jl + 16 * nb;
+            if (j >= GROUP_BATCH * TILE_USED) continue;
+            uint16_t *zj = zp + (j / TILE_USED) * TILE_ROWS * EP + j % TILE_USED;
 #pragma unroll
-        for (int mb = 0; mb < 5; mb++)
+            for (int mb = 0; mb < 5; mb++)
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                const int m = 16 * mb + 4 * kg + rr;
-                zj[m * EP] = (uint16_t)mod_q_classes(acc[mb][nb][0][rr], acc[mb][nb][1][rr], acc[mb][nb][2][rr], qd, qinv);
+                for (int rr = 0; rr < 4; rr++) {
+                    const int m = 16 * mb + 4 * kg + rr;
+                    zj[m * EP] = (uint16_t)reduce(acc[mb][nb][0][rr], acc[mb][nb][1][rr], acc[mb][nb][2][rr]);
+                }
+            if (kg == 0) {
+                const v4i xl = xacc[nb][0], xh = xacc[nb][1];
+                zj[80 * EP] = (uint16_t)reduce(xl[0], xh[0] + xl[1], xh[1]);
+                zj[81 * EP] = (uint16_t)reduce(xl[2], xh[2] + xl[3], xh[3]);
             }
-        if (kg == 0) {
-            const v4i xl = xacc[nb][0], xh = xacc[nb][1];
-            zj[80 * EP] = (uint16_t)mod_q_classes(xl[0], xh[0] + xl[1], xh[1], qd, qinv);
-            zj[81 * EP] = (uint16_t)mod_q_classes(xl[2], xh[2] + xl[3], xh[3], qd, qinv);
         }
-    }
+    };
+    if (q < UNSIGNED_DIGITS_Q && kbs <= KB_U32)  // block-uniform
+        epilogue1([&](int ll, int mid, int hh) { return mod_q_short(ll, mid, hh, qd, qinv); });
+    else
+        epilogue1([&](int ll, int mid, int hh) { return mod_q_classes(ll, mid, hh, qd, qinv); });
     __builtin_amdgcn_s_waitcnt(0);  // this thread's copies for the next item are complete
     __syncthreads();  // Z, and the next item's LUT and A copies
     if (nx.has) {
