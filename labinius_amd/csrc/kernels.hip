@@ -190,17 +190,30 @@ __device__ __forceinline__ int mod_q_classes(int ll, int mid, int hh, double qd,
     return (int)__builtin_fma(-t, qd, z);
 }
 
-// The same for non-negative classes of a short K: with 7-bit digits, ll + 2^7 mid is at most
-// K x 127^2 x 257 < 2^32 for K <= KB_U32 steps of 64, so it is summed exactly as a u32 first
-// (one conversion and one f64 fma fewer, which is what bounds the epilogue: every f64
-// instruction and conversion issues at half the rate of a 32-bit one).
+// The same for non-negative classes (the digits of q < UNSIGNED_DIGITS_Q), offset by
+// M = 1.5 x 2^52: the double whose bits are M's with a u32 x in the low word is M + x, and every
+// sum below stays an exact integer in [2^52, 2^53), where the unit is 1. So D = M + z costs one
+// or two conversions instead of three, and with k0 = M / q rounded to a double, fma(D, 1/q, -k0)
+// is z / q within q ulp(k0) / 2 <= 3/4 of a unit of q: the remainder M + r = fma(-t, q, D) is
+// exact with |r| <= (q + 1) / 2, and r is the low word of its bits (no conversion back). Every f64
+// instruction and conversion issues at half the rate of a 32-bit add. For K <= KB_U32 steps of
+// 64, ll + 2^7 mid <= K x 127^2 x 257 < 2^32 is one u32 sum.
 constexpr uint32_t KB_U32 = 4294967295u / (KSTEP * 127u * 127u * 257u);  // 16 steps, K = 1024
 static_assert(KB_U32 >= 16, "the fold's 1024 columns take the short form");
-__device__ __forceinline__ int mod_q_short(int ll, int mid, int hh, double qd, double qinv) {
-    const uint32_t w = (uint32_t)ll + ((uint32_t)mid << 7);
-    const double z = __builtin_fma(16384.0, (double)hh, (double)w);
-    const double t = __builtin_rint(z * qinv);
-    return (int)__builtin_fma(-t, qd, z);
+constexpr uint64_t M_BITS = 0x4338000000000000ull;  // 1.5 x 2^52
+constexpr double M_MAGIC = 6755399441055744.0;
+__device__ __forceinline__ double m_plus(uint32_t x) { return __builtin_bit_cast(double, M_BITS | x); }
+__device__ __forceinline__ int mod_q_offset(double D, double qd, double qinv, double k0) {
+    const double t = __builtin_rint(__builtin_fma(D, qinv, -k0));
+    return (int)(uint32_t)__builtin_bit_cast(uint64_t, __builtin_fma(-t, qd, D));
+}
+__device__ __forceinline__ int mod_q_short(int ll, int mid, int hh, double qd, double qinv, double k0) {
+    const double D = m_plus((uint32_t)ll + ((uint32_t)mid << 7));
+    return mod_q_offset(__builtin_fma(16384.0, (double)(uint32_t)hh, D), qd, qinv, k0);
+}
+__device__ __forceinline__ int mod_q_unsigned(int ll, int mid, int hh, double qd, double qinv, double k0) {
+    const double D = __builtin_fma(128.0, (double)(uint32_t)mid, m_plus((uint32_t)ll));
+    return mod_q_offset(__builtin_fma(16384.0, (double)(uint32_t)hh, D), qd, qinv, k0);
 }
 
 // x mod q in [0, q) for |x| < 2^52, the same way
@@ -610,8 +623,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
             }
         }
     };
+    const double k0 = M_MAGIC * qinv;
     if (q < UNSIGNED_DIGITS_Q && kbs <= KB_U32)  // block-uniform
-        epilogue1([&](int ll, int mid, int hh) { return mod_q_short(ll, mid, hh, qd, qinv); });
+        epilogue1([&](int ll, int mid, int hh) { return mod_q_short(ll, mid, hh, qd, qinv, k0); });
+    else if (q < UNSIGNED_DIGITS_Q)
+        epilogue1([&](int ll, int mid, int hh) { return mod_q_unsigned(ll, mid, hh, qd, qinv, k0); });
     else
         epilogue1([&](int ll, int mid, int hh) { return mod_q_classes(ll, mid, hh, qd, qinv); });
     __builtin_amdgcn_s_waitcnt(0);  // this thread's copies for the next item are complete
