@@ -272,6 +272,14 @@ struct lbn_prover {
     uint64_t *h_pts = nullptr;       // pinned staging of p0 and p1 (one upload)
     lbn_challenge *h_ch = nullptr;   // pinned staging of the folding challenges
     uint64_t *d_wscratch = nullptr;  // host-witness uploads
+    // uploads from pageable host memory, one share a thread: its stream, two pinned chunk buffers
+    // and the events of their copies (made at the first large upload)
+    struct Upload {
+        hipStream_t st = nullptr;
+        uint8_t *buf[2] = {};
+        hipEvent_t ev[2] = {};
+    };
+    std::vector<Upload> up;
     // the bit-dropped digits (grow-only): raw rows from the host, centered coefficients, top + digits
     uint32_t *d_bd_raw = nullptr;
     int16_t *d_bd_cen = nullptr;

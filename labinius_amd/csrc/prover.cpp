@@ -5,6 +5,7 @@
 
 #include "internal.hpp"
 #include "launch.hpp"
+#include "threads.hpp"
 
 namespace lbn {
 
@@ -360,6 +361,75 @@ static int build_device(lbn_pp *pp) {
 // ---------------------------------------------------------------------------------------------
 
 
+// ---------------------------------------------------------------------------------------------
+// uploads from pageable host memory
+// ---------------------------------------------------------------------------------------------
+// A large upload (a witness at 2^24 is 403 MB) is split over UP_SHARES threads, each copying its
+// chunks into two pinned buffers in turn and sending them on its own stream: the copies to
+// pinned memory run on several cores and the DMAs overlap them, where one pageable hipMemcpy
+// reaches about 30 GB/s. Smaller uploads take the plain copy.
+constexpr size_t UP_CHUNK = size_t(8) << 20, UP_MIN = size_t(32) << 20, UP_SHARES = 4;
+
+static void upload_free(std::vector<lbn_prover::Upload> &up) {
+    for (auto &u : up) {
+        for (int b = 0; b < 2; b++) {
+            if (u.ev[b]) (void)hipEventDestroy(u.ev[b]);
+            if (u.buf[b]) (void)hipHostFree(u.buf[b]);
+        }
+        if (u.st) (void)hipStreamDestroy(u.st);
+    }
+    up.clear();
+}
+
+static bool upload_staging(lbn_prover *pr) {
+    if (!pr->up.empty()) return true;
+    std::vector<lbn_prover::Upload> up(UP_SHARES);
+    bool ok = true;
+    for (auto &u : up) {
+        ok = ok && hipStreamCreate(&u.st) == hipSuccess;
+        for (int b = 0; b < 2 && ok; b++)
+            ok = hipHostMalloc((void **)&u.buf[b], UP_CHUNK, hipHostMallocDefault) == hipSuccess &&
+                 hipEventCreateWithFlags(&u.ev[b], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+        upload_free(up);
+        (void)hipGetLastError();  // the failed allocation is not reported by the next call
+        return false;
+    }
+    pr->up = std::move(up);
+    return true;
+}
+
+// dst (device) <- src (pageable host memory), complete when this returns
+static int upload_host(lbn_prover *pr, void *dst, const void *src, size_t bytes) {
+    int dev = 0;
+    if (bytes < UP_MIN || !upload_staging(pr) || hipGetDevice(&dev) != hipSuccess)
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? LBN_OK
+                                                                              : fail(LBN_ERR_HIP, "witness upload");
+    const size_t chunks = (bytes + UP_CHUNK - 1) / UP_CHUNK;
+    std::atomic<bool> ok{true};
+    const bool ran = run_shares(UP_SHARES, [&](size_t k) {
+        lbn_prover::Upload &u = pr->up[k];
+        if (hipSetDevice(dev) != hipSuccess) {
+            ok = false;
+            return;
+        }
+        const size_t c0 = chunks * k / UP_SHARES, c1 = chunks * (k + 1) / UP_SHARES;
+        for (size_t c = c0; c < c1 && ok; c++) {
+            const int b = (int)((c - c0) & 1);
+            if (c - c0 >= 2 && hipEventSynchronize(u.ev[b]) != hipSuccess) ok = false;  // the buffer's last DMA
+            const size_t off = c * UP_CHUNK, n = std::min(UP_CHUNK, bytes - off);
+            if (!ok) break;
+            std::memcpy(u.buf[b], (const uint8_t *)src + off, n);
+            if (hipMemcpyAsync((uint8_t *)dst + off, u.buf[b], n, hipMemcpyHostToDevice, u.st) != hipSuccess ||
+                hipEventRecord(u.ev[b], u.st) != hipSuccess)
+                ok = false;
+        }
+        if (hipStreamSynchronize(u.st) != hipSuccess) ok = false;
+    });
+    return ran && ok ? LBN_OK : fail(LBN_ERR_HIP, "witness upload");
+}
+
 static void prover_free(lbn_prover *pr) {
     void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
                     pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_eqf, pr->d_u, pr->d_wscratch,
@@ -376,6 +446,7 @@ static void prover_free(lbn_prover *pr) {
     verify_free(pr->vfy);
     for (hipEvent_t &e : pr->ev)
         if (e) (void)hipEventDestroy(e);
+    upload_free(pr->up);
     if (pr->stream) (void)hipStreamDestroy(pr->stream);
     if (pr->pp && --pr->pp->refs == 0) pp_free(pr->pp);
     delete pr;
@@ -650,9 +721,10 @@ int witness_alloc_upload(lbn_prover *pr, const lbn_f162 *w, size_t n, size_t col
         delete d;
         return fail(LBN_ERR_OOM, "witness allocation");
     }
-    if (hipMemcpy(d->d_w, w, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    const int rc = upload_host(pr, d->d_w, w, bytes);
+    if (rc) {
         lbn_dwitness_release(d);
-        return fail(LBN_ERR_HIP, "witness upload");
+        return rc;
     }
     *out = d;
     return LBN_OK;
@@ -845,7 +917,9 @@ int lbn_commit(lbn_prover *pr, const lbn_f162 *w, size_t n, int16_t *comp, uint3
         if (hipMalloc((void **)&pr->d_wscratch, n * sizeof(lbn_f162)) != hipSuccess)
             return fail(LBN_ERR_OOM, "witness scratch");
     }
-    HIP_TRY(hipMemcpyAsync(pr->d_wscratch, w, n * sizeof(lbn_f162), hipMemcpyHostToDevice, pr->stream));
+    HIP_TRY(hipStreamSynchronize(pr->stream));  // nothing of an earlier call still reads the scratch
+    const int rc = upload_host(pr, pr->d_wscratch, w, n * sizeof(lbn_f162));
+    if (rc) return rc;
     if (opening) *opening = nullptr;
     return commit_device(pr, pr->d_wscratch, comp, raw, opening);
 }
