@@ -280,6 +280,8 @@ struct lbn_prover {
         hipEvent_t ev[2] = {};
     };
     std::vector<Upload> up;
+    uint8_t *h_out = nullptr;  // pinned staging of the outputs copied back to host memory (grown on demand)
+    size_t h_out_cap = 0;
     // the bit-dropped digits (grow-only): raw rows from the host, centered coefficients, top + digits
     uint32_t *d_bd_raw = nullptr;
     int16_t *d_bd_cen = nullptr;
