@@ -506,13 +506,18 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
     // land while the products run; the compiler waits for them before the next LDS access.
     // The next step's A pieces (both digits, and the sixth block's) are read ahead, at the end of
     // the step before.
-    auto read_a0 = [&](const uint8_t *ab) {
+    // ab is the staged step's LDS byte address (block-uniform, the kernel's LDS starting at 0):
+    // a lane's two base addresses cost one add each, the pieces' offsets are immediates
+    auto read_a0 = [&](uint32_t ab) {
+        typedef __attribute__((address_space(3))) const uint8_t lds_u8;
+        ab = __builtin_amdgcn_readfirstlane(ab);  // computed by the scalar unit
+        lds_u8 *pa = (lds_u8 *)(uintptr_t)(ab + (uint32_t)aoff);
 #pragma unroll
-        for (int mb = 0; mb < 5; mb++) a0[mb] = *(const v4i *)(ab + aoff + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+        for (int mb = 0; mb < 5; mb++) a0[mb] = *(const __attribute__((address_space(3))) v4i *)(pa + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
 #pragma unroll
         for (int mb = 0; mb < 5; mb++)
-            a1[mb] = *(const v4i *)(ab + aoff + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
-        ax = *(const v4i *)(ab + xoff);
+            a1[mb] = *(const __attribute__((address_space(3))) v4i *)(pa + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
+        ax = *(const __attribute__((address_space(3))) v4i *)((lds_u8 *)(uintptr_t)(ab + (uint32_t)xoff));
     };
     // Set-up: the LUT, A(0), waves 4-7's part of A(1), the bytes of step 0, then A(0)'s pieces and
     // step 0's operands. The block's item before did all of it during its epilogue (ready).
@@ -522,7 +527,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
         if (wgrp) dma_a(kbs > 1 ? 1 : 0, 1);
         __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
         __syncthreads();                // the LUT, A(0) and waves 4-7's part of A(1)
-        read_a0(abuf);
+        read_a0(ABUF_OFF);
         build();
     }
     // step kb; FIRST: the step that starts the accumulators
@@ -574,7 +579,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
             __syncthreads();
         }
         build();                                 // step kb + 1 (its bytes were loaded before the products)
-        read_a0(abuf + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
+        read_a0(ABUF_OFF + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
     };
     step(0, std::true_type{});
     __builtin_amdgcn_sched_barrier(0);
@@ -633,7 +638,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
     __builtin_amdgcn_s_waitcnt(0);  // this thread's copies for the next item are complete
     __syncthreads();  // Z, and the next item's LUT and A copies
     if (nx.has) {
-        read_a0(abuf);
+        read_a0(ABUF_OFF);
         build_with(sel_of(tdu));
     }
     // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (n 0..39 and
