@@ -189,6 +189,7 @@ struct lbn_batch_ws {
 struct lbn_switch_ws {
     size_t cap_n = 0;
     uint64_t *d_eq = nullptr, *d_a = nullptr, *d_p = nullptr, *d_small = nullptr;
+    uint64_t *h_small = nullptr;  // pinned staging of the host's side of d_small (the round trips)
 };
 
 // The wire encoder's device scratch (grow-only, wire.cpp).

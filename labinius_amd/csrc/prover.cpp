@@ -488,6 +488,7 @@ static void prover_free(lbn_prover *pr) {
     if (pr->h_pts) (void)hipHostFree(pr->h_pts);
     if (pr->h_ch) (void)hipHostFree(pr->h_ch);
     if (pr->h_out) (void)hipHostFree(pr->h_out);
+    if (pr->sw.h_small) (void)hipHostFree(pr->sw.h_small);
     for (uint8_t *b : pr->free_bytes_f) (void)hipFree(b);  // a pooled prover keeps none
     batch_free(pr->batch);
     wire_free(pr->wire);

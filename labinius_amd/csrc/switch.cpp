@@ -135,6 +135,9 @@ int ilog2(size_t n) {
 }
 
 // The prover's switch workspace (grow-only).
+// words of d_small (and of its pinned host side): parts, psi, the partials, the round messages
+constexpr size_t SW_SMALL = (size_t)4 * 256 * 2 + 16 * 256 * 3 + 256 + 6 * 32;
+
 int sw_reserve(lbn_prover *pr, size_t n, size_t np) {
     lbn_switch_ws &w = pr->sw;
     int rc;
@@ -151,7 +154,11 @@ int sw_reserve(lbn_prover *pr, size_t n, size_t np) {
     }
     if (!w.d_small) {
         // parts (up to 4 x 256 B128), psi (16 x 256 F162), v (128 B128), one message per round
-        if ((rc = dalloc(&w.d_small, (size_t)4 * 256 * 2 + 16 * 256 * 3 + 256 + 6 * 32))) return rc;
+        if ((rc = dalloc(&w.d_small, SW_SMALL))) return rc;
+    }
+    if (!w.h_small && hipHostMalloc((void **)&w.h_small, SW_SMALL * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+        w.h_small = nullptr;
+        return fail(LBN_ERR_OOM, "the switch's pinned staging");
     }
     (void)np;
     return LBN_OK;
@@ -180,6 +187,10 @@ int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *wt, const uint64_t *r_l
     if ((rc = sw_reserve(pr, n, np))) return rc;
     lbn_switch_ws &ws = pr->sw;
     uint64_t *d_parts = ws.d_small, *d_psi = d_parts + 4 * 256 * 2, *d_v = d_psi + 16 * 256 * 3, *d_msg = d_v + 256;
+    // the same layout in pinned host memory: every copy is asynchronous and every wait a poll of the
+    // stream (a copy to or from pageable memory goes through the runtime's bounce buffer, and the
+    // sumcheck makes one round trip a round)
+    uint64_t *h_parts = ws.h_small, *h_psi = h_parts + 4 * 256 * 2, *h_v = h_psi + 16 * 256 * 3, *h_msg = h_v + 256;
     hipStream_t st = pr->stream;
     Transcript &t = tr->t;
     const B128 *rl = (const B128 *)r_lo, *rh = (const B128 *)r_hi;
@@ -187,7 +198,8 @@ int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *wt, const uint64_t *r_l
     statement(t, rl, r_lo_len, rh, l, cb);
 
     // eq(r_hi): bit b of j takes coordinate l - 1 - b; part c = bits 8c .. 8c + 7
-    std::vector<uint64_t> parts(np * 256 * 2, 0);
+    uint64_t *parts = h_parts;
+    std::memset(parts, 0, np * 256 * 2 * sizeof(uint64_t));
     for (size_t c = 0; c < np; c++) {
         const int nb = (int)std::min<size_t>(8, l - 8 * c);
         std::vector<B128> rc8(nb);
@@ -198,25 +210,27 @@ int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *wt, const uint64_t *r_l
             parts[2 * (c * 256 + m) + 1] = e[m].w[1];
         }
     }
-    HIP_TRY(hipMemcpyAsync(d_parts, parts.data(), parts.size() * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_parts, parts, np * 256 * 2 * 8, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_switch_eq(st, d_parts, (uint32_t)np, (uint32_t)n, ws.d_eq));
     // the partial evaluations
     HIP_TRY(launch_fill_u32(st, (uint32_t *)d_v, 2 * 256, 0));
     HIP_TRY(launch_switch_partials(st, wt->d_w, ws.d_eq, (uint32_t)n, d_v));
-    HIP_TRY(hipMemcpyAsync(partials, d_v, 128 * 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpyAsync(h_v, d_v, 128 * 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(stream_wait(st));
+    std::memcpy(partials, h_v, 128 * 16);
     absorb_b128(t, (const B128 *)partials, 128);
     const std::vector<F162> r7 = sample_f162(t, "labinius/switch-batch", 7);
     const std::vector<F162> batch = eq_f162_msb(r7.data(), 7);
     // psi[c][m]: sum over the bits b of m of batch[8c + b] (crossfield.rs:117-131)
-    std::vector<uint64_t> psi(16 * 256 * 3, 0);
+    uint64_t *psi = h_psi;
+    std::memset(psi, 0, 16 * 256 * 3 * sizeof(uint64_t));
     for (int c = 0; c < 16; c++)
         for (int m = 1; m < 256; m++) {
             const int b = __builtin_ctz(m);
             for (int k = 0; k < 3; k++)
                 psi[3 * (c * 256 + m) + k] = psi[3 * (c * 256 + (m ^ (1 << b))) + k] ^ batch[8 * c + b].w[k];
         }
-    HIP_TRY(hipMemcpyAsync(d_psi, psi.data(), psi.size() * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_psi, psi, 16 * 256 * 3 * 8, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_switch_psi(st, ws.d_eq, (uint32_t)n, d_psi, ws.d_a));
     // the sumcheck: a against the lifted trace (its third word read as zero), the folded trace in
     // d_p. Round i's fold runs fused with round i + 1's message; the last fold (two elements) is
@@ -226,9 +240,9 @@ int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *wt, const uint64_t *r_l
     HIP_TRY(launch_switch_msg(st, ws.d_a, wt->d_w, (uint32_t)(n / 2), 0, d_msg));
     size_t half = n / 2;
     for (size_t round = 0; round < l; round++) {
-        uint64_t msg[6];
-        HIP_TRY(hipMemcpyAsync(msg, d_msg + 6 * round, 48, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
+        const uint64_t *msg = h_msg;
+        HIP_TRY(hipMemcpyAsync(h_msg, d_msg + 6 * round, 48, hipMemcpyDeviceToHost, st));
+        HIP_TRY(stream_wait(st));
         std::memcpy(rounds[2 * round].w, msg, 24);
         std::memcpy(rounds[2 * round + 1].w, msg + 3, 24);
         absorb_f162(t, (const F162 *)msg, 2);
@@ -241,8 +255,9 @@ int lbn_switch_prove(lbn_prover *pr, const lbn_dwitness *wt, const uint64_t *r_l
     }
     // the last fold: p[0] + r (p[0] + p[1])
     F162 pl[2];
-    HIP_TRY(hipMemcpyAsync(pl, ws.d_p, 48, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpyAsync(h_msg, ws.d_p, 48, hipMemcpyDeviceToHost, st));
+    HIP_TRY(stream_wait(st));
+    std::memcpy(pl, h_msg, 48);
     const F162 op = f162_add(pl[0], f162_mul(r_pp[l - 1], f162_add(pl[0], pl[1])));
     std::memcpy(opened->w, op.w, 24);
     msb_first(r_pp, cl, p0, p1);
