@@ -73,3 +73,10 @@ def test_suite_xl_round_bit_exact(lbn, orc):
     verifier.verify_evaluation(point, claimed, row)
     verifier.verify_opening(commitment, challenges, point, fc, folded, fr)
     say("verified")
+    # Prover::commit(&Witness): the host witness uploaded through the prover's staged copies (four
+    # threads and streams, pinned chunks) and the outputs copied back through its pinned buffer
+    c_host, op_host = prover.commit(w)
+    assert np.array_equal(c_host.raw, commitment.raw)
+    assert np.array_equal(c_host.components, commitment.components)
+    prover.recycle(op_host)
+    say("host-witness commit equal")
