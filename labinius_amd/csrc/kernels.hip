@@ -165,10 +165,10 @@ constexpr int A_LDS = A_STEP;
 static_assert(A_STEP == 768 * 16 && GEMM_THREADS == 512, "two copies a thread of waves 0-3, one of waves 4-7");
 // LDS (the kernel's only LDS is the dynamic block, at address 0):
 //   [0, 64 K) replicated LUT | plain LUT (1 KB) | three A buffers | E | Z
-// E is loaded at item start and read in the epilogue. Z has its own region, so that the next
-// item's first A step can be copied into buffer 0 while the epilogue runs. A block whose next
-// item has the same tile (a persistent grid over one tile's groups) keeps the LUT and E from the
-// item before.
+// E is loaded when a block's tile changes and read in the epilogue. Z has its own region, so that
+// the next item's first A step can be copied into buffer 0 while the epilogue runs. A block whose
+// next item has the same tile (a persistent grid over one tile's groups) keeps the LUT and E from
+// the item before.
 // The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
 // address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
 constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
