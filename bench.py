@@ -41,14 +41,23 @@ def tile_ops(params):
     return 2 * 4 * per_ring * (params.witness_len() // 4)
 
 
+def kpad(length):
+    """a K index length padded to a power of two of at least one step of 64"""
+    p = 64
+    while p < length:
+        p *= 2
+    return p
+
+
 def executed_ops(params):
-    """int8 MFMA ops the commit GEMM executes, padding included: per (tile, group of 3 columns) and
-    K step of 64 ring elements, 8 waves x 44 v_mfma_i32_16x16x64_i8 of 2 x 16 x 16 x 64 ops (per
-    wave and 16-input block: five 16-row blocks x 4 digit products for rows 0..79, and 2 for a
-    block holding rows 80 and 81's digits; a group is 256 inputs of which 243)."""
+    """int8 MFMA ops the commit GEMM executes, padding included: per (tile, group of 256 columns)
+    and K step of 64 (sub-ring coefficient, ring element) pairs, 81 x kpad(L) / 64 steps, 8 waves x
+    44 v_mfma_i32_16x16x64_i8 of 2 x 16 x 16 x 64 ops (per wave and 16-column block: five 16-row
+    blocks x 4 digit products for rows 0..79, and 2 for a block holding rows 80 and 81's
+    digits)."""
     ntiles = sum(16 if (q - 1) % 972 == 0 and (q - 1) % 1944 != 0 else 8 for q in params.primes())
-    groups = -(-params.columns() // 3)
-    kbs = -(-params.len_ring() // 64)
+    groups = -(-params.columns() // 256)
+    kbs = 81 * kpad(params.len_ring()) // 64
     return ntiles * groups * kbs * 8 * 44 * 2 * 16 * 16 * 64
 
 
@@ -57,16 +66,18 @@ def stage_rooflines(params, stage, ms):
     MFMA at 5 POP/s), the achieved rate and the fraction; then the whole path's fraction, the sum
     of the stages' floors over the measured step. Bytes are what each kernel must move at least:
     W and the two byte layouts for the layout pass, yacc in and raw + components out for the
-    finish, the challenges' fragments, v-hat in and v out, eq, and W again for the row values.
+    finish, the challenges' fragments and the fold's A', v-hat in and v out, eq, and W again for
+    the row values.
     The smallest kernels (challenges, inverse NTT, eq) are bound by their latency, far from either
     roofline; they are listed for completeness."""
     n, r, L, k = params.witness_len(), params.columns(), params.len_ring(), len(params.primes())
-    layout = lambda batches, kdim: -(-batches // 3) * 8 * -(-kdim // 64) * 2 * 1024
+    layout = lambda batches, kdim: -(-batches // 256) * 8 * (81 * kpad(kdim) // 64) * 2 * 1024
     work = {
         "commit_layout": ("hbm", 24 * n + layout(r, L) + layout(L, r)),
         "commit_gemm": ("mfma", tile_ops(params)),
         "commit_finish": ("hbm", r * 2 * k * 648 * 4 + k * r * 648 * 4 + r * 4 * k * 162 * 2),
-        "fold_challenges": ("hbm", 8 * -(-r // 64) * 12288),
+        # the challenges' transform written and read, and the fold's A' (E folded in) written
+        "fold_challenges": ("hbm", 8 * (2 * -(-r // 64) + 81 * kpad(r) // 64) * 12288),
         "fold_gemm": ("mfma", 8 * 81 * 81 * 4 * 2 * L * r),
         "fold_intt": ("hbm", L * 648 * 4 + L * 648 * 2),
         "eval_eq": ("hbm", (n // r) * 24),
@@ -319,7 +330,7 @@ def batched(lbn, reps):
             "one_at_a_time_p99_ms": round(float(np.percentile(lat, 99)), 3),
             "speedup": round(one_ms / batch_ms, 3),
             "autotuned": {"tiles_per_xcd_round": info["tiles_per_xcd_round"],
-                          "persistent_blocks": info["persistent_blocks"], "tile_major": info["tile_major"]},
+                          "k_ranges": info["k_ranges"]},
             "graph": "one hipGraph launch per batch (layout, GEMM, finish)"}
 
 

@@ -185,15 +185,15 @@ int lbn_prover_timings(const lbn_prover *prover, float *ms, size_t n);
  * is uploaded with lbn_witness_upload_params; lbn_commit_batch commits `count` of them in one
  * launch sequence (layout, GEMM, finish; captured once as a HIP graph and replayed when the same
  * total width comes twice in a row): components[i] ([cols_i][4][limb][162]) and raw[i]
- * ([limb][cols_i][648]) may be NULL. The GEMM's block schedule is autotuned per power-of-two class
- * of the total column count. */
+ * ([limb][cols_i][648]) may be NULL. The GEMM's schedule is autotuned per power-of-two class of
+ * the total column count. */
 int lbn_witness_upload_params(lbn_prover *prover, const lbn_params *params, const lbn_f162 *w, size_t n,
                               lbn_dwitness **out);
 int lbn_commit_batch(lbn_prover *prover, const lbn_dwitness *const *ws, size_t count, int16_t *const *components,
                      uint32_t *const *raw);
-/* The last batch's schedule (tiles per XCD round; persistent blocks or 0, with bit 31 set when its
- * rounds run tile-major) and its device ms. */
-int lbn_batch_info(const lbn_prover *prover, uint32_t *sched_t, uint32_t *persistent_blocks, float *ms);
+/* The last batch's schedule (tiles per XCD round; the GEMM's K ranges, whose results are added)
+ * and its device ms. */
+int lbn_batch_info(const lbn_prover *prover, uint32_t *sched_t, uint32_t *k_ranges, float *ms);
 
 /* The serving loop (no counterpart in the reference, whose callers run one round at a time): a
  * server commits, evaluates and folds requests of many sizes that share one key (the same

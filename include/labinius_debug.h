@@ -24,9 +24,9 @@ int lbn_debug_tables(const lbn_params *params, const uint8_t seed[32], int which
  * uses: a is 16 x 64 (row-major [m][k]), b is 64 x 16 ([k][n]), d receives the 16 x 16 int32 product. */
 int lbn_debug_mfma_probe(const int8_t *a, const int8_t *b, int32_t *d);
 
-/* The tile GEMM's K chunk in steps of 64 (0: the default, the largest the int32 accumulators hold
- * exactly). Tests set a small chunk to run the multi-launch K path on small shapes. Process-wide;
- * not for concurrent use with running provers. */
+/* The tile GEMMs' longest K range in steps of 64 (0: the default, the largest the int32
+ * accumulators hold exactly). Tests set a short one to run many K ranges (added into the output)
+ * on small shapes. Process-wide; not for concurrent use with running provers. */
 int lbn_debug_gemm_chunk(uint32_t kb);
 
 /* Measurement, no counterpart in the reference: the device's int8 MFMA rate in TOP/s

@@ -544,10 +544,9 @@ class Prover:
         return out
 
     def batch_info(self):
-        t, pb, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
-        _check(lib().lbn_batch_info(ctypes.c_void_p(self._h), ctypes.byref(t), ctypes.byref(pb), ctypes.byref(ms)))
-        return {"tiles_per_xcd_round": t.value, "persistent_blocks": pb.value & 0x7FFFFFFF,
-                "tile_major": bool(pb.value >> 31), "ms": ms.value}
+        t, sp, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
+        _check(lib().lbn_batch_info(ctypes.c_void_p(self._h), ctypes.byref(t), ctypes.byref(sp), ctypes.byref(ms)))
+        return {"tiles_per_xcd_round": t.value, "k_ranges": sp.value, "ms": ms.value}
 
     def _buffers(self):
         p = self.params

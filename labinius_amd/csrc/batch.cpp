@@ -5,8 +5,8 @@
 // a batch is one commitment over the union of their columns: the layout pass reads column c
 // through a table of column addresses, and the GEMM, the finish and the outputs run over all
 // columns at once. The sequence (layout, GEMM, finish) is captured once per batch width in
-// a HIP graph and replayed; the GEMM's block schedule is chosen per width by timing the
-// candidates on the device the first time that width is seen.
+// a HIP graph and replayed; the GEMM's schedule (tiles per XCD round, K ranges) is chosen per
+// width class by timing the candidates on the device the first time that class is seen.
 #include <algorithm>
 #include <cstring>
 
@@ -53,7 +53,7 @@ static int batch_reserve(lbn_prover *pr, size_t cols) {
     b->graph = nullptr;
     b->graph_def = nullptr;
     b->graph_cols = 0;
-    const size_t nb = bytes_size((uint32_t)cap, pp->kb_c);
+    const size_t nb = bytes_size((uint32_t)cap, pp->kpad_c);
     int rc;
     if ((rc = dalloc(&b->d_bytes, nb))) return rc;
     HIP_TRY(hipMemset(b->d_bytes, 0, nb));  // padding inputs are never written and must read as zero
@@ -66,12 +66,10 @@ static int batch_reserve(lbn_prover *pr, size_t cols) {
     return LBN_OK;
 }
 
-// A schedule's persistent-grid word: the block count, and this bit when the rounds run tile-major
-// (gemm_item_of), so that a block keeps its tile's LUT and E over consecutive items.
-constexpr uint32_t TILE_MAJOR = 0x80000000u;
-
-// The kernel sequence of one batched commitment over R columns.
-static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t persist, bool with_prepass) {
+// The kernel sequence of one batched commitment over R columns: the GEMM with sched_t tiles per
+// XCD round and at least min_splits K ranges (its count of them to *splits).
+static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t min_splits, bool with_prepass,
+                        uint32_t *splits) {
     const lbn_pp *pp = pr->pp;
     const Shape &s = pp->shape;
     const size_t limbs = s.primes.size();
@@ -79,24 +77,11 @@ static int batch_launch(lbn_prover *pr, size_t R, uint32_t sched_t, uint32_t per
     hipStream_t st = pr->stream;
     if (with_prepass)
         HIP_TRY(launch_prepass(st, nullptr, b->d_cols, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)R,
-                               b->d_bytes, pp->kb_c, nullptr, 0));
-    GemmArgs g{};
-    g.tiles = pp->d_tiles;
-    g.afrag = pp->d_afrag;
-    g.lut = pp->d_lut;
-    g.ee = pp->d_ee;
-    g.out = pp->d_out;
-    g.bytes = b->d_bytes;
-    g.kbs = pp->kb_c;
-    g.a_kb_stride = A_STEP;
-    g.y = b->d_yacc;
-    g.y_batch = YACC_PLANES * limbs * N;
-    g.batches = (uint32_t)R;
-    g.groups = (uint32_t)((R + GROUP_BATCH - 1) / GROUP_BATCH);
-    g.ntiles = pp->ntiles;
+                               b->d_bytes, pp->kpad_c, nullptr, 0));
+    GemmArgs g = commit_gemm_args(pp, b->d_bytes, b->d_yacc, R);
     g.sched_t = sched_t;
-    g.tile_major = (persist & TILE_MAJOR) ? 1 : 0;
-    HIP_TRY(launch_tile_gemm(st, g, persist & ~TILE_MAJOR));
+    g.splits = min_splits;
+    HIP_TRY(launch_tile_gemm(st, g, splits));
     HIP_TRY(launch_commit_finish(st, b->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, b->d_raw,
                                  b->d_comp, (uint32_t)R));
     return LBN_OK;
@@ -110,10 +95,9 @@ static uint32_t width_class(size_t R) {
     return c;
 }
 
-// Candidate schedules, timed on this device for this width class: tiles per XCD round and a full
-// or a persistent grid (one or two blocks' worth of rounds per CU), the rounds of a persistent
-// grid walking the tile sets first or (tile-major) the groups first. A block whose consecutive
-// items share their tile keeps its LUT and E in LDS.
+// Candidate schedules, timed on this device for this width class: tiles per XCD round (the rest of
+// an XCD's 32 blocks are column blocks) and the launcher's K ranges or twice as many (more, shorter
+// items for a narrow batch). The result is the tiles per round and the ranges asked for.
 static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p) {
     lbn_batch_ws *b = pr->batch;
     for (const auto &e : b->tuned)
@@ -123,16 +107,17 @@ static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p
             return LBN_OK;
         }
     static const uint32_t ts[] = {4, 8, 16};
-    static const uint32_t ps[] = {0, 256, 512, 256 | TILE_MAJOR, 512 | TILE_MAJOR};
     float best = 1e30f;
     *best_t = 8;
     *best_p = 0;
     int rc;
-    if ((rc = batch_launch(pr, R, 8, 0, true))) return rc;  // the layout, and a warm-up
+    uint32_t auto_splits = 1;
+    if ((rc = batch_launch(pr, R, 8, 0, true, &auto_splits))) return rc;  // the layout, and a warm-up
+    const uint32_t ps[] = {0, 2 * auto_splits};
     for (uint32_t t : ts)
         for (uint32_t p : ps) {
             HIP_TRY(hipEventRecord(b->ev[0], pr->stream));
-            if ((rc = batch_launch(pr, R, t, p, false))) return rc;
+            if ((rc = batch_launch(pr, R, t, p, false, nullptr))) return rc;
             HIP_TRY(hipEventRecord(b->ev[1], pr->stream));
             HIP_TRY(hipEventSynchronize(b->ev[1]));
             const float ms = elapsed(b->ev[0], b->ev[1]);
@@ -198,7 +183,7 @@ int lbn_commit_batch(lbn_prover *pr, const lbn_dwitness *const *ws, size_t count
     b->last_cols = R;
     if (!replay) {
         HIP_TRY(hipEventRecord(b->ev[0], st));
-        if ((rc = batch_launch(pr, R, t, p, true))) return rc;
+        if ((rc = batch_launch(pr, R, t, p, true, &b->splits))) return rc;
         HIP_TRY(hipEventRecord(b->ev[1], st));
     } else if (!b->graph || b->graph_cols != R || b->graph_t != t || b->graph_persist != p) {
         if (b->graph) (void)hipGraphExecDestroy(b->graph);
@@ -206,7 +191,7 @@ int lbn_commit_batch(lbn_prover *pr, const lbn_dwitness *const *ws, size_t count
         b->graph = nullptr;
         b->graph_def = nullptr;
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = batch_launch(pr, R, t, p, true);
+        rc = batch_launch(pr, R, t, p, true, &b->graph_splits);
         hipGraph_t gdef = nullptr;
         const hipError_t ec = hipStreamEndCapture(st, &gdef);
         if (rc) {
@@ -224,6 +209,7 @@ int lbn_commit_batch(lbn_prover *pr, const lbn_dwitness *const *ws, size_t count
         HIP_TRY(hipEventRecord(b->ev[0], st));
         HIP_TRY(hipGraphLaunch(b->graph, st));
         HIP_TRY(hipEventRecord(b->ev[1], st));
+        b->splits = b->graph_splits;
     }
     // outputs: components [R][4][limb][162] are contiguous per witness; raw [limb][R][648] per limb
     c = 0;
@@ -241,14 +227,13 @@ int lbn_commit_batch(lbn_prover *pr, const lbn_dwitness *const *ws, size_t count
     HIP_TRY(hipStreamSynchronize(st));
     b->ms = elapsed(b->ev[0], b->ev[1]);
     b->sched_t = t;
-    b->persist = p;
     return LBN_OK;
 }
 
-int lbn_batch_info(const lbn_prover *pr, uint32_t *sched_t, uint32_t *persistent_blocks, float *ms) {
+int lbn_batch_info(const lbn_prover *pr, uint32_t *sched_t, uint32_t *splits, float *ms) {
     if (!pr || !pr->batch) return fail(LBN_ERR_ARG, "no batch has run on this prover");
     if (sched_t) *sched_t = pr->batch->sched_t;
-    if (persistent_blocks) *persistent_blocks = pr->batch->persist;
+    if (splits) *splits = pr->batch->splits;
     if (ms) *ms = pr->batch->ms;
     return LBN_OK;
 }

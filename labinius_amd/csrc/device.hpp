@@ -9,16 +9,19 @@
 // 81 a + s is a function of the even (a = 0) or odd (a = 1) bits of the same byte s. What is left
 // is, per 81-slot block, an 81 x 81 linear map E (slot u <- sub-ring coefficient n), so
 //
-//     y[u] = sum_n E[u][n] * Z[u][n],        Z[u][n] = sum_i A[i][u] * x_i[n],
+//     y[u] = sum_n E[u][n] sum_i A[i][u] x_i[n] = sum_(n, i) A'[(n, i)][u] x_i[n],
+//     A'[(n, i)][u] = E[u][n] A[i][u] mod q,
 //
-// where Z is an integer GEMM (K = ring elements) run on int8 MFMA with both operands split into
-// two int8 digits (UNSIGNED_DIGITS_Q below). A "tile" is one such 96 x 81 (81 x 81 used) GEMM: 8 per splitting
-// limb, 16 per quadratic limb (4 sub-rings x 2 halves of the A rows x 2 parities of the inputs).
-// The fold is the same GEMM with K = columns, A = the challenges' transform, 8 tiles on the base.
+// an integer GEMM with K = (n, i), 81 x the ring elements (commit) or columns (fold), run on int8
+// MFMA with both operands split into two int8 digits (UNSIGNED_DIGITS_Q below). A "tile" is one
+// such 96 x batch (81 or 82 rows used) GEMM with its own A' and LUT: 8 per splitting limb, 16 per
+// quadratic limb (4 sub-rings x 2 halves of the A rows x 2 parities of the inputs). The fold is
+// the same GEMM with A = the challenges' transform, 8 tiles on the base.
 //
-// A tile's A is shared by every batch element (column for the commit, ring element for the
-// fold), so the GEMM's N runs over batch elements and sub-ring coefficients together: a "group"
-// is 3 batch elements = 243 inputs, padded to 256 = 8 fragments of 32 inputs.
+// A tile's A' is shared by every batch element (column for the commit, ring element for the
+// fold), so the GEMM's N runs over batch elements: a "group" is 256 of them = 8 fragments of 32.
+// The long K of an output is split into ranges of at most KB_EXACT steps whose results are added
+// into y (u32 atomics of residues, reduced mod q by y's readers).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,64 +39,70 @@ constexpr int BLK = 32;          // staged fragment block (32 rows or inputs x 3
 constexpr int NBLK = TILE / BLK; // 3 blocks per tile side
 constexpr int FRAG = 1024;       // bytes of one staged 32 x 32 int8 fragment (64 lanes x 16 B)
 constexpr int A_STEP = 2 * 2 * NBLK * FRAG;  // A fragments per K step: [kh][digit][mb]
-constexpr int GROUP_BATCH = 3;               // batch elements per group
-constexpr int GROUP_COLS = 256;              // N of a group (243 used)
+constexpr int GROUP_COLS = 256;              // batch elements of a group (the GEMM's N per item)
 constexpr int GROUP_FRAGS = GROUP_COLS / BLK;
 constexpr int TILE_ROWS = 82;               // rows a tile may use (a quadratic commit tile's first half: 82)
-constexpr int EP = 88;                      // E and Z rows padded to 176 bytes (16-byte aligned; zero from n = 81)
+constexpr int EP = 88;                      // E rows padded to 176 bytes (zero from n = 81)
 constexpr int EE_TILE = TILE_ROWS * EP;      // E of a tile, [m 82][n 88] u16
 
-// The input bytes in MFMA B-fragment order. Byte n of batch element b at K index k lives at
+// The K order: index k = n kpad + i for sub-ring coefficient n < 81 and i < kpad, the ring
+// element (commit) or column (fold) padded to a power of two of at least one step, so that a
+// step holds one n and runs of steps of one n are a power of two.
+inline uint32_t gemm_kpad(uint64_t len) {
+    uint32_t p = KSTEP;
+    while (p < len) p <<= 1;
+    return p;
+}
+__host__ __device__ inline uint32_t gemm_kbs(uint32_t kpad) { return TILE_USED * kpad / KSTEP; }
+
+// The input bytes in MFMA B-fragment order. Byte n of batch element b at index i lives at
 //
 //   ((((g * 8 + jg) * kbs + kb) * 2 + kh) * 64 + lane) * 16 + (k & 15)
 //
-// with g = b / 3, j = 81 (b % 3) + n, jg = j / 32, lane = (j & 31) + 32 ((k >> 4) & 1),
-// kh = (k >> 5) & 1, kb = k / 64: one 16-byte load per lane gives a lane its B fragment.
-__host__ __device__ inline uint64_t byte_offset(uint32_t b, uint32_t n, uint32_t k, uint32_t kbs) {
-    const uint32_t g = b / GROUP_BATCH, j = TILE_USED * (b % GROUP_BATCH) + n;
-    const uint32_t lane = (j & 31) + 32 * ((k >> 4) & 1);
-    return ((((uint64_t)g * GROUP_FRAGS + (j >> 5)) * kbs + (k >> 6)) * 2 + ((k >> 5) & 1)) * 1024 +
+// with k = n kpad + i, g = b / 256, j = b % 256, jg = j / 32, lane = (j & 31) + 32 ((k >> 4) & 1),
+// kh = (k >> 5) & 1, kb = k / 64, kbs = gemm_kbs(kpad): one 16-byte load per lane gives a lane
+// its B fragment.
+__host__ __device__ inline uint64_t byte_offset(uint32_t b, uint32_t n, uint32_t i, uint32_t kpad) {
+    const uint32_t g = b / GROUP_COLS, j = b % GROUP_COLS;
+    const uint64_t k = (uint64_t)n * kpad + i;
+    const uint32_t lane = (j & 31) + 32 * (uint32_t)((k >> 4) & 1);
+    return ((((uint64_t)g * GROUP_FRAGS + (j >> 5)) * gemm_kbs(kpad) + (k >> 6)) * 2 + ((k >> 5) & 1)) * 1024 +
            lane * 16 + (k & 15);
 }
-inline uint64_t bytes_size(uint32_t batches, uint32_t kbs) {
-    return (uint64_t)((batches + GROUP_BATCH - 1) / GROUP_BATCH) * GROUP_FRAGS * kbs * 2 * 1024;
+inline uint64_t bytes_size(uint32_t batches, uint32_t kpad) {
+    return (uint64_t)((batches + GROUP_COLS - 1) / GROUP_COLS) * GROUP_FRAGS * gemm_kbs(kpad) * 2 * 1024;
 }
 
 // One tile of a GEMM launch.
 struct TileDesc {
-    uint64_t a_off;     // byte offset of the tile's A fragments: [kb][kh 2][digit 2][mb 3][lane 64][16]
+    uint64_t a_off;     // byte offset of the tile's A' fragments: [kb][kh 2][digit 2][mb 3][lane 64][16]
     uint32_t lut_off;   // element offset (u16) of the tile's two 256-entry LUTs
-    uint32_t ee_off;    // element offset (u16) of the tile's E, [m 82][n 88]
+    uint32_t ee_off;    // element offset (u16) of the tile's E, [m 82][n 88] (A' is built from it)
     uint32_t out_off;   // element offset (i16) of the tile's 96 output slots (-1 = padding row)
     uint32_t q;
-    uint32_t par;       // input n reads LUT (par + n) & 1
+    uint32_t par;       // sub-ring coefficient n reads LUT (par + n) & 1
     uint32_t y_off;     // word offset of the tile's limb inside one batch's output rows
 };
 
 struct GemmArgs {
     const TileDesc *tiles;
-    const int8_t *afrag;      // all tiles' A fragments
+    const int8_t *afrag;      // all tiles' A' fragments
     const uint16_t *lut;      // all tiles' LUTs (low byte x_l, high byte x_h, both int8)
-    const uint16_t *ee;       // all tiles' E (mod q)
     const int16_t *out;       // all tiles' output slots
     const uint8_t *bytes;     // the input bytes, fragment order (byte_offset)
-    uint32_t kbs;             // K steps of the layouts
-    uint32_t kb0, kbn;        // this launch's steps: kb0 .. kb0 + kbn - 1 (set by the launcher)
-    uint32_t add;             // 1: y = (y + result) mod q (a later K chunk), 0: y = result
-    uint32_t a_kb_stride;     // bytes of A fragments per K step of a tile (A_STEP)
-    uint32_t *y;              // output rows; every (batch element, tile output) has its own word
+    uint32_t kbs;             // K steps of the layouts, gemm_kbs(kpad)
+    uint32_t run_shift;       // log2(kpad / 64): step kb holds sub-ring coefficient kb >> run_shift
+    uint32_t a_kb_stride;     // bytes of A' fragments per K step of a tile (A_STEP)
+    uint32_t *y;              // output rows: every (batch element, tile output) has its own word
     uint64_t y_batch;         // words per batch element in y
     uint32_t batches;
-    uint32_t groups;          // ceil(batches / 3)
+    uint32_t groups;          // ceil(batches / 256)
     uint32_t ntiles;
-    uint32_t sched_t;         // tiles per XCD round (1 .. 32, a power of two); the rest are groups
-    uint32_t tile_major;      // 1: consecutive rounds share their tiles (and walk the groups)
-    uint32_t items;           // work items (set by the launcher)
-    // set by the launcher: log2(sched_t); the tile sets and group blocks of the XCD rounds; and
-    // for each the multiplier ceil(2^32 / d) that divides every round index by d as a product's
-    // high word (0 where rounds x d reaches 2^32 and that is not exact: the kernel then divides)
-    uint32_t t_shift, tgroups, gblocks;
-    uint64_t tg_magic, gb_magic;
+    uint32_t sched_t;         // tiles per XCD round (4 .. 32, a power of two); 32 / sched_t column blocks
+    // set by the launcher: K in `splits` ranges of kps steps (the last may be shorter); column
+    // blocks (group, range) = groups x splits; the tile sets and column-block sets of the XCD
+    // rounds; the work items
+    uint32_t splits, kps, cblocks, tsets, csets, items;
 };
 
 // Operands mod q are split into two int8 digits, x = x_l + 128 x_h: for q < UNSIGNED_DIGITS_Q
@@ -102,14 +111,15 @@ struct GemmArgs {
 constexpr uint32_t UNSIGNED_DIGITS_Q = 16384;
 // The int32 accumulators are exact while K x 32258 < 2^31: the largest class, mid = A_l x_h +
 // A_h x_l, adds at most 2 x 127 x 127 = 32258 per K index (non-negative digits; the centred ones
-// add at most 2 x 64 x 76). The launcher splits longer K into chunks of KB_EXACT steps and the
-// later chunks add their outputs mod q.
+// add at most 2 x 64 x 76). The launcher splits longer K into ranges of at most KB_EXACT steps.
 constexpr uint32_t KB_EXACT = 2147483647u / (KSTEP * 32258u);  // 1040 steps, K = 66 560
 
 // The commit GEMM's output is y[c][limb][plane][648]. A splitting limb's 8 tiles write each of
-// the 648 slots once, into plane 0. A quadratic limb's slot 2j + t is written by the two tiles of
-// input parity b = 0 and 1, each into plane b; the finish adds the planes. Every word then has
-// one writer, so the GEMM stores without atomics and y needs no clearing.
+// the 648 slots, into plane 0. A quadratic limb's slot 2j + t is written by the two tiles of
+// input parity b = 0 and 1, each into plane b; the finish adds the planes. With one K range a
+// word has one writer, which stores its residue; with several, y is cleared first and each range
+// adds its residue in [0, q), so a word is below splits x q (far below 2^32) and its reader
+// reduces it mod q.
 constexpr int YACC_PLANES = 2;
 
 // Per-limb constants of the commit finish (decomposition) kernel.

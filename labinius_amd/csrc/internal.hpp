@@ -135,8 +135,10 @@ struct lbn_pp {
     std::vector<int16_t> key;  // [limb][len_ring][648], centered
     int device = 0;
     bool host_only = false;  // the key and shape only (a verifier on a host without a GPU)
-    uint32_t lpad = 0, rpad = 0, kb_c = 0, kb_f = 0;
-    // commit GEMM
+    // K index lengths: ring elements and columns padded to a step (the A and challenge fragments)
+    // and to a power of two (the GEMMs' K order, device.hpp); the GEMMs' steps gemm_kbs(kpad)
+    uint32_t lpad = 0, rpad = 0, kpad_c = 0, kpad_f = 0, kb_c = 0, kb_f = 0;
+    // commit GEMM: A' resident, ntiles x kb_c steps
     uint32_t ntiles = 0;
     int8_t *d_afrag = nullptr;
     uint16_t *d_lut = nullptr, *d_ee = nullptr;
@@ -177,11 +179,11 @@ struct lbn_batch_ws {
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     size_t graph_cols = 0, last_cols = 0;
-    uint32_t graph_t = 0, graph_persist = 0;
-    std::vector<std::array<uint32_t, 3>> tuned;  // (width class, sched_t, persistent blocks)
+    uint32_t graph_t = 0, graph_persist = 0, graph_splits = 1;  // the graph's schedule and its K ranges
+    std::vector<std::array<uint32_t, 3>> tuned;  // (width class, sched_t, K ranges asked for)
     hipEvent_t ev[2] = {};
     float ms = 0;
-    uint32_t sched_t = 8, persist = 0;
+    uint32_t sched_t = 8, splits = 1;  // the last batch's schedule and K ranges
 };
 
 // The cross-field switch's workspace (grow-only): eq(r_hi), the batched eq image a, the folded
@@ -241,9 +243,12 @@ void batch_free(lbn_batch_ws *b);
 // bd/digits.rs:121-182 on the device into pr->d_bd_out (dropped.cpp)
 int bd_digits_device(lbn_prover *pr, const Shape &s, const uint32_t *d_raw);
 void wire_free(lbn_wire_ws &w);
+// The commit GEMM's arguments over ncol columns of a layout into y ([ncol][limb][plane][648]);
+// sched_t 8 and no minimum of K ranges (prover.cpp).
+GemmArgs commit_gemm_args(const lbn_pp *pp, const uint8_t *bytes, uint32_t *y, size_t ncol);
 // Prepass, GEMM and finish of ncol columns (column_len F162 each, at d_w) under the prover's key,
 // into d_comp ([c][4][limb][162]) and d_raw ([limb][c][648]) on the prover's stream (prover.cpp).
-// d_bytes holds bytes_size(ncol, kb_c) bytes whose padding reads as zero; d_yacc ncol x
+// d_bytes holds bytes_size(ncol, kpad_c) bytes whose padding reads as zero; d_yacc ncol x
 // YACC_PLANES x limbs x 648 words.
 int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint8_t *d_bytes, uint32_t *d_yacc,
                           uint32_t *d_raw, int16_t *d_comp);
@@ -264,7 +269,8 @@ struct lbn_prover {
     uint32_t *d_yacc = nullptr, *d_raw = nullptr;
     int16_t *d_comp = nullptr;
     lbn_challenge *d_ch = nullptr;
-    int8_t *d_afrag_f = nullptr;
+    int8_t *d_chat = nullptr;     // the challenges' transform, ntiles_f x rpad / 64 steps
+    int8_t *d_afrag_f = nullptr;  // the fold's A', ntiles_f x kb_f steps
     uint32_t *d_vhat = nullptr;
     int16_t *d_v = nullptr;
     uint64_t *d_p0 = nullptr, *d_eq = nullptr, *d_eqpart = nullptr, *d_u = nullptr;

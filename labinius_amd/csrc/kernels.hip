@@ -33,18 +33,18 @@ namespace lbn {
 
 // The byte of ring element i of column c at position n < 81: bit t is coefficient 81 t + n, and
 // coefficient 4 m + k of a ring element is bit m of its F162 element k (f162.rs:1-16). Written
-// twice in fragment order (byte_offset): the commit's (batch c, K = i) and the fold's (batch i,
-// K = c). A block covers 16 ring elements x 16 columns, one thread per ring element, and stages
-// the bytes in LDS, ring elements fastest for the commit's stores and then, in the same 22 KB,
-// columns fastest for the fold's, so that each store gathers its 16 bytes with one aligned
-// 16-byte read. Rows are padded to 272 bytes, which puts 16 consecutive n on distinct banks.
+// twice in fragment order (byte_offset): the commit's (batch c, K = (n, i)) and the fold's
+// (batch i, K = (n, c)). A block covers 16 ring elements x 16 columns, one thread per (ring
+// element, column), and stages the bytes in LDS, ring elements fastest for the commit's stores
+// and then, in the same 22 KB, columns fastest for the fold's, so that each store gathers its 16
+// bytes with one aligned 16-byte read; the 16 stores of one n are 256 contiguous bytes.
 constexpr int PRE_I = 16, PRE_C = 16, PRE_ROW = PRE_I * PRE_C + 16;
 
 __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *__restrict__ w,
                                                                 const uint64_t *const *__restrict__ cols,
                                                                 uint32_t column_len, uint32_t L, uint32_t r,
-                                                                uint8_t *__restrict__ bytes_c, uint32_t kbs_c,
-                                                                uint8_t *__restrict__ bytes_f, uint32_t kbs_f) {
+                                                                uint8_t *__restrict__ bytes_c, uint32_t kpad_c,
+                                                                uint8_t *__restrict__ bytes_f, uint32_t kpad_f) {
     __shared__ __attribute__((aligned(16))) uint8_t stc[TILE_USED][PRE_ROW];  // [n][c * 16 + i], then [n][i * 16 + c]
     const int il = threadIdx.x & (PRE_I - 1), cl = threadIdx.x / PRE_I;
     const uint32_t i0 = blockIdx.y * PRE_I, c0 = blockIdx.x * PRE_C;
@@ -114,23 +114,26 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 #pragma unroll
     for (int n = 0; n < TILE_USED; n++) stc[n][cl * PRE_I + il] = (uint8_t)(x[n >> 2] >> (8 * (n & 3)));
     __syncthreads();
-    // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16)
+    // commit: for each (c, n) the 16 bytes of ring elements i0 .. i0 + 15 (i0 is a multiple of 16);
+    // the 16 columns of one n are consecutive lanes of a fragment, 256 contiguous bytes, so
+    // consecutive threads take them
     if (bytes_c)
         for (int u = threadIdx.x; u < PRE_C * TILE_USED; u += PRE_I * PRE_C) {
-            const int cc = u / TILE_USED, n = u % TILE_USED;
+            const int cc = u % PRE_C, n = u / PRE_C;
             if (c0 + cc >= r) continue;
-            *(uint4 *)(bytes_c + byte_offset(c0 + cc, n, i0, kbs_c)) = *(const uint4 *)&stc[n][cc * PRE_I];
+            *(uint4 *)(bytes_c + byte_offset(c0 + cc, n, i0, kpad_c)) = *(const uint4 *)&stc[n][cc * PRE_I];
         }
     if (!bytes_f) return;
     __syncthreads();
 #pragma unroll
     for (int n = 0; n < TILE_USED; n++) stc[n][il * PRE_C + cl] = (uint8_t)(x[n >> 2] >> (8 * (n & 3)));
     __syncthreads();
-    // fold: for each (i, n) the 16 bytes of columns c0 .. c0 + 15 (c0 is a multiple of 16)
+    // fold: for each (i, n) the 16 bytes of columns c0 .. c0 + 15 (c0 is a multiple of 16), the 16
+    // ring elements of one n by consecutive threads
     for (int u = threadIdx.x; u < PRE_I * TILE_USED; u += PRE_I * PRE_C) {
-        const int ii = u / TILE_USED, n = u % TILE_USED;
+        const int ii = u % PRE_I, n = u / PRE_I;
         if (i0 + ii >= L) continue;
-        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0, kbs_f)) = *(const uint4 *)&stc[n][ii * PRE_C];
+        *(uint4 *)(bytes_f + byte_offset(i0 + ii, n, c0, kpad_f)) = *(const uint4 *)&stc[n][ii * PRE_C];
     }
 }
 
@@ -138,15 +141,14 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // the tile GEMM
 // =============================================================================================
 
-// One tile over one group of 3 batch elements: Z[m][j] = sum_k A[k][m] x_k[j] for the group's
-// 256 inputs j = 81 b + n (243 used) over all K, then y[b][out[m]] = sum_n E[m][n] Z[m][81 b + n]
-// mod q.
+// One tile over one group of 256 batch elements and one range of K steps:
+// y[b][out[m]] (+)= sum_k A'[k][m] x_b[k] mod q, with k = (n, i) as in device.hpp.
 //
-// Eight waves, wave w owning inputs 32 w .. 32 w + 31 as two 16-input blocks. A (both digits) is
-// staged per K step in LDS, double-buffered and shared by the eight waves. Each wave builds its
-// own B fragments in registers: two 16-byte loads per lane give the 16 bytes of its two inputs'
-// K range, and the tile's LUT (in LDS) turns every byte into its two digits. Z is kept as three
-// int32 accumulators per element: ll = A_l x_l, mid = A_l x_h + A_h x_l, hh = A_h x_h.
+// Eight waves, wave w owning inputs 32 w .. 32 w + 31 as two 16-input blocks. A' (both digits) is
+// staged per K step in LDS and shared by the eight waves. Each wave builds its own B fragments
+// in registers: two 16-byte loads per lane give the 16 bytes of its two inputs' K range, and the
+// tile's LUT (in LDS) turns every byte into its two digits. A product is kept as three int32
+// accumulators per element: ll = A_l x_l, mid = A_l x_h + A_h x_l, hh = A_h x_h.
 //
 // The products are v_mfma_i32_16x16x64_i8. Rows 0..79 are five 16-row blocks with four digit
 // products each; rows 80 and 81 (81 = 5 x 16 + 1, and a quadratic commit tile's first half uses
@@ -154,32 +156,23 @@ __global__ __launch_bounds__(PRE_I * PRE_C) void prepass_kernel(const uint64_t *
 // and with x_h) give all eight of their digit products. A step is 44 MFMAs of 32 K ops a wave,
 // where 96 rows of 32 x 32 x 32 blocks took 24 of 64 K: 17% less MFMA work.
 //
-// Blocks are ordered so that the 32 blocks an XCD runs at once are 8 tiles x 4 groups: each
-// A and each group's bytes then come from L2 for 4 and 8 blocks respectively.
+// Items are ordered so that the 32 blocks an XCD runs at once are t tiles x 32/t column blocks
+// (groups of one K range): each tile's A' range then comes from L2 for 32/t blocks and each
+// group's bytes for t.
 constexpr int GEMM_THREADS = 512;
-// The epilogue's Z in LDS: [batch 3][m 82][n 88] u16 (centred, |Z| <= (q + 1) / 2), rows of 176 bytes.
-constexpr int ZP_BYTES = (GROUP_BATCH * TILE_ROWS * EP * 2 + 15) / 16 * 16;
-// A is staged per K step in an LDS buffer of A_STEP bytes: 768 16-byte words, the first 512 one
-// a thread, the other 256 one a thread of waves 0-3.
+// A' is staged per K step in an LDS buffer of A_STEP bytes: 768 16-byte words, the first 512 one
+// a thread, the other 256 one a thread of waves 0-3. A' streams from HBM, so its copies are issued
+// A_PRE steps earlier than the step barriers need them, into A_PRE + 3 buffers.
 constexpr int A_LDS = A_STEP;
+constexpr int A_PRE = 4, A_BUFS = A_PRE + 3;
 static_assert(A_STEP == 768 * 16 && GEMM_THREADS == 512, "two copies a thread of waves 0-3, one of waves 4-7");
 // LDS (the kernel's only LDS is the dynamic block, at address 0):
-//   [0, 64 K) replicated LUT | plain LUT (1 KB) | three A buffers | E | Z
-// E is loaded when a block's tile changes and read in the epilogue. Z has its own region, so that
-// the next item's first A step can be copied into buffer 0 while the epilogue runs. A block whose
-// next item has the same tile (a persistent grid over one tile's groups) keeps the LUT and E from
-// the item before.
+//   [0, 64 K) replicated LUT | plain LUT (1 KB) | the tile's output slots (96 i16) | A_BUFS A' buffers
 // The replicated LUT holds entry (byte, parity, copy) at byte * 256 + parity * 128 + copy * 4: at
 // address 0 a lookup's address is one byte permute of the input word and the lane's own bits.
-constexpr int LUTR_OFF = 0, LUT_OFF = 65536, ABUF_OFF = LUT_OFF + 1024;
-constexpr int EE_OFF = ABUF_OFF + 3 * A_LDS;
-constexpr int ZP_OFF = EE_OFF + EE_TILE * 2;
-constexpr int GEMM_LDS = ZP_OFF + ZP_BYTES;
-static_assert(ZP_OFF % 16 == 0 && GEMM_LDS <= 160 * 1024, "the GEMM's LDS fits a CU");
-
-__host__ __device__ inline uint32_t gemm_items(uint32_t ntiles, uint32_t groups, uint32_t t) {
-    return 256 * ((ntiles + t - 1) / t) * ((groups + 256 / t - 1) / (256 / t));
-}
+constexpr int LUTR_OFF = 0, LUT_OFF = 65536, OUT_OFF = LUT_OFF + 1024, ABUF_OFF = OUT_OFF + 256;
+constexpr int GEMM_LDS = ABUF_OFF + A_BUFS * A_LDS;
+static_assert(GEMM_LDS <= 160 * 1024, "the GEMM's LDS fits a CU");
 
 // ll + 2^7 mid + 2^14 hh mod q as a centred residue, from the three int32 accumulators of a
 // product. The sum is exact in f64 (|.| < 2^46); t = rint(z / q) is within 2^-17 of z / q's
@@ -199,7 +192,6 @@ __device__ __forceinline__ int mod_q_classes(int ll, int mid, int hh, double qd,
 // instruction and conversion issues at half the rate of a 32-bit add. For K <= KB_U32 steps of
 // 64, ll + 2^7 mid <= K x 127^2 x 257 < 2^32 is one u32 sum.
 constexpr uint32_t KB_U32 = 4294967295u / (KSTEP * 127u * 127u * 257u);  // 16 steps, K = 1024
-static_assert(KB_U32 >= 16, "the fold's 1024 columns take the short form");
 constexpr uint64_t M_BITS = 0x4338000000000000ull;  // 1.5 x 2^52
 constexpr double M_MAGIC = 6755399441055744.0;
 __device__ __forceinline__ double m_plus(uint32_t x) { return __builtin_bit_cast(double, M_BITS | x); }
@@ -224,46 +216,26 @@ __device__ __forceinline__ uint32_t mod_q_i64(long long x, double qd, double qin
     return (uint32_t)(r < 0 ? r + (int)qd : r);
 }
 
-// The block order: item -> (tile, group). The 32 items an XCD runs at once (item & 7 is the XCD a
-// block lands on when the grid is a multiple of 8) are t tiles x 32/t groups. Rounds walk the tile
-// sets first (consecutive rounds share their groups' bytes in L2) or, tile-major, the groups
-// first: on a persistent grid of 256 blocks, block b's items b + 256 k are then rounds k with one
-// tile for ceil(groups / (256 / t)) rounds in a row, which keep that tile's LUT and E.
-// All of it scalar work without a division (t is a power of two; round / d is a product's high
-// word), so that a block finds its next item in a few instructions.
-__device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, uint32_t &tile, uint32_t &grp) {
-    const uint32_t sh = g.t_shift, xcd = item & 7, s = item >> 3;
-    const uint32_t round = s >> 5, within = s & 31;
-    const uint32_t d = g.tile_major ? g.gblocks : g.tgroups;
-    const uint64_t magic = g.tile_major ? g.gb_magic : g.tg_magic;
-    const uint32_t major = magic ? (uint32_t)(((uint64_t)round * magic) >> 32) : round / d;
-    const uint32_t minor = round - major * d;
-    const uint32_t ts = g.tile_major ? major : minor;  // round / gblocks or round % tgroups
-    const uint32_t gb = g.tile_major ? minor : major;  // round % gblocks or round / tgroups
-    tile = (ts << sh) + (within & ((1u << sh) - 1));
-    grp = ((8 * gb + xcd) << (5 - sh)) + (within >> sh);
-    return tile < g.ntiles && grp < g.groups;
+__host__ __device__ inline uint32_t gemm_items(uint32_t tsets, uint32_t csets) {
+    return 256 * (uint32_t)(((uint64_t)tsets * csets + 7) / 8);
 }
 
-// The block's next item: whether there is one, its tile and group.
-struct NextItem {
-    bool has;
-    uint32_t tile, grp;
-};
+// The block order: item -> (tile, column block). The 32 items an XCD runs at once (item & 7 is
+// the XCD a block lands on; the grid is a multiple of 8) are one unit of t tiles x 32/t column
+// blocks; units go round-robin over the XCDs, tile sets fastest. Column block cb is group
+// cb % groups of K range cb / groups, so the column blocks of a unit share their K range (and so
+// the tiles' A' ranges) unless the unit straddles two ranges.
+__device__ __forceinline__ bool gemm_item_of(const GemmArgs &g, uint32_t item, uint32_t &tile, uint32_t &cb) {
+    const uint32_t sh = __builtin_ctz(g.sched_t), xcd = item & 7, k = item >> 3;
+    const uint32_t unit = (k >> 5) * 8 + xcd, within = k & 31;
+    const uint32_t ts = unit % g.tsets, cs = unit / g.tsets;
+    tile = (ts << sh) + (within & (g.sched_t - 1));
+    cb = (cs << (5 - sh)) + (within >> sh);
+    return tile < g.ntiles && cb < g.cblocks;
+}
 
-// What an item hands to the next on its block: the registers of the first step (A(0)'s pieces
-// and the packed operands of step 0) and the input bytes, filled in by the item before (ready)
-// during its epilogue.
-struct ItemRegs {
-    v4i raw[2], a0[5], a1[5], ax, bl[2], bh[2];
-};
-
-template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td, TileDesc &tdu, int slot, uint32_t tile,
-                                          uint32_t grp, uint8_t *smem, bool ready, const NextItem &nx, ItemRegs &R);
-
-// A tile descriptor read by vector loads, as scalar (wave-uniform) values: taken once its loads
-// are complete, so that the item that uses it starts without waiting on a load.
+// A tile descriptor read by vector loads, as scalar (wave-uniform) values (the kernel writes y,
+// so the compiler does not read the descriptor with scalar loads).
 __device__ __forceinline__ TileDesc uniform_desc(const TileDesc &t) {
     TileDesc u;
     u.a_off = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(t.a_off >> 32)) << 32) |
@@ -283,6 +255,16 @@ __device__ __forceinline__ TileDesc uniform_desc(const TileDesc &t) {
 // for its own before the barrier that publishes them.
 constexpr int waitcnt_vm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
 
+// A workgroup barrier that waits for this wave's LDS accesses and for all but its last N
+// vector-memory operations. __syncthreads() waits for every one (its fence covers copies into
+// LDS still in flight); the GEMM's loop keeps its A' copies in flight across step barriers. The
+// memory clobber keeps the compiler's LDS accesses on their side of the barrier.
+template <int N>
+__device__ __forceinline__ void barrier_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt holds 0 .. 63");
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
 // The fourth word of a raw buffer descriptor (no stride, no format conversion) on this family.
 constexpr int BUF_RSRC_WORD3 = 0x00020000;
 
@@ -296,101 +278,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void *p, ui
                                              BUF_RSRC_WORD3);
 }
 
-// A tile's E into LDS (read only in the epilogue: 82 x 88 x 2 bytes, 902 16-byte words, two a
-// thread) and this thread's output slot (two epilogue threads an output; -1 for none). The
-// loads are issued before `wait` runs (a barrier where the LDS area is still being read), the
-// stores after it.
-static_assert(EE_TILE * 2 <= 2 * 16 * GEMM_THREADS && (EE_TILE * 2) % 16 == 0, "E fits two loads a thread");
-static_assert(2 * GROUP_BATCH * TILE_ROWS <= GEMM_THREADS, "two epilogue threads an output");
-template <typename Wait>
-__device__ __forceinline__ int load_tile_consts(const GemmArgs &g, const TileDesc &td, uint8_t *smem, Wait wait) {
-    constexpr int EE_V = EE_TILE * 2 / 16;
-    const int tid = threadIdx.x;
-    const uint4 *eesrc = (const uint4 *)(g.ee + td.ee_off);
-    const uint4 ee0 = eesrc[tid];
-    const uint4 ee1 = tid + GEMM_THREADS < EE_V ? eesrc[tid + GEMM_THREADS] : make_uint4(0, 0, 0, 0);
-    const int slot = tid < 2 * GROUP_BATCH * TILE_ROWS ? g.out[td.out_off + (tid >> 1) % TILE_ROWS] : -1;
-    wait();
-    uint4 *ee = (uint4 *)(smem + EE_OFF);
-    ee[tid] = ee0;
-    if (tid + GEMM_THREADS < EE_V) ee[tid + GEMM_THREADS] = ee1;
-    return slot;
-}
-
-// REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
-// half-wave read distinct banks; its set-up (eight 16-byte stores a thread) pays from 8 K steps.
-template <bool REP>
-__global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
-    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
-    // a full grid runs one item per block; a persistent grid (a multiple of 8 blocks) strides.
-    // Each item sets up the block's next item during its own epilogue (its LUT, A copies, input
-    // bytes and the first step's operands), so that the next item starts with its first products.
-    uint32_t item = blockIdx.x, tile = 0, grp = 0;
-    while (item < g.items && !gemm_item_of(g, item, tile, grp)) item += gridDim.x;
-    bool ready = false;
-    ItemRegs R;
-    // An item's tile descriptor is read during the item before, and E and the output
-    // slots are loaded when the tile changes, between two items: no item starts by waiting on a
-    // load. E is first read in the first item's epilogue, after its K steps' barriers.
-    if (item >= g.items) return;
-    TileDesc td = uniform_desc(g.tiles[tile]);
-    int slot = load_tile_consts(g, td, smem, [] {});
-    while (item < g.items) {
-        NextItem nx;
-        uint32_t nitem = item + gridDim.x;
-        nx.tile = nx.grp = 0;
-        while (nitem < g.items && !gemm_item_of(g, nitem, nx.tile, nx.grp)) nitem += gridDim.x;
-        nx.has = nitem < g.items;
-        TileDesc tdu;
-        gemm_item<REP>(g, td, tdu, slot, tile, grp, smem, ready, nx, R);
-        // A new tile's E overwrites this one's once every wave is past its E.Z sums. Otherwise
-        // the next item writes the LDS only through its K steps (A copies into buffers that the
-        // epilogue does not read), and no wave passes its first step barrier before every wave
-        // has finished this epilogue, so this item's Z is safe too.
-        if (nx.has && nx.tile != tile) slot = load_tile_consts(g, tdu, smem, [] { __syncthreads(); });
-        tile = nx.tile;
-        grp = nx.grp;
-        item = nitem;
-        ready = nx.has;
-        td = tdu;
-    }
-}
-
 // The LDS word at byte address a. The GEMM's dynamic LDS block is the kernel's only LDS, so it
 // starts at address 0 and the LUT offsets above are LDS addresses.
 typedef __attribute__((address_space(3))) const uint32_t lds_u32;
 __device__ __forceinline__ lds_u32 *lds_word(uint32_t a) { return (lds_u32 *)(uintptr_t)(a + LUTR_OFF); }
 
-template <bool REP>
-__device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td, TileDesc &tdu, int slot, uint32_t tile,
-                                          uint32_t grp, uint8_t *smem, bool ready, const NextItem &nx, ItemRegs &R) {
-    v4i(&raw)[2] = R.raw;
-    v4i(&a0)[5] = R.a0;
-    v4i(&a1)[5] = R.a1;
-    v4i &ax = R.ax;
-    v4i(&bl)[2] = R.bl;
-    v4i(&bh)[2] = R.bh;
-
-    uint16_t *lut = (uint16_t *)(smem + LUT_OFF);    // [parity 2][byte 256]
-    uint8_t *abuf = smem + ABUF_OFF;                  // [3][A_LDS]
-    uint16_t *zp = (uint16_t *)(smem + ZP_OFF);       // [3][82][88], after the loop
-    uint16_t *ee = (uint16_t *)(smem + EE_OFF);       // [82][88]
-
+// The work of one item for the waves of one phase (WG 0: waves 0-3, 1: waves 4-7), compiled
+// apart so that the compiler counts each phase's own outstanding loads exactly (the two phases
+// issue different numbers of A' copies a step). The kernel has set up the LUT and the tile's
+// output slots in LDS; the first barrier here publishes them.
+template <bool REP, int WG>
+__device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td, uint32_t grp, uint32_t kb0,
+                                          uint32_t kbs, uint8_t *smem) {
+    const uint16_t *lut = (const uint16_t *)(smem + LUT_OFF);  // [parity 2][byte 256]
+    const int16_t *outs = (const int16_t *)(smem + OUT_OFF);   // [row 96]
+    uint8_t *abuf = smem + ABUF_OFF;                            // [A_BUFS][A_LDS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // a tile's LUT into LDS: entry (parity tid >> 8, byte tid & 255), replicated or plain
-    auto set_lut = [&](const TileDesc &t) {
-        const uint32_t lv = g.lut[t.lut_off + tid];
-        if (REP) {
-            // its 32 copies: eight 16-byte stores of one value
-            const uint4 v4 = make_uint4(lv, lv, lv, lv);
-            uint8_t *row = smem + LUTR_OFF + (tid & 255) * 256 + (tid >> 8) * 128;
-#pragma unroll
-            for (int c = 0; c < 8; c++) *(uint4 *)(row + 16 * c) = v4;
-        } else {
-            lut[tid] = (uint16_t)lv;
-        }
-    };
-    if (!ready) set_lut(td);
+    constexpr int wgrp = WG;
 
     // v_mfma_i32_16x16x64_i8 operands: lane holds row (A) or input (B) c16 = lane & 15 of a 16-block
     // at K group kg = lane >> 4, i.e. K 16 kg .. 16 kg + 15 of the step's 64. A wave owns 32 inputs
@@ -399,48 +303,33 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
     // the 16-byte piece of (row or input x, K group kg) sits at lane (x & 31) + 32 (kg & 1) of
     // half kh = kg >> 1 there.
     const int c16 = lane & 15, kg = lane >> 4;
-    const int wgrp = __builtin_amdgcn_readfirstlane(wave >> 2);  // waves 0-3: 0, waves 4-7: 1
     const int jl = 32 * wave + c16;  // this lane's input of block nb = 0 within the group
     const int off32 = c16 + 32 * (kg & 1), kh = kg >> 1;
-    // byte 0 of a lookup address: the input's parity and this lane's LUT copy; byte 1 comes from
-    // the input word
-    struct Sel {
-        uint32_t lsel[2];
-        const uint16_t *L[2];
-    };
-    auto sel_of = [&](const TileDesc &t) {
-        Sel x;
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++) {
-            const int par = ((int)t.par + (jl + 16 * nb) % TILE_USED) & 1;
-            x.lsel[nb] = (uint32_t)(par << 7) | (uint32_t)((lane & 31) << 2);
-            x.L[nb] = lut + (par << 8);
-        }
-        return x;
-    };
-    const Sel sel = sel_of(td);
+    // A step holds one sub-ring coefficient n, whose bytes read LUT (par + n) & 1: byte 0 of a
+    // lookup address is that parity and this lane's LUT copy, byte 1 comes from the input word
+    const uint32_t copy_sel = (uint32_t)((lane & 31) << 2);
+    auto par_of = [&](uint32_t kb) { return (td.par + ((kb0 + kb) >> g.run_shift)) & 1u; };
 
     // the (group, wave) byte fragments: [kb][kh 2][lane 64][16]; this lane's two pieces a step. Both
     // streams are raw buffer loads: the region's base in the (uniform) descriptor, the step in the
     // scalar offset, the lane's fixed offset in the vector one, so a step's address is no VALU work.
     const uint32_t wave_u = __builtin_amdgcn_readfirstlane(wave);
-    auto bytes_rsrc = [&](uint32_t gr) {
-        const uint8_t *bbase = g.bytes + ((uint64_t)gr * GROUP_FRAGS + wave_u) * g.kbs * 2 * 64 * 16;
-        return uniform_rsrc(bbase, g.kbs * 2048u);
-    };
-    const __amdgpu_buffer_rsrc_t brs = bytes_rsrc(grp);
+    const uint8_t *bbase = g.bytes + ((uint64_t)grp * GROUP_FRAGS + wave_u) * g.kbs * 2 * 64 * 16;
+    const __amdgpu_buffer_rsrc_t brs = uniform_rsrc(bbase, g.kbs * 2048u);
     const uint32_t boff = (kh * 64 + off32) * 16;
     const __amdgpu_buffer_rsrc_t ars = uniform_rsrc(g.afrag + td.a_off, g.kbs * g.a_kb_stride);
 
-    auto load_raw_from = [&](const __amdgpu_buffer_rsrc_t &rs, uint32_t kb) {
+    v4i raw[2], a0[5], a1[5], ax, bl[2], bh[2];
+    auto load_raw = [&](uint32_t kb) {
 #pragma unroll
         for (int nb = 0; nb < 2; nb++)
-            raw[nb] = __builtin_amdgcn_raw_buffer_load_b128(rs, boff + 256 * nb, (g.kb0 + kb) * 2048u, 0);
+            raw[nb] = __builtin_amdgcn_raw_buffer_load_b128(brs, boff + 256 * nb, (kb0 + kb) * 2048u, 0);
     };
-    auto load_raw = [&](uint32_t kb) { load_raw_from(brs, kb); };
-    // the operands of one step from its input bytes (raw) through the LUT, with a tile's lookup
-    // selectors
-    auto build_with = [&](const Sel &x) {
+    // the operands of step kb from its input bytes (raw) through the LUT
+    auto build = [&](uint32_t kb) {
+        const uint32_t par = par_of(kb);
+        const uint32_t lsel = copy_sel | (par << 7);
+        const uint16_t *L = lut + (par << 8);
         uint32_t v[2][16];
         // all 32 lookups first, so that their latencies overlap
 #pragma unroll
@@ -452,8 +341,8 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const uint32_t r = w4[d];
-                    v[nb][4 * d + e] = REP ? *lds_word(__builtin_amdgcn_perm(r, x.lsel[nb], 0x0c0c0400u + 0x100u * e))
-                                           : x.L[nb][(r >> (8 * e)) & 255];
+                    v[nb][4 * d + e] = REP ? *lds_word(__builtin_amdgcn_perm(r, lsel, 0x0c0c0400u + 0x100u * e))
+                                           : L[(r >> (8 * e)) & 255];
                 }
         }
 #pragma unroll
@@ -470,42 +359,24 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
             bh[nb] = (v4i){oh[0], oh[1], oh[2], oh[3]};
         }
     };
-    auto build = [&]() { build_with(sel); };
-    // A goes from global memory straight into an LDS buffer (buffer loads to LDS, no registers): a
+    // A' goes from global memory straight into an LDS buffer (buffer loads to LDS, no registers): a
     // step's 768 16-byte words, word w by thread w % 512, a wave 64 consecutive ones (the words
     // from 512 on by waves 0-3).
-    auto dma_a_from = [&](const __amdgpu_buffer_rsrc_t &rs, uint32_t kb, int buf) {
+    auto dma_a = [&](uint32_t kb, int buf) {
 #pragma unroll
         for (int i = 0; i < 2; i++)
-            if (i == 0 || wave_u < 4)
+            if (i == 0 || WG == 0)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rs, (__attribute__((address_space(3))) void *)(abuf + buf * A_LDS + (64 * wave_u + GEMM_THREADS * i) * 16),
-                    16, (tid + GEMM_THREADS * i) * 16, (g.kb0 + kb) * g.a_kb_stride, 0, 0);
+                    ars, (__attribute__((address_space(3))) void *)(abuf + buf * A_LDS + (64 * wave_u + GEMM_THREADS * i) * 16),
+                    16, (tid + GEMM_THREADS * i) * 16, (kb0 + kb) * g.a_kb_stride, 0, 0);
     };
-    auto dma_a = [&](uint32_t kb, int buf) { dma_a_from(ars, kb, buf); };
-    // this lane's A pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
+    // this lane's A' pieces in a staged step ([kh][digit][mb 3][lane 64][16]): rows 16 mb16 + c16
     // (mb16 < 5, rows 0..79) and, for the sixth block, rows 80 and 81: lane c16 < 4 holds digit
-    // c16 & 1 of row 80 + (c16 >> 1); the other lanes read row 82, which every tile's A keeps zero
+    // c16 & 1 of row 80 + (c16 >> 1); the other lanes read row 82, which every tile's A' keeps zero
     // (a tile uses at most 82 rows)
     const int aoff = (kh * 2 * NBLK * 64 + off32) * 16;
     const int xdig = c16 & 1, xrow = c16 < 4 ? (c16 >> 1) & 1 : 2;
     const int xoff = ((kh * 2 + xdig) * NBLK * 64 + 2 * 64 + 16 + xrow + 32 * (kg & 1)) * 16;
-
-    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the sixth block's [nb][B digit];
-    // the first step's products start them from zero (an inline constant, no clearing moves)
-    v4i acc[5][2][3], xacc[2][2];
-
-    const uint32_t kbs = g.kbn;  // steps of this launch, counted from kb0
-    // The two waves of a SIMD (w and w + 4) run each step's halves, the products of step kb and
-    // the lookups of step kb + 1, in opposite phase: waves 0-3 meet the step barrier between the
-    // two, waves 4-7 before the products, so one wave's lookups run while the other's products
-    // are in the MFMA pipe. A is then read for two steps at once (waves 0-3 one step ahead) and
-    // is staged in three buffers; the step that a thread stages is kb + 1 (waves 0-3) or kb + 2
-    // (waves 4-7), both into the buffer that no wave reads in that interval. A wave issues its A
-    // copy and the next step's input bytes just before its products, which read no LDS, so both
-    // land while the products run; the compiler waits for them before the next LDS access.
-    // The next step's A pieces (both digits, and the sixth block's) are read ahead, at the end of
-    // the step before.
     // ab is the staged step's LDS byte address (block-uniform, the kernel's LDS starting at 0):
     // a lane's two base addresses cost one add each, the pieces' offsets are immediates
     auto read_a0 = [&](uint32_t ab) {
@@ -519,28 +390,44 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
             a1[mb] = *(const __attribute__((address_space(3))) v4i *)(pa + NBLK * FRAG + ((mb >> 1) * 64 + 16 * (mb & 1)) * 16);
         ax = *(const __attribute__((address_space(3))) v4i *)((lds_u8 *)(uintptr_t)(ab + (uint32_t)xoff));
     };
-    // Set-up: the LUT, A(0), waves 4-7's part of A(1), the bytes of step 0, then A(0)'s pieces and
-    // step 0's operands. The block's item before did all of it during its epilogue (ready).
-    if (!ready) {
-        dma_a(0, 0);
-        load_raw(0);
-        if (wgrp) dma_a(kbs > 1 ? 1 : 0, 1);
-        __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
-        __syncthreads();                // the LUT, A(0) and waves 4-7's part of A(1)
-        read_a0(ABUF_OFF);
-        build();
-    }
-    // step kb; FIRST: the step that starts the accumulators
+
+    // accumulators: [mb16 5][nb 2][class 3] 16 x 16 blocks, and the sixth block's [nb][B digit];
+    // the first step's products start them from zero (an inline constant, no clearing moves)
+    v4i acc[5][2][3], xacc[2][2];
+
+    // Set-up: the bytes of step 0, A'(0 .. A_PRE) and waves 4-7's part of A'(A_PRE + 1), then (after
+    // the barrier that publishes them and the LUT) A'(0)'s pieces and step 0's operands.
+    load_raw(0);
+#pragma unroll
+    for (int st = 0; st <= A_PRE + 1; st++)
+        if (st <= A_PRE || wgrp) dma_a(min((uint32_t)st, kbs - 1), st);
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
+    __syncthreads();                // the LUT, A'(0) and waves 4-7's part of A'(1)
+    read_a0(ABUF_OFF);
+    build(0);
+
+    // The two waves of a SIMD (w and w + 4) run each step's halves, the products of step kb and
+    // the lookups of step kb + 1, in opposite phase: waves 0-3 meet the step barrier between the
+    // two, waves 4-7 before the products, so one wave's lookups run while the other's products
+    // are in the MFMA pipe. Barrier kb publishes A'(kb + 1). A thread stages step
+    // kb + 1 + A_PRE (waves 0-3) or kb + 2 + A_PRE (waves 4-7) into the buffer of step kb - 2
+    // (waves 0-3) or kb - 1 (waves 4-7), which every wave has read: with three buffers and no
+    // A_PRE that is the opposite-phase schedule's only free buffer, and A_PRE more buffers move
+    // each copy A_PRE steps earlier. A wave issues the next step's input bytes and then its A'
+    // copy just before its products, which read no LDS; the bytes (the older loads) are waited for
+    // before the lookups, the copy only before the barrier that publishes it, A_PRE steps later.
+    // The next step's A' pieces (both digits, and the sixth block's) are read ahead, at the end of
+    // the step before.
     auto step = [&](uint32_t kb, auto first) {
         constexpr bool FIRST = decltype(first)::value;
         const v4i z = {0, 0, 0, 0};
         auto c = [&](const v4i &x) -> v4i { return FIRST ? z : x; };
-        if (wgrp) {
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // this wave's copy of A(kb + 1) is in LDS
-            __syncthreads();
-        }
-        dma_a(min(kb + 1 + wgrp, kbs - 1), (int)((kb + 1 + wgrp) % 3));
+        // Per step a thread of waves 0-3 issues 2 byte loads and 2 copies, one of waves 4-7 2 and 1:
+        // the copy of A'(kb + 1) has 4 A_PRE (waves 0-3, issued in step kb - A_PRE) or 3 A_PRE
+        // (waves 4-7, step kb - 1 - A_PRE) loads after it.
+        if (wgrp) barrier_vm<3 * A_PRE>();  // this wave's copy of A'(kb + 1) is in LDS
         load_raw(min(kb + 1, kbs - 1));
+        dma_a(min(kb + 1 + wgrp + A_PRE, kbs - 1), (int)((kb + 1 + wgrp + A_PRE) % A_BUFS));
         // the wave issuing products goes first at the SIMD's arbiter, the other wave's lookups
         // fill in around them (the wave's own priority; the stream's stays the default)
         __builtin_amdgcn_s_setprio(1);
@@ -574,109 +461,135 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
             for (int nb = 0; nb < 2; nb++)
                 acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
         __builtin_amdgcn_s_setprio(0);
-        if (!wgrp) {
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // this wave's copy of A(kb + 1) is in LDS
-            __syncthreads();
-        }
-        build();                                 // step kb + 1 (its bytes were loaded before the products)
-        read_a0(ABUF_OFF + ((kb + 1) % 3) * A_LDS);  // A(kb + 1) is complete: both phases are past barrier kb
+        if (!wgrp) barrier_vm<4 * A_PRE>();  // this wave's copy of A'(kb + 1) is in LDS
+        build(kb + 1);                                    // its bytes were loaded before the products
+        read_a0(ABUF_OFF + ((kb + 1) % A_BUFS) * A_LDS);  // A'(kb + 1) is complete: both phases are past barrier kb
     };
     step(0, std::true_type{});
-    __builtin_amdgcn_sched_barrier(0);
-    // the next item's tile descriptor, read after step 0's barrier: a barrier waits for every load
-    // a wave has issued, and the next one is a step away
-    const TileDesc tdn = g.tiles[nx.has ? nx.tile : tile];
+    __builtin_amdgcn_sched_barrier(0);  // the peeled first step stays apart from the loop
     for (uint32_t kb = 1; kb < kbs; kb++) step(kb, std::false_type{});
     __builtin_amdgcn_s_waitcnt(0);  // the last (clamped) copies into LDS are complete
-    __syncthreads();  // every wave is past its last read of the A buffers and of the LUT
-    tdu = uniform_desc(tdn);  // its loads are complete
-    // The block's next item, set up now (the epilogue uses neither the LUT, the A buffers nor the
-    // operand registers): its LUT when the tile changes, A(0) into buffer 0, waves 4-7's part of
-    // A(1) into buffer 1, the bytes of step 0; after the epilogue's barrier, A(0)'s pieces and
-    // step 0's operands.
-    if (nx.has) {
-        if (nx.tile != tile) set_lut(tdu);
-        const __amdgpu_buffer_rsrc_t nars = uniform_rsrc(g.afrag + tdu.a_off, g.kbs * g.a_kb_stride);
-        dma_a_from(nars, 0, 0);
-        if (wgrp) dma_a_from(nars, kbs > 1 ? 1 : 0, 1);
-        load_raw_from(bytes_rsrc(nx.grp), 0);
-    }
 
-    // epilogue 1: Z = ll + 2^7 mid + 2^14 hh mod q, centred (int16), into zp[b][m][n] for this lane's inputs
-    // j = 81 b + n; the D block of a 16 x 16 product holds, in register r, row 4 kg + r and
-    // column c16. In the sixth block, rows 0..3 (lanes of K group 0) are row 80's and row 81's
-    // digit products: with B's low digit [l80, h80, l81, h81] x l, with its high digit x h.
+    // The epilogue: each of the lane's products ll + 2^7 mid + 2^14 hh mod q, as a residue in
+    // [0, q), to its output word. The D block of a 16 x 16 product holds, in register r, row
+    // 4 kg + r and column c16. In the sixth block, rows 0..3 (lanes of K group 0) are row 80's and
+    // row 81's digit products: with B's low digit [l80, h80, l81, h81] x l, with its high digit x h.
     const uint32_t q = td.q;
-    const double qd = (double)q, qinv = 1.0 / qd;
-    auto epilogue1 = [&](auto reduce) {
+    const double qd = (double)q, qinv = 1.0 / qd, k0 = M_MAGIC * qinv;
+    auto epilogue = [&](auto reduce) {
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) {
-            const int j = jl + 16 * nb;
-            if (j >= GROUP_BATCH * TILE_USED) continue;
-            uint16_t *zj = zp + (j / TILE_USED) * TILE_ROWS * EP + j % TILE_USED;
+            const uint32_t batch = grp * GROUP_COLS + jl + 16 * nb;
+            if (batch >= g.batches) continue;
+            uint32_t *yb = g.y + (size_t)batch * g.y_batch + td.y_off;
+            auto put = [&](int s, int r) {
+                if (s < 0) return;  // a padding row
+                const uint32_t u = (uint32_t)(r < 0 ? r + (int)q : r);
+                if (g.splits == 1)  // block-uniform
+                    yb[s] = u;
+                else
+                    atomicAdd(yb + s, u);
+            };
 #pragma unroll
             for (int mb = 0; mb < 5; mb++)
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int m = 16 * mb + 4 * kg + rr;
-                    zj[m * EP] = (uint16_t)reduce(acc[mb][nb][0][rr], acc[mb][nb][1][rr], acc[mb][nb][2][rr]);
-                }
+                for (int rr = 0; rr < 4; rr++)
+                    put(outs[16 * mb + 4 * kg + rr], reduce(acc[mb][nb][0][rr], acc[mb][nb][1][rr], acc[mb][nb][2][rr]));
             if (kg == 0) {
                 const v4i xl = xacc[nb][0], xh = xacc[nb][1];
-                zj[80 * EP] = (uint16_t)reduce(xl[0], xh[0] + xl[1], xh[1]);
-                zj[81 * EP] = (uint16_t)reduce(xl[2], xh[2] + xl[3], xh[3]);
+                put(outs[80], reduce(xl[0], xh[0] + xl[1], xh[1]));
+                put(outs[81], reduce(xl[2], xh[2] + xl[3], xh[3]));
             }
         }
     };
-    const double k0 = M_MAGIC * qinv;
     if (q < UNSIGNED_DIGITS_Q && kbs <= KB_U32)  // block-uniform
-        epilogue1([&](int ll, int mid, int hh) { return mod_q_short(ll, mid, hh, qd, qinv, k0); });
+        epilogue([&](int ll, int mid, int hh) { return mod_q_short(ll, mid, hh, qd, qinv, k0); });
     else if (q < UNSIGNED_DIGITS_Q)
-        epilogue1([&](int ll, int mid, int hh) { return mod_q_unsigned(ll, mid, hh, qd, qinv, k0); });
+        epilogue([&](int ll, int mid, int hh) { return mod_q_unsigned(ll, mid, hh, qd, qinv, k0); });
     else
-        epilogue1([&](int ll, int mid, int hh) { return mod_q_classes(ll, mid, hh, qd, qinv); });
-    __builtin_amdgcn_s_waitcnt(0);  // this thread's copies for the next item are complete
-    __syncthreads();  // Z, and the next item's LUT and A copies
-    if (nx.has) {
-        read_a0(ABUF_OFF);
-        build_with(sel_of(tdu));
-    }
-    // epilogue 2: y[b][out[m]] = sum_n E[m][n] Z[81 b + n][m], two threads an output (n 0..39 and
-    // 40..87; E is zero from n = 81), reading four pairs of n at a time (16 bytes), each pair one
-    // v_dot2_i32_i16: 0 <= E < q and |Z| <= (q + 1) / 2 with q <= 19441, so four pairs (eight
-    // products) stay within an int32 before it is added to the int64 sum
-    {
-        const int o = tid >> 1, half = tid & 1;
-        const int b = o / TILE_ROWS, m = o % TILE_ROWS;
-        static_assert(EP * 2 == 11 * 16, "a row of E or Z is eleven 16-byte words");
-        const uint4 *ep = (const uint4 *)(ee + m * EP) + 5 * half;
-        const uint4 *zq = (const uint4 *)(zp + (b * TILE_ROWS + m) * EP) + 5 * half;
-        static_assert(8ll * (19441 - 1) * ((19441 + 1) / 2) < (1ll << 31), "four pairs fit an int32 (q <= 19441)");
-        long long sum = 0;
-        if (tid < 2 * GROUP_BATCH * TILE_ROWS) {
+        epilogue([&](int ll, int mid, int hh) { return mod_q_classes(ll, mid, hh, qd, qinv); });
+}
+
+// REP: the LUT is replicated 32 times (4-byte entries, copy = lane & 31) so that the 32 lanes of a
+// half-wave read distinct banks; its set-up (eight 16-byte stores a thread) pays from 8 K steps.
+template <bool REP>
+__global__ __launch_bounds__(GEMM_THREADS, 1) void tile_gemm_kernel(GemmArgs g) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    uint32_t tile, cb;
+    if (!gemm_item_of(g, blockIdx.x, tile, cb)) return;  // block-uniform
+    const TileDesc td = uniform_desc(g.tiles[tile]);
+    const uint32_t grp = cb % g.groups, kb0 = (cb / g.groups) * g.kps;
+    const uint32_t kbs = min(g.kps, g.kbs - kb0);  // this item's steps kb0 .. kb0 + kbs - 1
+    const int tid = threadIdx.x;
+    if (tid < TILE) ((int16_t *)(smem + OUT_OFF))[tid] = g.out[td.out_off + tid];
+    // the tile's LUT into LDS: entry (parity tid >> 8, byte tid & 255), replicated or plain
+    const uint32_t lv = g.lut[td.lut_off + tid];
+    if (REP) {
+        // its 32 copies: eight 16-byte stores of one value
+        const uint4 v4 = make_uint4(lv, lv, lv, lv);
+        uint8_t *row = smem + LUTR_OFF + (tid & 255) * 256 + (tid >> 8) * 128;
 #pragma unroll
-            for (int c = 0; c < 6; c++) {
-                if (c == 5 && !half) break;  // half 0 has five words, half 1 six
-                const uint4 e = ep[c], z = zq[c];
-                int s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.x), __builtin_bit_cast(s16x2, z.x), 0, false);
-                s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.y), __builtin_bit_cast(s16x2, z.y), s4, false);
-                s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.z), __builtin_bit_cast(s16x2, z.z), s4, false);
-                s4 = __builtin_amdgcn_sdot2(__builtin_bit_cast(s16x2, e.w), __builtin_bit_cast(s16x2, z.w), s4, false);
-                sum += s4;
-            }
-        }
-        sum += __shfl_xor(sum, 1, 64);
-        const uint32_t batch = grp * GROUP_BATCH + b;
-        if (half || slot < 0 || batch >= g.batches) return;  // slot < 0 for tid >= 492 too
-        const uint32_t v = mod_q_i64(sum, qd, qinv);  // |sum| < 81 q^2 / 2 < 2^34
-        uint32_t *yp = g.y + (size_t)batch * g.y_batch + td.y_off + slot;
-        if (g.add) {
-            const uint32_t a = *yp + v;  // both in [0, q)
-            *yp = a >= q ? a - q : a;
-        } else {
-            *yp = v;
-        }
+        for (int c = 0; c < 8; c++) *(uint4 *)(row + 16 * c) = v4;
+    } else {
+        ((uint16_t *)(smem + LUT_OFF))[tid] = (uint16_t)lv;
     }
+    if (__builtin_amdgcn_readfirstlane(tid >> 8))  // waves 4-7
+        gemm_item<REP, 1>(g, td, grp, kb0, kbs, smem);
+    else
+        gemm_item<REP, 0>(g, td, grp, kb0, kbs, smem);
+}
+
+// A' fragments (device.hpp): step kb' = n 2^run_shift + kb of a tile holds
+// E[m][n] A[64 kb + k][m] mod q as two digits, from the tile's A fragments (src_steps steps of
+// the same layout, src_tile bytes a tile; zero from there on) and its E. A thread per
+// (step, kh, mb, lane) of a tile: 16 values, both digits.
+constexpr int AF_THREADS = 2 * NBLK * 64;  // 384
+__device__ __forceinline__ int digits_of(uint32_t r, uint32_t q, int &hi) {
+    if (q < UNSIGNED_DIGITS_Q) {
+        hi = (int)(r >> 7);
+        return (int)(r & 127);
+    }
+    const int c = r > (q - 1) / 2 ? (int)r - (int)q : (int)r;
+    const int lo = ((c + 64) & 127) - 64;
+    hi = (c - lo) >> 7;  // exact: c - lo is a multiple of 128
+    return lo;
+}
+__global__ __launch_bounds__(AF_THREADS) void afold_kernel(const int8_t *__restrict__ src, uint64_t src_tile,
+                                                           uint32_t src_steps, const TileDesc *__restrict__ tiles,
+                                                           const uint16_t *__restrict__ ee, uint32_t run_shift,
+                                                           int8_t *__restrict__ dst) {
+    const uint32_t kbp = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+    const uint32_t kh = tid / (NBLK * 64), mb = (tid / 64) % NBLK, lane = tid % 64;
+    const uint32_t n = kbp >> run_shift, kb = kbp & ((1u << run_shift) - 1);
+    const TileDesc td = tiles[t];
+    const uint32_t q = td.q, m = 32 * mb + (lane & 31);
+    const uint32_t e = m < TILE_ROWS ? ee[td.ee_off + m * EP + n] : 0;
+    const size_t piece = (size_t)((kh * 2) * NBLK + mb) * FRAG + lane * 16;
+    uint32_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+    if (kb < src_steps && e != 0) {
+        const int8_t *s = src + t * src_tile + (size_t)kb * A_STEP + piece;
+        const uint4 sl = *(const uint4 *)s, sh = *(const uint4 *)(s + NBLK * FRAG);
+        const uint32_t wl[4] = {sl.x, sl.y, sl.z, sl.w}, wh[4] = {sh.x, sh.y, sh.z, sh.w};
+        const float qf = 1.0f / (float)q;
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int xl = (int8_t)(wl[w] >> (8 * b)), xh = (int8_t)(wh[w] >> (8 * b));
+                // |x| < q and e < q (q <= 19441): |x e| < 2^29, and the float quotient is within
+                // q 2^-23 of p / q, so the remainder is within one q of [0, q)
+                const int p = (xl + 128 * xh) * (int)e;
+                int r = p - (int)__builtin_rintf((float)p * qf) * (int)q;
+                r += r < 0 ? (int)q : 0;
+                int dh;
+                const int dl = digits_of((uint32_t)r, q, dh);
+                lo[w] |= (uint32_t)(uint8_t)dl << (8 * b);
+                hi[w] |= (uint32_t)(uint8_t)dh << (8 * b);
+            }
+    }
+    int8_t *o = dst + td.a_off + (size_t)kbp * A_STEP + piece;
+    *(uint4 *)o = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    *(uint4 *)(o + NBLK * FRAG) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
 }
 
 // =============================================================================================
@@ -836,7 +749,8 @@ __global__ __launch_bounds__(256) void intt_kernel(const uint32_t *__restrict__ 
     const uint32_t i = blockIdx.x;
     const uint32_t q = ic.q;
     const float qf = ic.qinv;
-    for (int x = threadIdx.x; x < 648; x += 256) u[x] = vhat[(size_t)i * 648 + x];  // in [0, q)
+    // the fold GEMM's K ranges each added a residue in [0, q)
+    for (int x = threadIdx.x; x < 648; x += 256) u[x] = vhat[(size_t)i * 648 + x] % q;
     for (uint32_t x = threadIdx.x; x < ic.twi_count; x += 256) tw[x] = twi[x];
     __syncthreads();
     const int radix_s[7] = {2, 2, 2, 3, 3, 3, 3};
@@ -2254,47 +2168,71 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v) {
 }
 
 hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *const *cols, uint32_t column_len,
-                          uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
-                          uint32_t kbs_f) {
+                          uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kpad_c, uint8_t *bytes_f,
+                          uint32_t kpad_f) {
     dim3 grid((r + PRE_C - 1) / PRE_C, (L + PRE_I - 1) / PRE_I);
-    LBN_LAUNCH(prepass_kernel, grid, dim3(PRE_I * PRE_C), 0, s, w, cols, column_len, L, r, bytes_c, kbs_c, bytes_f,
-               kbs_f);
+    LBN_LAUNCH(prepass_kernel, grid, dim3(PRE_I * PRE_C), 0, s, w, cols, column_len, L, r, bytes_c, kpad_c, bytes_f,
+               kpad_f);
     return hipSuccess;
 }
 
 uint32_t gemm_kb_chunk = 0;
 
-hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks) {
+// The K ranges of a launch: at least the exact minimum (and the caller's g.splits), then the
+// fewest that minimise a full grid's makespan on 256 CUs, ceil(items / 256) rounds of
+// ceil(kbs / splits) steps.
+static uint32_t gemm_splits(const GemmArgs &g) {
+    const uint32_t cap = gemm_kb_chunk ? gemm_kb_chunk : KB_EXACT;
+    const uint32_t s_min = std::max((g.kbs + cap - 1) / cap, std::max(g.splits, 1u));
+    const uint64_t base = (uint64_t)g.ntiles * g.groups;
+    const uint32_t s_max = std::max(s_min, std::min<uint32_t>(g.kbs, s_min + 2 * (uint32_t)((512 + base - 1) / base)));
+    uint32_t best = s_min;
+    uint64_t best_cost = ~0ull;
+    for (uint32_t sp = s_min; sp <= s_max; sp++) {
+        const uint64_t cost = ((base * sp + 255) / 256) * ((g.kbs + sp - 1) / sp);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = sp;
+        }
+    }
+    return best;
+}
+
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t *splits_out) {
     static const hipError_t attr0 = hipFuncSetAttribute((const void *)tile_gemm_kernel<false>,
                                                         hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
     static const hipError_t attr1 = hipFuncSetAttribute((const void *)tile_gemm_kernel<true>,
                                                         hipFuncAttributeMaxDynamicSharedMemorySize, GEMM_LDS);
     if (attr0 != hipSuccess) return attr0;
     if (attr1 != hipSuccess) return attr1;
-    if (g.groups == 0 || g.ntiles == 0) return hipSuccess;
+    if (g.groups == 0 || g.ntiles == 0 || g.kbs == 0) return hipSuccess;
     GemmArgs a = g;
-    if (a.sched_t == 0 || a.sched_t > 32 || (a.sched_t & (a.sched_t - 1))) a.sched_t = 8;
-    a.items = gemm_items(a.ntiles, a.groups, a.sched_t);
-    a.t_shift = (uint32_t)__builtin_ctz(a.sched_t);
-    a.tgroups = (a.ntiles + a.sched_t - 1) / a.sched_t;
-    a.gblocks = (a.groups + 256 / a.sched_t - 1) / (256 / a.sched_t);
-    // n / d = (n * ceil(2^32 / d)) >> 32 exactly while n * d < 2^32 (n < the rounds, tgroups x gblocks)
-    const uint64_t rounds = (uint64_t)a.tgroups * a.gblocks;
-    auto magic = [&](uint32_t d) { return rounds * d < (1ull << 32) ? ((1ull << 32) + d - 1) / d : 0ull; };
-    a.tg_magic = magic(a.tgroups);
-    a.gb_magic = magic(a.gblocks);
-    uint32_t grid = a.items;
-    if (persistent_blocks && persistent_blocks < grid) grid = (persistent_blocks + 7) / 8 * 8;
-    // K in chunks the int32 accumulators hold exactly; the later chunks add mod q
-    const uint32_t chunk = gemm_kb_chunk ? gemm_kb_chunk : KB_EXACT;
-    for (a.kb0 = 0; a.kb0 < g.kbs; a.kb0 += chunk) {
-        a.kbn = std::min(chunk, g.kbs - a.kb0);
-        a.add = a.kb0 > 0;
-        if (a.kbn >= 8)
-            LBN_LAUNCH(tile_gemm_kernel<true>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
-        else
-            LBN_LAUNCH(tile_gemm_kernel<false>, dim3(grid), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    if (a.sched_t < 4 || a.sched_t > 32 || (a.sched_t & (a.sched_t - 1))) a.sched_t = 8;
+    const uint32_t sp = gemm_splits(a);
+    a.kps = (a.kbs + sp - 1) / sp;
+    a.splits = (a.kbs + a.kps - 1) / a.kps;  // no empty range
+    a.cblocks = a.groups * a.splits;
+    a.tsets = (a.ntiles + a.sched_t - 1) / a.sched_t;
+    a.csets = (a.cblocks + 32 / a.sched_t - 1) / (32 / a.sched_t);
+    a.items = gemm_items(a.tsets, a.csets);
+    if (splits_out) *splits_out = a.splits;
+    // several K ranges add their residues into y, which starts from zero
+    if (a.splits > 1) {
+        const hipError_t e = launch_fill_u32(s, a.y, (size_t)a.batches * a.y_batch, 0);
+        if (e != hipSuccess) return e;
     }
+    if (a.kps >= 8)
+        LBN_LAUNCH(tile_gemm_kernel<true>, dim3(a.items), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    else
+        LBN_LAUNCH(tile_gemm_kernel<false>, dim3(a.items), dim3(GEMM_THREADS), GEMM_LDS, s, a);
+    return hipSuccess;
+}
+
+hipError_t launch_afold(hipStream_t s, const int8_t *src, uint64_t src_tile, uint32_t src_steps,
+                        const TileDesc *tiles, uint32_t ntiles, const uint16_t *ee, uint32_t kpad, int8_t *dst) {
+    if (!ntiles || kpad < KSTEP || (kpad & (kpad - 1))) return ntiles ? hipErrorInvalidValue : hipSuccess;
+    LBN_LAUNCH(afold_kernel, dim3(gemm_kbs(kpad), ntiles), dim3(AF_THREADS), 0, s, src, src_tile, src_steps, tiles, ee,
+               (uint32_t)__builtin_ctz(kpad / KSTEP), dst);
     return hipSuccess;
 }
 

@@ -22,13 +22,19 @@ hipError_t launch_mfma_f4_probe(hipStream_t s, const uint32_t *A, const uint32_t
 hipError_t launch_mfma_f4_rate(hipStream_t s, int random, uint32_t blocks, uint32_t iters, float *sink);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 // w + c * column_len (cols == NULL) or cols[c] is column c; either output may be NULL
+// the byte layouts for K index lengths padded to kpad_c (ring elements) and kpad_f (columns)
 hipError_t launch_prepass(hipStream_t s, const uint64_t *w, const uint64_t *const *cols, uint32_t column_len,
-                          uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kbs_c, uint8_t *bytes_f,
-                          uint32_t kbs_f);
-// persistent_blocks == 0: one block per work item; otherwise that many blocks (rounded up to 8) loop.
-// K runs in chunks of gemm_kb_chunk steps (0: KB_EXACT), one launch each.
-hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t persistent_blocks);
+                          uint32_t L, uint32_t r, uint8_t *bytes_c, uint32_t kpad_c, uint8_t *bytes_f,
+                          uint32_t kpad_f);
+// One block per work item; K in ranges of at most gemm_kb_chunk steps (0: KB_EXACT), at least
+// g.splits of them (0: no minimum), whose count goes to *splits (may be NULL). With several, y
+// (batches x y_batch words) is cleared first and the ranges add into it.
+hipError_t launch_tile_gemm(hipStream_t s, const GemmArgs &g, uint32_t *splits);
 extern uint32_t gemm_kb_chunk;  // tests only (lbn_debug_gemm_chunk)
+// The A' fragments of ntiles tiles (device.hpp; tiles[t].a_off into dst, E at tiles[t].ee_off) from
+// their A fragments, src_steps steps a tile, src_tile bytes apart; kpad a power of two >= 64.
+hipError_t launch_afold(hipStream_t s, const int8_t *src, uint64_t src_tile, uint32_t src_steps,
+                        const TileDesc *tiles, uint32_t ntiles, const uint16_t *ee, uint32_t kpad, int8_t *dst);
 hipError_t launch_commit_finish(hipStream_t s, const uint32_t *yacc, uint32_t limbs,
                                 const FinishLimb *fl, const uint16_t *slot648, const uint16_t *qcs,
                                 uint32_t *raw, int16_t *comp, uint32_t cols);

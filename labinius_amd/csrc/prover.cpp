@@ -176,8 +176,8 @@ static void pp_free(lbn_pp *pp) {
 
 // Everything the GEMMs read, built on the host.
 struct HostTables {
-    uint32_t lpad = 0, rpad = 0, kb_c = 0, kb_f = 0;
-    std::vector<int8_t> afrag;
+    uint32_t lpad = 0, rpad = 0, kpad_c = 0, kpad_f = 0, kb_c = 0, kb_f = 0;
+    std::vector<int8_t> afrag;  // the commit tiles' A fragments, lpad / 64 steps a tile
     std::vector<uint16_t> lut, ee;
     std::vector<int16_t> out;
     std::vector<TileDesc> desc;
@@ -215,8 +215,10 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     const size_t limbs = s.primes.size();
     H.lpad = (uint32_t)round_up(s.len_ring, KSTEP);
     H.rpad = (uint32_t)round_up(s.r, KSTEP);
-    H.kb_c = H.lpad / KSTEP;
-    H.kb_f = H.rpad / KSTEP;
+    H.kpad_c = gemm_kpad(s.len_ring);
+    H.kpad_f = gemm_kpad(s.r);
+    H.kb_c = gemm_kbs(H.kpad_c);
+    H.kb_f = gemm_kbs(H.kpad_f);
 
     // commit tiles
     std::vector<HostTile> tiles;
@@ -237,7 +239,7 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
             }
         }
     }
-    const size_t a_tile = (size_t)H.kb_c * A_STEP;
+    const size_t a_tile = (size_t)(H.lpad / KSTEP) * A_STEP;  // A; A' has kb_c steps a tile
     const size_t ee_tile = (size_t)EE_TILE;
     H.afrag.assign(a_tile * tiles.size(), 0);
     H.lut.assign(512 * tiles.size(), 0);
@@ -246,12 +248,12 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     H.desc.resize(tiles.size());
     for (size_t ti = 0; ti < tiles.size(); ti++) {
         const uint32_t k = tile_limb[ti], q = s.primes[k];
-        pack_key(tiles[ti], key.data() + (size_t)k * s.len_ring * N, s.len_ring, H.kb_c, q,
+        pack_key(tiles[ti], key.data() + (size_t)k * s.len_ring * N, s.len_ring, H.lpad / KSTEP, q,
                  H.afrag.data() + ti * a_tile);
         std::memcpy(H.lut.data() + 512 * ti, tiles[ti].lut, 1024);
         pack_ee(tiles[ti], H.ee.data() + ee_tile * ti);
         for (int m = 0; m < TILE; m++) H.out[TILE * ti + m] = (int16_t)tiles[ti].out[m];
-        H.desc[ti] = TileDesc{ti * a_tile, (uint32_t)(512 * ti), (uint32_t)(ee_tile * ti), (uint32_t)(TILE * ti),
+        H.desc[ti] = TileDesc{ti * H.kb_c * A_STEP, (uint32_t)(512 * ti), (uint32_t)(ee_tile * ti), (uint32_t)(TILE * ti),
                               q, tiles[ti].par, (uint32_t)((YACC_PLANES * k + tiles[ti].par) * N)};
     }
 
@@ -290,7 +292,7 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     } else {
         for (int beta = 0; beta < 8; beta++) ft.push_back(split_tile(B, beta));
     }
-    const size_t af_tile = (size_t)H.kb_f * A_STEP;
+    const size_t af_tile = (size_t)H.kb_f * A_STEP;  // A' (the challenges' transform has rpad / 64 steps)
     H.lut_f.assign(512 * ft.size(), 0);
     H.ee_f.assign(ee_tile * ft.size(), 0);
     H.expo_f.assign(TILE * ft.size(), 0);
@@ -318,6 +320,25 @@ static void build_host_tables(const Shape &s, const std::vector<int16_t> &key, H
     H.twi = H.twi_limb[0];
 }
 
+// The commit tiles' A' (E folded into A, device.hpp), resident: built on the device from the
+// A fragments, which are then freed.
+static int build_afrag(lbn_pp *pp, const HostTables &H) {
+    const size_t steps = H.lpad / KSTEP;
+    int8_t *src = nullptr;
+    HIP_TRY(hipMalloc((void **)&src, std::max<size_t>(1, H.afrag.size())));
+    hipError_t e = hipMemcpy(src, H.afrag.data(), H.afrag.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)&pp->d_afrag, (size_t)pp->ntiles * pp->kb_c * A_STEP);
+    if (e == hipSuccess) {
+        pp->allocations.push_back(pp->d_afrag);
+        e = launch_afold(nullptr, src, steps * A_STEP, (uint32_t)steps, pp->d_tiles, pp->ntiles, pp->d_ee, pp->kpad_c,
+                         pp->d_afrag);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    (void)hipFree(src);
+    HIP_TRY(e);
+    return LBN_OK;
+}
+
 static int build_device(lbn_pp *pp) {
     HostTables H;
     build_host_tables(pp->shape, pp->key, H);
@@ -325,6 +346,8 @@ static int build_device(lbn_pp *pp) {
         if (ic.twi_count > INTT_TWI_MAX) return fail(LBN_ERR_HIP, "inverse twiddles exceed the kernel's table");
     pp->lpad = H.lpad;
     pp->rpad = H.rpad;
+    pp->kpad_c = H.kpad_c;
+    pp->kpad_f = H.kpad_f;
     pp->kb_c = H.kb_c;
     pp->kb_f = H.kb_f;
     pp->ntiles = (uint32_t)H.desc.size();
@@ -334,11 +357,11 @@ static int build_device(lbn_pp *pp) {
     pp->intt = H.intt;
     const Tables &t = tables();
     int rc;
-    if ((rc = upload(pp, &pp->d_afrag, H.afrag.data(), H.afrag.size()))) return rc;
     if ((rc = upload(pp, &pp->d_lut, H.lut.data(), H.lut.size()))) return rc;
     if ((rc = upload(pp, &pp->d_ee, H.ee.data(), H.ee.size()))) return rc;
     if ((rc = upload(pp, &pp->d_out, H.out.data(), H.out.size()))) return rc;
     if ((rc = upload(pp, &pp->d_tiles, H.desc.data(), H.desc.size()))) return rc;
+    if ((rc = build_afrag(pp, H))) return rc;
     if ((rc = upload(pp, &pp->d_finish, H.finish.data(), H.finish.size()))) return rc;
     if ((rc = upload(pp, &pp->d_slot648, &t.slot_648[0][0], 4 * N162))) return rc;
     if ((rc = upload(pp, &pp->d_qcs, &t.quad_class_slot[0][0], 2 * N162))) return rc;
@@ -478,7 +501,7 @@ static hipError_t download_host(lbn_prover *pr, hipStream_t st, const HostOut *o
 }
 
 static void prover_free(lbn_prover *pr) {
-    void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_afrag_f, pr->d_vhat,
+    void *bufs[] = {pr->d_bytes_c, pr->d_yacc, pr->d_raw, pr->d_comp, pr->d_ch, pr->d_chat, pr->d_afrag_f, pr->d_vhat,
                     pr->d_v, pr->d_p0, pr->d_eq, pr->d_eqpart, pr->d_eqf, pr->d_u, pr->d_wscratch,
                     pr->d_eqpart1, pr->d_t,
                     pr->d_bd_raw, pr->d_bd_cen, pr->d_bd_out, pr->sw.d_eq, pr->sw.d_a, pr->sw.d_p,
@@ -501,15 +524,15 @@ static void prover_free(lbn_prover *pr) {
     delete pr;
 }
 
-static size_t bytes_f_size(const lbn_pp *pp) { return bytes_size((uint32_t)pp->shape.len_ring, pp->kb_f); }
+static size_t bytes_f_size(const lbn_pp *pp) { return bytes_size((uint32_t)pp->shape.len_ring, pp->kpad_f); }
 
 // The fold's layout buffer: recycled by the prover, or from its pool. Padding columns are never
 // written by the layout pass and must read as zero: a fresh buffer, or a pooled one that last held
-// another layout, is cleared.
+// another layout (another ring element or column count), is cleared.
 static int take_bytes_f(lbn_prover *pr, uint8_t **out) {
     const size_t sz = bytes_f_size(pr->pp);
     if (pr->pool) {
-        const uint64_t tag = ((uint64_t)pr->pp->kb_f << 32) | pr->pp->shape.len_ring;
+        const uint64_t tag = ((uint64_t)pr->pp->shape.r << 32) | pr->pp->shape.len_ring;
         bool same = false;
         void *p = nullptr;
         int rc = pool_alloc(pr->pool, sz, tag, &p, &same);
@@ -554,19 +577,25 @@ struct BytesGuard {
     ~BytesGuard() { give_bytes_f(pr, b); }
 };
 
-// The commit GEMM's schedule: 8 tiles x 4 groups an XCD round, or, with under 32 groups (a
-// few columns), 32 tiles x 1 group, so that the groups spread over the 8 XCDs instead of two.
-static uint32_t commit_sched(uint32_t groups) { return groups < 32 ? 32 : 8; }
-
-// The commit GEMM of a wide layout runs tile-major on a persistent grid of one block a CU: each
-// block keeps one tile (its replicated LUT and its E in LDS) for ceil(groups / 32) items in a row,
-// while the XCD's 8 tiles x 4 groups a round still share A and the groups' bytes in L2.
-constexpr uint32_t COMMIT_BLOCKS = 256;
-static void commit_schedule(GemmArgs &g) {
-    g.sched_t = commit_sched(g.groups);
-    g.tile_major = g.groups >= 64;
+// The commit GEMM over ncol columns of a layout (bytes) into y ([ncol][limb][plane][648]).
+GemmArgs commit_gemm_args(const lbn_pp *pp, const uint8_t *bytes, uint32_t *y, size_t ncol) {
+    GemmArgs g{};
+    g.tiles = pp->d_tiles;
+    g.afrag = pp->d_afrag;
+    g.lut = pp->d_lut;
+    g.out = pp->d_out;
+    g.bytes = bytes;
+    g.kbs = pp->kb_c;
+    g.run_shift = (uint32_t)__builtin_ctz(pp->kpad_c / KSTEP);
+    g.a_kb_stride = A_STEP;
+    g.y = y;
+    g.y_batch = YACC_PLANES * pp->shape.primes.size() * N;
+    g.batches = (uint32_t)ncol;
+    g.groups = (uint32_t)((ncol + GROUP_COLS - 1) / GROUP_COLS);
+    g.ntiles = pp->ntiles;
+    g.sched_t = 8;
+    return g;
 }
-static uint32_t commit_blocks(const GemmArgs &g) { return g.tile_major ? COMMIT_BLOCKS : 0; }
 
 static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uint32_t *raw, lbn_opening **opening) {
     lbn_pp *pp = pr->pp;
@@ -581,24 +610,10 @@ static int commit_device(lbn_prover *pr, const uint64_t *d_w, int16_t *comp, uin
     hipStream_t st = pr->stream;
     HIP_TRY(hipEventRecord(pr->ev[0], st));
     HIP_TRY(launch_prepass(st, d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)s.r,
-                           pr->d_bytes_c, pp->kb_c, bytes_f, pp->kb_f));
+                           pr->d_bytes_c, pp->kpad_c, bytes_f, pp->kpad_f));
     HIP_TRY(hipEventRecord(pr->ev[1], st));
-    GemmArgs g{};
-    g.tiles = pp->d_tiles;
-    g.afrag = pp->d_afrag;
-    g.lut = pp->d_lut;
-    g.ee = pp->d_ee;
-    g.out = pp->d_out;
-    g.bytes = pr->d_bytes_c;
-    g.kbs = pp->kb_c;
-    g.a_kb_stride = A_STEP;
-    g.y = pr->d_yacc;
-    g.y_batch = YACC_PLANES * limbs * N;
-    g.batches = (uint32_t)s.r;
-    g.groups = (uint32_t)((s.r + GROUP_BATCH - 1) / GROUP_BATCH);
-    g.ntiles = pp->ntiles;
-    commit_schedule(g);
-    HIP_TRY(launch_tile_gemm(st, g, commit_blocks(g)));
+    GemmArgs g = commit_gemm_args(pp, pr->d_bytes_c, pr->d_yacc, s.r);
+    HIP_TRY(launch_tile_gemm(st, g, nullptr));
     HIP_TRY(hipEventRecord(pr->ev[2], st));
     HIP_TRY(launch_commit_finish(st, pr->d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, pr->d_raw,
                                  pr->d_comp, (uint32_t)s.r));
@@ -631,33 +646,13 @@ int commit_columns_device(lbn_prover *pr, const uint64_t *d_w, size_t ncol, uint
     const size_t limbs = s.primes.size();
     hipStream_t st = pr->stream;
     HIP_TRY(launch_prepass(st, d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring, (uint32_t)ncol, d_bytes,
-                           pp->kb_c, nullptr, 0));
-    GemmArgs g{};
-    g.tiles = pp->d_tiles;
-    g.afrag = pp->d_afrag;
-    g.lut = pp->d_lut;
-    g.ee = pp->d_ee;
-    g.out = pp->d_out;
-    g.bytes = d_bytes;
-    g.kbs = pp->kb_c;
-    g.a_kb_stride = A_STEP;
-    g.y = d_yacc;
-    g.y_batch = YACC_PLANES * limbs * N;
-    g.batches = (uint32_t)ncol;
-    g.groups = (uint32_t)((ncol + GROUP_BATCH - 1) / GROUP_BATCH);
-    g.ntiles = pp->ntiles;
-    commit_schedule(g);
-    HIP_TRY(launch_tile_gemm(st, g, commit_blocks(g)));
+                           pp->kpad_c, nullptr, 0));
+    GemmArgs g = commit_gemm_args(pp, d_bytes, d_yacc, ncol);
+    HIP_TRY(launch_tile_gemm(st, g, nullptr));
     HIP_TRY(launch_commit_finish(st, d_yacc, (uint32_t)limbs, pp->d_finish, pp->d_slot648, pp->d_qcs, d_raw, d_comp,
                                  (uint32_t)ncol));
     return LBN_OK;
 }
-
-// The fold GEMM runs on a persistent grid of 256 blocks (one a CU): its 8 tiles over 256 blocks
-// give every block one tile for all its items (gemm_item_of with 8 tiles an XCD round), so the
-// block replicates that tile's LUT and loads its E once, where a full grid paid both per item of
-// only ceil(columns / 64) K steps.
-constexpr uint32_t FOLD_BLOCKS = 256;
 
 static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch, size_t r, int16_t *v) {
     lbn_pp *pp = pr->pp;
@@ -667,24 +662,26 @@ static int fold_device(lbn_prover *pr, lbn_opening *op, const lbn_challenge *ch,
     HIP_TRY(hipMemcpyAsync(pr->d_ch, pr->h_ch, r * sizeof(lbn_challenge), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(pr->ev[4], st));
     HIP_TRY(launch_chat(st, (const lbn_challenge_dev *)pr->d_ch, (uint32_t)r, pp->rpad, pp->d_expo_f, pp->d_pw_f,
-                        pp->cond_f, pp->mult_f, s.primes[0], pr->d_afrag_f, pp->ntiles_f));
+                        pp->cond_f, pp->mult_f, s.primes[0], pr->d_chat, pp->ntiles_f));
+    HIP_TRY(launch_afold(st, pr->d_chat, (size_t)(pp->rpad / KSTEP) * A_STEP, pp->rpad / KSTEP, pp->d_tiles_f,
+                         pp->ntiles_f, pp->d_ee_f, pp->kpad_f, pr->d_afrag_f));
     HIP_TRY(hipEventRecord(pr->ev[5], st));
     GemmArgs g{};
     g.tiles = pp->d_tiles_f;
     g.afrag = pr->d_afrag_f;
     g.lut = pp->d_lut_f;
-    g.ee = pp->d_ee_f;
     g.out = pp->d_out_f;
     g.bytes = op->d_bytes_f;
     g.kbs = pp->kb_f;
+    g.run_shift = (uint32_t)__builtin_ctz(pp->kpad_f / KSTEP);
     g.a_kb_stride = A_STEP;
     g.y = pr->d_vhat;
     g.y_batch = N;
     g.batches = (uint32_t)s.len_ring;
-    g.groups = (uint32_t)((s.len_ring + GROUP_BATCH - 1) / GROUP_BATCH);
+    g.groups = (uint32_t)((s.len_ring + GROUP_COLS - 1) / GROUP_COLS);
     g.ntiles = pp->ntiles_f;
     g.sched_t = 8;
-    HIP_TRY(launch_tile_gemm(st, g, FOLD_BLOCKS));
+    HIP_TRY(launch_tile_gemm(st, g, nullptr));
     HIP_TRY(hipEventRecord(pr->ev[6], st));
     HIP_TRY(launch_intt(st, pr->d_vhat, (uint32_t)s.len_ring, pp->intt, pp->d_twi, pr->d_v));
     HIP_TRY(hipEventRecord(pr->ev[7], st));
@@ -866,13 +863,14 @@ int lbn_prover_create(lbn_pp *pp, lbn_prover **out) {
                                                                                    : fail(LBN_ERR_HIP, "stream"));
     for (hipEvent_t &e : pr->ev)
         TRY_RC(hipEventCreate(&e) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "event"));
-    const size_t bc = bytes_size((uint32_t)s.r, pp->kb_c);
+    const size_t bc = bytes_size((uint32_t)s.r, pp->kpad_c);
     TRY_RC(dalloc(&pr->d_bytes_c, bc));
     TRY_RC(hipMemset(pr->d_bytes_c, 0, bc) == hipSuccess ? LBN_OK : fail(LBN_ERR_HIP, "memset"));
     TRY_RC(dalloc(&pr->d_yacc, s.r * YACC_PLANES * limbs * N));
     TRY_RC(dalloc(&pr->d_raw, s.r * limbs * N));
     TRY_RC(dalloc(&pr->d_comp, s.r * 4 * limbs * N162));
     TRY_RC(dalloc(&pr->d_ch, s.r));
+    TRY_RC(dalloc(&pr->d_chat, (size_t)pp->ntiles_f * (pp->rpad / KSTEP) * A_STEP));
     TRY_RC(dalloc(&pr->d_afrag_f, (size_t)pp->ntiles_f * pp->kb_f * A_STEP));
     TRY_RC(dalloc(&pr->d_vhat, s.len_ring * N));
     TRY_RC(dalloc(&pr->d_v, s.len_ring * N));
@@ -999,7 +997,7 @@ int lbn_fold_resident(lbn_prover *pr, const lbn_dwitness *w, const lbn_challenge
     int rc = take_bytes_f(pr, &op.d_bytes_f);
     if (rc) return rc;
     const hipError_t e = launch_prepass(pr->stream, w->d_w, nullptr, (uint32_t)s.column_len, (uint32_t)s.len_ring,
-                                        (uint32_t)s.r, nullptr, 0, op.d_bytes_f, pr->pp->kb_f);
+                                        (uint32_t)s.r, nullptr, 0, op.d_bytes_f, pr->pp->kpad_f);
     rc = e == hipSuccess ? fold_device(pr, &op, ch, r, v) : fail(LBN_ERR_HIP, hipGetErrorString(e));
     give_bytes_f(pr, op.d_bytes_f);
     return rc;
@@ -1112,7 +1110,7 @@ int lbn_debug_mfma_f4_rate(int random_operands, double *tflops) {
     return rc;
 }
 
-// Test-only (include/labinius_debug.h): the GEMM's K chunk in steps, 0 for the exact-range default.
+// Test-only (include/labinius_debug.h): the GEMMs' longest K range in steps, 0 for the exact-range default.
 int lbn_debug_gemm_chunk(uint32_t kb) {
     if (kb > KB_EXACT) return fail(LBN_ERR_ARG, "a K chunk beyond the accumulators' exact range");
     gemm_kb_chunk = kb;

@@ -88,8 +88,7 @@ static lbn_params size_params(const lbn_params &key, uint32_t wl, uint32_t dropp
 }
 
 static uint64_t request_bytes(const Shape &s) {
-    const uint32_t kb_f = (uint32_t)(round_up(s.r, KSTEP) / KSTEP);
-    return pool_rounded(s.n * sizeof(lbn_f162)) + pool_rounded(bytes_size((uint32_t)s.len_ring, kb_f));
+    return pool_rounded(s.n * sizeof(lbn_f162)) + pool_rounded(bytes_size((uint32_t)s.len_ring, gemm_kpad(s.r)));
 }
 
 // The device pp of a size (created on first use; immutable and shared by the workers' provers).

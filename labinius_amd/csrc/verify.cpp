@@ -51,7 +51,7 @@ int verify_alloc(lbn_prover *pr) {
     const size_t limbs = s.primes.size();
     if (limbs > (size_t)GARNER_MAX) return fail(LBN_ERR_SHAPE, "too many moduli");
     int rc;
-    const size_t nbytes = bytes_size((uint32_t)MAX_COLS, pp->kb_c);
+    const size_t nbytes = bytes_size((uint32_t)MAX_COLS, pp->kpad_c);
     if ((rc = dalloc(&w.d_v, s.len_ring * N))) return rc;
     if ((rc = dalloc(&w.d_w, MAX_COLS * s.column_len * 3))) return rc;
     if ((rc = dalloc(&w.d_bytes, nbytes))) return rc;

@@ -91,10 +91,11 @@ def test_commit_fold_eval_parity(lbn, orc, wl, cl, base, extra):
 
 @pytest.mark.parametrize("chunk,wl,cl,extra", [(1, 13, 3, [2917]), (3, 16, 8, [9721]), (8, 18, 2, [2917])])
 def test_gemm_k_chunks(lbn, orc, chunk, wl, cl, extra):
-    """K run as several launches whose later ones add mod q (the path for K past the int32
-    accumulators' exact range, 220 736 ring elements or columns), here with chunks of 1, 3 and 8
-    steps: commitments and folds stay bit-exact. (16, 8) splits the fold's K; (18, 2) the commit's
-    64 steps into 8-step launches on the replicated-LUT path."""
+    """K split into many ranges whose residues are added into the output (the path every K past
+    the int32 accumulators' exact range takes), here ranges of at most 1, 3 and 8 steps:
+    commitments and folds stay bit-exact. (1, 13, 3) runs every step as its own range on the plain
+    LUT; (18, 2) the commit's 20 736 steps (16 384 ring elements) as 8-step ranges on the
+    replicated-LUT path."""
     say(f"case: K chunk {chunk} wl={wl} cl={cl} extra={extra}")
     L = lbn.lib()
     assert L.lbn_debug_gemm_chunk(ctypes.c_uint32(chunk)) == 0

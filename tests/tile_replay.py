@@ -98,8 +98,9 @@ def replay_commit(lbn, params, seed, w):
     y = np.zeros((cols, YACC_PLANES * limbs * N), np.int64)
     written = np.zeros(y.shape, bool)
     a_tile = kbs * A_STEP
-    for d in desc:
-        A = a_matrix(afrag[d["a_off"]:d["a_off"] + a_tile], kbs)
+    for ti, d in enumerate(desc):
+        # the host's A fragments, lpad / 64 steps a tile (a_off places the device's A', E folded in)
+        A = a_matrix(afrag[ti * a_tile:(ti + 1) * a_tile], kbs)
         E = ee_matrix(ee[d["ee_off"]:d["ee_off"] + EE_TILE])
         lt = lut[d["lut_off"]:d["lut_off"] + 512].astype(np.int64)
         o = out[d["out_off"]:d["out_off"] + TILE]
@@ -110,7 +111,7 @@ def replay_commit(lbn, params, seed, w):
             rows = tile_rows(A, X, E, int(d["q"]))
             for m in range(TILE):
                 if o[m] >= 0:
-                    # the GEMM stores without atomics: every word has exactly one writer
+                    # every word has exactly one (tile, row) writer
                     assert not written[c, d["y_off"] + o[m]]
                     written[c, d["y_off"] + o[m]] = True
                     y[c, d["y_off"] + o[m]] = rows[m]
