@@ -98,18 +98,18 @@ static uint32_t width_class(size_t R) {
 // Candidate schedules, timed on this device for this width class: tiles per XCD round (the rest of
 // an XCD's 32 blocks are column blocks) and the launcher's K ranges or twice as many (more, shorter
 // items for a narrow batch). The result is the tiles per round and the ranges asked for.
-static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p) {
+static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_s) {
     lbn_batch_ws *b = pr->batch;
     for (const auto &e : b->tuned)
         if (e[0] == width_class(R)) {
             *best_t = e[1];
-            *best_p = e[2];
+            *best_s = e[2];
             return LBN_OK;
         }
     static const uint32_t ts[] = {4, 8, 16};
     float best = 1e30f;
     *best_t = 8;
-    *best_p = 0;
+    *best_s = 0;
     int rc;
     uint32_t auto_splits = 1;
     if ((rc = batch_launch(pr, R, 8, 0, true, &auto_splits))) return rc;  // the layout, and a warm-up
@@ -124,10 +124,10 @@ static int autotune(lbn_prover *pr, size_t R, uint32_t *best_t, uint32_t *best_p
             if (ms < best) {
                 best = ms;
                 *best_t = t;
-                *best_p = p;
+                *best_s = p;
             }
         }
-    b->tuned.push_back({width_class(R), *best_t, *best_p});
+    b->tuned.push_back({width_class(R), *best_t, *best_s});
     return LBN_OK;
 }
 
@@ -185,7 +185,7 @@ int lbn_commit_batch(lbn_prover *pr, const lbn_dwitness *const *ws, size_t count
         HIP_TRY(hipEventRecord(b->ev[0], st));
         if ((rc = batch_launch(pr, R, t, p, true, &b->splits))) return rc;
         HIP_TRY(hipEventRecord(b->ev[1], st));
-    } else if (!b->graph || b->graph_cols != R || b->graph_t != t || b->graph_persist != p) {
+    } else if (!b->graph || b->graph_cols != R || b->graph_t != t || b->graph_min_splits != p) {
         if (b->graph) (void)hipGraphExecDestroy(b->graph);
         if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
         b->graph = nullptr;
@@ -203,7 +203,7 @@ int lbn_commit_batch(lbn_prover *pr, const lbn_dwitness *const *ws, size_t count
         HIP_TRY(hipGraphInstantiate(&b->graph, gdef, nullptr, nullptr, 0));
         b->graph_cols = R;
         b->graph_t = t;
-        b->graph_persist = p;
+        b->graph_min_splits = p;
     }
     if (replay) {
         HIP_TRY(hipEventRecord(b->ev[0], st));

@@ -179,7 +179,7 @@ struct lbn_batch_ws {
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     size_t graph_cols = 0, last_cols = 0;
-    uint32_t graph_t = 0, graph_persist = 0, graph_splits = 1;  // the graph's schedule and its K ranges
+    uint32_t graph_t = 0, graph_min_splits = 0, graph_splits = 1;  // the graph's schedule: tiles a round, K ranges asked for and run
     std::vector<std::array<uint32_t, 3>> tuned;  // (width class, sched_t, K ranges asked for)
     hipEvent_t ev[2] = {};
     float ms = 0;

@@ -424,7 +424,9 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
         auto c = [&](const v4i &x) -> v4i { return FIRST ? z : x; };
         // Per step a thread of waves 0-3 issues 2 byte loads and 2 copies, one of waves 4-7 2 and 1:
         // the copy of A'(kb + 1) has 4 A_PRE (waves 0-3, issued in step kb - A_PRE) or 3 A_PRE
-        // (waves 4-7, step kb - 1 - A_PRE) loads after it.
+        // (waves 4-7, step kb - 1 - A_PRE) loads after it. The counts hold only if every wave
+        // issues these loads every step and uses the bytes (a wave that skipped its products would
+        // let the compiler drop its byte loads).
         if (wgrp) barrier_vm<3 * A_PRE>();  // this wave's copy of A'(kb + 1) is in LDS
         load_raw(min(kb + 1, kbs - 1));
         dma_a(min(kb + 1 + wgrp + A_PRE, kbs - 1), (int)((kb + 1 + wgrp + A_PRE) % A_BUFS));
