@@ -2,6 +2,7 @@
 //
 //   prepass_kernel        witness (F162, 24 B each) -> the two byte layouts the GEMMs stream
 //   tile_gemm_kernel      the int8-MFMA tile GEMM behind both the commitment and the fold
+//   afold_kernel          a GEMM's A' = E o A (the key's once, the challenges' per fold)
 //   commit_finish_kernel  y mod q -> raw rows and the 648 -> 4 x 162 decomposition
 //   chat_kernel           the challenges' transform c_j(-X^4) at every slot, as MFMA A fragments
 //   intt_kernel           a limb's inverse transform -> centered coefficients (the folded witness)
@@ -651,7 +652,8 @@ __global__ void commit_finish_kernel(const uint32_t *__restrict__ yacc, uint32_t
 // the fold's challenge transform
 // =============================================================================================
 
-// The A fragments of the fold GEMM: tile tau, K = column c, M = row m; the value at a row is
+// The challenges' transform in A fragment order (afold_kernel folds E into it for the fold GEMM):
+// tile tau, K = column c, M = row m; the value at a row is
 // c(-X^4) at the row's root, sum_i (-1)^{p_i} (1 - 2 s_i) root^{mult expo[m] p_i} (fold.rs:287-300
 // embeds c as the coefficient (-1)^m c_m of X^{4m}). One thread per (tile, column, row).
 // A block per (16 challenges, tile): the challenges and the root powers staged in LDS; thread
