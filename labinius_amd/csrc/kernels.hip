@@ -305,6 +305,11 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
     // half kh = kg >> 1 there.
     const int c16 = lane & 15, kg = lane >> 4;
     const int jl = 32 * wave + c16;  // this lane's input of block nb = 0 within the group
+    // A wave whose 32 batch elements all lie past the last (a narrow commitment's group) issues no
+    // products and no lookups. It still issues every load and copy and waits for its input bytes
+    // where the others use them (an empty asm statement reads them), so that the step barriers'
+    // load counts hold for it too.
+    const bool active = __builtin_amdgcn_readfirstlane(grp * GROUP_COLS + 32 * wave) < g.batches;
     const int off32 = c16 + 32 * (kg & 1), kh = kg >> 1;
     // A step holds one sub-ring coefficient n, whose bytes read LUT (par + n) & 1: byte 0 of a
     // lookup address is that parity and this lane's LUT copy, byte 1 comes from the input word
@@ -321,6 +326,7 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
     const __amdgpu_buffer_rsrc_t ars = uniform_rsrc(g.afrag + td.a_off, g.kbs * g.a_kb_stride);
 
     v4i raw[2], a0[5], a1[5], ax, bl[2], bh[2];
+    auto consume_raw = [&]() { asm volatile("" ::"v"(raw[0]), "v"(raw[1])); };
     auto load_raw = [&](uint32_t kb) {
 #pragma unroll
         for (int nb = 0; nb < 2; nb++)
@@ -404,8 +410,12 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
         if (st <= A_PRE || wgrp) dma_a(min((uint32_t)st, kbs - 1), st);
     __builtin_amdgcn_s_waitcnt(0);  // this thread's copies into LDS are complete
     __syncthreads();                // the LUT, A'(0) and waves 4-7's part of A'(1)
-    read_a0(ABUF_OFF);
-    build(0);
+    if (active) {
+        read_a0(ABUF_OFF);
+        build(0);
+    } else {
+        consume_raw();
+    }
 
     // The two waves of a SIMD (w and w + 4) run each step's halves, the products of step kb and
     // the lookups of step kb + 1, in opposite phase: waves 0-3 meet the step barrier between the
@@ -426,47 +436,53 @@ __device__ __forceinline__ void gemm_item(const GemmArgs &g, const TileDesc &td,
         // Per step a thread of waves 0-3 issues 2 byte loads and 2 copies, one of waves 4-7 2 and 1:
         // the copy of A'(kb + 1) has 4 A_PRE (waves 0-3, issued in step kb - A_PRE) or 3 A_PRE
         // (waves 4-7, step kb - 1 - A_PRE) loads after it. The counts hold only if every wave
-        // issues these loads every step and uses the bytes (a wave that skipped its products would
-        // let the compiler drop its byte loads).
+        // issues these loads every step and reads the bytes (an idle wave through consume_raw;
+        // with no reader the compiler would move its byte loads into the branch that reads them).
         if (wgrp) barrier_vm<3 * A_PRE>();  // this wave's copy of A'(kb + 1) is in LDS
         load_raw(min(kb + 1, kbs - 1));
         dma_a(min(kb + 1 + wgrp + A_PRE, kbs - 1), (int)((kb + 1 + wgrp + A_PRE) % A_BUFS));
         // the wave issuing products goes first at the SIMD's arbiter, the other wave's lookups
         // fill in around them (the wave's own priority; the stream's stays the default)
-        __builtin_amdgcn_s_setprio(1);
-        // The digit classes in turn over all ten (row block, input block) pairs, so that the two
-        // products sharing a mid accumulator are 24 MFMAs apart: a dependent MFMA waits for its
-        // accumulator in the pipe, and 16 x 16 x 64 products need about 16 in between to run at
-        // full rate.
+        if (active) {
+            __builtin_amdgcn_s_setprio(1);
+            // The digit classes in turn over all ten (row block, input block) pairs, so that the two
+            // products sharing a mid accumulator are 24 MFMAs apart: a dependent MFMA waits for its
+            // accumulator in the pipe, and 16 x 16 x 64 products need about 16 in between to run at
+            // full rate.
 #pragma unroll
-        for (int mb = 0; mb < 5; mb++)
+            for (int mb = 0; mb < 5; mb++)
 #pragma unroll
-            for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bl[nb], c(acc[mb][nb][0]), 0, 0, 0);
+                for (int nb = 0; nb < 2; nb++)
+                    acc[mb][nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bl[nb], c(acc[mb][nb][0]), 0, 0, 0);
 #pragma unroll
-        for (int mb = 0; mb < 5; mb++)
+            for (int mb = 0; mb < 5; mb++)
 #pragma unroll
-            for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], c(acc[mb][nb][1]), 0, 0, 0);
+                for (int nb = 0; nb < 2; nb++)
+                    acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[mb], bh[nb], c(acc[mb][nb][1]), 0, 0, 0);
 #pragma unroll
-        for (int nb = 0; nb < 2; nb++) {
-            xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], c(xacc[nb][0]), 0, 0, 0);
-            xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], c(xacc[nb][1]), 0, 0, 0);
+            for (int nb = 0; nb < 2; nb++) {
+                xacc[nb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bl[nb], c(xacc[nb][0]), 0, 0, 0);
+                xacc[nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ax, bh[nb], c(xacc[nb][1]), 0, 0, 0);
+            }
+#pragma unroll
+            for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++)
+                    acc[mb][nb][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bh[nb], c(acc[mb][nb][2]), 0, 0, 0);
+#pragma unroll
+            for (int mb = 0; mb < 5; mb++)
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++)
+                    acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
+            __builtin_amdgcn_s_setprio(0);
         }
-#pragma unroll
-        for (int mb = 0; mb < 5; mb++)
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bh[nb], c(acc[mb][nb][2]), 0, 0, 0);
-#pragma unroll
-        for (int mb = 0; mb < 5; mb++)
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++)
-                acc[mb][nb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1[mb], bl[nb], acc[mb][nb][1], 0, 0, 0);
-        __builtin_amdgcn_s_setprio(0);
         if (!wgrp) barrier_vm<4 * A_PRE>();  // this wave's copy of A'(kb + 1) is in LDS
-        build(kb + 1);                                    // its bytes were loaded before the products
-        read_a0(ABUF_OFF + ((kb + 1) % A_BUFS) * A_LDS);  // A'(kb + 1) is complete: both phases are past barrier kb
+        if (active) {
+            build(kb + 1);                                    // its bytes were loaded before the products
+            read_a0(ABUF_OFF + ((kb + 1) % A_BUFS) * A_LDS);  // A'(kb + 1) is complete: both phases are past barrier kb
+        } else {
+            consume_raw();
+        }
     };
     step(0, std::true_type{});
     __builtin_amdgcn_sched_barrier(0);  // the peeled first step stays apart from the loop
